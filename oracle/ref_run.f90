@@ -1,0 +1,109 @@
+! Test infrastructure only (checker and CPU baseline, never the product path).
+!
+! Runs the reference osinco3d time loop on the CPU with the reference's own
+! modules, compiled from the reference sources in place by oracle/Makefile.
+! It follows the step sequence of src/osinco3d_main.f90:97-188 (old_values,
+! predict_velocity, correct_pression, correct_velocity, the two diagnostic
+! divergences with function_stats, print_velocity_values, compute_cfl) but
+! leaves out the file output of the driver (slices, XDMF, checkpoints,
+! gnuplot), which needs no timing and writes outside the run directory.
+! The full driver cannot be built here: src/initial_conditions.f90 needs
+! src/noise_module.f90, which needs FFTW3, absent from this image.
+!
+! Usage (run directory holds parameters.o3d, read by the reference's own
+! parameters() from src/initialization.f90:114-151):
+!   o3d_ref_run <out.bin> [dump_every]
+! typesim = 2 : Taylor-Green initial field, same formula as
+!               src/initial_conditions.f90:157-162 (restated here because that
+!               module cannot be built without FFTW);
+! typesim = 0 : initial field read by read_fields (src/IOfunctions.f90:406-470)
+!               from fields.bin.
+! Output: stream file of records [itime(int32), nx, ny, nz (int32), time,
+! ux, uy, uz, pp (float64, column-major)]: the initial field (itime 0), then
+! every step that is a multiple of dump_every, and the last step.
+! Per-step wall time goes to stdout as "O3D_STEP_SECONDS <itime> <seconds>".
+program o3d_ref_run
+  use initialization
+  use integration
+  use diffoper
+  use functions
+  use utils
+  use IOfunctions
+  implicit none
+  character(len=256) :: outname, arg
+  integer :: dump_every, i, j, k, iunit
+  integer(8) :: c0, c1, crate
+  real(kind=8) :: t_step
+
+  call get_command_argument(1, outname)
+  dump_every = 0
+  if (command_argument_count() >= 2) then
+     call get_command_argument(2, arg)
+     read(arg, *) dump_every
+  end if
+
+  call parameters()
+  call variables()
+  call schemes()
+
+  if (typesim == 2) then
+     do k = 1, nz
+        do j = 1, ny
+           do i = 1, nx
+              ux(i,j,k) =  ratio * u0/l0 * sin(x(i)) * cos(y(j))* cos(z(k))
+              uy(i,j,k) = -ratio * u0/l0 * cos(x(i)) * sin(y(j))* cos(z(k))
+              uz(i,j,k) = 0.d0
+              pp(i,j,k) = 0.0625d0 * (cos(2.d0*x(i)) + cos(2.d0*y(j))) * &
+                   (cos(2.d0*z(k)) + 2.d0)
+           end do
+        end do
+     end do
+     time0 = 0.d0
+  else
+     fields_name_file = 'fields.bin'
+     call read_fields(x, y, z, ux, uy, uz, pp, phi, nx, ny, nz, time0, fields_name_file)
+     time = time0
+     phi = 0.d0
+  end if
+  nu_t = 0.d0
+
+  open(newunit=iunit, file=trim(outname), status='replace', access='stream', &
+       form='unformatted', action='write')
+  write(iunit) int(0, 4), int(nx, 4), int(ny, 4), int(nz, 4)
+  write(iunit) time0
+  write(iunit) ux, uy, uz, pp
+  call system_clock(count_rate=crate)
+  do itime = itstart, itstop
+     call system_clock(c0)
+     time = time0 + itime * dt
+     call old_values(ux, uy, uz, old_ux, old_uy, old_uz, nx, ny, nz)
+     call predict_velocity(ux_pred, uy_pred, uz_pred, ux, uy, uz, &
+          fux, fuy, fuz, re, adt, bdt, cdt, itime, itscheme, &
+          dx, dy, dz, nx, ny, nz, iles, cs, delta, nu_t)
+     call correct_pression(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, &
+          nx, ny, nz, dt, omega, eps, kmax, idyn, multigrid)
+     call correct_velocity(ux, uy, uz, ux_pred, uy_pred, uz_pred, &
+          pp, dt, dx, dy, dz, nx, ny, nz)
+     call divergence(divu_pred, ux_pred, uy_pred, uz_pred, &
+          dx, dy, dz, nx, ny, nz, 1)
+     divu_stats = function_stats(divu_pred, nx, ny, nz)
+     call print_divu_statistics(divu_stats, .true.)
+     call divergence(divu, ux, uy, uz, dx, dy, dz, nx, ny, nz, 1)
+     divu_stats = function_stats(divu, nx, ny, nz)
+     call print_divu_statistics(divu_stats, .false.)
+     call print_velocity_values(ux, uy, uz)
+     call compute_cfl(cflx, cfly, cflz, ux, uy, uz, dx, dy, dz, dt)
+     call print_cfl(cflx, cfly, cflz)
+     call system_clock(c1)
+     t_step = real(c1 - c0, kind=8) / real(crate, kind=8)
+     write(*, '(A17,I8,ES16.8)') 'O3D_STEP_SECONDS ', itime, t_step
+     if ((dump_every > 0 .and. modulo(itime, max(dump_every, 1)) == 0) .or. &
+          itime == itstop) then
+        write(iunit) int(itime, 4), int(nx, 4), int(ny, 4), int(nz, 4)
+        write(iunit) time
+        write(iunit) ux, uy, uz, pp
+        flush(iunit)
+     end if
+  end do
+  close(iunit)
+end program o3d_ref_run
