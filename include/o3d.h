@@ -1,0 +1,192 @@
+/* osinco3d time-step path on MI355X: the C ABI of libo3d.so.
+ *
+ * This is the drop-in boundary.  The reference is a serial Fortran code whose
+ * time step calls module procedures by name and derivative / Poisson routines
+ * through procedure pointers (src/initialization.f90:86-110, bound in
+ * schemes() at :226-304).  The Fortran shim modules in osinco3d_amd/fortran/
+ * keep those module and procedure names and dummy-argument lists and call the
+ * functions below through ISO_C_BINDING.
+ *
+ * Conventions
+ *  - Fields are contiguous column-major (nx, ny, nz) arrays, x fastest, as in
+ *    the reference.  float64 unless the name ends in _f32.
+ *  - Pointers may be host or device memory: host arrays are staged through
+ *    device buffers (correct, slower); device arrays are used in place.
+ *  - bc[3]: per-axis boundary type, 0 = periodic, 1 = free-slip
+ *    (src/initialization.f90:77-78).  Both faces of an axis share one type,
+ *    as schemes() requires.
+ *  - Status: 0 = ok; 1 = velocity diverged (the reference prints
+ *    "Velocity has diverged" and stops, src/integration.f90:309-325);
+ *    < 0 = runtime or argument error, text from o3d_last_error().
+ *  - One caller thread, one HIP stream; every call returns with its host
+ *    outputs valid.  No C++ exception crosses this interface.
+ */
+#ifndef O3D_H
+#define O3D_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- runtime */
+int o3d_init(int device);
+/* Launch stream (a hipStream_t); NULL = the default stream. */
+int o3d_set_stream(void *stream);
+const char *o3d_last_error(void);
+int o3d_sync(void);
+/* Library version string. */
+const char *o3d_version(void);
+
+/* ------------------------------------------------------ operator entry points */
+
+/* One routine of src/derivation.f90 (interface der_type(df, f, d),
+ * src/initialization.f90:86-91).  which = axis*6 + (order-1)*3 + variant,
+ * variant 0 = "_00" periodic, 1 = "p_11" free-slip even, 2 = "i_11" free-slip
+ * odd; 18 = derz_2dsim, 19 = derzz_2dsim.  E.g. which 0 = derx_00
+ * (src/derivation.f90:6-60), 8 = deryi_11 (:269-321), 17 = derzzi_11
+ * (:836-883). */
+int o3d_der(int which, double *df, const double *f, int nx, int ny, int nz, double d);
+int o3d_der_f32(int which, float *df, const float *f, int nx, int ny, int nz, double d);
+
+/* divergence(divf, fx, fy, fz, dx, dy, dz, nx, ny, nz, odd),
+ * src/differential_operators.f90:7-38.  divf = div / divisor (divisor = 1 for
+ * the plain operator, dt for the Poisson rhs of src/integration.f90:239).
+ * divf may be NULL (statistics only).  stats6 (host, may be NULL) receives
+ * function_stats of the undivided divergence (src/functions.f90:27-63). */
+int o3d_divergence(double *divf, const double *fx, const double *fy, const double *fz,
+                   double dx, double dy, double dz, int nx, int ny, int nz, const int *bc,
+                   int odd, double divisor, double *stats6);
+
+/* calculate_nu_t(nu_t, ux, uy, uz, dx, dy, dz, cs, delta),
+ * src/les_turbulence.f90:10-97; stats6 (host, may be NULL) = function_stats
+ * of nu_t (the reference prints min, max, mean). */
+int o3d_nu_t(double *nu_t, const double *ux, const double *uy, const double *uz, double dx,
+             double dy, double dz, double cs, double delta, int nx, int ny, int nz,
+             const int *bc, double *stats6);
+
+/* predict_velocity, src/integration.f90:14-197, same argument meaning:
+ * fux/fuy/fuz are (nx,ny,nz,3) history arrays, slot 1 = newest, shifted by
+ * copy at the end as the reference does; adt/bdt/cdt are the 3-element
+ * coefficient arrays of src/initialization.f90:194-202.  Returns -2 for an
+ * unrecognized itscheme (the reference stops). */
+int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz_pred, const double *ux,
+                         const double *uy, const double *uz, double *fux, double *fuy,
+                         double *fuz, double re, const double *adt, const double *bdt,
+                         const double *cdt, int itime, int itscheme, double dx, double dy,
+                         double dz, int nx, int ny, int nz, int iles, double cs, double delta,
+                         double *nu_t, const int *bc);
+
+/* poisson_solver_0000 / _0011 / _111111 (interface poi_type,
+ * src/initialization.f90:93-102; bodies src/poisson.f90:6-381), red-black
+ * ordering.  solver: 0 = "0000", 1 = "0011", 2 = "111111".  omega is in/out
+ * (dynamic omega persists, idyn = 1).  exit_iter = the iteration the
+ * reference prints on exit (kmax+1 when kmax ran out); reason 0 = dmax < eps,
+ * 1 = stagnation, 2 = kmax.  Any of the three out pointers may be NULL. */
+int o3d_poisson(double *pp, const double *rhs, double dx, double dy, double dz, int nx, int ny,
+                int nz, double *omega, double eps, int kmax, int idyn, int solver,
+                int *exit_iter, double *dmax, int *reason);
+
+/* correct_pression, src/integration.f90:199-255 (SOR branch): divergence of
+ * the predicted velocity (odd variants), rhs = div/dt, then the Poisson solver
+ * that schemes() binds for bc (src/initialization.f90:283-301). */
+int o3d_correct_pression(double *pp, const double *ux_pred, const double *uy_pred,
+                         const double *uz_pred, double dx, double dy, double dz, int nx, int ny,
+                         int nz, double dt, double *omega, double eps, int kmax, int idyn,
+                         const int *bc, int *exit_iter, double *dmax, int *reason);
+
+/* correct_velocity, src/integration.f90:257-330.  vstats (host, may be
+ * NULL) receives per component c: [4c+0] min, [4c+1] max, [4c+2] max |u|,
+ * [4c+3] NaN flag.  Returns 1 when the reference's guard (NaN, or
+ * maxval(u) > 1000) fires. */
+int o3d_correct_velocity(double *ux, double *uy, double *uz, const double *ux_pred,
+                         const double *uy_pred, const double *uz_pred, const double *pp,
+                         double dt, double dx, double dy, double dz, int nx, int ny, int nz,
+                         const int *bc, double *vstats);
+
+/* function_stats, src/functions.f90:27-63: min, max, mean, then the 1-based
+ * (i, j, k) of the first maximum in column-major order. */
+int o3d_function_stats(const double *f, int nx, int ny, int nz, double *stats6);
+
+/* statistics_calc, src/utils.f90:243-375: out17 = t, E_k, eps, eps2,
+ * enstrophy, <u^2>, <v^2>, <w^2>, then the nine <(du_c/dx_a)^2> in the order
+ * of the stats.dat columns (src/IOfunctions.f90:552-555). */
+int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx, int ny, int nz,
+                   double dx, double dy, double dz, double re, double t, const int *bc,
+                   double *out17);
+
+/* ------------------------------------------------- device-resident solver */
+
+/* Precision forms. */
+#define O3D_FP64 0  /* plain FP64: the drop-in (parity with the reference) */
+#define O3D_MIXED 1 /* FP32 stencils, AB update and SOR sweeps; FP64-refined Poisson */
+
+/* Warm start of the Poisson solve. */
+#define O3D_WARM_PREVIOUS 0 /* previous step's pressure (the reference's behaviour) */
+#define O3D_WARM_LEARNED 1  /* learned correction, previous pressure as fallback */
+
+typedef struct o3d_params {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  int bc[3];
+  int itscheme;         /* 1 Euler, 2 AB2, 3 AB3 */
+  double dt, re;
+  double adt[3], bdt[3], cdt[3];
+  int iles;
+  double cs, delta;
+  double omega, eps;
+  int kmax, idyn;
+  int precision;        /* O3D_FP64 or O3D_MIXED */
+  int warm_start;       /* O3D_WARM_PREVIOUS or O3D_WARM_LEARNED */
+  int inner_max;        /* O3D_MIXED: FP32 sweeps between FP64 refinements (0 = default) */
+} o3d_params;
+
+typedef struct o3d_step_info {
+  int sor_exit_iter;      /* as the reference prints it */
+  int sor_reason;         /* 0 eps, 1 stagnation, 2 kmax */
+  int sweeps_f64;         /* FP64 red-black sweeps this step */
+  int sweeps_f32;         /* FP32 red-black sweeps this step (O3D_MIXED) */
+  int refinements;        /* FP64 residual refinements (O3D_MIXED) */
+  int warm_used;          /* 1 when the learned warm start was accepted */
+  int diverged;           /* reference's "Velocity has diverged" guard */
+  int has_nan;
+  double sor_dmax;        /* dmax of the last (FP64) sweep */
+  double omega;           /* omega after the solve */
+  double poisson_ms;      /* device time of the Poisson solve */
+  double step_ms;         /* device time of the whole step */
+  double warm_dmax[2];    /* first-sweep dmax: [0] previous pressure, [1] learned */
+  double divu_pred[6];    /* function_stats of div(u*) */
+  double divu[6];         /* function_stats of div(u) */
+  double vel_min[3], vel_max[3], vel_amax[3];
+  double cfl[3];          /* max|u_c| dt / d_c, src/utils.f90:178-208 */
+  double nu_t[6];         /* function_stats of nu_t (iles = 1) */
+} o3d_step_info;
+
+int o3d_solver_create(const o3d_params *p, void **handle);
+int o3d_solver_destroy(void *handle);
+/* Host or device float64 arrays in, copied into the solver's own fields. */
+int o3d_solver_upload(void *handle, const double *ux, const double *uy, const double *uz,
+                      const double *pp);
+int o3d_solver_download(void *handle, double *ux, double *uy, double *uz, double *pp);
+/* One time step of src/osinco3d_main.f90:97-128: predict_velocity,
+ * correct_pression, correct_velocity and the driver's per-step diagnostics
+ * (both divergences, velocity min/max, CFL), with itime as in the driver. */
+int o3d_solver_step(void *handle, int itime, o3d_step_info *info);
+int o3d_solver_statistics(void *handle, double t, double *out17);
+/* Device pointer of a solver field: 0..2 u, 3..5 u*, 6 p (float64), 7 rhs,
+ * 8 nu_t.  Element type is float64 for O3D_FP64 and float32 otherwise
+ * (p is float64 in both forms). */
+void *o3d_solver_field(void *handle, int which);
+/* Time of SOR sweeps measured with HIP events on the launch stream over the
+ * last step: total ms of complete sweep batches and the number of sweeps in
+ * them, per precision (index 0 FP64, 1 FP32). */
+int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2);
+/* Attach the learned warm start (an o3d_nn handle, see o3d_nn.h). */
+int o3d_solver_set_warm_model(void *handle, void *nn);
+/* Keep a copy of each step's (rhs, pressure) pair for training: when
+ * enabled, the pair is written to the given device buffers (float32, n cells
+ * each) at slot `slot`. */
+int o3d_solver_capture(void *handle, float *rhs_out, float *p_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1151 @@
+// C ABI of libo3d.so (declarations and conventions in include/o3d.h).
+//
+// Two layers:
+//  * operator entry points with the reference routines' argument lists; host
+//    arrays are staged through device buffers, device arrays used in place;
+//  * a device-resident solver that runs whole time steps with all fields in
+//    HBM (history ring rotated by index, not copied; statistics reduced on
+//    the device; one small read-back per SOR batch and per step).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/o3d.h"
+#include "o3d_nn.h"
+#include "o3d_sor.h"
+#include "o3d_stencil.h"
+
+using namespace o3d;
+
+namespace {
+
+hipStream_t g_stream = nullptr;
+std::string g_err;
+bool g_init = false;
+
+int fail(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HIPCHK(...)                                                                       \
+  do {                                                                                    \
+    hipError_t e_ = (__VA_ARGS__);                                                        \
+    if (e_ != hipSuccess)                                                                 \
+      return fail("%s failed: %s (%s:%d)", #__VA_ARGS__, hipGetErrorString(e_), __FILE__,   \
+                  __LINE__);                                                              \
+  } while (0)
+#define CHK(x)                \
+  do {                        \
+    int r_ = (x);             \
+    if (r_ != 0) return r_;   \
+  } while (0)
+
+int ensure_init() {
+  if (g_init) return 0;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail("no HIP device available (%s)", hipGetErrorString(e));
+  HIPCHK(hipSetDevice(0));
+  g_init = true;
+  return 0;
+}
+
+bool is_device_ptr(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.isManaged;
+}
+
+int check_grid(int nx, int ny, int nz) {
+  if (nx < 6 || ny < 6 || nz < 6)
+    return fail("grid %dx%dx%d: every axis needs at least 6 nodes (6th-order stencil)", nx, ny,
+                nz);
+  if ((long long)nx * ny * nz >= (1ll << 31)) return fail("grid too large for 32-bit indexing");
+  return 0;
+}
+
+int check_bc(const int *bc) {
+  if (!bc) return fail("bc is NULL");
+  for (int a = 0; a < 3; ++a)
+    if (bc[a] != PERIODIC && bc[a] != FREE_SLIP)
+      return fail("Unrecognized boundary type %d on axis %d", bc[a], a);
+  return 0;
+}
+
+// Staging of host arrays through device buffers for one call.
+struct Staged {
+  struct Out { void *dev; void *host; size_t bytes; };
+  std::vector<void *> allocs;
+  std::vector<Out> outs;
+  ~Staged() {
+    for (void *p : allocs) (void)hipFree(p);
+  }
+  template <typename T>
+  int in(const T *p, size_t n, const T **dev) {
+    if (!p) { *dev = nullptr; return 0; }
+    if (is_device_ptr(p)) { *dev = p; return 0; }
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(T)));
+    allocs.push_back(d);
+    HIPCHK(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
+    *dev = (const T *)d;
+    return 0;
+  }
+  // Output (copy_in: the array is also read, e.g. inout).
+  template <typename T>
+  int out(T *p, size_t n, T **dev, bool copy_in) {
+    if (!p) { *dev = nullptr; return 0; }
+    if (is_device_ptr(p)) { *dev = p; return 0; }
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(T)));
+    allocs.push_back(d);
+    if (copy_in) HIPCHK(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
+    outs.push_back({d, (void *)p, n * sizeof(T)});
+    *dev = (T *)d;
+    return 0;
+  }
+  template <typename T>
+  int scratch(size_t n, T **dev) {
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(T)));
+    allocs.push_back(d);
+    *dev = (T *)d;
+    return 0;
+  }
+  int finish() {
+    for (const Out &o : outs)
+      HIPCHK(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+  }
+};
+
+template <typename T>
+void fill_coef(T dst[3][3], const double h[3], bool second) {
+  for (int a = 0; a < 3; ++a) {
+    if (second) {
+      const D2Coef c = d2coef(h[a]);
+      dst[a][0] = T(c.a); dst[a][1] = T(c.b); dst[a][2] = T(c.c);
+    } else {
+      const D1Coef c = d1coef(h[a]);
+      dst[a][0] = T(c.a); dst[a][1] = T(c.b); dst[a][2] = T(c.c);
+    }
+  }
+}
+
+// function_stats layout from a finalized record (src/functions.f90:57-62).
+void stats6_from(const FieldPartial &r, int nx, int ny, int nz, double *st) {
+  st[0] = r.vmin;
+  st[1] = r.vmax;
+  st[2] = r.sum / (double)((long long)nx * ny * nz);
+  const int ix = r.imax == 0x7fffffff ? 0 : r.imax;
+  st[3] = ix % nx + 1;
+  st[4] = (ix / nx) % ny + 1;
+  st[5] = ix / (nx * ny) + 1;
+}
+
+// Solver kind that schemes() binds from the x and y boundary types.
+int solver_from_bc(const int *bc) {
+  if (bc[0] == PERIODIC && bc[1] == PERIODIC) return SOR_0000;
+  if (bc[0] == PERIODIC && bc[1] == FREE_SLIP) return SOR_0011;
+  if (bc[0] == FREE_SLIP && bc[1] == FREE_SLIP) return SOR_111111;
+  return -1;
+}
+
+int check_red_black(int kind, int nx, int ny, int nz) {
+  int per[3];
+  solver_periodic(kind, per);
+  const int n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a)
+    if (per[a] && (n[a] & 1))
+      return fail("red-black SOR needs an even node count along periodic axis %d (n=%d)", a,
+                  n[a]);
+  return 0;
+}
+
+struct SorConst { double ox, oy, oz, A; };
+SorConst sor_const(double dx, double dy, double dz) {
+  // src/poisson.f90:42-51
+  SorConst s;
+  const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+  s.ox = 1.0 / dx2; s.oy = 1.0 / dy2; s.oz = 1.0 / dz2;
+  const double tx = 2.0 * s.ox, ty = 2.0 * s.oy, tz = 2.0 * s.oz;
+  s.A = -(tx + ty + tz);
+  return s;
+}
+
+// Batched launcher of red-black iterations with device-side stop control.
+// Buffers: buf[0] holds the start iterate; iteration it reads buf[(it-1)&1].
+template <typename T, typename TR>
+struct SorRun {
+  SorArgs<T, TR> a;
+  SorState *st = nullptr;    // device
+  SorState *hst = nullptr;   // pinned host copy
+  T *buf[2] = {nullptr, nullptr};
+  // Timing of complete batches (all launches did a sweep).
+  double ms_full = 0.0;
+  int sweeps_full = 0;
+  std::vector<hipEvent_t> *events = nullptr;
+
+  int run(double omega, int first_batch, int *sweeps_out) {
+    HIPCHK(launch_sor_init(st, omega, 0, g_stream));
+    int it = 1;
+    int batch = std::max(1, first_batch);
+    int done_before = 0;
+    while (true) {
+      hipEvent_t e0 = (*events)[0], e1 = (*events)[1];
+      HIPCHK(hipEventRecord(e0, g_stream));
+      for (int b = 0; b < batch; ++b, ++it) {
+        a.it = it;
+        a.pin = buf[(it - 1) & 1];
+        a.pout = buf[it & 1];
+        HIPCHK(launch_sor_iteration<T, TR>(a, g_stream));
+        if (it > a.kmax) { ++it; break; }
+      }
+      HIPCHK(hipEventRecord(e1, g_stream));
+      HIPCHK(hipMemcpyAsync(hst, st, sizeof(SorState), hipMemcpyDeviceToHost, g_stream));
+      HIPCHK(hipStreamSynchronize(g_stream));
+      const int launched = it - 1;  // iterations numbered 1..launched were launched
+      if (!hst->done) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        ms_full += ms;
+        sweeps_full += launched - done_before;
+        done_before = launched;
+        if (launched > a.kmax) return fail("SOR state did not close after kmax");
+        batch = 8;
+        continue;
+      }
+      break;
+    }
+    *sweeps_out = hst->sweeps;
+    return 0;
+  }
+};
+
+}  // namespace
+
+// ================================================================ runtime
+extern "C" int o3d_init(int device) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail("device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  g_init = true;
+  return 0;
+}
+
+extern "C" int o3d_set_stream(void *stream) {
+  g_stream = (hipStream_t)stream;
+  return 0;
+}
+
+extern "C" const char *o3d_last_error(void) { return g_err.c_str(); }
+
+extern "C" int o3d_sync(void) {
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+extern "C" const char *o3d_version(void) { return "osinco3d-amd 0.1 gfx950"; }
+
+// ================================================================ operators
+extern "C" int o3d_der(int which, double *df, const double *f, int nx, int ny, int nz,
+                       double d) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  if (which < 0 || which > 19) return fail("derivative selector %d out of range", which);
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  const double *fd;
+  double *dfd;
+  CHK(s.in(f, n, &fd));
+  CHK(s.out(df, n, &dfd, false));
+  HIPCHK(launch_der<double>(which, dfd, fd, nx, ny, nz, d, g_stream));
+  return s.finish();
+}
+
+extern "C" int o3d_der_f32(int which, float *df, const float *f, int nx, int ny, int nz,
+                           double d) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  if (which < 0 || which > 19) return fail("derivative selector %d out of range", which);
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  const float *fd;
+  float *dfd;
+  CHK(s.in(f, n, &fd));
+  CHK(s.out(df, n, &dfd, false));
+  HIPCHK(launch_der<float>(which, dfd, fd, nx, ny, nz, d, g_stream));
+  return s.finish();
+}
+
+namespace {
+int finalize_to_host(const FieldPartial *part, int nparts, int stride, int first,
+                     FieldPartial *dev_out, FieldPartial *host_out) {
+  HIPCHK(launch_finalize_partials(part, nparts, stride, first, dev_out, g_stream));
+  HIPCHK(hipMemcpyAsync(host_out, dev_out, sizeof(FieldPartial), hipMemcpyDeviceToHost,
+                        g_stream));
+  return 0;
+}
+}  // namespace
+
+extern "C" int o3d_divergence(double *divf, const double *fx, const double *fy, const double *fz,
+                              double dx, double dy, double dz, int nx, int ny, int nz,
+                              const int *bc, int odd, double divisor, double *stats6) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  DivArgs<double> a;
+  CHK(s.in(fx, n, &a.f[0]));
+  CHK(s.in(fy, n, &a.f[1]));
+  CHK(s.in(fz, n, &a.f[2]));
+  CHK(s.out(divf, n, &a.out, false));
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.odd = odd != 0;
+  a.tl = make_tiling(nx, ny, nz);
+  const double h[3] = {dx, dy, dz};
+  fill_coef<double>(a.d1, h, false);
+  a.divisor = divisor;
+  FieldPartial *fin;
+  CHK(s.scratch(a.tl.blocks(), &a.part));
+  CHK(s.scratch(1, &fin));
+  HIPCHK(launch_divergence<double>(a, g_stream));
+  FieldPartial r;
+  CHK(finalize_to_host(a.part, a.tl.blocks(), 1, 0, fin, &r));
+  CHK(s.finish());
+  if (stats6) stats6_from(r, nx, ny, nz, stats6);
+  return 0;
+}
+
+namespace {
+// Shared by o3d_nu_t and o3d_predict_velocity: the fused predict kernel with
+// the reference's slot semantics.
+int predict_common(double *up[3], const double *u[3], double *f1[3], const double *f2[3],
+                   const double *f3[3], double *nut, int nh, bool les, double re, double adu,
+                   double bdu, double cdu, double cs, double delta, const double h[3], int nx,
+                   int ny, int nz, const int *bc, FieldPartial *nut_part) {
+  PredictArgs<double> a;
+  for (int c = 0; c < 3; ++c) {
+    a.u[c] = u[c]; a.up[c] = up[c]; a.f1[c] = f1[c]; a.f2[c] = f2[c]; a.f3[c] = f3[c];
+  }
+  a.nut = nut;
+  a.nut_part = nut_part;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.tl = make_tiling(nx, ny, nz);
+  fill_coef<double>(a.d1, h, false);
+  fill_coef<double>(a.d2, h, true);
+  a.adu = adu; a.bdu = bdu; a.cdu = cdu;
+  a.onere = 1.0 / re;
+  const double csd = cs * delta;
+  a.csd2 = csd * csd;
+  HIPCHK(launch_predict<double>(a, nh, les, g_stream));
+  return 0;
+}
+}  // namespace
+
+extern "C" int o3d_nu_t(double *nu_t, const double *ux, const double *uy, const double *uz,
+                        double dx, double dy, double dz, double cs, double delta, int nx, int ny,
+                        int nz, const int *bc, double *stats6) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  const double *u[3];
+  double *nut;
+  CHK(s.in(ux, n, &u[0]));
+  CHK(s.in(uy, n, &u[1]));
+  CHK(s.in(uz, n, &u[2]));
+  CHK(s.out(nu_t, n, &nut, false));
+  // nu_t shares the fused kernel: the velocity outputs go to scratch.
+  double *up[3], *f1[3];
+  for (int c = 0; c < 3; ++c) {
+    CHK(s.scratch(n, &up[c]));
+    CHK(s.scratch(n, &f1[c]));
+  }
+  const double *f2[3] = {nullptr, nullptr, nullptr};
+  const Tiling tl = make_tiling(nx, ny, nz);
+  FieldPartial *part, *fin;
+  CHK(s.scratch(tl.blocks(), &part));
+  CHK(s.scratch(1, &fin));
+  const double h[3] = {dx, dy, dz};
+  CHK(predict_common(up, u, f1, f2, f2, nut, 1, true, 1.0, 0.0, 0.0, 0.0, cs, delta, h, nx, ny,
+                     nz, bc, part));
+  FieldPartial r;
+  CHK(finalize_to_host(part, tl.blocks(), 1, 0, fin, &r));
+  CHK(s.finish());
+  if (stats6) stats6_from(r, nx, ny, nz, stats6);
+  return 0;
+}
+
+extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz_pred,
+                                    const double *ux, const double *uy, const double *uz,
+                                    double *fux, double *fuy, double *fuz, double re,
+                                    const double *adt, const double *bdt, const double *cdt,
+                                    int itime, int itscheme, double dx, double dy, double dz,
+                                    int nx, int ny, int nz, int iles, double cs, double delta,
+                                    double *nu_t, const int *bc) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  // Coefficient choice, src/integration.f90:84-105.
+  int ci;
+  if (itscheme == 1 || itime == 1) ci = 0;
+  else if (itscheme == 2 || itime == 2) ci = 1;
+  else if (itscheme == 3) ci = 2;
+  else { fail("itscheme: %d unrecognized", itscheme); return -2; }
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  const double *u[3];
+  double *up[3], *F[3], *nut;
+  CHK(s.in(ux, n, &u[0]));
+  CHK(s.in(uy, n, &u[1]));
+  CHK(s.in(uz, n, &u[2]));
+  CHK(s.out(ux_pred, n, &up[0], false));
+  CHK(s.out(uy_pred, n, &up[1], false));
+  CHK(s.out(uz_pred, n, &up[2], false));
+  CHK(s.out(fux, 3 * n, &F[0], true));
+  CHK(s.out(fuy, 3 * n, &F[1], true));
+  CHK(s.out(fuz, 3 * n, &F[2], true));
+  CHK(s.out(nu_t, n, &nut, false));
+  double *f1[3];
+  const double *f2[3], *f3[3];
+  for (int c = 0; c < 3; ++c) { f1[c] = F[c]; f2[c] = F[c] + n; f3[c] = F[c] + 2 * n; }
+  const Tiling tl = make_tiling(nx, ny, nz);
+  FieldPartial *part;
+  CHK(s.scratch(tl.blocks(), &part));
+  const double h[3] = {dx, dy, dz};
+  const bool les = iles == 1;
+  if (!les && nut) HIPCHK(launch_fill_f64(nut, 0.0, (int)n, g_stream));
+  CHK(predict_common(up, u, f1, f2, f3, les ? nut : nullptr, ci + 1, les, re, adt[ci], bdt[ci],
+                     cdt[ci], cs, delta, h, nx, ny, nz, bc, part));
+  // History shift by copy, src/integration.f90:176-188.
+  for (int c = 0; c < 3; ++c) {
+    if (itscheme == 2) {
+      HIPCHK(launch_copy<double>(F[c] + n, F[c], (int)n, g_stream));
+    } else if (itscheme == 3) {
+      HIPCHK(launch_copy<double>(F[c] + 2 * n, F[c] + n, (int)n, g_stream));
+      HIPCHK(launch_copy<double>(F[c] + n, F[c], (int)n, g_stream));
+    }
+  }
+  return s.finish();
+}
+
+namespace {
+int poisson_dev(double *pp, const double *rhs, double dx, double dy, double dz, int nx, int ny,
+                int nz, double *omega, double eps, int kmax, int idyn, int solver, int *exit_iter,
+                double *dmax, int *reason, Staged &s) {
+  if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
+  CHK(check_red_black(solver, nx, ny, nz));
+  if (kmax < 1) return fail("kmax must be >= 1");
+  const int n = nx * ny * nz;
+  SorRun<double, double> r;
+  CHK(s.scratch(1, &r.st));
+  HIPCHK(hipHostMalloc((void **)&r.hst, sizeof(SorState)));
+  std::vector<hipEvent_t> ev(2);
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  r.events = &ev;
+  double *other;
+  CHK(s.scratch(n, &other));
+  r.buf[0] = pp;
+  r.buf[1] = other;
+  SorArgs<double, double> &a = r.a;
+  a.rhs = rhs;
+  a.st = r.st;
+  a.log = nullptr;
+  a.log_len = 0;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  solver_periodic(solver, a.per);
+  a.tl = make_sor_tiling(nx, ny, nz);
+  const SorConst c = sor_const(dx, dy, dz);
+  a.ox = c.ox; a.oy = c.oy; a.oz = c.oz; a.A = c.A;
+  a.eps = eps;
+  a.factor = solver == SOR_0011 ? 1.01 : 1.05;  // src/poisson.f90:38,162,285
+  a.kmax = kmax;
+  a.idyn = idyn;
+  int sweeps = 0;
+  int rc = r.run(*omega, 16, &sweeps);
+  if (rc == 0 && (sweeps & 1)) rc = launch_copy<double>(pp, other, n, g_stream) == hipSuccess ? 0 : fail("copy failed");
+  if (rc == 0) {
+    *omega = r.hst->omega_final;
+    if (exit_iter) *exit_iter = r.hst->exit_iter;
+    if (dmax) *dmax = r.hst->dmax_final;
+    if (reason) *reason = r.hst->reason;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipHostFree(r.hst);
+  return rc;
+}
+}  // namespace
+
+extern "C" int o3d_poisson(double *pp, const double *rhs, double dx, double dy, double dz, int nx,
+                           int ny, int nz, double *omega, double eps, int kmax, int idyn,
+                           int solver, int *exit_iter, double *dmax, int *reason) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  if (!omega) return fail("omega is NULL");
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  double *p;
+  const double *r;
+  CHK(s.out(pp, n, &p, true));
+  CHK(s.in(rhs, n, &r));
+  CHK(poisson_dev(p, r, dx, dy, dz, nx, ny, nz, omega, eps, kmax, idyn, solver, exit_iter, dmax,
+                  reason, s));
+  return s.finish();
+}
+
+extern "C" int o3d_correct_pression(double *pp, const double *ux_pred, const double *uy_pred,
+                                    const double *uz_pred, double dx, double dy, double dz,
+                                    int nx, int ny, int nz, double dt, double *omega, double eps,
+                                    int kmax, int idyn, const int *bc, int *exit_iter,
+                                    double *dmax, int *reason) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  if (!omega) return fail("omega is NULL");
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  double *p, *rhs;
+  DivArgs<double> a;
+  CHK(s.out(pp, n, &p, true));
+  CHK(s.in(ux_pred, n, &a.f[0]));
+  CHK(s.in(uy_pred, n, &a.f[1]));
+  CHK(s.in(uz_pred, n, &a.f[2]));
+  CHK(s.scratch(n, &rhs));
+  a.out = rhs;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.odd = 1;  // src/integration.f90:235-236
+  a.tl = make_tiling(nx, ny, nz);
+  const double h[3] = {dx, dy, dz};
+  fill_coef<double>(a.d1, h, false);
+  a.divisor = dt;
+  CHK(s.scratch(a.tl.blocks(), &a.part));
+  HIPCHK(launch_divergence<double>(a, g_stream));
+  CHK(poisson_dev(p, rhs, dx, dy, dz, nx, ny, nz, omega, eps, kmax, idyn, solver_from_bc(bc),
+                  exit_iter, dmax, reason, s));
+  return s.finish();
+}
+
+extern "C" int o3d_correct_velocity(double *ux, double *uy, double *uz, const double *ux_pred,
+                                    const double *uy_pred, const double *uz_pred,
+                                    const double *pp, double dt, double dx, double dy, double dz,
+                                    int nx, int ny, int nz, const int *bc, double *vstats) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  CorrectArgs<double, double> a;
+  CHK(s.out(ux, n, &a.u[0], false));
+  CHK(s.out(uy, n, &a.u[1], false));
+  CHK(s.out(uz, n, &a.u[2], false));
+  CHK(s.in(ux_pred, n, &a.up[0]));
+  CHK(s.in(uy_pred, n, &a.up[1]));
+  CHK(s.in(uz_pred, n, &a.up[2]));
+  CHK(s.in(pp, n, &a.p));
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.tl = make_tiling(nx, ny, nz);
+  const double h[3] = {dx, dy, dz};
+  fill_coef<double>(a.d1, h, false);
+  a.dt = dt;
+  FieldPartial *fin;
+  CHK(s.scratch(3 * a.tl.blocks(), &a.part));
+  CHK(s.scratch(3, &fin));
+  HIPCHK(launch_correct<double, double>(a, g_stream));
+  FieldPartial r[3];
+  for (int c = 0; c < 3; ++c) CHK(finalize_to_host(a.part, a.tl.blocks(), 3, c, fin + c, &r[c]));
+  CHK(s.finish());
+  int diverged = 0;
+  for (int c = 0; c < 3; ++c) {
+    if (vstats) {
+      vstats[4 * c + 0] = r[c].vmin;
+      vstats[4 * c + 1] = r[c].vmax;
+      vstats[4 * c + 2] = r[c].amax;
+      vstats[4 * c + 3] = r[c].nan;
+    }
+    if (r[c].nan || r[c].vmax > 1000.0) diverged = 1;
+  }
+  return diverged;
+}
+
+extern "C" int o3d_function_stats(const double *f, int nx, int ny, int nz, double *stats6) {
+  CHK(ensure_init());
+  if (!stats6) return fail("stats6 is NULL");
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  const double *fd;
+  CHK(s.in(f, n, &fd));
+  const int nb = std::max(1, std::min(2048, (int)((n + 4095) / 4096)));
+  FieldPartial *part, *fin;
+  CHK(s.scratch(nb, &part));
+  CHK(s.scratch(1, &fin));
+  HIPCHK(launch_field_stats<double>(fd, (int)n, part, nb, g_stream));
+  FieldPartial r;
+  CHK(finalize_to_host(part, nb, 1, 0, fin, &r));
+  CHK(s.finish());
+  stats6_from(r, nx, ny, nz, stats6);
+  return 0;
+}
+
+namespace {
+template <typename T>
+int statistics_dev(const T *u[3], int nx, int ny, int nz, double dx, double dy, double dz,
+                   double re, double t, const int *bc, double *out17, Staged &s) {
+  StatsArgs<T> a;
+  for (int c = 0; c < 3; ++c) a.u[c] = u[c];
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.tl = make_tiling(nx, ny, nz);
+  const double h[3] = {dx, dy, dz};
+  fill_coef<T>(a.d1, h, false);
+  fill_coef<T>(a.d2, h, true);
+  a.xnu = 1.0 / re;
+  double *fin;
+  CHK(s.scratch((size_t)16 * a.tl.blocks(), &a.part));
+  CHK(s.scratch(16, &fin));
+  HIPCHK(launch_statistics<T>(a, g_stream));
+  HIPCHK(launch_finalize_sums(a.part, a.tl.blocks(), 16, fin, g_stream));
+  double h16[16];
+  HIPCHK(hipMemcpyAsync(h16, fin, sizeof(h16), hipMemcpyDeviceToHost, g_stream));
+  CHK(s.finish());
+  const double N = (double)((long long)nx * ny * nz);
+  out17[0] = t;
+  out17[1] = h16[0] / N;  // E_k
+  out17[2] = h16[1] / N;  // eps
+  out17[3] = h16[2] / N;  // eps2
+  out17[4] = h16[3] / N;  // enstrophy
+  for (int m = 0; m < 3; ++m) out17[5 + m] = h16[4 + m] / N;
+  for (int m = 0; m < 9; ++m) out17[8 + m] = h16[7 + m] / N;
+  return 0;
+}
+}  // namespace
+
+extern "C" int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
+                              int ny, int nz, double dx, double dy, double dz, double re,
+                              double t, const int *bc, double *out17) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  if (!out17) return fail("out17 is NULL");
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  const double *u[3];
+  CHK(s.in(ux, n, &u[0]));
+  CHK(s.in(uy, n, &u[1]));
+  CHK(s.in(uz, n, &u[2]));
+  return statistics_dev<double>(u, nx, ny, nz, dx, dy, dz, re, t, bc, out17, s);
+}
+
+// ====================================================== device-resident solver
+namespace {
+
+struct Solver {
+  o3d_params p;
+  int n = 0;
+  bool mixed = false;
+  Tiling tl;
+  int kind = 0;
+  SorConst sc;
+  double omega = 0.0;
+  // Fields (element size 8 for FP64, 4 for the FP32 form; p always FP64).
+  void *u[3] = {}, *up[3] = {}, *F[3][3] = {}, *rhs = nullptr, *nut = nullptr;
+  double *P[2] = {nullptr, nullptr};
+  int cur = 0;                 // P[cur] holds the pressure
+  float *E[2] = {nullptr, nullptr}, *R = nullptr;  // mixed: correction ping-pong, residual
+  int ring = 0;                // predict calls so far (history ring position)
+  // Reductions.
+  FieldPartial *part_div = nullptr, *part_div2 = nullptr, *part_nut = nullptr,
+               *part_cor = nullptr, *fin = nullptr;
+  FieldPartial *hfin = nullptr;  // pinned, 6 records
+  SorState *st = nullptr, *st64 = nullptr, *hst = nullptr;
+  std::vector<hipEvent_t> ev;
+  int last_sweeps[2] = {0, 0};
+  double sor_ms[2] = {0, 0};
+  int sor_sweeps[2] = {0, 0};
+  void *nn = nullptr;          // learned warm start
+  float *cap_rhs = nullptr, *cap_p = nullptr;
+  std::vector<void *> allocs;
+
+  size_t esz() const { return mixed ? 4 : 8; }
+  template <typename T>
+  int alloc(T **p_, size_t count) {
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, count * sizeof(T)));
+    HIPCHK(hipMemset(d, 0, count * sizeof(T)));
+    allocs.push_back(d);
+    *p_ = (T *)d;
+    return 0;
+  }
+  int allocv(void **p_, size_t bytes) {
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, bytes));
+    HIPCHK(hipMemset(d, 0, bytes));
+    allocs.push_back(d);
+    *p_ = d;
+    return 0;
+  }
+  ~Solver() {
+    for (void *a : allocs) (void)hipFree(a);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (hfin) (void)hipHostFree(hfin);
+    if (hst) (void)hipHostFree(hst);
+  }
+};
+
+template <typename T>
+int solver_predict(Solver &S, int itime) {
+  const o3d_params &p = S.p;
+  int ci;
+  if (p.itscheme == 1 || itime == 1) ci = 0;
+  else if (p.itscheme == 2 || itime == 2) ci = 1;
+  else if (p.itscheme == 3) ci = 2;
+  else { fail("itscheme: %d unrecognized", p.itscheme); return -2; }
+  // History ring: F1 of this step goes to slot ring % 3; slots of the two
+  // previous steps hold F2 and F3 (the reference shifts by copying,
+  // src/integration.f90:176-188; unused slots start at zero).
+  const int s1 = S.ring % 3, s2 = (S.ring + 2) % 3, s3 = (S.ring + 1) % 3;
+  PredictArgs<T> a;
+  for (int c = 0; c < 3; ++c) {
+    a.u[c] = (const T *)S.u[c];
+    a.up[c] = (T *)S.up[c];
+    a.f1[c] = (T *)S.F[s1][c];
+    a.f2[c] = (const T *)S.F[s2][c];
+    a.f3[c] = (const T *)S.F[s3][c];
+  }
+  const bool les = p.iles == 1;
+  a.nut = les ? (T *)S.nut : nullptr;
+  a.nut_part = S.part_nut;
+  a.nx = p.nx; a.ny = p.ny; a.nz = p.nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = p.bc[q];
+  a.tl = S.tl;
+  const double h[3] = {p.dx, p.dy, p.dz};
+  fill_coef<T>(a.d1, h, false);
+  fill_coef<T>(a.d2, h, true);
+  a.adu = T(p.adt[ci]); a.bdu = T(p.bdt[ci]); a.cdu = T(p.cdt[ci]);
+  a.onere = T(1.0 / p.re);
+  const double csd = p.cs * p.delta;
+  a.csd2 = T(csd * csd);
+  HIPCHK(launch_predict<T>(a, ci + 1, les, g_stream));
+  if (p.itscheme >= 2) S.ring++;
+  return 0;
+}
+
+template <typename T>
+int solver_divergence(Solver &S, const void *const f[3], T *out, double divisor,
+                      FieldPartial *part) {
+  DivArgs<T> a;
+  for (int c = 0; c < 3; ++c) a.f[c] = (const T *)f[c];
+  a.out = out;
+  a.part = part;
+  a.nx = S.p.nx; a.ny = S.p.ny; a.nz = S.p.nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = S.p.bc[q];
+  a.odd = 1;
+  a.tl = S.tl;
+  const double h[3] = {S.p.dx, S.p.dy, S.p.dz};
+  fill_coef<T>(a.d1, h, false);
+  a.divisor = T(divisor);
+  HIPCHK(launch_divergence<T>(a, g_stream));
+  return 0;
+}
+
+template <typename T>
+int solver_correct(Solver &S) {
+  CorrectArgs<T, double> a;
+  for (int c = 0; c < 3; ++c) { a.u[c] = (T *)S.u[c]; a.up[c] = (const T *)S.up[c]; }
+  a.p = S.P[S.cur];
+  a.part = S.part_cor;
+  a.nx = S.p.nx; a.ny = S.p.ny; a.nz = S.p.nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = S.p.bc[q];
+  a.tl = S.tl;
+  const double h[3] = {S.p.dx, S.p.dy, S.p.dz};
+  fill_coef<double>(a.d1, h, false);
+  a.dt = S.p.dt;
+  HIPCHK(launch_correct<T, double>(a, g_stream));
+  return 0;
+}
+
+template <typename T, typename TR>
+void sor_args_common(Solver &S, SorArgs<T, TR> &a, const TR *rhs, SorState *st, int kmax) {
+  a.rhs = rhs;
+  a.st = st;
+  a.log = nullptr;
+  a.log_len = 0;
+  a.nx = S.p.nx; a.ny = S.p.ny; a.nz = S.p.nz;
+  solver_periodic(S.kind, a.per);
+  a.tl = make_sor_tiling(S.p.nx, S.p.ny, S.p.nz);
+  a.ox = S.sc.ox; a.oy = S.sc.oy; a.oz = S.sc.oz; a.A = S.sc.A;
+  a.eps = S.p.eps;
+  a.factor = S.kind == SOR_0011 ? 1.01 : 1.05;
+  a.kmax = kmax;
+  a.idyn = S.p.idyn;
+}
+
+// Plain FP64 Poisson solve on P[cur] (src/poisson.f90 semantics, red-black).
+int solver_poisson_f64(Solver &S, o3d_step_info *info) {
+  SorRun<double, double> r;
+  r.st = S.st;
+  r.hst = S.hst;
+  r.events = &S.ev;
+  r.buf[0] = S.P[S.cur];
+  r.buf[1] = S.P[1 - S.cur];
+  sor_args_common<double, double>(S, r.a, (const double *)S.rhs, S.st, S.p.kmax);
+  int sweeps = 0;
+  const int first = S.last_sweeps[0] > 0 ? std::max(4, S.last_sweeps[0] - 2) : 16;
+  CHK(r.run(S.omega, first, &sweeps));
+  if (sweeps & 1) S.cur = 1 - S.cur;
+  S.last_sweeps[0] = sweeps;
+  S.omega = S.hst->omega_final;
+  S.sor_ms[0] += r.ms_full;
+  S.sor_sweeps[0] += r.sweeps_full;
+  info->sor_exit_iter = S.hst->exit_iter;
+  info->sor_reason = S.hst->reason;
+  info->sweeps_f64 = sweeps;
+  info->sweeps_f32 = 0;
+  info->refinements = 0;
+  info->sor_dmax = S.hst->dmax_final;
+  info->omega = S.omega;
+  return 0;
+}
+
+// Mixed precision: FP32 red-black sweeps on the correction e of A e = r,
+// r = rhs - A p formed in FP64; p += e in FP64; convergence judged by an FP64
+// red-black sweep of p with the reference's stop rules.
+int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
+  const o3d_params &p = S.p;
+  const int n = S.n;
+  const int inner_max = p.inner_max > 0 ? p.inner_max : 400;
+  int total = 0, sw32 = 0, sw64 = 0, cycles = 0;
+  int reason = 2;
+  double judge_prev = 1609.0, dlast = 0.0;
+  while (total < p.kmax) {
+    int per[3];
+    solver_periodic(S.kind, per);
+    HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx, p.ny,
+                                      p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
+    SorRun<float, float> r;
+    r.st = S.st;
+    r.hst = S.hst;
+    r.events = &S.ev;
+    r.buf[0] = S.E[0];
+    r.buf[1] = S.E[1];
+    const int kin = std::min(inner_max, p.kmax - total);
+    sor_args_common<float, float>(S, r.a, S.R, S.st, kin);
+    int sweeps = 0;
+    const int first = cycles == 0 && S.last_sweeps[1] > 0
+                          ? std::min(kin, std::max(4, S.last_sweeps[1] - 2))
+                          : std::min(kin, 16);
+    CHK(r.run(S.omega, first, &sweeps));
+    S.sor_ms[1] += r.ms_full;
+    S.sor_sweeps[1] += r.sweeps_full;
+    const int rin = S.hst->reason;
+    S.omega = S.hst->omega_final;
+    total += sweeps;
+    sw32 += sweeps;
+    ++cycles;
+    HIPCHK(launch_fold(S.P[S.cur], S.E[sweeps & 1], n, g_stream));
+    if (rin == 2 && total < p.kmax) continue;  // cycle cap: refresh the residual
+    // FP64 judge sweep (applied, like the reference's last sweep).
+    SorRun<double, float> j;
+    j.st = S.st64;
+    j.hst = S.hst;
+    j.events = &S.ev;
+    j.buf[0] = S.P[S.cur];
+    j.buf[1] = S.P[1 - S.cur];
+    sor_args_common<double, float>(S, j.a, (const float *)S.rhs, S.st64, 1);
+    int jsw = 0;
+    CHK(j.run(S.omega, 2, &jsw));
+    if (jsw & 1) S.cur = 1 - S.cur;
+    sw64 += jsw;
+    total += jsw;
+    dlast = S.hst->dmax_final;
+    if (dlast < p.eps) { reason = 0; break; }
+    if (fabs(judge_prev - dlast) < p.eps / 1000.0) { reason = 1; break; }
+    judge_prev = dlast;
+  }
+  S.last_sweeps[1] = sw32;
+  info->sor_exit_iter = reason == 2 ? p.kmax + 1 : total;
+  info->sor_reason = reason;
+  info->sweeps_f64 = sw64;
+  info->sweeps_f32 = sw32;
+  info->refinements = cycles;
+  info->sor_dmax = dlast;
+  info->omega = S.omega;
+  return 0;
+}
+
+template <typename T>
+int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
+  const o3d_params &p = S.p;
+  HIPCHK(hipEventRecord(S.ev[2], g_stream));
+  CHK(solver_predict<T>(S, itime));
+  CHK(solver_divergence<T>(S, S.up, (T *)S.rhs, p.dt, S.part_div));
+  HIPCHK(launch_finalize_partials(S.part_div, S.tl.blocks(), 1, 0, S.fin + 0, g_stream));
+  if (p.iles == 1)
+    HIPCHK(launch_finalize_partials(S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5, g_stream));
+  if (S.cap_rhs) HIPCHK((launch_convert<float, T>(S.cap_rhs, (const T *)S.rhs, S.n, g_stream)));
+  HIPCHK(hipEventRecord(S.ev[4], g_stream));
+  int warm = 0;
+  if (S.nn) CHK(o3d_nn_warm_start(S.nn, &S, itime, &warm, info->warm_dmax));
+  if (S.mixed) CHK(solver_poisson_mixed(S, info));
+  else CHK(solver_poisson_f64(S, info));
+  info->warm_used = warm;
+  HIPCHK(hipEventRecord(S.ev[5], g_stream));
+  if (S.cap_p) HIPCHK((launch_convert<float, double>(S.cap_p, S.P[S.cur], S.n, g_stream)));
+  CHK(solver_correct<T>(S));
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(launch_finalize_partials(S.part_cor, S.tl.blocks(), 3, c, S.fin + 2 + c, g_stream));
+  CHK(solver_divergence<T>(S, S.u, nullptr, 1.0, S.part_div2));
+  HIPCHK(launch_finalize_partials(S.part_div2, S.tl.blocks(), 1, 0, S.fin + 1, g_stream));
+  HIPCHK(hipEventRecord(S.ev[3], g_stream));
+  HIPCHK(hipMemcpyAsync(S.hfin, S.fin, 6 * sizeof(FieldPartial), hipMemcpyDeviceToHost,
+                        g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
+  info->step_ms = ms;
+  HIPCHK(hipEventElapsedTime(&ms, S.ev[4], S.ev[5]));
+  info->poisson_ms = ms;
+  stats6_from(S.hfin[0], p.nx, p.ny, p.nz, info->divu_pred);
+  stats6_from(S.hfin[1], p.nx, p.ny, p.nz, info->divu);
+  if (p.iles == 1) stats6_from(S.hfin[5], p.nx, p.ny, p.nz, info->nu_t);
+  else for (int q = 0; q < 6; ++q) info->nu_t[q] = 0.0;
+  const double h[3] = {p.dx, p.dy, p.dz};
+  info->diverged = 0;
+  info->has_nan = 0;
+  for (int c = 0; c < 3; ++c) {
+    const FieldPartial &r = S.hfin[2 + c];
+    info->vel_min[c] = r.vmin;
+    info->vel_max[c] = r.vmax;
+    info->vel_amax[c] = r.amax;
+    info->cfl[c] = r.amax * p.dt / h[c];
+    if (r.nan) info->has_nan = 1;
+    if (r.nan || r.vmax > 1000.0) info->diverged = 1;
+  }
+  return info->diverged ? 1 : 0;
+}
+
+}  // namespace
+
+// Accessors used by the learned warm start (o3d_nn.hip).
+namespace o3d {
+double *solver_pressure(void *s) { return ((Solver *)s)->P[((Solver *)s)->cur]; }
+double *solver_pressure_other(void *s) { return ((Solver *)s)->P[1 - ((Solver *)s)->cur]; }
+void solver_swap_pressure(void *s) { ((Solver *)s)->cur = 1 - ((Solver *)s)->cur; }
+const void *solver_rhs(void *s) { return ((Solver *)s)->rhs; }
+bool solver_mixed(void *s) { return ((Solver *)s)->mixed; }
+void solver_dims(void *s, int *nx, int *ny, int *nz) {
+  *nx = ((Solver *)s)->p.nx; *ny = ((Solver *)s)->p.ny; *nz = ((Solver *)s)->p.nz;
+}
+int solver_kind(void *s) { return ((Solver *)s)->kind; }
+void solver_sor_const(void *s, double *c4) {
+  const SorConst &c = ((Solver *)s)->sc;
+  c4[0] = c.ox; c4[1] = c.oy; c4[2] = c.oz; c4[3] = c.A;
+}
+double solver_omega(void *s) { return ((Solver *)s)->omega; }
+double solver_eps(void *s) { return ((Solver *)s)->p.eps; }
+hipStream_t current_stream() { return g_stream; }
+int set_error(const char *msg) { return fail("%s", msg); }
+}  // namespace o3d
+
+extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
+  CHK(ensure_init());
+  if (!p || !handle) return fail("NULL argument");
+  CHK(check_grid(p->nx, p->ny, p->nz));
+  CHK(check_bc(p->bc));
+  if (p->itscheme < 1 || p->itscheme > 3) {
+    fail("itscheme: %d unrecognized", p->itscheme);
+    return -2;
+  }
+  if (p->precision != O3D_FP64 && p->precision != O3D_MIXED)
+    return fail("unknown precision form %d", p->precision);
+  const int kind = solver_from_bc(p->bc);
+  if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");
+  CHK(check_red_black(kind, p->nx, p->ny, p->nz));
+  Solver *S = new Solver();
+  S->p = *p;
+  S->n = p->nx * p->ny * p->nz;
+  S->mixed = p->precision == O3D_MIXED;
+  S->tl = make_tiling(p->nx, p->ny, p->nz);
+  S->kind = kind;
+  S->sc = sor_const(p->dx, p->dy, p->dz);
+  S->omega = p->omega;
+  const size_t n = S->n, es = S->esz();
+  int rc = 0;
+  for (int c = 0; c < 3 && !rc; ++c) {
+    rc |= S->allocv(&S->u[c], n * es);
+    rc |= S->allocv(&S->up[c], n * es);
+    for (int s = 0; s < 3 && !rc; ++s) rc |= S->allocv(&S->F[s][c], n * es);
+  }
+  if (!rc) rc |= S->allocv(&S->rhs, n * es);
+  if (!rc && p->iles == 1) rc |= S->allocv(&S->nut, n * es);
+  if (!rc) rc |= S->alloc(&S->P[0], n);
+  if (!rc) rc |= S->alloc(&S->P[1], n);
+  if (!rc && S->mixed) {
+    rc |= S->alloc(&S->E[0], n);
+    rc |= S->alloc(&S->E[1], n);
+    rc |= S->alloc(&S->R, n);
+  }
+  const int nb = S->tl.blocks();
+  if (!rc) rc |= S->alloc(&S->part_div, nb);
+  if (!rc) rc |= S->alloc(&S->part_div2, nb);
+  if (!rc) rc |= S->alloc(&S->part_nut, nb);
+  if (!rc) rc |= S->alloc(&S->part_cor, 3 * nb);
+  if (!rc) rc |= S->alloc(&S->fin, 6);
+  if (!rc) rc |= S->alloc(&S->st, 1);
+  if (!rc) rc |= S->alloc(&S->st64, 1);
+  if (!rc && hipHostMalloc((void **)&S->hfin, 6 * sizeof(FieldPartial)) != hipSuccess)
+    rc = fail("hipHostMalloc failed");
+  if (!rc && hipHostMalloc((void **)&S->hst, sizeof(SorState)) != hipSuccess)
+    rc = fail("hipHostMalloc failed");
+  S->ev.resize(6);
+  for (auto &e : S->ev)
+    if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail("hipEventCreate failed");
+  if (rc) { delete S; return rc; }
+  *handle = S;
+  return 0;
+}
+
+extern "C" int o3d_solver_destroy(void *handle) {
+  if (handle) {
+    (void)hipStreamSynchronize(g_stream);
+    delete (Solver *)handle;
+  }
+  return 0;
+}
+
+namespace {
+int upload_field(Solver &S, void *dst, const double *src) {
+  const size_t n = S.n;
+  Staged s;
+  const double *d;
+  CHK(s.in(src, n, &d));
+  if (S.mixed) HIPCHK((launch_convert<float, double>((float *)dst, d, (int)n, g_stream)));
+  else HIPCHK(hipMemcpyAsync(dst, d, n * 8, hipMemcpyDeviceToDevice, g_stream));
+  return s.finish();
+}
+int download_field(Solver &S, double *dst, const void *src) {
+  const size_t n = S.n;
+  Staged s;
+  double *d;
+  CHK(s.out(dst, n, &d, false));
+  if (S.mixed) HIPCHK((launch_convert<double, float>(d, (const float *)src, (int)n, g_stream)));
+  else HIPCHK(hipMemcpyAsync(d, src, n * 8, hipMemcpyDeviceToDevice, g_stream));
+  return s.finish();
+}
+}  // namespace
+
+extern "C" int o3d_solver_upload(void *handle, const double *ux, const double *uy,
+                                 const double *uz, const double *pp) {
+  if (!handle) return fail("NULL handle");
+  Solver &S = *(Solver *)handle;
+  const double *u[3] = {ux, uy, uz};
+  for (int c = 0; c < 3; ++c)
+    if (u[c]) CHK(upload_field(S, S.u[c], u[c]));
+  if (pp) {
+    Staged s;
+    const double *d;
+    CHK(s.in(pp, S.n, &d));
+    HIPCHK(hipMemcpyAsync(S.P[S.cur], d, (size_t)S.n * 8, hipMemcpyDeviceToDevice, g_stream));
+    CHK(s.finish());
+  }
+  return 0;
+}
+
+extern "C" int o3d_solver_download(void *handle, double *ux, double *uy, double *uz, double *pp) {
+  if (!handle) return fail("NULL handle");
+  Solver &S = *(Solver *)handle;
+  double *u[3] = {ux, uy, uz};
+  for (int c = 0; c < 3; ++c)
+    if (u[c]) CHK(download_field(S, u[c], S.u[c]));
+  if (pp) {
+    Staged s;
+    double *d;
+    CHK(s.out(pp, S.n, &d, false));
+    HIPCHK(hipMemcpyAsync(d, S.P[S.cur], (size_t)S.n * 8, hipMemcpyDeviceToDevice, g_stream));
+    CHK(s.finish());
+  }
+  return 0;
+}
+
+extern "C" int o3d_solver_step(void *handle, int itime, o3d_step_info *info) {
+  if (!handle || !info) return fail("NULL argument");
+  Solver &S = *(Solver *)handle;
+  memset(info, 0, sizeof(*info));
+  S.sor_ms[0] = S.sor_ms[1] = 0.0;
+  S.sor_sweeps[0] = S.sor_sweeps[1] = 0;
+  return S.mixed ? solver_step_t<float>(S, itime, info) : solver_step_t<double>(S, itime, info);
+}
+
+extern "C" int o3d_solver_statistics(void *handle, double t, double *out17) {
+  if (!handle || !out17) return fail("NULL argument");
+  Solver &S = *(Solver *)handle;
+  Staged s;
+  if (S.mixed) {
+    const float *u[3] = {(const float *)S.u[0], (const float *)S.u[1], (const float *)S.u[2]};
+    return statistics_dev<float>(u, S.p.nx, S.p.ny, S.p.nz, S.p.dx, S.p.dy, S.p.dz, S.p.re, t,
+                                 S.p.bc, out17, s);
+  }
+  const double *u[3] = {(const double *)S.u[0], (const double *)S.u[1], (const double *)S.u[2]};
+  return statistics_dev<double>(u, S.p.nx, S.p.ny, S.p.nz, S.p.dx, S.p.dy, S.p.dz, S.p.re, t,
+                                S.p.bc, out17, s);
+}
+
+extern "C" void *o3d_solver_field(void *handle, int which) {
+  if (!handle) return nullptr;
+  Solver &S = *(Solver *)handle;
+  switch (which) {
+    case 0: case 1: case 2: return S.u[which];
+    case 3: case 4: case 5: return S.up[which - 3];
+    case 6: return S.P[S.cur];
+    case 7: return S.rhs;
+    case 8: return S.nut;
+    default: return nullptr;
+  }
+}
+
+extern "C" int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2) {
+  if (!handle || !ms2 || !sweeps2) return fail("NULL argument");
+  Solver &S = *(Solver *)handle;
+  for (int q = 0; q < 2; ++q) { ms2[q] = S.sor_ms[q]; sweeps2[q] = S.sor_sweeps[q]; }
+  return 0;
+}
+
+extern "C" int o3d_solver_set_warm_model(void *handle, void *nn) {
+  if (!handle) return fail("NULL handle");
+  Solver &S = *(Solver *)handle;
+  S.nn = nn;
+  return 0;
+}
+
+extern "C" int o3d_solver_capture(void *handle, float *rhs_out, float *p_out) {
+  if (!handle) return fail("NULL handle");
+  Solver &S = *(Solver *)handle;
+  S.cap_rhs = rhs_out;
+  S.cap_p = p_out;
+  return 0;
+}

@@ -1,0 +1,81 @@
+// Host-side interface of the red-black SOR kernels (o3d_sor.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "o3d_common.h"
+
+namespace o3d {
+
+// Device-resident solver state.  Iteration `it` (1-based) is one launch; it
+// decides from the state written by earlier launches whether the reference's
+// stop rule has fired (src/poisson.f90:110-114), so the host can queue many
+// launches without reading anything back.
+struct SorState {
+  unsigned long long dmax_bits[4];  // IEEE bits of dmax of iteration it, ring by it & 3
+  double omega[4];                  // omega used by iteration it, ring by it & 3
+  double omega_final;               // omega after the solve (persists, idyn = 1)
+  double dmax_final;                // dmax of the last sweep
+  int done;                         // stop rule fired
+  int exit_iter;                    // the reference's printed exit iteration
+  int reason;                       // 0 eps, 1 stagnation, 2 kmax
+  int sweeps;                       // sweeps applied to the field
+  int base;                         // iteration offset for graph-replayed batches
+  int pad;
+};
+
+// The three neighbour rules of src/poisson.f90 (chosen from x and y
+// boundary types only, src/initialization.f90:283-301).
+enum SolverKind : int { SOR_0000 = 0, SOR_0011 = 1, SOR_111111 = 2 };
+inline void solver_periodic(int kind, int per[3]) {
+  per[0] = kind != SOR_111111;
+  per[1] = kind == SOR_0000;
+  per[2] = kind != SOR_111111;
+}
+
+struct SorTiling {
+  int ntx, nty, ntz, zc;
+  int blocks() const { return ntx * nty * ntz; }
+};
+SorTiling make_sor_tiling(int nx, int ny, int nz);
+
+template <typename T, typename TR>
+struct SorArgs {
+  const T *pin;  // iterate before the sweep (not modified)
+  T *pout;       // iterate after the sweep
+  const TR *rhs;
+  SorState *st;
+  double *log;   // dmax per iteration (may be null), log[it-1]
+  int log_len;
+  int nx, ny, nz, per[3];
+  SorTiling tl;
+  double ox, oy, oz, A;  // 1/dx^2, 1/dy^2, 1/dz^2, -(2/dx^2+2/dy^2+2/dz^2)
+  double eps, factor;
+  int kmax, idyn;
+  int it;        // iteration number of this launch (added to st->base)
+};
+
+template <typename T, typename TR>
+hipError_t launch_sor_iteration(const SorArgs<T, TR> &a, hipStream_t s);
+hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s);
+
+// Mixed precision helpers.  r = rhs - L p (FP64 arithmetic) stored in FP32,
+// and e = 0; and p += e.
+template <typename TR>
+hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *e, int nx,
+                               int ny, int nz, const int per[3], double ox, double oy,
+                               double oz, double A, hipStream_t s);
+hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s);
+// FP64 residual norms: out[0] = max |r|, out[1] = sum r^2 (r = rhs - L p).
+template <typename TR>
+hipError_t launch_residual_norm(const double *p, const TR *rhs, double *part, int nblocks,
+                                int nx, int ny, int nz, const int per[3], double ox,
+                                double oy, double oz, double A, hipStream_t s);
+
+template <typename T>
+hipError_t launch_copy(T *dst, const T *src, int n, hipStream_t s);
+template <typename TD, typename TS>
+hipError_t launch_convert(TD *dst, const TS *src, int n, hipStream_t s);
+hipError_t launch_fill_f64(double *dst, double v, int n, hipStream_t s);
+hipError_t launch_fill_f32(float *dst, float v, int n, hipStream_t s);
+
+}  // namespace o3d

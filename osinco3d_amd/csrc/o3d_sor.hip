@@ -1,0 +1,403 @@
+// Red-black SOR for the pressure Poisson equation (src/poisson.f90), MI355X.
+//
+// One launch performs one full iteration (red sweep, then black sweep) and
+// touches HBM once per iteration: it reads the old iterate and the rhs and
+// writes the new iterate (24 B/cell in FP64, 12 B/cell in FP32) to a second
+// buffer (ping-pong).  Inside a block:
+//  * the block owns a 64x8 x-y tile and marches a chunk of z planes;
+//  * red values are computed on the tile plus a 1-wide halo from the old
+//    iterate (which no block modifies), so the black update of the tile needs
+//    no exchange with neighbouring blocks: the halo red values are recomputed
+//    here bit-identically to the owning block (same inputs, same order);
+//  * the march lags black by one plane: at step s the block computes red on
+//    plane s, then black on plane s-1, whose red neighbours (s-2, s-1, s) are
+//    all new.  Old planes, new red planes and rhs planes live in small LDS
+//    rings (4, 4 and 3 planes).
+// Colour of a node is (i+j+k) & 1 (0-based); colour 0 is swept first.  This is
+// a valid two-colouring with the reference's neighbour rules when every
+// periodic axis has an even node count (checked on the host).
+// The per-iteration maximum update dmax (un-relaxed Gauss-Seidel value minus
+// old value, src/poisson.f90:100-101) is reduced with wavefront shuffles and
+// one 64-bit atomicMax per block on the IEEE bit pattern (dmax >= 0).
+#include <algorithm>
+
+#include "o3d_sor.h"
+
+namespace o3d {
+
+constexpr int SX = 64, SY = 8;
+constexpr int R2W = SX + 4, R2H = SY + 4;  // old iterate, halo 2
+constexpr int R1W = SX + 2, R1H = SY + 2;  // new red values and rhs, halo 1
+constexpr int R2N = R2W * R2H, R1N = R1W * R1H;
+constexpr int SNT = SX * SY;
+constexpr int SNW = SNT / O3D_WAVE;
+static_assert(R2N <= 2 * SNT && R1N <= 2 * SNT, "two cells per thread at most");
+
+SorTiling make_sor_tiling(int nx, int ny, int nz) {
+  SorTiling t;
+  t.ntx = (nx + SX - 1) / SX;
+  t.nty = (ny + SY - 1) / SY;
+  const int layer = t.ntx * t.nty;
+  int ntz = (2048 + layer - 1) / layer;
+  ntz = std::max(1, std::min(ntz, std::max(1, nz / 32)));
+  t.zc = (nz + ntz - 1) / ntz;
+  t.ntz = (nz + t.zc - 1) / t.zc;
+  return t;
+}
+
+// Neighbour index for the SOR rules: periodic wrap, or the free-slip mirror
+// m1 = p1 = 2 at node 1 and m1 = p1 = n-1 at node n (src/poisson.f90:196-206).
+// Deeper halo positions use the same mirror; positions more than 2 beyond the
+// domain belong to idle lanes of a partial tile and are clamped.
+__device__ __forceinline__ int sor_ghost(int q, int n, int periodic) {
+  if (q > n + 1) return n - 1;
+  if (q < -2) return 0;
+  if (periodic) {
+    if (q < 0) return q + n;
+    if (q >= n) return q - n;
+    return q;
+  }
+  if (q < 0) return -q;
+  if (q > n - 1) return 2 * (n - 1) - q;
+  return q;
+}
+
+__device__ __forceinline__ double bits_to_double(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+
+// The reference's loop tail for iteration it-1 (src/poisson.f90:110-122).
+// Returns true when the solve is over; sets om to omega for iteration it.
+__device__ __forceinline__ bool sor_decide(const SorState *st, int it, double eps, double factor,
+                                           int kmax, int idyn, double &om, int &reason,
+                                           double &dlast) {
+  reason = -1;
+  dlast = 0.0;
+  if (it <= 1) { om = st->omega[0]; return false; }
+  const double d1 = bits_to_double(st->dmax_bits[(it - 1) & 3]);
+  const double d0 = (it - 1 == 1) ? 1609.0 : bits_to_double(st->dmax_bits[(it - 2) & 3]);
+  dlast = d1;
+  om = st->omega[(it - 1) & 3];
+  if (d1 < eps) { reason = 0; return true; }
+  if (fabs(d0 - d1) < eps / 1000.0) { reason = 1; return true; }
+  if (it - 1 > 1 && idyn == 1) {
+    if (d1 > d0) om = om * (2.0 - factor);
+    else if (d1 < 0.1 * d0) om = fmin(om * factor, 2.0);
+  }
+  if (it > kmax) { reason = 2; return true; }
+  return false;
+}
+
+template <typename T, typename TR>
+__global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
+  __shared__ T oldr[4][R2N];
+  __shared__ T redr[4][R1N];
+  __shared__ TR rhsr[3][R1N];
+  __shared__ double scr[SNW];
+  SorState *st = a.st;
+  if (st->done) return;
+  const int it = a.it + st->base;
+  double om, dlast;
+  int reason;
+  if (sor_decide(st, it, a.eps, a.factor, a.kmax, a.idyn, om, reason, dlast)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = 1;
+      st->reason = reason;
+      st->exit_iter = reason == 2 ? a.kmax + 1 : it - 1;
+      st->sweeps = it - 1;
+      st->dmax_final = dlast;
+      st->omega_final = om;
+      if (a.log && it - 2 < a.log_len) a.log[it - 2] = dlast;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->omega[it & 3] = om;
+    st->dmax_bits[(it + 1) & 3] = 0ull;
+    if (a.log && it >= 2 && it - 2 < a.log_len) a.log[it - 2] = dlast;
+  }
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int plane = nx * ny;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tix = t % a.tl.ntx;
+  const int rr = t / a.tl.ntx;
+  const int tiy = rr % a.tl.nty;
+  const int tiz = rr / a.tl.nty;
+  const int x0 = tix * SX, y0 = tiy * SY, z0 = tiz * a.tl.zc;
+  const int z1 = min(z0 + a.tl.zc, nz);
+  const int tid = threadIdx.x;
+  const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A);
+  const T w = T(om);
+  const T w1 = T(1.0 - om);
+
+  // Per-thread load descriptors: up to two R2 cells and two R1 cells.
+  int col2[2], col1[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int c2 = tid + q * SNT;
+    const int a2 = c2 % R2W, b2 = c2 / R2W;
+    col2[q] = (c2 < R2N) ? sor_ghost(x0 - 2 + a2, nx, a.per[0]) +
+                               nx * sor_ghost(y0 - 2 + b2, ny, a.per[1])
+                         : -1;
+    const int c1 = tid + q * SNT;
+    const int a1 = c1 % R1W, b1 = c1 / R1W;
+    col1[q] = (c1 < R1N) ? sor_ghost(x0 - 1 + a1, nx, a.per[0]) +
+                               nx * sor_ghost(y0 - 1 + b1, ny, a.per[1])
+                         : -1;
+  }
+  // Black-pass cell of this thread (tile position).
+  const int tx = tid % SX, ty = tid / SX;
+  const int bi = x0 + tx, bj = y0 + ty;
+  const bool bown = bi < nx && bj < ny;
+  const int b1i = (ty + 1) * R1W + tx + 1;
+  const int b2i = (ty + 2) * R2W + tx + 2;
+
+  auto load_old = [&](int zp) {
+    const int gz = sor_ghost(zp, nz, a.per[2]);
+    T *dst = oldr[zp & 3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (col2[q] >= 0) dst[tid + q * SNT] = a.pin[col2[q] + plane * gz];
+  };
+
+  double dmax = 0.0;
+  load_old(z0 - 2);
+  load_old(z0 - 1);
+  for (int s = z0 - 1; s <= z1; ++s) {
+    // Phase 1: old plane s+1 on R2, rhs plane s on R1.
+    load_old(s + 1);
+    {
+      const int gz = sor_ghost(s, nz, a.per[2]);
+      TR *dst = rhsr[(s + 3) % 3];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        if (col1[q] >= 0) dst[tid + q * SNT] = a.rhs[col1[q] + plane * gz];
+    }
+    __syncthreads();
+    // Phase 2: red (colour 0) on plane s over R1, from old values only.
+    {
+      const T *om_ = oldr[(s + 4 - 1) & 3];
+      const T *o0 = oldr[s & 3];
+      const T *op = oldr[(s + 1) & 3];
+      const TR *rh = rhsr[(s + 3) % 3];
+      T *rd = redr[s & 3];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c1 = tid + q * SNT;
+        if (c1 >= R1N) continue;
+        const int a1 = c1 % R1W, b1 = c1 / R1W;
+        const int X = x0 - 1 + a1, Y = y0 - 1 + b1;
+        if (((X + Y + s) & 1) != 0) continue;
+        const int i2 = (b1 + 1) * R2W + a1 + 1;
+        const T pc = o0[i2];
+        // src/poisson.f90:95-98
+        const T pn = (-ox * (o0[i2 - 1] + o0[i2 + 1]) - oy * (o0[i2 - R2W] + o0[i2 + R2W]) -
+                      oz * (om_[i2] + op[i2]) + T(rh[c1])) /
+                     A;
+        rd[c1] = w1 * pc + w * pn;
+        if (a1 >= 1 && a1 <= SX && b1 >= 1 && b1 <= SY && X < nx && Y < ny && s >= z0 && s < z1)
+          dmax = fmax(dmax, (double)fabs(pn - pc));
+      }
+    }
+    __syncthreads();
+    // Phase 3: black (colour 1) on plane s-1 over the tile; write plane s-1.
+    const int sb = s - 1;
+    if (sb >= z0 && bown) {
+      const T *rm = redr[(sb + 4 - 1) & 3];
+      const T *r0 = redr[sb & 3];
+      const T *rp = redr[s & 3];
+      T v;
+      if (((bi + bj + sb) & 1) != 0) {
+        const T pc = oldr[sb & 3][b2i];
+        const T pn = (-ox * (r0[b1i - 1] + r0[b1i + 1]) - oy * (r0[b1i - R1W] + r0[b1i + R1W]) -
+                      oz * (rm[b1i] + rp[b1i]) + T(rhsr[(sb + 3) % 3][b1i])) /
+                     A;
+        v = w1 * pc + w * pn;
+        dmax = fmax(dmax, (double)fabs(pn - pc));
+      } else {
+        v = r0[b1i];
+      }
+      a.pout[bi + nx * bj + plane * sb] = v;
+    }
+  }
+  // Block max, one atomic per block.
+  dmax = block_max<SNW>(dmax, scr);
+  if (threadIdx.x == 0 && dmax > 0.0)
+    atomicMax(&st->dmax_bits[it & 3], (unsigned long long)__double_as_longlong(dmax));
+  if (threadIdx.x == 0 && dmax != dmax)
+    atomicMax(&st->dmax_bits[it & 3], (unsigned long long)__double_as_longlong(dmax));
+}
+
+template <typename T, typename TR>
+hipError_t launch_sor_iteration(const SorArgs<T, TR> &a, hipStream_t s) {
+  sor_kernel<T, TR><<<dim3(a.tl.blocks()), dim3(SNT), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+__global__ void sor_init_kernel(SorState *st, double omega, int base) {
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) { st->dmax_bits[q] = 0ull; st->omega[q] = omega; }
+    st->omega_final = omega;
+    st->dmax_final = 0.0;
+    st->done = 0;
+    st->exit_iter = 0;
+    st->reason = -1;
+    st->sweeps = 0;
+    st->base = base;
+  }
+}
+
+hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s) {
+  sor_init_kernel<<<1, 64, 0, s>>>(st, omega, base);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------ mixed precision
+template <typename TR>
+__global__ void __launch_bounds__(256) residual_f32_kernel(const double *__restrict__ p,
+                                                           const TR *__restrict__ rhs,
+                                                           float *__restrict__ r,
+                                                           float *__restrict__ e, int nx, int ny,
+                                                           int nz, int px, int py, int pz,
+                                                           double ox, double oy, double oz,
+                                                           double A) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = nx * ny * nz;
+  if (idx >= N) return;
+  const int i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
+  const int plane = nx * ny;
+  const int base = idx - i - nx * j - plane * k;
+  (void)base;
+  const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+  const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+  const int km = sor_ghost(k - 1, nz, pz), kp = sor_ghost(k + 1, nz, pz);
+  const int rowk = plane * k, rowj = nx * j;
+  const double lap = ox * (p[im + rowj + rowk] + p[ip + rowj + rowk]) +
+                     oy * (p[i + nx * jm + rowk] + p[i + nx * jp + rowk]) +
+                     oz * (p[i + rowj + plane * km] + p[i + rowj + plane * kp]) + A * p[idx];
+  r[idx] = (float)((double)rhs[idx] - lap);
+  e[idx] = 0.0f;
+}
+
+template <typename TR>
+hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *e, int nx, int ny,
+                               int nz, const int per[3], double ox, double oy, double oz,
+                               double A, hipStream_t s) {
+  const int N = nx * ny * nz;
+  residual_f32_kernel<TR><<<(N + 255) / 256, 256, 0, s>>>(p, rhs, r, e, nx, ny, nz, per[0],
+                                                          per[1], per[2], ox, oy, oz, A);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) fold_kernel(double *__restrict__ p,
+                                                   const float *__restrict__ e, int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) p[idx] = p[idx] + (double)e[idx];
+}
+
+hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s) {
+  fold_kernel<<<(n + 255) / 256, 256, 0, s>>>(p, e, n);
+  return hipGetLastError();
+}
+
+template <typename TR>
+__global__ void __launch_bounds__(256) residual_norm_kernel(const double *__restrict__ p,
+                                                            const TR *__restrict__ rhs,
+                                                            double *part, int nx, int ny, int nz,
+                                                            int px, int py, int pz, double ox,
+                                                            double oy, double oz, double A) {
+  __shared__ double sc[4];
+  const int N = nx * ny * nz;
+  const int plane = nx * ny;
+  double mx = 0.0, ss = 0.0;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += gridDim.x * blockDim.x) {
+    const int i = idx % nx, j = (idx / nx) % ny, k = idx / plane;
+    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+    const int km = sor_ghost(k - 1, nz, pz), kp = sor_ghost(k + 1, nz, pz);
+    const int rowk = plane * k, rowj = nx * j;
+    const double lap = ox * (p[im + rowj + rowk] + p[ip + rowj + rowk]) +
+                       oy * (p[i + nx * jm + rowk] + p[i + nx * jp + rowk]) +
+                       oz * (p[i + rowj + plane * km] + p[i + rowj + plane * kp]) + A * p[idx];
+    const double r = (double)rhs[idx] - lap;
+    mx = fmax(mx, fabs(r));
+    ss += r * r;
+  }
+  mx = block_max<4>(mx, sc);
+  ss = block_sum<4>(ss, sc);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = mx; part[2 * blockIdx.x + 1] = ss; }
+}
+
+template <typename TR>
+hipError_t launch_residual_norm(const double *p, const TR *rhs, double *part, int nblocks, int nx,
+                                int ny, int nz, const int per[3], double ox, double oy,
+                                double oz, double A, hipStream_t s) {
+  residual_norm_kernel<TR><<<nblocks, 256, 0, s>>>(p, rhs, part, nx, ny, nz, per[0], per[1],
+                                                   per[2], ox, oy, oz, A);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) copy_kernel(T *__restrict__ d, const T *__restrict__ s,
+                                                   int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) d[idx] = s[idx];
+}
+template <typename T>
+hipError_t launch_copy(T *dst, const T *src, int n, hipStream_t s) {
+  copy_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+template <typename TD, typename TS>
+__global__ void __launch_bounds__(256) convert_kernel(TD *__restrict__ d,
+                                                      const TS *__restrict__ s, int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) d[idx] = (TD)s[idx];
+}
+template <typename TD, typename TS>
+hipError_t launch_convert(TD *dst, const TS *src, int n, hipStream_t s) {
+  convert_kernel<TD, TS><<<(n + 255) / 256, 256, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) fill_kernel(T *__restrict__ d, T v, int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) d[idx] = v;
+}
+hipError_t launch_fill_f64(double *dst, double v, int n, hipStream_t s) {
+  fill_kernel<double><<<(n + 255) / 256, 256, 0, s>>>(dst, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f32(float *dst, float v, int n, hipStream_t s) {
+  fill_kernel<float><<<(n + 255) / 256, 256, 0, s>>>(dst, v, n);
+  return hipGetLastError();
+}
+
+template hipError_t launch_sor_iteration<double, double>(const SorArgs<double, double> &,
+                                                         hipStream_t);
+template hipError_t launch_sor_iteration<double, float>(const SorArgs<double, float> &,
+                                                        hipStream_t);
+template hipError_t launch_sor_iteration<float, float>(const SorArgs<float, float> &,
+                                                       hipStream_t);
+template hipError_t launch_residual_f32<double>(const double *, const double *, float *, float *,
+                                                int, int, int, const int *, double, double,
+                                                double, double, hipStream_t);
+template hipError_t launch_residual_f32<float>(const double *, const float *, float *, float *,
+                                               int, int, int, const int *, double, double, double,
+                                               double, hipStream_t);
+template hipError_t launch_residual_norm<double>(const double *, const double *, double *, int,
+                                                 int, int, int, const int *, double, double,
+                                                 double, double, hipStream_t);
+template hipError_t launch_residual_norm<float>(const double *, const float *, double *, int, int,
+                                                int, int, const int *, double, double, double,
+                                                double, hipStream_t);
+template hipError_t launch_copy<double>(double *, const double *, int, hipStream_t);
+template hipError_t launch_copy<float>(float *, const float *, int, hipStream_t);
+template hipError_t launch_convert<float, double>(float *, const double *, int, hipStream_t);
+template hipError_t launch_convert<double, float>(double *, const float *, int, hipStream_t);
+template hipError_t launch_convert<float, float>(float *, const float *, int, hipStream_t);
+
+}  // namespace o3d

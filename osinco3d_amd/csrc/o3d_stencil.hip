@@ -1,0 +1,704 @@
+// Stencil kernels of the osinco3d time-step path for MI355X (gfx950).
+//
+// Design (memory-bound work, ~1.7 flop/B in FP64):
+//  * z-marching 2.5D tiles: a 64x8 block (one wave per x-row) owns an x-y tile
+//    and walks a chunk of z planes.  Each thread keeps a 7-deep register queue
+//    of its own column per field (z-derivatives, 6th order needs k-3..k+3) and
+//    the current plane is staged once in LDS with a 3-wide cross-shaped halo
+//    for the x- and y-derivatives.  Each field value is read from HBM once per
+//    chunk; halo re-reads come from L2 of the same XCD (xcd_tile).
+//  * Boundaries are folded into the halo fill: periodic wrap and free-slip
+//    mirror (with the sign flip of an odd field) are applied when a value is
+//    loaded, so every node, boundary rows included, uses the interior formula.
+//    With the reference's operation order this reproduces all 18 boundary
+//    variants of src/derivation.f90 bit for bit (FP contraction is off).
+//  * Several derivatives are fused per field read: the predict kernel computes
+//    9 first and 9 second derivatives, Smagorinsky nu_t, convection, diffusion
+//    and the Euler/AB2/AB3 update from one read of (u, v, w).
+#include <float.h>
+#include <limits.h>
+
+#include <algorithm>
+
+#include "o3d_stencil.h"
+
+namespace o3d {
+
+Tiling make_tiling(int nx, int ny, int nz) {
+  Tiling t;
+  t.ntx = (nx + kTX - 1) / kTX;
+  t.nty = (ny + kTY - 1) / kTY;
+  const int layer = t.ntx * t.nty;
+  int ntz = (2048 + layer - 1) / layer;  // aim at >= 2048 workgroups
+  ntz = std::max(1, std::min(ntz, std::max(1, nz / 32)));  // chunks of >= 32 planes
+  t.zc = (nz + ntz - 1) / ntz;
+  t.ntz = (nz + t.zc - 1) / t.zc;
+  return t;
+}
+
+// Ghost mapping that stays in bounds for threads of a partial tile that lie
+// more than 3 nodes beyond the domain (their values are never used).
+__device__ __forceinline__ int ghost_safe(int q, int n, int bc, bool &flip) {
+  if (q > n + 2) { flip = false; return n - 1; }
+  return ghost_index(q, n, bc, flip);
+}
+
+struct TilePos {
+  int x0, y0, z0, z1, tx, ty, i, j, gi, gj;
+  bool fi, fj, valid;
+};
+
+__device__ __forceinline__ TilePos tile_pos(const Tiling &tl, int nx, int ny, int nz,
+                                            const int *bc) {
+  TilePos p;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tix = t % tl.ntx;
+  const int r = t / tl.ntx;
+  const int tiy = r % tl.nty;
+  const int tiz = r / tl.nty;
+  p.tx = threadIdx.x % kTX;
+  p.ty = threadIdx.x / kTX;
+  p.x0 = tix * kTX;
+  p.y0 = tiy * kTY;
+  p.z0 = tiz * tl.zc;
+  p.z1 = min(p.z0 + tl.zc, nz);
+  p.i = p.x0 + p.tx;
+  p.j = p.y0 + p.ty;
+  p.valid = p.i < nx && p.j < ny;
+  p.gi = ghost_safe(p.i, nx, bc[0], p.fi);
+  p.gj = ghost_safe(p.j, ny, bc[1], p.fj);
+  return p;
+}
+
+// 6th-order first derivative from 7 consecutive values v(-3..3), in the
+// reference's order a*(f+3 - f-3) - b*(f+2 - f-2) + c*(f+1 - f-1).
+template <typename T>
+__device__ __forceinline__ T d1(T m3, T m2, T m1, T p1, T p2, T p3, const T *c) {
+  return c[0] * (p3 - m3) - c[1] * (p2 - m2) + c[2] * (p1 - m1);
+}
+// 4th-order second derivative: -a*(f-2 + f+2) + b*(f-1 + f+1) - c*f.
+template <typename T>
+__device__ __forceinline__ T d2(T m2, T m1, T z, T p1, T p2, const T *c) {
+  return -c[0] * (m2 + p2) + c[1] * (m1 + p1) - c[2] * z;
+}
+
+// ------------------------------------------------------------ generic D1/D2
+// One routine of src/derivation.f90 per launch (the procedure-pointer
+// drop-ins).  which = axis*6 + (order-1)*3 + variant, variant 0 = _00,
+// 1 = p_11 (even), 2 = i_11 (odd); 18/19 = the 2D stubs derz_2dsim and
+// derzz_2dsim (src/derivation.f90:481-495, 934-948).
+template <typename T, int AXIS, int ORDER>
+__global__ void __launch_bounds__(256) der_kernel(T *__restrict__ df, const T *__restrict__ f,
+                                                  int nx, int ny, int nz, int variant, T ca,
+                                                  T cb, T cc) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = nx * ny * nz;
+  if (idx >= N) return;
+  const int i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
+  const int n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+  const int p = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+  const int stride = AXIS == 0 ? 1 : (AXIS == 1 ? nx : nx * ny);
+  const int base = idx - p * stride;
+  const int bc = variant == 0 ? PERIODIC : FREE_SLIP;
+  const bool odd = variant == 2;
+  auto F = [&](int s) -> T {
+    bool fl;
+    const int q = ghost_index(p + s, n, bc, fl);
+    const T v = f[base + q * stride];
+    return (fl && odd) ? -v : v;
+  };
+  const T c[3] = {ca, cb, cc};
+  if (ORDER == 1) {
+    if (variant == 1 && (p == 0 || p == n - 1)) { df[idx] = T(0); return; }
+    df[idx] = d1<T>(F(-3), F(-2), F(-1), F(1), F(2), F(3), c);
+  } else {
+    df[idx] = d2<T>(F(-2), F(-1), f[idx], F(1), F(2), c);
+  }
+}
+
+template <typename T>
+__global__ void der2dsim_kernel(T *__restrict__ df, const T *__restrict__ f, int N, T h) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < N) df[idx] = T(0) * f[idx] * h;
+}
+
+template <typename T>
+hipError_t launch_der(int which, T *df, const T *f, int nx, int ny, int nz, double h,
+                      hipStream_t s) {
+  const int N = nx * ny * nz;
+  const dim3 blk(256), grd((N + 255) / 256);
+  if (which == 18 || which == 19) {
+    der2dsim_kernel<T><<<grd, blk, 0, s>>>(df, f, N, T(h));
+    return hipGetLastError();
+  }
+  if (which < 0 || which > 19) return hipErrorInvalidValue;
+  const int axis = which / 6, order = (which % 6) / 3 + 1, variant = which % 3;
+  double a, b, c;
+  if (order == 1) { D1Coef k = d1coef(h); a = k.a; b = k.b; c = k.c; }
+  else { D2Coef k = d2coef(h); a = k.a; b = k.b; c = k.c; }
+#define O3D_DER(AX, OR)                                                                   \
+  der_kernel<T, AX, OR><<<grd, blk, 0, s>>>(df, f, nx, ny, nz, variant, T(a), T(b), T(c))
+  if (axis == 0) { if (order == 1) O3D_DER(0, 1); else O3D_DER(0, 2); }
+  else if (axis == 1) { if (order == 1) O3D_DER(1, 1); else O3D_DER(1, 2); }
+  else { if (order == 1) O3D_DER(2, 1); else O3D_DER(2, 2); }
+#undef O3D_DER
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ block partials
+__device__ __forceinline__ void partial_init(FieldPartial &p) {
+  p.vmin = DBL_MAX;
+  p.vmax = -DBL_MAX;
+  p.sum = 0.0;
+  p.amax = 0.0;
+  p.imax = INT_MAX;
+  p.nan = 0;
+}
+__device__ __forceinline__ void partial_add(FieldPartial &p, double v, int idx) {
+  if (v != v) { p.nan = 1; return; }
+  p.sum += v;
+  if (v < p.vmin) p.vmin = v;
+  if (v > p.vmax) { p.vmax = v; p.imax = idx; }
+  p.amax = fmax(p.amax, fabs(v));
+}
+
+constexpr int kNW = kTX * kTY / O3D_WAVE;  // waves per stencil block
+
+struct ReduceScratch {
+  double v[kNW];
+  int ix[kNW];
+};
+
+__device__ __forceinline__ void partial_store(FieldPartial p, FieldPartial *out,
+                                              ReduceScratch &sc) {
+  const double vmin = block_min<kNW>(p.vmin, sc.v);
+  const double vmax0 = p.vmax;
+  double vmax = vmax0;
+  int imax = p.imax;
+  block_argmax<kNW>(vmax, imax, sc.v, sc.ix);
+  const double sum = block_sum<kNW>(p.sum, sc.v);
+  const double amax = block_max<kNW>(p.amax, sc.v);
+  const double nan = block_max<kNW>((double)p.nan, sc.v);
+  if (threadIdx.x == 0) {
+    FieldPartial r;
+    r.vmin = vmin; r.vmax = vmax; r.sum = sum; r.amax = amax; r.imax = imax;
+    r.nan = nan > 0.0 ? 1 : 0;
+    *out = r;
+  }
+}
+
+// ------------------------------------------------------------------ predict
+// src/integration.f90:14-197 (+ src/les_turbulence.f90:10-97 when LES).
+template <typename T, int NH, bool LES>
+__global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> A) {
+  __shared__ T pl[3][kTY + 6][kTX + 6];
+  __shared__ ReduceScratch sc;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const TilePos P = tile_pos(A.tl, nx, ny, nz, A.bc);
+  const int plane = nx * ny;
+  const int col = P.gi + nx * P.gj;
+  const int tx = P.tx, ty = P.ty;
+  // u_c is odd along axis c (parity table of src/integration.f90:118-165).
+  const bool own_neg[2] = {P.fi, P.fj};
+
+  // Halo duties of this thread: at most one x-halo and one y-halo cell.
+  int hx_col = -1, hx_g = 0;
+  bool hx_f = false;
+  if (tx < 3) { hx_col = tx; hx_g = ghost_safe(P.x0 + tx - 3, nx, A.bc[0], hx_f); }
+  else if (tx >= kTX - 3) { hx_col = tx + 6; hx_g = ghost_safe(P.x0 + tx + 3, nx, A.bc[0], hx_f); }
+  int hy_row = -1, hy_g = 0;
+  bool hy_f = false;
+  if (ty < 3) { hy_row = ty; hy_g = ghost_safe(P.y0 + ty - 3, ny, A.bc[1], hy_f); }
+  else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
+  const int hx_idx = hx_g + nx * P.gj;
+  const int hy_idx = P.gi + nx * hy_g;
+
+  // Forced zeros of the even free-slip first derivative (src/derivation.f90:87,105).
+  const bool edge_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  const bool edge_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+
+  FieldPartial nut_p;
+  partial_init(nut_p);
+
+  T q[3][7];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    bool fz;
+    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
+    q[0][s] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+    q[1][s] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+    q[2][s] = load_ghost(A.u[2], col + plane * kk, fz);
+  }
+  const T *c1x = A.d1[0], *c1y = A.d1[1], *c1z = A.d1[2];
+  const T *c2x = A.d2[0], *c2y = A.d2[1], *c2z = A.d2[2];
+
+  for (int k = P.z0; k < P.z1; ++k) {
+    {
+      bool fz;
+      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
+      q[0][6] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+      q[1][6] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+      q[2][6] = load_ghost(A.u[2], col + plane * kk, fz);
+    }
+    const int pk = plane * k;
+    T hx[3], hy[3];
+    if (hx_col >= 0) {
+      hx[0] = load_ghost(A.u[0], hx_idx + pk, hx_f);
+      hx[1] = load_ghost(A.u[1], hx_idx + pk, own_neg[1]);
+      hx[2] = A.u[2][hx_idx + pk];
+    }
+    if (hy_row >= 0) {
+      hy[0] = load_ghost(A.u[0], hy_idx + pk, own_neg[0]);
+      hy[1] = load_ghost(A.u[1], hy_idx + pk, hy_f);
+      hy[2] = A.u[2][hy_idx + pk];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pl[c][ty + 3][tx + 3] = q[c][3];
+      if (hx_col >= 0) pl[c][ty + 3][hx_col] = hx[c];
+      if (hy_row >= 0) pl[c][hy_row][tx + 3] = hy[c];
+    }
+    __syncthreads();
+    if (P.valid) {
+      const int idx = P.i + nx * P.j + pk;
+      const int r = ty + 3, cc = tx + 3;
+      T g[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        g[c][0] = d1<T>(pl[c][r][cc - 3], pl[c][r][cc - 2], pl[c][r][cc - 1], pl[c][r][cc + 1],
+                        pl[c][r][cc + 2], pl[c][r][cc + 3], c1x);
+        g[c][1] = d1<T>(pl[c][r - 3][cc], pl[c][r - 2][cc], pl[c][r - 1][cc], pl[c][r + 1][cc],
+                        pl[c][r + 2][cc], pl[c][r + 3][cc], c1y);
+        g[c][2] = d1<T>(q[c][0], q[c][1], q[c][2], q[c][4], q[c][5], q[c][6], c1z);
+      }
+      if (edge_x) { g[1][0] = T(0); g[2][0] = T(0); }
+      if (edge_y) { g[0][1] = T(0); g[2][1] = T(0); }
+      if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) { g[0][2] = T(0); g[1][2] = T(0); }
+      T nu_eff = A.onere;
+      if (LES) {
+        // src/les_turbulence.f90:74-85
+        const T s11 = g[0][0], s22 = g[1][1], s33 = g[2][2];
+        const T s12 = T(0.5) * (g[0][1] + g[1][0]);
+        const T s13 = T(0.5) * (g[0][2] + g[2][0]);
+        const T s23 = T(0.5) * (g[1][2] + g[2][1]);
+        const T smag = sqrt(T(2) * (s11 * s11 + s22 * s22 + s33 * s33 +
+                                    T(2) * (s12 * s12 + s13 * s13 + s23 * s23)));
+        const T nut = A.csd2 * smag;
+        nu_eff = A.onere + nut;
+        if (A.nut) A.nut[idx] = nut;
+        partial_add(nut_p, (double)nut, idx);
+      }
+      const T ux = q[0][3], uy = q[1][3], uz = q[2][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T lx = d2<T>(pl[c][r][cc - 2], pl[c][r][cc - 1], pl[c][r][cc], pl[c][r][cc + 1],
+                           pl[c][r][cc + 2], c2x);
+        const T ly = d2<T>(pl[c][r - 2][cc], pl[c][r - 1][cc], pl[c][r][cc], pl[c][r + 1][cc],
+                           pl[c][r + 2][cc], c2y);
+        const T lz = d2<T>(q[c][1], q[c][2], q[c][3], q[c][4], q[c][5], c2z);
+        // src/integration.f90:129-134 (and :149-154, :169-174)
+        const T F = nu_eff * (lx + ly + lz) - (ux * g[c][0] + uy * g[c][1] + uz * g[c][2]);
+        T un = q[c][3] + A.adu * F;
+        if (NH >= 2) un = un + A.bdu * A.f2[c][idx];
+        if (NH >= 3) un = un + A.cdu * A.f3[c][idx];
+        A.f1[c][idx] = F;
+        A.up[c][idx] = un;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int s = 0; s < 6; ++s) q[c][s] = q[c][s + 1];
+    }
+  }
+  if (LES) partial_store(nut_p, A.nut_part + blockIdx.x, sc);
+}
+
+template <typename T>
+hipError_t launch_predict(const PredictArgs<T> &a, int nh, bool les, hipStream_t s) {
+  const dim3 blk(kTX * kTY), grd(a.tl.blocks());
+#define O3D_PRED(NH, LES) predict_kernel<T, NH, LES><<<grd, blk, 0, s>>>(a)
+  if (les) {
+    if (nh == 1) O3D_PRED(1, true); else if (nh == 2) O3D_PRED(2, true); else O3D_PRED(3, true);
+  } else {
+    if (nh == 1) O3D_PRED(1, false); else if (nh == 2) O3D_PRED(2, false); else O3D_PRED(3, false);
+  }
+#undef O3D_PRED
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------- divergence
+// src/differential_operators.f90:7-38; with divisor = dt this is the
+// Poisson right-hand side of src/integration.f90:235-239.  Statistics of the
+// (undivided) divergence are reduced per block (function_stats).
+template <typename T>
+__global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
+  __shared__ T su[kTY][kTX + 6];
+  __shared__ T sv[kTY + 6][kTX];
+  __shared__ ReduceScratch sc;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const TilePos P = tile_pos(A.tl, nx, ny, nz, A.bc);
+  const int plane = nx * ny;
+  const int col = P.gi + nx * P.gj;
+  const int tx = P.tx, ty = P.ty;
+  const bool odd = A.odd != 0;
+  int hx_col = -1, hx_g = 0;
+  bool hx_f = false;
+  if (tx < 3) { hx_col = tx; hx_g = ghost_safe(P.x0 + tx - 3, nx, A.bc[0], hx_f); }
+  else if (tx >= kTX - 3) { hx_col = tx + 6; hx_g = ghost_safe(P.x0 + tx + 3, nx, A.bc[0], hx_f); }
+  int hy_row = -1, hy_g = 0;
+  bool hy_f = false;
+  if (ty < 3) { hy_row = ty; hy_g = ghost_safe(P.y0 + ty - 3, ny, A.bc[1], hy_f); }
+  else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
+  const int hx_idx = hx_g + nx * P.gj;
+  const int hy_idx = P.gi + nx * hy_g;
+  const bool zero_x = !odd && A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  const bool zero_y = !odd && A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+
+  FieldPartial st;
+  partial_init(st);
+  T qw[7];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    bool fz;
+    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
+    qw[s] = load_ghost(A.f[2], col + plane * kk, odd && fz);
+  }
+  for (int k = P.z0; k < P.z1; ++k) {
+    {
+      bool fz;
+      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
+      qw[6] = load_ghost(A.f[2], col + plane * kk, odd && fz);
+    }
+    const int pk = plane * k;
+    su[ty][tx + 3] = load_ghost(A.f[0], col + pk, odd && P.fi);
+    sv[ty + 3][tx] = load_ghost(A.f[1], col + pk, odd && P.fj);
+    if (hx_col >= 0) su[ty][hx_col] = load_ghost(A.f[0], hx_idx + pk, odd && hx_f);
+    if (hy_row >= 0) sv[hy_row][tx] = load_ghost(A.f[1], hy_idx + pk, odd && hy_f);
+    __syncthreads();
+    if (P.valid) {
+      const int idx = P.i + nx * P.j + pk;
+      const int cc = tx + 3, r = ty + 3;
+      T dfx = d1<T>(su[ty][cc - 3], su[ty][cc - 2], su[ty][cc - 1], su[ty][cc + 1],
+                    su[ty][cc + 2], su[ty][cc + 3], A.d1[0]);
+      T dfy = d1<T>(sv[r - 3][tx], sv[r - 2][tx], sv[r - 1][tx], sv[r + 1][tx], sv[r + 2][tx],
+                    sv[r + 3][tx], A.d1[1]);
+      T dfz = d1<T>(qw[0], qw[1], qw[2], qw[4], qw[5], qw[6], A.d1[2]);
+      if (zero_x) dfx = T(0);
+      if (zero_y) dfy = T(0);
+      if (!odd && A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) dfz = T(0);
+      const T div = dfx + dfy + dfz;
+      if (A.out) A.out[idx] = div / A.divisor;
+      partial_add(st, (double)div, idx);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 6; ++s) qw[s] = qw[s + 1];
+  }
+  partial_store(st, A.part + blockIdx.x, sc);
+}
+
+template <typename T>
+hipError_t launch_divergence(const DivArgs<T> &a, hipStream_t s) {
+  div_kernel<T><<<dim3(a.tl.blocks()), dim3(kTX * kTY), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ correct
+// src/integration.f90:257-330: grad p with the even ("p") variants, then
+// u = u* - dt*grad p; the NaN / max>1000 guard and the driver's velocity
+// min/max and CFL scans (src/IOfunctions.f90:322-330, src/utils.f90:178-208)
+// are reduced in the same pass.
+template <typename T, typename TP>
+__global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, TP> A) {
+  __shared__ TP pl[kTY + 6][kTX + 6];
+  __shared__ ReduceScratch sc;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const TilePos P = tile_pos(A.tl, nx, ny, nz, A.bc);
+  const int plane = nx * ny;
+  const int col = P.gi + nx * P.gj;
+  const int tx = P.tx, ty = P.ty;
+  int hx_col = -1, hx_g = 0;
+  bool hx_f = false;
+  if (tx < 3) { hx_col = tx; hx_g = ghost_safe(P.x0 + tx - 3, nx, A.bc[0], hx_f); }
+  else if (tx >= kTX - 3) { hx_col = tx + 6; hx_g = ghost_safe(P.x0 + tx + 3, nx, A.bc[0], hx_f); }
+  int hy_row = -1, hy_g = 0;
+  bool hy_f = false;
+  if (ty < 3) { hy_row = ty; hy_g = ghost_safe(P.y0 + ty - 3, ny, A.bc[1], hy_f); }
+  else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
+  const int hx_idx = hx_g + nx * P.gj;
+  const int hy_idx = P.gi + nx * hy_g;
+  const bool zero_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  const bool zero_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+
+  FieldPartial st[3];
+  partial_init(st[0]); partial_init(st[1]); partial_init(st[2]);
+  TP q[7];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    bool fz;
+    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
+    q[s] = A.p[col + plane * kk];
+  }
+  for (int k = P.z0; k < P.z1; ++k) {
+    {
+      bool fz;
+      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
+      q[6] = A.p[col + plane * kk];
+    }
+    const int pk = plane * k;
+    pl[ty + 3][tx + 3] = q[3];
+    if (hx_col >= 0) pl[ty + 3][hx_col] = A.p[hx_idx + pk];
+    if (hy_row >= 0) pl[hy_row][tx + 3] = A.p[hy_idx + pk];
+    __syncthreads();
+    if (P.valid) {
+      const int idx = P.i + nx * P.j + pk;
+      const int cc = tx + 3, r = ty + 3;
+      TP g[3];
+      g[0] = d1<TP>(pl[r][cc - 3], pl[r][cc - 2], pl[r][cc - 1], pl[r][cc + 1], pl[r][cc + 2],
+                    pl[r][cc + 3], A.d1[0]);
+      g[1] = d1<TP>(pl[r - 3][cc], pl[r - 2][cc], pl[r - 1][cc], pl[r + 1][cc], pl[r + 2][cc],
+                    pl[r + 3][cc], A.d1[1]);
+      g[2] = d1<TP>(q[0], q[1], q[2], q[4], q[5], q[6], A.d1[2]);
+      if (zero_x) g[0] = TP(0);
+      if (zero_y) g[1] = TP(0);
+      if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) g[2] = TP(0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T un = T(TP(A.up[c][idx]) - A.dt * g[c]);
+        A.u[c][idx] = un;
+        partial_add(st[c], (double)un, idx);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 6; ++s) q[s] = q[s + 1];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) partial_store(st[c], A.part + 3 * blockIdx.x + c, sc);
+}
+
+template <typename T, typename TP>
+hipError_t launch_correct(const CorrectArgs<T, TP> &a, hipStream_t s) {
+  correct_kernel<T, TP><<<dim3(a.tl.blocks()), dim3(kTX * kTY), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------- statistics
+// src/utils.f90:243-375: per block, the 16 sums behind E_k, eps, eps2,
+// enstrophy and the mean squares (FP64 accumulation).
+// Slot order: 0 e_k, 1 eps, 2 eps2, 3 dzeta, 4..6 u^2 v^2 w^2, 7..15 (du_c/dx_a)^2.
+template <typename T>
+__global__ void __launch_bounds__(kTX *kTY) stats_kernel(const StatsArgs<T> A) {
+  __shared__ T pl[3][kTY + 6][kTX + 6];
+  __shared__ ReduceScratch sc;
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const TilePos P = tile_pos(A.tl, nx, ny, nz, A.bc);
+  const int plane = nx * ny;
+  const int col = P.gi + nx * P.gj;
+  const int tx = P.tx, ty = P.ty;
+  const bool own_neg[2] = {P.fi, P.fj};
+  int hx_col = -1, hx_g = 0;
+  bool hx_f = false;
+  if (tx < 3) { hx_col = tx; hx_g = ghost_safe(P.x0 + tx - 3, nx, A.bc[0], hx_f); }
+  else if (tx >= kTX - 3) { hx_col = tx + 6; hx_g = ghost_safe(P.x0 + tx + 3, nx, A.bc[0], hx_f); }
+  int hy_row = -1, hy_g = 0;
+  bool hy_f = false;
+  if (ty < 3) { hy_row = ty; hy_g = ghost_safe(P.y0 + ty - 3, ny, A.bc[1], hy_f); }
+  else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
+  const int hx_idx = hx_g + nx * P.gj;
+  const int hy_idx = P.gi + nx * hy_g;
+  const bool edge_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  const bool edge_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  double acc[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) acc[m] = 0.0;
+  T q[3][7];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    bool fz;
+    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
+    q[0][s] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+    q[1][s] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+    q[2][s] = load_ghost(A.u[2], col + plane * kk, fz);
+  }
+  for (int k = P.z0; k < P.z1; ++k) {
+    {
+      bool fz;
+      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
+      q[0][6] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+      q[1][6] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+      q[2][6] = load_ghost(A.u[2], col + plane * kk, fz);
+    }
+    const int pk = plane * k;
+    T hx[3], hy[3];
+    if (hx_col >= 0) {
+      hx[0] = load_ghost(A.u[0], hx_idx + pk, hx_f);
+      hx[1] = load_ghost(A.u[1], hx_idx + pk, own_neg[1]);
+      hx[2] = A.u[2][hx_idx + pk];
+    }
+    if (hy_row >= 0) {
+      hy[0] = load_ghost(A.u[0], hy_idx + pk, own_neg[0]);
+      hy[1] = load_ghost(A.u[1], hy_idx + pk, hy_f);
+      hy[2] = A.u[2][hy_idx + pk];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pl[c][ty + 3][tx + 3] = q[c][3];
+      if (hx_col >= 0) pl[c][ty + 3][hx_col] = hx[c];
+      if (hy_row >= 0) pl[c][hy_row][tx + 3] = hy[c];
+    }
+    __syncthreads();
+    if (P.valid) {
+      const int r = ty + 3, cc = tx + 3;
+      double g[3][3], l[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        g[c][0] = d1<T>(pl[c][r][cc - 3], pl[c][r][cc - 2], pl[c][r][cc - 1], pl[c][r][cc + 1],
+                        pl[c][r][cc + 2], pl[c][r][cc + 3], A.d1[0]);
+        g[c][1] = d1<T>(pl[c][r - 3][cc], pl[c][r - 2][cc], pl[c][r - 1][cc], pl[c][r + 1][cc],
+                        pl[c][r + 2][cc], pl[c][r + 3][cc], A.d1[1]);
+        g[c][2] = d1<T>(q[c][0], q[c][1], q[c][2], q[c][4], q[c][5], q[c][6], A.d1[2]);
+        l[c][0] = d2<T>(pl[c][r][cc - 2], pl[c][r][cc - 1], pl[c][r][cc], pl[c][r][cc + 1],
+                        pl[c][r][cc + 2], A.d2[0]);
+        l[c][1] = d2<T>(pl[c][r - 2][cc], pl[c][r - 1][cc], pl[c][r][cc], pl[c][r + 1][cc],
+                        pl[c][r + 2][cc], A.d2[1]);
+        l[c][2] = d2<T>(q[c][1], q[c][2], q[c][3], q[c][4], q[c][5], A.d2[2]);
+      }
+      if (edge_x) { g[1][0] = 0.0; g[2][0] = 0.0; }
+      if (edge_y) { g[0][1] = 0.0; g[2][1] = 0.0; }
+      if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) { g[0][2] = 0.0; g[1][2] = 0.0; }
+      const double ux = q[0][3], uy = q[1][3], uz = q[2][3];
+      const double xnu = A.xnu;
+      acc[0] += 0.5 * (ux * ux + uy * uy + uz * uz);
+      const double a2 = 2.0 * g[0][0], b2 = 2.0 * g[1][1], c2 = 2.0 * g[2][2];
+      const double s12 = g[0][1] + g[1][0], s13 = g[0][2] + g[2][0], s23 = g[1][2] + g[2][1];
+      acc[1] += 0.5 * xnu * (a2 * a2 + b2 * b2 + c2 * c2 + 2.0 * (s12 * s12) +
+                             2.0 * (s13 * s13) + 2.0 * (s23 * s23));
+      acc[2] += (-xnu) * (ux * (l[0][0] + l[0][1] + l[0][2]) + uy * (l[1][0] + l[1][1] + l[1][2]) +
+                          uz * (l[2][0] + l[2][1] + l[2][2]));
+      const double w1 = g[2][1] - g[1][2], w2 = g[0][2] - g[2][0], w3 = g[1][0] - g[0][1];
+      acc[3] += 0.5 * (w1 * w1 + w2 * w2 + w3 * w3);
+      acc[4] += ux * ux;
+      acc[5] += uy * uy;
+      acc[6] += uz * uz;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[7 + 3 * c + a] += g[c][a] * g[c][a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int s = 0; s < 6; ++s) q[c][s] = q[c][s + 1];
+  }
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const double v = block_sum<kNW>(acc[m], sc.v);
+    if (threadIdx.x == 0) A.part[16 * blockIdx.x + m] = v;
+  }
+}
+
+template <typename T>
+hipError_t launch_statistics(const StatsArgs<T> &a, hipStream_t s) {
+  stats_kernel<T><<<dim3(a.tl.blocks()), dim3(kTX * kTY), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------- plain field scan
+template <typename T>
+__global__ void __launch_bounds__(kTX *kTY) field_stats_kernel(const T *__restrict__ f, int n,
+                                                               FieldPartial *part) {
+  __shared__ ReduceScratch sc;
+  FieldPartial st;
+  partial_init(st);
+  // Contiguous chunk per block so that per-thread sums run in index order.
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(n, lo + per);
+  for (int idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) partial_add(st, (double)f[idx], idx);
+  partial_store(st, part + blockIdx.x, sc);
+}
+
+template <typename T>
+hipError_t launch_field_stats(const T *f, int n, FieldPartial *part, int nblocks, hipStream_t s) {
+  field_stats_kernel<T><<<dim3(nblocks), dim3(kTX * kTY), 0, s>>>(f, n, part);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- finalize
+__global__ void __launch_bounds__(1024) finalize_partials_kernel(const FieldPartial *part,
+                                                                 int nparts, int stride,
+                                                                 int first, FieldPartial *out) {
+  __shared__ double sv[16];
+  __shared__ int si[16];
+  FieldPartial st;
+  partial_init(st);
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+    const FieldPartial p = part[(size_t)b * stride + first];
+    st.sum += p.sum;
+    st.vmin = fmin(st.vmin, p.vmin);
+    argmax_merge(st.vmax, st.imax, p.vmax, p.imax);
+    st.amax = fmax(st.amax, p.amax);
+    st.nan |= p.nan;
+  }
+  const double vmin = block_min<16>(st.vmin, sv);
+  double vmax = st.vmax;
+  int imax = st.imax;
+  block_argmax<16>(vmax, imax, sv, si);
+  const double sum = block_sum<16>(st.sum, sv);
+  const double amax = block_max<16>(st.amax, sv);
+  const double nan = block_max<16>((double)st.nan, sv);
+  if (threadIdx.x == 0) {
+    FieldPartial r;
+    r.vmin = vmin; r.vmax = vmax; r.imax = imax; r.sum = sum; r.amax = amax;
+    r.nan = nan > 0.0 ? 1 : 0;
+    *out = r;
+  }
+}
+
+hipError_t launch_finalize_partials(const FieldPartial *part, int nparts, int stride, int first,
+                                    FieldPartial *out, hipStream_t s) {
+  finalize_partials_kernel<<<1, 1024, 0, s>>>(part, nparts, stride, first, out);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(1024) finalize_sums_kernel(const double *part, int nparts,
+                                                             int width, double *out) {
+  __shared__ double sv[16];
+  for (int m = 0; m < width; ++m) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += part[(size_t)b * width + m];
+    s = block_sum<16>(s, sv);
+    if (threadIdx.x == 0) out[m] = s;
+  }
+}
+
+hipError_t launch_finalize_sums(const double *part, int nparts, int width, double *out,
+                                hipStream_t s) {
+  finalize_sums_kernel<<<1, 1024, 0, s>>>(part, nparts, width, out);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------- instantiations
+template hipError_t launch_der<double>(int, double *, const double *, int, int, int, double,
+                                       hipStream_t);
+template hipError_t launch_der<float>(int, float *, const float *, int, int, int, double,
+                                      hipStream_t);
+template hipError_t launch_predict<double>(const PredictArgs<double> &, int, bool, hipStream_t);
+template hipError_t launch_predict<float>(const PredictArgs<float> &, int, bool, hipStream_t);
+template hipError_t launch_divergence<double>(const DivArgs<double> &, hipStream_t);
+template hipError_t launch_divergence<float>(const DivArgs<float> &, hipStream_t);
+template hipError_t launch_correct<double, double>(const CorrectArgs<double, double> &,
+                                                   hipStream_t);
+template hipError_t launch_correct<float, double>(const CorrectArgs<float, double> &,
+                                                  hipStream_t);
+template hipError_t launch_statistics<double>(const StatsArgs<double> &, hipStream_t);
+template hipError_t launch_statistics<float>(const StatsArgs<float> &, hipStream_t);
+template hipError_t launch_field_stats<double>(const double *, int, FieldPartial *, int,
+                                               hipStream_t);
+template hipError_t launch_field_stats<float>(const float *, int, FieldPartial *, int,
+                                              hipStream_t);
+
+}  // namespace o3d

@@ -1,0 +1,139 @@
+"""Host side of the time loop: the reference's driver sequence on the device.
+
+``Solver`` owns one device-resident solver of libo3d.so built from a
+``Config`` (the reference's parameters.o3d) and runs whole time steps of
+src/osinco3d_main.f90:97-128 on the GPU.  The initial fields are the
+reference's (Taylor-Green: src/initial_conditions.f90:157-162) or come from a
+fields.bin checkpoint (typesim = 0).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import capi
+from .config import Config
+
+FP64 = 0
+MIXED = 1
+WARM_PREVIOUS = 0
+WARM_LEARNED = 1
+
+SOR_REASONS = {0: "eps", 1: "stagnation", 2: "kmax"}
+
+
+def grid(cfg: Config):
+    """Node coordinates x(i) = x0 + (i-1) dx (src/initialization.f90:211-219)."""
+    dx, dy, dz = cfg.spacing()
+    x = cfg.x0 + np.arange(cfg.nx, dtype=np.float64) * dx
+    y = cfg.y0 + np.arange(cfg.ny, dtype=np.float64) * dy
+    z = cfg.z0 + np.arange(cfg.nz, dtype=np.float64) * dz
+    return x, y, z
+
+
+def taylor_green(cfg: Config) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """u = sin x cos y cos z, v = -cos x sin y cos z, w = 0 and the
+    pressure 1/16 (cos 2x + cos 2y)(cos 2z + 2)
+    (src/initial_conditions.f90:157-162).  Arrays of shape (nz, ny, nx)."""
+    x, y, z = grid(cfg)
+    amp = cfg.ratio * cfg.u0 / cfg.l0
+    sx, cx = np.sin(x), np.cos(x)
+    sy, cy = np.sin(y), np.cos(y)
+    cz = np.cos(z)
+    ux = ((amp * sx)[None, None, :] * cy[None, :, None]) * cz[:, None, None]
+    uy = ((-amp * cx)[None, None, :] * sy[None, :, None]) * cz[:, None, None]
+    uz = np.zeros_like(ux)
+    pp = 0.0625 * (np.cos(2 * x)[None, None, :] + np.cos(2 * y)[None, :, None]) * \
+        (np.cos(2 * z)[:, None, None] + 2.0)
+    return (np.ascontiguousarray(ux), np.ascontiguousarray(uy), uz,
+            np.ascontiguousarray(pp))
+
+
+def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
+                inner_max: int = 0) -> capi.O3DParams:
+    p = capi.O3DParams()
+    p.nx, p.ny, p.nz = cfg.nx, cfg.ny, cfg.nz
+    p.dx, p.dy, p.dz = cfg.spacing()
+    p.bc = (C.c_int * 3)(*cfg.bc())
+    p.itscheme = cfg.itscheme
+    dt = cfg.time_step_overrides()
+    p.dt = dt
+    p.re = cfg.re
+    c2 = Config(**{**cfg.__dict__, "dt": dt})
+    adt, bdt, cdt = c2.ab_coefficients()
+    p.adt = (C.c_double * 3)(*adt)
+    p.bdt = (C.c_double * 3)(*bdt)
+    p.cdt = (C.c_double * 3)(*cdt)
+    p.iles = cfg.iles
+    p.cs = cfg.cs
+    p.delta = cfg.delta()
+    p.omega = cfg.omega
+    p.eps = cfg.eps
+    p.kmax = cfg.kmax
+    p.idyn = cfg.idyn
+    p.precision = precision
+    p.warm_start = warm
+    p.inner_max = inner_max
+    return p
+
+
+class Solver:
+    """Device-resident time stepping of one configuration."""
+
+    def __init__(self, cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
+                 inner_max: int = 0):
+        self.cfg = cfg
+        self.lib = capi.load()
+        self.params = make_params(cfg, precision, warm, inner_max)
+        self.handle = C.c_void_p(0)
+        capi.check(self.lib.o3d_solver_create(C.byref(self.params), C.byref(self.handle)))
+        self.dt = self.params.dt
+
+    def close(self):
+        if self.handle:
+            self.lib.o3d_solver_destroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, ux=None, uy=None, uz=None, pp=None):
+        capi.check(self.lib.o3d_solver_upload(self.handle, capi.ptr(ux), capi.ptr(uy),
+                                              capi.ptr(uz), capi.ptr(pp)))
+
+    def download(self):
+        shape = (self.cfg.nz, self.cfg.ny, self.cfg.nx)
+        out = [np.empty(shape) for _ in range(4)]
+        capi.check(self.lib.o3d_solver_download(self.handle, *[capi.ptr(a) for a in out]))
+        return out
+
+    def init_taylor_green(self):
+        self.upload(*taylor_green(self.cfg))
+
+    def step(self, itime: int) -> dict:
+        info = capi.O3DStepInfo()
+        rc = capi.check(self.lib.o3d_solver_step(self.handle, int(itime), C.byref(info)))
+        d = info.to_dict()
+        d["rc"] = rc
+        return d
+
+    def statistics(self, t: float) -> np.ndarray:
+        out = np.zeros(17)
+        capi.check(self.lib.o3d_solver_statistics(self.handle, C.c_double(t),
+                                                  out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def field_ptr(self, which: int) -> int:
+        return int(self.lib.o3d_solver_field(self.handle, int(which)) or 0)
+
+    def sor_timing(self):
+        ms = (C.c_double * 2)()
+        sw = (C.c_int * 2)()
+        capi.check(self.lib.o3d_solver_sor_timing(self.handle, ms, sw))
+        return list(ms), list(sw)

@@ -1,0 +1,260 @@
+"""GPU parity of every operator entry point against the oracle.
+
+The FP64 kernels keep the reference's operation order and are compiled
+without FP contraction, so stencil outputs must be bit-identical to the C
+restatement (which is itself bit-identical to the reference Fortran, see
+test_oracle.py).  Reductions (sums) differ only in summation order.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from osinco3d_amd import capi
+from tests.helpers import rand_fields, solver_code
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(9, 11, 13), (12, 10, 70), (16, 12, 8)]   # (nz, ny, nx)
+BCS = [(0, 0, 0), (0, 1, 0), (1, 1, 1), (1, 0, 1)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    capi.load()
+    print(f"[gpu] libo3d {capi.load().o3d_version().decode()}", flush=True)
+    return torch
+
+
+def _kind(bc_axis, odd):
+    return 0 if bc_axis == 0 else (2 if odd else 1)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_derivatives_bit_exact(orc, shape):
+    f, = rand_fields(shape, 1, seed=3)
+    h = 0.173
+    for which in range(20):
+        df = np.full_like(f, np.nan)
+        capi.der(which, df, f, h)
+        if which >= 18:
+            assert np.array_equal(df, 0.0 * f * h)
+            continue
+        axis, order, kind = which // 6, (which % 6) // 3 + 1, which % 3
+        exp = orc.der1(f, axis, kind, h) if order == 1 else orc.der2(f, axis, kind, h)
+        assert np.array_equal(df, exp), (which, np.abs(df - exp).max())
+
+
+def test_derivatives_device_pointers(orc, gpu):
+    torch = gpu
+    f, = rand_fields((14, 18, 66), 1, seed=4)
+    ft = torch.from_numpy(f).cuda()
+    dft = torch.empty_like(ft)
+    for which in (0, 5, 7, 11, 14, 15):
+        capi.der(which, dft, ft, 0.05)
+        torch.cuda.synchronize()
+        axis, order, kind = which // 6, (which % 6) // 3 + 1, which % 3
+        exp = orc.der1(f, axis, kind, 0.05) if order == 1 else orc.der2(f, axis, kind, 0.05)
+        assert np.array_equal(dft.cpu().numpy(), exp)
+
+
+def test_derivatives_f32_close(orc):
+    f, = rand_fields((10, 12, 14), 1, seed=5)
+    f32 = f.astype(np.float32)
+    for which in range(18):
+        df = np.empty_like(f32)
+        capi.der(which, df, f32, 0.1)
+        axis, order, kind = which // 6, (which % 6) // 3 + 1, which % 3
+        exp = orc.der1(f32.astype(np.float64), axis, kind, 0.1) if order == 1 else \
+            orc.der2(f32.astype(np.float64), axis, kind, 0.1)
+        # FP32 tolerance: a few ulp of the largest term (coefficient * |f|).
+        scale = np.abs(exp).max() + (1 / 0.1 if order == 1 else 30 / (12 * 0.01)) * 4
+        assert np.abs(df - exp).max() <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("odd", [0, 1])
+def test_divergence(orc, bc, odd):
+    sh = (12, 10, 70)
+    fx, fy, fz = rand_fields(sh, 3, seed=7)
+    d = (0.11, 0.13, 0.17)
+    for divisor in (1.0, 0.37):
+        out = np.empty_like(fx)
+        st = capi.divergence(out, fx, fy, fz, d, bc, odd, divisor)
+        div = orc.divergence(fx, fy, fz, d, bc, odd)
+        assert np.array_equal(out, div / divisor)
+        ref = orc.function_stats(div)
+        assert st[0] == ref[0] and st[1] == ref[1]
+        assert np.array_equal(st[3:], ref[3:])
+        assert abs(st[2] - ref[2]) <= 1e-13 * np.abs(div).max()
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_nu_t(orc, bc):
+    ux, uy, uz = rand_fields((10, 12, 14), 3, seed=9)
+    d = (0.2, 0.21, 0.19)
+    out = np.empty_like(ux)
+    st = capi.nu_t(out, ux, uy, uz, d, 0.17, 0.2, bc)
+    exp = orc.nu_t(ux, uy, uz, d, 0.17, 0.2, bc)
+    assert np.array_equal(out, exp)
+    ref = orc.function_stats(exp)
+    assert st[0] == ref[0] and st[1] == ref[1] and np.array_equal(st[3:], ref[3:])
+
+
+@pytest.mark.parametrize("bc", BCS)
+@pytest.mark.parametrize("itscheme", [1, 2, 3])
+@pytest.mark.parametrize("iles", [0, 1])
+def test_predict_velocity(orc, bc, itscheme, iles):
+    sh = (10, 12, 14)
+    u = rand_fields(sh, 3, seed=11)
+    d = (0.11, 0.13, 0.17)
+    dt = 0.01
+    adt = [dt, 3 * dt / 2, 23 * dt / 12]
+    bdt = [0.0, -1 * dt / 2, -16 * dt / 12]
+    cdt = [0.0, 0.0, 5 * dt / 12]
+    rng = np.random.default_rng(12)
+    Fg = [rng.standard_normal((3,) + sh) for _ in range(3)]
+    Fo = [f.copy() for f in Fg]
+    for itime in (1, 2, 3, 4):
+        upg = [np.empty(sh) for _ in range(3)]
+        nutg = np.full(sh, np.nan)
+        nuto = np.zeros(sh)
+        capi.predict_velocity(upg, u, Fg, 1600.0, adt, bdt, cdt, itime, itscheme, d, iles, 0.17,
+                              0.2, nutg, bc)
+        upo = orc.predict(*u, *Fo, 1600.0, adt, bdt, cdt, itime, itscheme, d, iles, 0.17, 0.2,
+                          nuto, bc)
+        for a, b in zip(upg, upo):
+            assert np.array_equal(a, b), (itime, np.abs(a - b).max())
+        for a, b in zip(Fg, Fo):
+            assert np.array_equal(a, b)
+        assert np.array_equal(nutg, nuto)
+
+
+def test_predict_rejects_unknown_scheme():
+    sh = (8, 8, 8)
+    u = rand_fields(sh, 3)
+    F = [np.zeros((3,) + sh) for _ in range(3)]
+    up = [np.empty(sh) for _ in range(3)]
+    with pytest.raises(capi.O3DError, match="itscheme"):
+        capi.predict_velocity(up, u, F, 100.0, [1, 1, 1], [0, 0, 0], [0, 0, 0], 5, 7,
+                              (0.1, 0.1, 0.1), 0, 0.1, 0.1, np.zeros(sh), (0, 0, 0))
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_correct_velocity(orc, bc):
+    sh = (12, 10, 70)
+    up = rand_fields(sh, 3, seed=13)
+    pp, = rand_fields(sh, 1, seed=14)
+    d = (0.11, 0.13, 0.17)
+    ug = [np.empty(sh) for _ in range(3)]
+    rc, vs = capi.correct_velocity(ug, up, pp, 0.02, d, bc)
+    uo = [np.empty(sh) for _ in range(3)]
+    rco = orc.correct_velocity(*uo, *up, pp, 0.02, d, bc)
+    assert rc == rco == 0
+    for c in range(3):
+        assert np.array_equal(ug[c], uo[c])
+        assert vs[c, 0] == uo[c].min() and vs[c, 1] == uo[c].max()
+        assert vs[c, 2] == np.abs(uo[c]).max() and vs[c, 3] == 0
+
+
+def test_correct_velocity_guards(orc):
+    sh = (8, 8, 8)
+    up = rand_fields(sh, 3, seed=15)
+    pp = np.zeros(sh)
+    d = (0.1, 0.1, 0.1)
+    up[1][3, 4, 5] = 1500.0          # maxval(uy) > 1000 -> diverged
+    ug = [np.empty(sh) for _ in range(3)]
+    rc, vs = capi.correct_velocity(ug, up, pp, 0.02, d, (0, 0, 0))
+    assert rc == 1 and orc.correct_velocity(*[np.empty(sh) for _ in range(3)], *up, pp, 0.02, d,
+                                            (0, 0, 0)) == 1
+    up[1][3, 4, 5] = -1500.0         # signed maximum: a large negative value does not trip it
+    rc, _ = capi.correct_velocity(ug, up, pp, 0.02, d, (0, 0, 0))
+    assert rc == 0
+    up[2][1, 1, 1] = np.nan          # NaN -> diverged
+    rc, vs = capi.correct_velocity(ug, up, pp, 0.02, d, (0, 0, 0))
+    assert rc == 1 and vs[2, 3] == 1
+
+
+def test_function_stats(orc):
+    f, = rand_fields((13, 17, 71), 1, seed=16)
+    st = capi.function_stats(f)
+    ref = orc.function_stats(f)
+    assert st[0] == ref[0] and st[1] == ref[1] and np.array_equal(st[3:], ref[3:])
+    assert abs(st[2] - ref[2]) <= 1e-14 * np.abs(f).max()
+    g = np.zeros((9, 9, 9))
+    g[2, 3, 4] = 5.0
+    g[7, 1, 1] = 5.0                 # tie: the first in column-major order wins
+    st = capi.function_stats(g)
+    assert list(st[3:]) == [5, 4, 3] == list(orc.function_stats(g)[3:])
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_statistics(orc, bc):
+    u = rand_fields((10, 12, 14), 3, seed=17)
+    d = (0.2, 0.21, 0.19)
+    g = capi.statistics(*u, d, 1600.0, 1.5, bc)
+    o = orc.statistics(*u, d, 1600.0, 1.5, bc)
+    assert np.allclose(g, o, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("solver,shape", [("0000", (10, 12, 14)), ("0011", (10, 9, 14)),
+                                          ("111111", (9, 11, 13)), ("0000", (16, 16, 130))])
+@pytest.mark.parametrize("idyn", [0, 1])
+def test_sor_red_black_bit_exact(orc, solver, shape, idyn):
+    rhs, p0 = rand_fields(shape, 2, seed=19)
+    rhs -= rhs.mean()
+    d = (0.3, 0.35, 0.4)
+    for eps, kmax in ((1e-8, 400), (1e-30, 7)):
+        pg = p0.copy()
+        po = p0.copy()
+        it, dm, rs, om = capi.poisson(pg, rhs, d, 1.6, eps, kmax, idyn, solver)
+        ito, dmo, rso, omo, _ = orc.sor(po, rhs, d, solver, 1.6, eps, kmax, idyn, red_black=True)
+        assert (it, rs) == (ito, rso), (it, rs, ito, rso)
+        assert dm == dmo and om == omo
+        assert np.array_equal(pg, po)
+
+
+@pytest.mark.parametrize("solver", ["0000", "0011", "111111"])
+def test_sor_converges_like_reference(orc, solver):
+    """Red-black and lexicographic SOR reach the same field (up to the
+    constant null space) at the same eps."""
+    sh = (12, 12, 16)
+    rhs, = rand_fields(sh, 1, seed=21)
+    d = (0.3, 0.3, 0.3)
+    # A compatible rhs for every BC: the residual of a random field.
+    q, = rand_fields(sh, 1, seed=22)
+    rhs = -orc.residual(q, np.zeros(sh), d, solver)
+    eps = 1e-11
+    pg = np.zeros(sh)
+    po = np.zeros(sh)
+    capi.poisson(pg, rhs, d, 1.7, eps, 20000, 0, solver)
+    orc.sor(po, rhs, d, solver, 1.7, eps, 20000, 0, red_black=False)
+    a = pg - pg.mean()
+    b = po - po.mean()
+    assert np.abs(a - b).max() <= 1e4 * eps
+    assert np.abs(a - (q - q.mean())).max() <= 1e4 * eps
+
+
+def test_sor_rejects_odd_periodic():
+    sh = (9, 10, 10)
+    with pytest.raises(capi.O3DError, match="even"):
+        capi.poisson(np.zeros(sh), np.zeros(sh), (0.1, 0.1, 0.1), 1.5, 1e-8, 10, 0, "0000")
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
+def test_correct_pression(orc, bc):
+    sh = (10, 12, 14)
+    up = rand_fields(sh, 3, seed=23)
+    d = (0.3, 0.32, 0.29)
+    dt = 0.05
+    p0, = rand_fields(sh, 1, seed=24)
+    pg = p0.copy()
+    po = p0.copy()
+    it, dm, rs, om = capi.correct_pression(pg, up, d, dt, 1.7, 1e-8, 300, 0, bc)
+    rhs = orc.divergence(*up, d, bc, 1) / dt
+    ito, dmo, rso, omo, _ = orc.sor(po, rhs, d, solver_code(bc), 1.7, 1e-8, 300, 0,
+                                    red_black=True)
+    assert (it, rs, dm) == (ito, rso, dmo)
+    assert np.array_equal(pg, po)

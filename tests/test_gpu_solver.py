@@ -1,0 +1,116 @@
+"""Whole time steps on the GPU against the oracle.
+
+The FP64 solver uses red-black SOR, so it is compared bit for bit with the
+oracle's composition of the same steps in red-black order, and against the
+reference's lexicographic order within a tolerance tied to eps (the two
+orderings converge to the same field up to the constant null space).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from osinco3d_amd.config import Config, tgv_config
+from osinco3d_amd.solver import FP64, MIXED, Solver, taylor_green
+from tests.helpers import OracleStepper, rand_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg_free_slip_les():
+    cfg = tgv_config(16, 6, 3)
+    cfg.nbcx1 = cfg.nbcxn = cfg.nbcy1 = cfg.nbcyn = cfg.nbcz1 = cfg.nbczn = 1
+    cfg.xlx = cfg.yly = cfg.zlz = np.pi
+    cfg.iles = 1
+    cfg.dt = 0.01
+    cfg.omega = 1.7
+    return cfg
+
+
+def _cfg_mixing():
+    cfg = Config(nx=24, ny=16, nz=12, xlx=6.0, yly=4.0, zlz=3.0, y0=-2.0, nbcy1=1, nbcyn=1,
+                 itscheme=2, dt=0.02, re=3000.0, iles=1, cs=0.17, omega=1.8, eps=1e-8,
+                 kmax=3000, typesim=0)
+    return cfg
+
+
+def _init(cfg, seed=0):
+    if cfg.typesim == 2:
+        return list(taylor_green(cfg))
+    sh = (cfg.nz, cfg.ny, cfg.nx)
+    u = rand_fields(sh, 3, seed=seed, scale=0.05)
+    y = cfg.y0 + np.arange(cfg.ny) * cfg.spacing()[1]
+    u[0] += 0.5 * np.tanh(2 * y)[None, :, None]
+    return [*u, np.zeros(sh)]
+
+
+@pytest.mark.parametrize("make", [lambda: tgv_config(16, 6, 2), lambda: tgv_config(16, 6, 3),
+                                  _cfg_free_slip_les, _cfg_mixing])
+def test_fp64_steps_bit_exact(orc, make):
+    cfg = make()
+    print(f"[gpu] solver bit-exact {cfg.nx}x{cfg.ny}x{cfg.nz} bc={cfg.bc()} "
+          f"scheme={cfg.itscheme} les={cfg.iles}", flush=True)
+    ux, uy, uz, pp = _init(cfg)
+    s = Solver(cfg, FP64)
+    s.upload(ux, uy, uz, pp)
+    os_ = OracleStepper(orc, cfg, red_black=True)
+    u = [ux.copy(), uy.copy(), uz.copy()]
+    p = pp.copy()
+    for itime in range(1, 6):
+        info = s.step(itime)
+        u, p, up, oi = os_.step(itime, *u, p)
+        g = s.download()
+        for c in range(3):
+            assert np.array_equal(g[c], u[c]), (itime, c, np.abs(g[c] - u[c]).max())
+        assert np.array_equal(g[3], p)
+        assert info["sor_exit_iter"] == oi["exit_iter"]
+        assert info["sor_reason"] == oi["reason"]
+        assert info["diverged"] == 0
+        div = orc.divergence(*u, cfg.spacing(), list(cfg.bc()), 1)
+        st = orc.function_stats(div)
+        assert info["divu"][0] == st[0] and info["divu"][1] == st[1]
+        assert list(info["divu"][3:]) == list(st[3:])
+        for c in range(3):
+            assert info["vel_max"][c] == u[c].max() and info["vel_min"][c] == u[c].min()
+    s.close()
+
+
+def test_fp64_close_to_reference_ordering(orc):
+    """Reference (lexicographic) vs GPU (red-black) after 5 TGV steps."""
+    cfg = tgv_config(16, 5, 2, eps=1e-12, kmax=20000)
+    ux, uy, uz, pp = taylor_green(cfg)
+    s = Solver(cfg, FP64)
+    s.upload(ux, uy, uz, pp)
+    lex = OracleStepper(orc, cfg, red_black=False)
+    u = [ux.copy(), uy.copy(), uz.copy()]
+    p = pp.copy()
+    for itime in range(1, 6):
+        s.step(itime)
+        u, p, _, _ = lex.step(itime, *u, p)
+    g = s.download()
+    for c in range(3):
+        assert np.abs(g[c] - u[c]).max() <= 1e-8
+    assert np.abs((g[3] - g[3].mean()) - (p - p.mean())).max() <= 1e-8
+
+
+def test_mixed_precision_tracks_fp64():
+    cfg = tgv_config(32, 10, 3)
+    ux, uy, uz, pp = taylor_green(cfg)
+    a = Solver(cfg, FP64)
+    b = Solver(cfg, MIXED)
+    a.upload(ux, uy, uz, pp)
+    b.upload(ux, uy, uz, pp)
+    for itime in range(1, 11):
+        ia = a.step(itime)
+        ib = b.step(itime)
+        assert ib["sor_reason"] == 0 and ib["sor_dmax"] < cfg.eps
+        assert ib["sweeps_f32"] > 0 and ib["sweeps_f64"] >= 1
+    sa = a.statistics(10 * a.dt)
+    sb = b.statistics(10 * b.dt)
+    assert abs(sb[1] - sa[1]) <= 1e-6 * abs(sa[1])     # kinetic energy
+    assert abs(sb[4] - sa[4]) <= 1e-5 * abs(sa[4])     # enstrophy
+    ga, gb = a.download(), b.download()
+    for c in range(3):
+        assert np.abs(ga[c] - gb[c]).max() <= 1e-5
+    a.close()
+    b.close()
