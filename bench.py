@@ -1,0 +1,286 @@
+#!/usr/bin/env python3
+"""Benchmark of the osinco3d time-step path on one MI355X.
+
+Workload: Taylor-Green vortex, Re = 1600, 256^3 exactly periodic box, AB3,
+CFL 0.25, SOR omega = 2/(1+sin(pi/256)), eps = 1e-8, kmax = 5000.  A step is
+one full time step of the reference driver (predict_velocity,
+correct_pression with the Poisson solve to eps, correct_velocity and the
+driver's per-step diagnostics), with all fields resident in HBM.
+
+Each requested form is timed over the same steps:
+  fp64   plain FP64 (the drop-in; bit-identical operators to the reference)
+  mixed  FP32 stencils / AB update / SOR sweeps, FP64-refined Poisson
+  nn     mixed + learned warm start (needs a trained model file)
+The JSON line's value is the steps/s of the headline form (the last of
+--forms); every form's figures are included under "forms".
+
+roofline: the dominant kernel of the headline form is the red-black SOR
+sweep.  Its algorithmic bytes per launch are (read p, read rhs, write p) =
+3 values per cell: 24 B/cell in FP64, 12 B/cell in FP32, times the cell
+count.  Its average launch duration is measured with HIP events on the
+launch stream over complete sweep batches inside the timed steps.
+
+cpu_baseline: the reference Fortran (oracle/_ref/o3d_ref_run, built from the
+reference sources at -O3, single-threaded) on one host core: a bounded sample
+of one 256^3 step with kmax capped, whose phases are timed; the per-step
+time is extrapolated with the measured per-iteration SOR cost and the
+FP64 form's mean iteration count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import resource
+import statistics
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TGV 256³ steps/s + Poisson ms & iters to eps=1e-8; plain FP64 vs mixed+NN"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-kmax", type=int, default=10)
+    ap.add_argument("--model", default=os.path.join(ROOT, "gpurun_out", "warm_model.bin"))
+    return ap.parse_args()
+
+
+def pct(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = min(len(v) - 1, max(0, int(round(q / 100.0 * (len(v) - 1)))))
+    return v[k]
+
+
+def run_form(form, cfg, steps, warmup, model_path):
+    import torch
+    from osinco3d_amd import solver as S
+
+    prec = S.FP64 if form == "fp64" else S.MIXED
+    warm = S.WARM_LEARNED if form == "nn" else S.WARM_PREVIOUS
+    s = S.Solver(cfg, prec, warm)
+    if form == "nn":
+        from osinco3d_amd import warmstart
+        net = warmstart.load_or_none(model_path)
+        if net is None:
+            s.close()
+            return None
+        net.attach(s)
+    s.init_taylor_green()
+    for it in range(1, warmup + 1):
+        s.step(it)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = []
+    sor_ms = [0.0, 0.0]
+    sor_sw = [0, 0]
+    for k in range(1, steps + 1):
+        infos.append(s.step(warmup + k))
+        ms, sw = s.sor_timing()
+        for q in range(2):
+            sor_ms[q] += ms[q]
+            sor_sw[q] += sw[q]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    t_end = (warmup + steps) * s.dt
+    st = s.statistics(t_end)
+    s.close()
+    elapsed = t1 - t0
+    n = cfg.nx * cfg.ny * cfg.nz
+    iters = [i["sor_exit_iter"] for i in infos]
+    pms = [i["poisson_ms"] for i in infos]
+    out = {
+        "steps_per_s": steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "mcell_updates_per_s": n * steps / elapsed / 1e6,
+        "poisson_iters": {"mean": statistics.mean(iters), "p50": pct(iters, 50),
+                          "p95": pct(iters, 95)},
+        "poisson_ms": {"mean": statistics.mean(pms), "p50": pct(pms, 50), "p95": pct(pms, 95)},
+        "sweeps_f64_mean": statistics.mean(i["sweeps_f64"] for i in infos),
+        "sweeps_f32_mean": statistics.mean(i["sweeps_f32"] for i in infos),
+        "refinements_mean": statistics.mean(i["refinements"] for i in infos),
+        "sor_reasons": sorted({i["sor_reason"] for i in infos}),
+        "diverged": any(i["diverged"] for i in infos),
+        "kinetic_energy": st[1],
+        "enstrophy": st[4],
+        "t_end": t_end,
+    }
+    # SOR sweep kernel: average launch duration from events over full batches.
+    for q, name, bpc in ((0, "sor_f64", 24), (1, "sor_f32", 12)):
+        if sor_sw[q] > 0:
+            avg_ms = sor_ms[q] / sor_sw[q]
+            gbs = bpc * n / (avg_ms * 1e-3) / 1e9
+            out[name] = {"avg_launch_us": 1e3 * avg_ms, "launches_timed": sor_sw[q],
+                         "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs}
+    if form == "nn":
+        out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)
+    return out
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth: a 1 GiB device-to-device copy (read + write
+    counted), timed with events on the current stream."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(n, kmax, iters_per_step):
+    """Reference Fortran, one core, one n^3 step with kmax capped."""
+    from oracle import ref_run_path
+    from osinco3d_amd.config import tgv_config
+
+    exe = ref_run_path()
+    if exe is None:
+        return None
+    cfg = tgv_config(n, 1, 3)
+    cfg.kmax = kmax
+    with tempfile.TemporaryDirectory() as d:
+        cfg.write(os.path.join(d, "parameters.o3d"))
+
+        def unlimited_stack():
+            resource.setrlimit(resource.RLIMIT_STACK,
+                               (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+        cmd = [exe, "-"]
+        if os.path.exists("/usr/bin/taskset"):
+            cmd = ["/usr/bin/taskset", "-c", "0"] + cmd
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, cwd=d, capture_output=True, text=True,
+                           preexec_fn=unlimited_stack, timeout=600)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        return {"error": f"reference exited with {r.returncode}"}
+    ph = None
+    iters = None
+    for line in r.stdout.splitlines():
+        if line.startswith("O3D_PHASE_SECONDS"):
+            ph = [float(x) for x in line.split()[2:]]
+        m = re.match(r"\s*Exit iteration\s*:\s*(\d+)", line)
+        if m:
+            iters = int(m.group(1))
+    if ph is None or iters is None:
+        return {"error": "could not parse the reference output"}
+    predict, pression, correct, div, diag = ph
+    sweeps = min(iters, kmax)
+    t_iter = (pression - div) / sweeps
+    t_other = predict + div + correct + div + diag
+    t_step = t_other + iters_per_step * t_iter
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": 1.0 / t_step, "unit": "steps/s", "cores": 1, "kind": "reference",
+        "sample": (f"reference Fortran (amdflang -O3) on 1 core of '{cpu_model}' "
+                   f"({os.cpu_count()} visible): one {n}^3 TGV step with kmax={kmax} "
+                   f"({wall:.1f} s wall); phases predict {predict:.2f} s, correct_pression "
+                   f"{pression:.2f} s ({sweeps} sweeps, {t_iter:.3f} s/sweep), correct "
+                   f"{correct:.2f} s, diagnostics {div + diag:.2f} s; step time = "
+                   f"{t_other:.2f} s + {iters_per_step:.1f} sweeps x {t_iter:.3f} s"),
+        "seconds_per_step": t_step,
+    }
+
+
+def main():
+    args = parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}, "
+              f"WORLD_SIZE {world}); the path has no multi-GPU mode.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    from osinco3d_amd.config import tgv_config
+
+    cfg = tgv_config(args.n, 1000, 3)
+    copy_gbs = copy_bandwidth()
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    results = {}
+    for form in forms:
+        print(f"[bench] form {form}: {args.n}^3, warmup {args.warmup}, steps {args.steps}",
+              file=sys.stderr, flush=True)
+        r = run_form(form, cfg, args.steps, args.warmup, args.model)
+        if r is not None:
+            results[form] = r
+            print(f"[bench] form {form}: {r['steps_per_s']:.2f} steps/s, "
+                  f"{r['poisson_iters']['mean']:.1f} iters/step", file=sys.stderr, flush=True)
+    head_form = [f for f in forms if f in results][-1]
+    head = results[head_form]
+    ref = results.get("fp64")
+    if ref is not None:
+        for f, r in results.items():
+            r["speedup_vs_fp64"] = r["steps_per_s"] / ref["steps_per_s"]
+            r["kinetic_energy_rel_err_vs_fp64"] = abs(r["kinetic_energy"] - ref["kinetic_energy"]) / abs(ref["kinetic_energy"])
+            r["enstrophy_rel_err_vs_fp64"] = abs(r["enstrophy"] - ref["enstrophy"]) / abs(ref["enstrophy"])
+    dom = head.get("sor_f32") if head_form != "fp64" else head.get("sor_f64")
+    roofline = None
+    if dom:
+        roofline = {"bound": "hbm", "kernel": "sor_kernel (red-black sweep)",
+                    "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": None,
+                    "frac_of_measured_copy": dom["achieved_GBps"] / copy_gbs,
+                    "measured_copy_GBps": copy_gbs}
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline (reference Fortran, 1 core)", file=sys.stderr, flush=True)
+        iters = (ref or head)["poisson_iters"]["mean"]
+        cpu = cpu_baseline(args.n, args.cpu_kmax, iters)
+    line = {
+        "metric": METRIC,
+        "value": head["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head_form == "fp64" else "f32+f64",
+        "data": "synthetic (analytic Taylor-Green initial field)",
+        "config": {"workload": f"Taylor-Green vortex Re=1600, {args.n}^3 periodic, AB3, "
+                               f"CFL 0.25, SOR eps=1e-8", "grid": [args.n] * 3,
+                   "form": head_form, "parallelism": "none (1 GPU)"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": results,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

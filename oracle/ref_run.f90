@@ -12,7 +12,7 @@
 !
 ! Usage (run directory holds parameters.o3d, read by the reference's own
 ! parameters() from src/initialization.f90:114-151):
-!   o3d_ref_run <out.bin> [dump_every]
+!   o3d_ref_run <out.bin | -> [dump_every]      ('-': no field output)
 ! typesim = 2 : Taylor-Green initial field, same formula as
 !               src/initial_conditions.f90:157-162 (restated here because that
 !               module cannot be built without FFTW);
@@ -21,7 +21,9 @@
 ! Output: stream file of records [itime(int32), nx, ny, nz (int32), time,
 ! ux, uy, uz, pp (float64, column-major)]: the initial field (itime 0), then
 ! every step that is a multiple of dump_every, and the last step.
-! Per-step wall time goes to stdout as "O3D_STEP_SECONDS <itime> <seconds>".
+! Per-step wall time goes to stdout as "O3D_STEP_SECONDS <itime> <seconds>",
+! and per phase as "O3D_PHASE_SECONDS <itime> <predict> <correct_pression>
+! <correct_velocity> <one divergence> <remaining diagnostics>".
 program o3d_ref_run
   use initialization
   use integration
@@ -32,8 +34,9 @@ program o3d_ref_run
   implicit none
   character(len=256) :: outname, arg
   integer :: dump_every, i, j, k, iunit
-  integer(8) :: c0, c1, crate
-  real(kind=8) :: t_step
+  integer(8) :: c0, c1, crate, cp(6)
+  real(kind=8) :: t_step, tph(5)
+  logical :: dump
 
   call get_command_argument(1, outname)
   dump_every = 0
@@ -67,25 +70,33 @@ program o3d_ref_run
   end if
   nu_t = 0.d0
 
-  open(newunit=iunit, file=trim(outname), status='replace', access='stream', &
-       form='unformatted', action='write')
-  write(iunit) int(0, 4), int(nx, 4), int(ny, 4), int(nz, 4)
-  write(iunit) time0
-  write(iunit) ux, uy, uz, pp
+  dump = trim(outname) /= '-'
+  if (dump) then
+     open(newunit=iunit, file=trim(outname), status='replace', access='stream', &
+          form='unformatted', action='write')
+     write(iunit) int(0, 4), int(nx, 4), int(ny, 4), int(nz, 4)
+     write(iunit) time0
+     write(iunit) ux, uy, uz, pp
+  end if
   call system_clock(count_rate=crate)
   do itime = itstart, itstop
      call system_clock(c0)
+     cp(1) = c0
      time = time0 + itime * dt
      call old_values(ux, uy, uz, old_ux, old_uy, old_uz, nx, ny, nz)
      call predict_velocity(ux_pred, uy_pred, uz_pred, ux, uy, uz, &
           fux, fuy, fuz, re, adt, bdt, cdt, itime, itscheme, &
           dx, dy, dz, nx, ny, nz, iles, cs, delta, nu_t)
+     call system_clock(cp(2))
      call correct_pression(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, &
           nx, ny, nz, dt, omega, eps, kmax, idyn, multigrid)
+     call system_clock(cp(3))
      call correct_velocity(ux, uy, uz, ux_pred, uy_pred, uz_pred, &
           pp, dt, dx, dy, dz, nx, ny, nz)
+     call system_clock(cp(4))
      call divergence(divu_pred, ux_pred, uy_pred, uz_pred, &
           dx, dy, dz, nx, ny, nz, 1)
+     call system_clock(cp(5))
      divu_stats = function_stats(divu_pred, nx, ny, nz)
      call print_divu_statistics(divu_stats, .true.)
      call divergence(divu, ux, uy, uz, dx, dy, dz, nx, ny, nz, 1)
@@ -95,15 +106,22 @@ program o3d_ref_run
      call compute_cfl(cflx, cfly, cflz, ux, uy, uz, dx, dy, dz, dt)
      call print_cfl(cflx, cfly, cflz)
      call system_clock(c1)
+     cp(6) = c1
      t_step = real(c1 - c0, kind=8) / real(crate, kind=8)
+     do i = 1, 5
+        tph(i) = real(cp(i+1) - cp(i), kind=8) / real(crate, kind=8)
+     end do
      write(*, '(A17,I8,ES16.8)') 'O3D_STEP_SECONDS ', itime, t_step
-     if ((dump_every > 0 .and. modulo(itime, max(dump_every, 1)) == 0) .or. &
-          itime == itstop) then
+     ! predict, correct_pression, correct_velocity, one divergence, rest of
+     ! the diagnostics
+     write(*, '(A18,I8,5ES16.8)') 'O3D_PHASE_SECONDS ', itime, tph
+     if (dump .and. ((dump_every > 0 .and. modulo(itime, max(dump_every, 1)) == 0) .or. &
+          itime == itstop)) then
         write(iunit) int(itime, 4), int(nx, 4), int(ny, 4), int(nz, 4)
         write(iunit) time
         write(iunit) ux, uy, uz, pp
         flush(iunit)
      end if
   end do
-  close(iunit)
+  if (dump) close(iunit)
 end program o3d_ref_run

@@ -36,7 +36,10 @@ struct SorTiling {
   int ntx, nty, ntz, zc;
   int blocks() const { return ntx * nty * ntz; }
 };
-SorTiling make_sor_tiling(int nx, int ny, int nz);
+SorTiling make_sor_tiling(int nx, int ny, int nz, int slots);
+// Concurrent blocks of the sweep kernel on this device (occupancy x CUs).
+template <typename T, typename TR>
+int sor_slots();
 
 template <typename T, typename TR>
 struct SorArgs {

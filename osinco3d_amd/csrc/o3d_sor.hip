@@ -33,16 +33,52 @@ constexpr int SNT = SX * SY;
 constexpr int SNW = SNT / O3D_WAVE;
 static_assert(R2N <= 2 * SNT && R1N <= 2 * SNT, "two cells per thread at most");
 
-SorTiling make_sor_tiling(int nx, int ny, int nz) {
+// z-chunking: a block marches zc planes and re-reads 3 extra planes (red on
+// z0-1 and z1, old on z0-2 and z1+1), so long chunks save traffic, but the
+// grid should fill the machine in whole rounds of `slots` concurrent blocks
+// (a partial last round leaves CUs idle).  Pick the chunk count with the
+// least rounds * (zc + 3).
+SorTiling make_sor_tiling(int nx, int ny, int nz, int slots) {
   SorTiling t;
   t.ntx = (nx + SX - 1) / SX;
   t.nty = (ny + SY - 1) / SY;
   const int layer = t.ntx * t.nty;
-  int ntz = (2048 + layer - 1) / layer;
-  ntz = std::max(1, std::min(ntz, std::max(1, nz / 32)));
-  t.zc = (nz + ntz - 1) / ntz;
+  if (slots <= 0) slots = 1024;
+  double best = 1e30;
+  int best_ntz = 1;
+  for (int ntz = 1; ntz <= std::max(1, nz / 8); ++ntz) {
+    const int zc = (nz + ntz - 1) / ntz;
+    const int n_t = (nz + zc - 1) / zc;
+    const int blocks = layer * n_t;
+    const int rounds = (blocks + slots - 1) / slots;
+    // Time ~ rounds * (zc + 3) plane steps; normalised by the ideal
+    // nz * layer / slots.
+    const double cost = (double)rounds * (zc + 3.0) * slots / ((double)nz * layer);
+    if (cost < best - 1e-9) { best = cost; best_ntz = ntz; }
+  }
+  t.zc = (nz + best_ntz - 1) / best_ntz;
   t.ntz = (nz + t.zc - 1) / t.zc;
   return t;
+}
+
+template <typename T, typename TR>
+__global__ void sor_kernel(const SorArgs<T, TR> a);
+
+template <typename T, typename TR>
+int sor_slots() {
+  static int slots = 0;
+  if (slots == 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sor_kernel<T, TR>, SNT, 0) !=
+            hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return 1024;
+    }
+    slots = std::max(1, per_cu * cus);
+  }
+  return slots;
 }
 
 // Neighbour index for the SOR rules: periodic wrap, or the free-slip mirror
@@ -88,11 +124,34 @@ __device__ __forceinline__ bool sor_decide(const SorState *st, int it, double ep
   return false;
 }
 
+// Point update of src/poisson.f90:95-98.  The FP64 form divides by A as the
+// reference does (bit-identical results); the FP32 correction sweeps of the
+// mixed form multiply by the precomputed 1/A instead.
+template <typename T, typename TR>
+__device__ __forceinline__ T sor_gs(T w_, T e_, T s_, T n_, T b_, T t_, TR r, T ox, T oy, T oz,
+                                    T A, T invA) {
+  const T num = -ox * (w_ + e_) - oy * (s_ + n_) - oz * (b_ + t_) + T(r);
+  if constexpr (sizeof(T) == 8) return num / A;
+  else return num * invA;
+}
+
+__device__ __forceinline__ int ring3(int s) { return (s + 6) % 3; }
+
+template <int K>
+struct IC { static constexpr int value = K; };
+
+// Planes of the old iterate and rhs are prefetched PD steps ahead into
+// registers, so each block keeps PD planes of loads in flight across the
+// barriers of the march.
+constexpr int PD = 3;
+
 template <typename T, typename TR>
 __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
-  __shared__ T oldr[4][R2N];
-  __shared__ T redr[4][R1N];
-  __shared__ TR rhsr[3][R1N];
+  // LDS rings: old iterate on R2 (3 planes), new red values on R1 (3 planes),
+  // rhs on R1 (2 planes).  Plane s lives in slot ring3(s) (rhs: s & 1).
+  __shared__ T oldr[3][R2N];
+  __shared__ T redr[3][R1N];
+  __shared__ TR rhsr[2][R1N];
   __shared__ double scr[SNW];
   SorState *st = a.st;
   if (st->done) return;
@@ -127,105 +186,127 @@ __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
   const int x0 = tix * SX, y0 = tiy * SY, z0 = tiz * a.tl.zc;
   const int z1 = min(z0 + a.tl.zc, nz);
   const int tid = threadIdx.x;
-  const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A);
+  const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A), invA = T(1.0 / a.A);
   const T w = T(om);
   const T w1 = T(1.0 - om);
 
-  // Per-thread load descriptors: up to two R2 cells and two R1 cells.
-  int col2[2], col1[2];
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int c2 = tid + q * SNT;
-    const int a2 = c2 % R2W, b2 = c2 / R2W;
-    col2[q] = (c2 < R2N) ? sor_ghost(x0 - 2 + a2, nx, a.per[0]) +
-                               nx * sor_ghost(y0 - 2 + b2, ny, a.per[1])
-                         : -1;
-    const int c1 = tid + q * SNT;
-    const int a1 = c1 % R1W, b1 = c1 / R1W;
-    col1[q] = (c1 < R1N) ? sor_ghost(x0 - 1 + a1, nx, a.per[0]) +
-                               nx * sor_ghost(y0 - 1 + b1, ny, a.per[1])
-                         : -1;
-  }
-  // Black-pass cell of this thread (tile position).
+  // Global column offsets of this thread's R2 and R1 cells (the second of
+  // each exists for the first R2N-SNT / R1N-SNT threads; the others load a
+  // valid dummy address and skip the LDS store).
+  const int c2b = tid + SNT;
+  const bool v2b = c2b < R2N, v1b = c2b < R1N;
+  const int col2a = sor_ghost(x0 - 2 + tid % R2W, nx, a.per[0]) +
+                    nx * sor_ghost(y0 - 2 + tid / R2W, ny, a.per[1]);
+  const int col2b = v2b ? sor_ghost(x0 - 2 + c2b % R2W, nx, a.per[0]) +
+                              nx * sor_ghost(y0 - 2 + c2b / R2W, ny, a.per[1])
+                        : col2a;
+  const int col1a = sor_ghost(x0 - 1 + tid % R1W, nx, a.per[0]) +
+                    nx * sor_ghost(y0 - 1 + tid / R1W, ny, a.per[1]);
+  const int col1b = v1b ? sor_ghost(x0 - 1 + c2b % R1W, nx, a.per[0]) +
+                              nx * sor_ghost(y0 - 1 + c2b / R1W, ny, a.per[1])
+                        : col1a;
+  // Red cells of R1: thread tid < 330 owns the (tid % 33)-th red node of R1
+  // row tid / 33.
+  constexpr int RPR = R1W / 2;
+  const bool red_lane = tid < RPR * R1H;
+  const int rb1 = tid / RPR, rk = tid % RPR;
+  const int rpar = (x0 - 1 + y0 - 1 + rb1) & 1;
+  // Black pass: this thread's tile node.
   const int tx = tid % SX, ty = tid / SX;
   const int bi = x0 + tx, bj = y0 + ty;
   const bool bown = bi < nx && bj < ny;
   const int b1i = (ty + 1) * R1W + tx + 1;
   const int b2i = (ty + 2) * R2W + tx + 2;
+  const int bpar = (bi + bj) & 1;
 
-  auto load_old = [&](int zp) {
-    const int gz = sor_ghost(zp, nz, a.per[2]);
-    T *dst = oldr[zp & 3];
+  auto gz_of = [&](int zp) { return plane * sor_ghost(zp, nz, a.per[2]); };
+
+  T po[PD][2];
+  TR pr[PD][2];
+  {
+    const int g2 = gz_of(z0 - 2), g1 = gz_of(z0 - 1);
+    oldr[ring3(z0 - 2)][tid] = a.pin[col2a + g2];
+    oldr[ring3(z0 - 1)][tid] = a.pin[col2a + g1];
+    if (v2b) {
+      oldr[ring3(z0 - 2)][c2b] = a.pin[col2b + g2];
+      oldr[ring3(z0 - 1)][c2b] = a.pin[col2b + g1];
+    }
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
-      if (col2[q] >= 0) dst[tid + q * SNT] = a.pin[col2[q] + plane * gz];
-  };
+    for (int k = 0; k < PD; ++k) {
+      const int go = gz_of(z0 + k), gr = gz_of(z0 - 1 + k);
+      po[k][0] = a.pin[col2a + go];
+      po[k][1] = a.pin[col2b + go];
+      pr[k][0] = a.rhs[col1a + gr];
+      pr[k][1] = a.rhs[col1b + gr];
+    }
+  }
 
   double dmax = 0.0;
-  load_old(z0 - 2);
-  load_old(z0 - 1);
-  for (int s = z0 - 1; s <= z1; ++s) {
-    // Phase 1: old plane s+1 on R2, rhs plane s on R1.
-    load_old(s + 1);
+  auto step = [&](int s, auto K) {
+    constexpr int k = decltype(K)::value;
+    // Phase 1: publish old(s+1) and rhs(s); prefetch old(s+1+PD), rhs(s+PD).
     {
-      const int gz = sor_ghost(s, nz, a.per[2]);
-      TR *dst = rhsr[(s + 3) % 3];
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-        if (col1[q] >= 0) dst[tid + q * SNT] = a.rhs[col1[q] + plane * gz];
+      T *od = oldr[ring3(s + 1)];
+      TR *rd = rhsr[s & 1];
+      od[tid] = po[k][0];
+      rd[tid] = pr[k][0];
+      if (v2b) od[c2b] = po[k][1];
+      if (v1b) rd[c2b] = pr[k][1];
+      const int go = gz_of(s + 1 + PD), gr = gz_of(s + PD);
+      po[k][0] = a.pin[col2a + go];
+      po[k][1] = a.pin[col2b + go];
+      pr[k][0] = a.rhs[col1a + gr];
+      pr[k][1] = a.rhs[col1b + gr];
     }
     __syncthreads();
-    // Phase 2: red (colour 0) on plane s over R1, from old values only.
-    {
-      const T *om_ = oldr[(s + 4 - 1) & 3];
-      const T *o0 = oldr[s & 3];
-      const T *op = oldr[(s + 1) & 3];
-      const TR *rh = rhsr[(s + 3) % 3];
-      T *rd = redr[s & 3];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int c1 = tid + q * SNT;
-        if (c1 >= R1N) continue;
-        const int a1 = c1 % R1W, b1 = c1 / R1W;
-        const int X = x0 - 1 + a1, Y = y0 - 1 + b1;
-        if (((X + Y + s) & 1) != 0) continue;
-        const int i2 = (b1 + 1) * R2W + a1 + 1;
-        const T pc = o0[i2];
-        // src/poisson.f90:95-98
-        const T pn = (-ox * (o0[i2 - 1] + o0[i2 + 1]) - oy * (o0[i2 - R2W] + o0[i2 + R2W]) -
-                      oz * (om_[i2] + op[i2]) + T(rh[c1])) /
-                     A;
-        rd[c1] = w1 * pc + w * pn;
-        if (a1 >= 1 && a1 <= SX && b1 >= 1 && b1 <= SY && X < nx && Y < ny && s >= z0 && s < z1)
-          dmax = fmax(dmax, (double)fabs(pn - pc));
-      }
+    // Phase 2: red (colour 0) on plane s over R1 from old values only; keep
+    // this thread's old value and rhs of plane s-1 for the black pass.
+    if (red_lane) {
+      const int a1 = 2 * rk + ((rpar + s) & 1);
+      const int X = x0 - 1 + a1, Y = y0 - 1 + rb1;
+      const int i2 = (rb1 + 1) * R2W + a1 + 1;
+      const int i1 = rb1 * R1W + a1;
+      const T *o0 = oldr[ring3(s)];
+      const T *om_ = oldr[ring3(s - 1)];
+      const T *op = oldr[ring3(s + 1)];
+      const T pc = o0[i2];
+      const T pn = sor_gs<T, TR>(o0[i2 - 1], o0[i2 + 1], o0[i2 - R2W], o0[i2 + R2W], om_[i2],
+                                 op[i2], rhsr[s & 1][i1], ox, oy, oz, A, invA);
+      redr[ring3(s)][i1] = w1 * pc + w * pn;
+      if (a1 >= 1 && a1 <= SX && rb1 >= 1 && rb1 <= SY && X < nx && Y < ny && s >= z0 &&
+          s < z1)
+        dmax = fmax(dmax, (double)fabs(pn - pc));
     }
+    const T bold = oldr[ring3(s - 1)][b2i];
+    const TR brhs = rhsr[(s - 1) & 1][b1i];
     __syncthreads();
     // Phase 3: black (colour 1) on plane s-1 over the tile; write plane s-1.
     const int sb = s - 1;
     if (sb >= z0 && bown) {
-      const T *rm = redr[(sb + 4 - 1) & 3];
-      const T *r0 = redr[sb & 3];
-      const T *rp = redr[s & 3];
+      const T *rm = redr[ring3(sb - 1)];
+      const T *r0 = redr[ring3(sb)];
+      const T *rp = redr[ring3(s)];
       T v;
-      if (((bi + bj + sb) & 1) != 0) {
-        const T pc = oldr[sb & 3][b2i];
-        const T pn = (-ox * (r0[b1i - 1] + r0[b1i + 1]) - oy * (r0[b1i - R1W] + r0[b1i + R1W]) -
-                      oz * (rm[b1i] + rp[b1i]) + T(rhsr[(sb + 3) % 3][b1i])) /
-                     A;
-        v = w1 * pc + w * pn;
-        dmax = fmax(dmax, (double)fabs(pn - pc));
+      if (((bpar + sb) & 1) != 0) {
+        const T pn = sor_gs<T, TR>(r0[b1i - 1], r0[b1i + 1], r0[b1i - R1W], r0[b1i + R1W],
+                                   rm[b1i], rp[b1i], brhs, ox, oy, oz, A, invA);
+        v = w1 * bold + w * pn;
+        dmax = fmax(dmax, (double)fabs(pn - bold));
       } else {
         v = r0[b1i];
       }
       a.pout[bi + nx * bj + plane * sb] = v;
     }
+  };
+  static_assert(PD == 3, "the march below is unrolled for PD == 3");
+  for (int s = z0 - 1; s <= z1; s += PD) {
+    step(s, IC<0>{});
+    if (s + 1 <= z1) step(s + 1, IC<1>{});
+    if (s + 2 <= z1) step(s + 2, IC<2>{});
   }
   // Block max, one atomic per block.
   dmax = block_max<SNW>(dmax, scr);
-  if (threadIdx.x == 0 && dmax > 0.0)
-    atomicMax(&st->dmax_bits[it & 3], (unsigned long long)__double_as_longlong(dmax));
-  if (threadIdx.x == 0 && dmax != dmax)
+  if (threadIdx.x == 0 && dmax != 0.0)
     atomicMax(&st->dmax_bits[it & 3], (unsigned long long)__double_as_longlong(dmax));
 }
 
@@ -376,6 +457,9 @@ hipError_t launch_fill_f32(float *dst, float v, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
+template int sor_slots<double, double>();
+template int sor_slots<double, float>();
+template int sor_slots<float, float>();
 template hipError_t launch_sor_iteration<double, double>(const SorArgs<double, double> &,
                                                          hipStream_t);
 template hipError_t launch_sor_iteration<double, float>(const SorArgs<double, float> &,

@@ -1,0 +1,128 @@
+"""The drop-in boundary without a GPU: the C-ABI library loads and exports
+every entry point that include/o3d.h declares; host-side configuration and
+checkpoint formats round-trip."""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from osinco3d_amd import capi
+from osinco3d_amd.config import Config, mixing_layer_config, tgv_config
+from osinco3d_amd.fields_io import read_fields, write_fields
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(o3d_\w+)\s*\(", text)) - {"o3d_params", "o3d_step_info"})
+
+
+def test_library_exports_every_declared_function():
+    lib = capi.load()
+    declared = header_functions(os.path.join(ROOT, "include", "o3d.h"))
+    assert declared, "no declarations parsed"
+    missing = [f for f in declared if not hasattr(lib, f)]
+    assert not missing, missing
+    assert sorted(capi.EXPORTS) == declared
+
+
+def test_nn_header_functions_exported():
+    lib = capi.load()
+    p = os.path.join(ROOT, "include", "o3d_nn.h")
+    if not os.path.exists(p):
+        pytest.skip("no o3d_nn.h")
+    missing = [f for f in header_functions(p) if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert capi.load().o3d_version().decode().startswith("osinco3d-amd")
+
+
+def test_runtime_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    rc = capi.load().o3d_init(0)
+    assert rc < 0 and "device" in capi.last_error().lower()
+
+
+def test_struct_layouts_match_header():
+    # o3d_params: 3 ints, 3 doubles, int[3], int, 2 doubles, 3x double[3], int,
+    # 2 doubles, 2 doubles, 2 ints, 3 ints -> the ctypes mirror computes the
+    # same natural alignment as the C compiler.
+    assert ctypes.sizeof(capi.O3DParams) % 8 == 0
+    assert capi.O3DParams.adt.offset % 8 == 0
+    assert ctypes.sizeof(capi.O3DStepInfo) % 8 == 0
+
+
+def test_namelist_round_trip(tmp_path):
+    cfg = mixing_layer_config()
+    p = tmp_path / "parameters.o3d"
+    cfg.write(str(p))
+    back = Config.read(str(p))
+    assert back == cfg
+    text = p.read_text()
+    # Groups in the order the reference reads them.
+    order = [m.group(1) for m in re.finditer(r"^&(\w+)", text, re.M)]
+    assert order == ["FlowParam", "Domain", "BoundaryConditions", "AdvanceTime", "PoissonEq",
+                     "Scalar", "VisuParameters", "InitInflow", "LES"]
+
+
+def test_reads_reference_style_parameter_file():
+    text = """&FlowParam
+typesim = 2  !> comment
+re = 1600.
+/End
+&Domain
+nx = 185
+xlx = 3.141592653589793
+/End
+&PoissonEq
+eps = 1.e-4
+/End
+"""
+    cfg = Config.from_namelist(text)
+    assert cfg.nx == 185 and cfg.re == 1600.0 and cfg.eps == 1e-4 and cfg.typesim == 2
+
+
+def test_grid_constants():
+    cfg = tgv_config(64, 100, 2)
+    dx = cfg.spacing()[0]
+    assert math.isclose(dx, 2 * math.pi / 64, rel_tol=1e-15)       # exactly periodic
+    assert math.isclose(cfg.dt, 0.25 * dx, rel_tol=1e-15)
+    assert cfg.poisson_kind() == "0000"
+    assert mixing_layer_config().poisson_kind() == "0011"
+    c = Config(nbcx1=1, nbcxn=1, nbcy1=0, nbcyn=0)
+    assert c.poisson_kind() == ""                                  # no solver bound
+    with pytest.raises(ValueError):
+        Config(nbcx1=0, nbcxn=1).bc()
+
+
+def test_viscous_dt_override():
+    cfg = Config(nx=16, ny=16, nz=16, xlx=1.0, yly=1.0, zlz=1.0, re=0.01, dt=0.1)
+    dmin = 1.0 / 15
+    assert math.isclose(cfg.time_step_overrides(), 0.1 * dmin * dmin * 0.01)
+
+
+def test_fields_bin_round_trip(tmp_path):
+    cfg = Config(nx=7, ny=6, nz=5, xlx=1.0, yly=2.0, zlz=3.0, y0=-1.0)
+    rng = np.random.default_rng(0)
+    f = [rng.standard_normal((5, 6, 7)) for _ in range(5)]
+    p = tmp_path / "fields.bin"
+    write_fields(str(p), cfg, *f, time=1.25)
+    assert p.stat().st_size == 8 + 12 + 8 * (7 + 6 + 5) + 5 * 8 * 210
+    t, d = read_fields(str(p), cfg)
+    assert t == 1.25
+    for a, k in zip(f, ("ux", "uy", "uz", "pp", "phi")):
+        assert np.array_equal(a, d[k])
+    assert np.allclose(d["y"], -1.0 + np.arange(6) * 0.4)
+    with pytest.raises(ValueError):
+        read_fields(str(p), Config(nx=8, ny=6, nz=5))

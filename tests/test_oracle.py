@@ -1,0 +1,302 @@
+"""Pinning the oracle (CPU).
+
+The C restatement is checked against
+  * the golden fixtures written from the reference's own code
+    (tests/golden/gen_golden.py): bit-identical;
+  * the reference itself (oracle/_ref), when built: bit-identical on fresh
+    seeded inputs;
+  * the published t = 0 statistics row of the reference's Taylor-Green run
+    (examples/tgv_re1600_dns/tgv_stats_re1600_dns.dat:18);
+  * closed forms: modified wavenumbers of every stencil variant at every node
+    (boundaries included), telescoping and constant-field invariants, the
+    discrete Laplacian's eigenfunctions for SOR, a dense solve on a tiny grid.
+Each check is chosen to fail on a plausible slip: a wrong coefficient, a
+dropped term, a wrong ghost sign or mirror offset, a wrong period, a
+swapped parity, a wrong neighbour rule.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from osinco3d_amd.config import Config, tgv_config
+from osinco3d_amd.solver import taylor_green
+from tests.helpers import OracleStepper, rand_fields
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+# ---------------------------------------------------------------- fixtures
+def test_derivatives_match_reference_fixture(orc):
+    g = gold("derivatives.npz")
+    f, h = g["f"], float(g["h"])
+    for axis in range(3):
+        for order in (1, 2):
+            for kind in range(3):
+                exp = g[f"d_{axis}_{order}_{kind}"]
+                got = orc.der1(f, axis, kind, h) if order == 1 else orc.der2(f, axis, kind, h)
+                assert np.array_equal(got, exp), (axis, order, kind)
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
+def test_operators_match_reference_fixture(orc, bc):
+    g = gold("operators.npz")
+    tag = "".join(map(str, bc))
+    d = tuple(g["d"])
+    u = [g[f"u{c}"] for c in range(3)]
+    assert np.array_equal(orc.divergence(*u, d, bc, 1), g[f"div1_{tag}"])
+    assert np.array_equal(orc.divergence(*u, d, bc, 0), g[f"div0_{tag}"])
+    assert np.array_equal(orc.nu_t(*u, d, 0.17, 0.2, bc), g[f"nut_{tag}"])
+    a, b, c = list(g["adt"]), list(g["bdt"]), list(g["cdt"])
+    for its in (1, 2, 3):
+        for iles in (0, 1):
+            F = [np.ascontiguousarray(np.random.default_rng(220 + q).standard_normal((3, 8, 8, 8)))
+                 for q in range(3)]
+            nut = np.zeros((8, 8, 8))
+            for itime in (1, 2, 3):
+                up = orc.predict(*u, *F, 1600.0, a, b, c, itime, its, d, iles, 0.17, 0.2, nut, bc)
+                key = f"pred_{tag}_{its}_{iles}_{itime}"
+                for q in range(3):
+                    assert np.array_equal(up[q], g[f"{key}_up{q}"]), key
+                    if itime == 3:
+                        assert np.array_equal(F[q], g[f"{key}_F{q}"]), key
+                assert np.array_equal(nut, g[f"{key}_nut"])
+    v = [np.empty_like(u[0]) for _ in range(3)]
+    orc.correct_velocity(*v, *u, g["p"], 0.02, d, bc)
+    for q in range(3):
+        assert np.array_equal(v[q], g[f"corr_{tag}_u{q}"])
+    assert np.array_equal(orc.function_stats(u[0]), g["fstats"])
+
+
+def test_ab_coefficients_closed_form():
+    g = gold("operators.npz")
+    dt = 0.01
+    cfg = Config(dt=dt)
+    a, b, c = cfg.ab_coefficients()
+    assert np.array_equal(a, g["adt"]) and np.array_equal(b, g["bdt"]) and np.array_equal(c, g["cdt"])
+    # AB2 and AB3 weights sum to dt (consistency), Euler is (dt, 0, 0).
+    assert math.isclose(a[1] + b[1] + c[1], dt, rel_tol=1e-15)
+    assert math.isclose(a[2] + b[2] + c[2], dt, rel_tol=1e-15)
+
+
+@pytest.mark.parametrize("solver,bc", [("0000", (0, 0, 0)), ("0011", (0, 1, 0)),
+                                       ("111111", (1, 1, 1))])
+def test_sor_matches_reference_fixture(orc, solver, bc):
+    g = gold("sor.npz")
+    d = tuple(g["d"])
+    rhs, p0 = g[f"{solver}_rhs"], g[f"{solver}_p0"]
+    for kmax in (1, 7):
+        p = p0.copy()
+        orc.sor(p, rhs, d, solver, 1.6, 1e-30, kmax, 0)
+        assert np.array_equal(p, g[f"{solver}_k{kmax}"])
+    p = p0.copy()
+    orc.sor(p, rhs, d, solver, 1.6, 1e-9, 5000, 0)
+    assert np.array_equal(p, g[f"{solver}_conv"])
+
+
+def _cfg_from(g):
+    return Config.from_namelist(bytes(g["params"]).decode())
+
+
+@pytest.mark.parametrize("name", ["run_tgv16_ab2.npz", "run_tgv16fs_ab3_les.npz",
+                                  "run_mixing24_ab2_les.npz"])
+def test_time_loop_matches_reference_fixture(orc, name):
+    """The oracle's lexicographic time loop reproduces whole reference runs
+    bit for bit, SOR exit iterations included."""
+    g = gold(name)
+    cfg = _cfg_from(g)
+    st = OracleStepper(orc, cfg, red_black=False)
+    u = [g["ux0"].copy(), g["uy0"].copy(), g["uz0"].copy()]
+    p = g["pp0"].copy()
+    iters = []
+    for itime in range(1, cfg.itstop + 1):
+        u, p, _, info = st.step(itime, *u, p)
+        iters.append(info["exit_iter"])
+        if f"ux{itime}" in g:
+            for c, k in enumerate(("ux", "uy", "uz")):
+                assert np.array_equal(u[c], g[f"{k}{itime}"]), (itime, k)
+            assert np.array_equal(p, g[f"pp{itime}"])
+    assert iters == list(g["iters"])
+
+
+# ---------------------------------------------------- the published t = 0 row
+# examples/tgv_re1600_dns/tgv_stats_re1600_dns.dat:18, columns t, E_k, eps,
+# eps2, enstrophy, <u^2>, <v^2>, <w^2>, then the nine mean square gradients
+# (185^3 nodes on [0, pi]^3, free-slip on every face, Re = 1600).
+PUBLISHED_T0 = [0.0, 1.256720036326e-01, 4.729958738868e-04, 4.712700131764e-04,
+                3.729620752964e-01, 1.256720036326e-01, 1.256720036325e-01, 0.0,
+                1.270380036720e-01, 1.243206917656e-01, 1.243206917656e-01,
+                1.243206917656e-01, 1.270380036720e-01, 1.243206917655e-01, 0.0, 0.0, 0.0]
+
+
+def test_statistics_match_published_row(orc):
+    n = 185
+    cfg = Config(nx=n, ny=n, nz=n, xlx=math.pi, yly=math.pi, zlz=math.pi, re=1600.0,
+                 nbcx1=1, nbcxn=1, nbcy1=1, nbcyn=1, nbcz1=1, nbczn=1)
+    ux, uy, uz, _ = taylor_green(cfg)
+    out = orc.statistics(ux, uy, uz, cfg.spacing(), 1600.0, 0.0, (1, 1, 1))
+    for k, (got, pub) in enumerate(zip(out, PUBLISHED_T0)):
+        if pub == 0.0:
+            assert abs(got) < 1e-13, k
+        else:
+            # 13 significant digits are printed (es21.12).
+            assert abs(got - pub) <= 1.5e-12 * abs(pub), (k, got, pub)
+
+
+# ------------------------------------------------------------ closed forms
+def _mw1(kh, h):
+    return (45 * math.sin(kh) - 9 * math.sin(2 * kh) + math.sin(3 * kh)) / (30 * h)
+
+
+def _mw2(kh, h):
+    return (30 - 32 * math.cos(kh) + 2 * math.cos(2 * kh)) / (12 * h * h)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_periodic_modified_wavenumber(orc, axis):
+    n, k = 24, 3
+    h = 2 * math.pi / n          # exactly periodic sampling: period n nodes
+    x = np.arange(n) * h
+    shape = [5, 6, 7]
+    shape[2 - axis] = n
+    bshape = [1, 1, 1]
+    bshape[2 - axis] = n
+    f = np.ascontiguousarray(np.broadcast_to(np.sin(k * x).reshape(bshape), shape))
+    d1 = orc.der1(f, axis, 0, h)
+    d2 = orc.der2(f, axis, 0, h)
+    e1 = _mw1(k * h, h) * np.cos(k * x).reshape(bshape)
+    e2 = -_mw2(k * h, h) * np.sin(k * x).reshape(bshape)
+    assert np.abs(d1 - e1).max() < 1e-12
+    assert np.abs(d2 - e2).max() < 1e-11
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_free_slip_modified_wavenumber(orc, axis):
+    """On [0, pi] with the node-inclusive grid, sin(kx) is odd and cos(kx)
+    even about both end nodes, so the "i" and "p" variants are exact at every
+    node, boundary rows included."""
+    n, k = 19, 2
+    h = math.pi / (n - 1)
+    x = np.arange(n) * h
+    shape = [5, 6, 7]
+    shape[2 - axis] = n
+    bshape = [1, 1, 1]
+    bshape[2 - axis] = n
+    s = np.ascontiguousarray(np.broadcast_to(np.sin(k * x).reshape(bshape), shape))
+    c = np.ascontiguousarray(np.broadcast_to(np.cos(k * x).reshape(bshape), shape))
+    k1, k2 = _mw1(k * h, h), _mw2(k * h, h)
+    assert np.abs(orc.der1(s, axis, 2, h) - k1 * np.cos(k * x).reshape(bshape)).max() < 1e-12
+    assert np.abs(orc.der1(c, axis, 1, h) + k1 * np.sin(k * x).reshape(bshape)).max() < 1e-12
+    assert np.abs(orc.der2(s, axis, 2, h) + k2 * np.sin(k * x).reshape(bshape)).max() < 1e-11
+    assert np.abs(orc.der2(c, axis, 1, h) + k2 * np.cos(k * x).reshape(bshape)).max() < 1e-11
+
+
+def test_stencil_invariants(orc):
+    f, = rand_fields((10, 12, 14), 1, seed=31)
+    for axis in range(3):
+        # Telescoping: the periodic first derivative sums to zero.
+        assert abs(orc.der1(f, axis, 0, 0.1).sum()) < 1e-9
+        for kind in range(3):
+            one = np.ones_like(f)
+            assert np.abs(orc.der1(one, axis, kind, 0.1))[...].max() == 0.0 or kind == 2
+            assert np.abs(orc.der2(one, axis, 1, 0.1)).max() < 1e-12
+    # Second derivative of x^2 is 2 away from the boundary rows.
+    n = 16
+    h = 0.25
+    x = (np.arange(n) * h) ** 2
+    f = np.ascontiguousarray(np.broadcast_to(x, (6, 7, n)))
+    d2 = orc.der2(f, 0, 1, h)
+    assert np.abs(d2[:, :, 2:-2] - 2.0).max() < 1e-11
+
+
+@pytest.mark.parametrize("solver", ["0000", "0011", "111111"])
+def test_sor_eigenfunction(orc, solver):
+    """rhs = L q for a discrete eigenfunction q: SOR returns q up to the
+    null-space constant, lexicographic and red-black alike."""
+    nx, ny, nz = 16, 12, 10
+    d = (0.3, 0.25, 0.35)
+    i, j, k = np.arange(nx), np.arange(ny), np.arange(nz)
+    if solver == "0000":
+        q = (np.cos(2 * math.pi * 2 * k / nz)[:, None, None] *
+             np.cos(2 * math.pi * j / ny)[None, :, None] * np.sin(2 * math.pi * i / nx)[None, None, :])
+        lam = sum((2 - 2 * math.cos(2 * math.pi * m / n)) / h ** 2
+                  for m, n, h in ((1, nx, d[0]), (1, ny, d[1]), (2, nz, d[2])))
+    elif solver == "0011":
+        q = (np.cos(2 * math.pi * k / nz)[:, None, None] *
+             np.cos(math.pi * j / (ny - 1))[None, :, None] * np.cos(2 * math.pi * i / nx)[None, None, :])
+        lam = ((2 - 2 * math.cos(2 * math.pi / nx)) / d[0] ** 2 +
+               (2 - 2 * math.cos(math.pi / (ny - 1))) / d[1] ** 2 +
+               (2 - 2 * math.cos(2 * math.pi / nz)) / d[2] ** 2)
+    else:
+        q = (np.cos(math.pi * k / (nz - 1))[:, None, None] *
+             np.cos(math.pi * j / (ny - 1))[None, :, None] * np.cos(math.pi * 2 * i / (nx - 1))[None, None, :])
+        lam = ((2 - 2 * math.cos(2 * math.pi / (nx - 1))) / d[0] ** 2 +
+               (2 - 2 * math.cos(math.pi / (ny - 1))) / d[1] ** 2 +
+               (2 - 2 * math.cos(math.pi / (nz - 1))) / d[2] ** 2)
+    q = np.ascontiguousarray(q)
+    rhs = np.ascontiguousarray(-lam * q)
+    # The residual restatement agrees with the closed-form eigenvalue.
+    assert np.abs(orc.residual(q, rhs, d, solver)).max() < 1e-10
+    for rb in (False, True):
+        p = np.zeros_like(q)
+        orc.sor(p, rhs, d, solver, 1.8, 1e-13, 20000, 0, red_black=rb)
+        assert np.abs((p - p.mean()) - (q - q.mean())).max() < 1e-9
+
+
+def test_sor_dense_solve_tiny_grid(orc):
+    """Brute force on a 6^3 periodic grid: the SOR answer equals the
+    minimum-norm solution of the dense singular system plus a constant."""
+    n = 6
+    d = (0.5, 0.4, 0.3)
+    N = n ** 3
+    L = np.zeros((N, N))
+    idx = lambda i, j, k: i + n * (j + n * k)
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                r = idx(i, j, k)
+                for (di, dj, dk), h in (((1, 0, 0), d[0]), ((0, 1, 0), d[1]), ((0, 0, 1), d[2])):
+                    for s in (-1, 1):
+                        L[r, idx((i + s * di) % n, (j + s * dj) % n, (k + s * dk) % n)] += 1 / h ** 2
+                    L[r, r] -= 2 / h ** 2
+    rhs = np.random.default_rng(5).standard_normal(N)
+    rhs -= rhs.mean()
+    x = np.linalg.lstsq(L, rhs, rcond=None)[0]
+    p = np.zeros((n, n, n))
+    orc.sor(p, np.ascontiguousarray(rhs.reshape(n, n, n)), d, "0000", 1.5, 1e-14, 50000, 0,
+            red_black=True)
+    pv = p.reshape(-1)
+    assert np.abs((pv - pv.mean()) - (x - x.mean())).max() < 1e-10
+
+
+def test_function_stats_brute_force(orc):
+    f = np.random.default_rng(6).integers(0, 4, size=(5, 6, 7)).astype(np.float64)
+    st = orc.function_stats(f)
+    flat = f.reshape(-1)           # column-major (i fastest) order
+    first = int(np.argmax(flat))   # numpy's argmax returns the first maximum
+    i, j, k = first % 7, (first // 7) % 6, first // 42
+    assert list(st) == [f.min(), f.max(), flat.sum() / flat.size, i + 1, j + 1, k + 1]
+
+
+# ------------------------------------------------- live reference (if built)
+@pytest.mark.parametrize("bc", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
+def test_oracle_equals_reference_live(orc, ref, bc):
+    ref.set_bc(bc)
+    sh = (9, 10, 12)
+    u = rand_fields(sh, 3, seed=41)
+    d = (0.12, 0.14, 0.16)
+    for axis in range(3):
+        for kind in range(3):
+            ref.set_bc([1 if kind else 0] * 3)
+            assert np.array_equal(orc.der1(u[0], axis, kind, 0.1), ref.der(u[0], axis, 1, kind, 0.1))
+            assert np.array_equal(orc.der2(u[0], axis, kind, 0.1), ref.der(u[0], axis, 2, kind, 0.1))
+    ref.set_bc(bc)
+    assert np.array_equal(orc.divergence(*u, d, bc, 1), ref.divergence(*u, d, 1))
+    assert np.array_equal(orc.nu_t(*u, d, 0.2, 0.3, bc), ref.nu_t(*u, d, 0.2, 0.3))
