@@ -29,8 +29,11 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["o3d_stencil.hip", "o3d_sor.hip", "o3d_nn.hip", "o3d_api.hip"]
-FORTRAN_SOURCES = ["o3d_capi.f90", "derivation.f90", "poisson.f90", "differential_operators.f90",
-                   "les_turbulence.f90", "integration.f90"]
+# Modules that compile on their own.  differential_operators, les_turbulence
+# and integration use the reference's initialization module (as the
+# reference's own versions do); they are built together with it by
+# `make -C oracle dropin`.
+FORTRAN_SOURCES = ["o3d_capi.f90", "derivation.f90", "poisson.f90"]
 
 
 def _run(cmd, verbose):
@@ -86,6 +89,8 @@ def build_oracle(verbose: bool = False, reference: str = "/root/reference") -> N
     _run(["make", "-s", "-C", odir, "oracle"], verbose)
     if os.path.isdir(os.path.join(reference, "src")) and shutil.which("make"):
         _run(["make", "-s", "-C", odir, "ref", f"REF={reference}"], verbose)
+        if os.path.exists(LIB):
+            _run(["make", "-s", "-C", odir, "dropin", f"REF={reference}"], verbose)
 
 
 def build_all(verbose: bool = False) -> None:

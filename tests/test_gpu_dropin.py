@@ -1,0 +1,67 @@
+"""The Fortran drop-in on the GPU: the reference's own initialization,
+functions, IOfunctions and utils modules (built from its sources, with
+schemes() binding its procedure pointers to this package's derivation and
+poisson modules) and the reference time loop, with predict_velocity,
+correct_pression and correct_velocity served by libo3d.so.  Compared with
+the reference's own runs (golden fixtures)."""
+from __future__ import annotations
+
+import os
+import re
+import resource
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle.oracle import read_ref_run
+from osinco3d_amd.config import Config
+from osinco3d_amd.fields_io import write_fields
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "oracle", "_ref", "o3d_dropin_run")
+
+
+def _run(cfg, dump_every, fields=None):
+    with tempfile.TemporaryDirectory() as d:
+        cfg.write(os.path.join(d, "parameters.o3d"))
+        if fields is not None:
+            write_fields(os.path.join(d, "fields.bin"), cfg, *fields)
+
+        def unlimited():
+            resource.setrlimit(resource.RLIMIT_STACK,
+                               (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+        r = subprocess.run([EXE, "out.bin", str(dump_every)], cwd=d, capture_output=True,
+                           text=True, preexec_fn=unlimited, timeout=120)
+        assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+        recs = read_ref_run(os.path.join(d, "out.bin"))
+    iters = [int(m.group(1)) for m in re.finditer(r"Exit iteration\s*:\s*(\d+)", r.stdout)]
+    return recs, iters, r.stdout
+
+
+@pytest.mark.parametrize("name,dump", [("run_tgv16_ab2.npz", 10), ("run_tgv16fs_ab3_les.npz", 4),
+                                       ("run_mixing24_ab2_les.npz", 5)])
+def test_dropin_matches_reference_run(name, dump):
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/o3d_dropin_run not built (needs the reference tree)")
+    g = np.load(os.path.join(ROOT, "tests", "golden", name), allow_pickle=False)
+    cfg = Config.from_namelist(bytes(g["params"]).decode())
+    fields = None
+    if cfg.typesim == 0:
+        fields = (g["ux0"], g["uy0"], g["uz0"], g["pp0"])
+    print(f"[gpu] drop-in {name}", flush=True)
+    recs, iters, out = _run(cfg, dump, fields)
+    assert "Predict velocity" in out and "Exit iteration" in out
+    stagnated = "No converged" in out
+    tol_u, tol_p = (2e-3, 2e-2) if stagnated else (2e-6, 2e-5)
+    assert len(iters) == cfg.itstop
+    for itime, t, ux, uy, uz, pp in recs:
+        if itime == 0:
+            continue
+        for a, k in ((ux, "ux"), (uy, "uy"), (uz, "uz")):
+            assert np.abs(a - g[f"{k}{itime}"]).max() <= tol_u, (itime, k)
+        pr = g[f"pp{itime}"]
+        assert np.abs((pp - pp.mean()) - (pr - pr.mean())).max() <= tol_p
