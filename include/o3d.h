@@ -179,12 +179,19 @@ void *o3d_solver_field(void *handle, int which);
  * last step: total ms of complete sweep batches and the number of sweeps in
  * them, per precision (index 0 FP64, 1 FP32). */
 int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2);
-/* Attach the learned warm start (an o3d_nn handle, see o3d_nn.h). */
+/* Attach the learned warm start (an o3d_nn network, include/o3d_nn.h), or
+ * NULL to detach.  Each step then forms, before the Poisson solve, the
+ * residual r = rhs - L p_prev of the previous pressure and the previous
+ * step's increment e_prev = p_prev - p_prev2, feeds the network the two
+ * planes (r dx^2 and e_prev, each divided by its RMS), and offers
+ * p_prev + RMS(e_prev) * output as the initial guess; the previous pressure
+ * stays the initial guess unless the guess has the smaller L2 residual. */
 int o3d_solver_set_warm_model(void *handle, void *nn);
-/* Keep a copy of each step's (rhs, pressure) pair for training: when
- * enabled, the pair is written to the given device buffers (float32, n cells
- * each) at slot `slot`. */
-int o3d_solver_capture(void *handle, float *rhs_out, float *p_out);
+/* Training data capture: when set, every step writes float32 planes of n
+ * cells each: r_out = rhs - L p_prev (before the solve), eprev_out = the
+ * previous step's pressure increment, e_out = this step's increment
+ * p_new - p_prev.  NULL pointers disable the capture. */
+int o3d_solver_capture(void *handle, float *r_out, float *eprev_out, float *e_out);
 
 #ifdef __cplusplus
 }

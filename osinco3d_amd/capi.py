@@ -78,6 +78,10 @@ EXPORTS = [
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
 ]
+NN_EXPORTS = ["o3d_nn_create", "o3d_nn_destroy", "o3d_nn_resize", "o3d_nn_num_params",
+              "o3d_nn_get_params", "o3d_nn_set_params", "o3d_nn_get_grads", "o3d_nn_zero_grad",
+              "o3d_nn_forward", "o3d_nn_backward", "o3d_nn_mse", "o3d_nn_adam_step",
+              "o3d_nn_last_timing"]
 
 _lib = None
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/o3d.h"
+#include "../../include/o3d_nn.h"
 #include "o3d_nn.h"
 #include "o3d_sor.h"
 #include "o3d_stencil.h"
@@ -688,7 +689,17 @@ struct Solver {
   double sor_ms[2] = {0, 0};
   int sor_sweeps[2] = {0, 0};
   void *nn = nullptr;          // learned warm start
-  float *cap_rhs = nullptr, *cap_p = nullptr;
+  float *cap_r = nullptr, *cap_eprev = nullptr, *cap_e = nullptr;
+  // Learned warm start / capture state.
+  double *Pstart = nullptr;    // pressure at the start of the solve (p_prev)
+  float *Eprev = nullptr;      // previous step's increment
+  float *Rl = nullptr;         // residual of p_prev (float32)
+  float *Feat = nullptr;       // network input, 2 planes
+  float *Out = nullptr;        // network output
+  double *npart = nullptr;     // reduction partials
+  double *hnpart = nullptr;    // pinned copy
+  int hist = 0;                // steps with an increment available
+  double nn_ms = 0.0, nn_flops = 0.0;
   std::vector<void *> allocs;
 
   size_t esz() const { return mixed ? 4 : 8; }
@@ -714,6 +725,7 @@ struct Solver {
     for (auto e : ev) (void)hipEventDestroy(e);
     if (hfin) (void)hipHostFree(hfin);
     if (hst) (void)hipHostFree(hst);
+    if (hnpart) (void)hipHostFree(hnpart);
   }
 };
 
@@ -898,6 +910,148 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   return 0;
 }
 
+// ---- learned warm start and training capture ------------------------------
+__global__ void lw_features_kernel(const float *r, const float *e, float *f, int n, float sr,
+                                   float se) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  f[i] = r[i] * sr;
+  f[n + i] = e[i] * se;
+}
+__global__ void lw_guess_kernel(const double *p, const float *o, double *q, int n, double s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = p[i] + s * (double)o[i];
+}
+__global__ void lw_increment_kernel(const double *p, const double *p0, float *e, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] = (float)(p[i] - p0[i]);
+}
+__global__ void __launch_bounds__(256) lw_sumsq_kernel(const float *a, int n, double *part) {
+  __shared__ double sc[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += (double)a[i] * a[i];
+  s = block_sum<4>(s, sc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+constexpr int kLwBlocks = 512;
+
+int lw_alloc(Solver &S) {
+  if (S.Pstart) return 0;
+  const size_t n = S.n;
+  CHK(S.alloc(&S.Pstart, n));
+  CHK(S.alloc(&S.Eprev, n));
+  CHK(S.alloc(&S.Rl, n));
+  CHK(S.alloc(&S.Feat, 2 * n));
+  CHK(S.alloc(&S.Out, n));
+  CHK(S.alloc(&S.npart, 2 * kLwBlocks));
+  HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
+  return 0;
+}
+
+int lw_sumsq(Solver &S, const float *a, double *out) {
+  lw_sumsq_kernel<<<kLwBlocks, 256, 0, g_stream>>>(a, S.n, S.npart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(S.hnpart, S.npart, kLwBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                        g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  double s = 0.0;
+  for (int b = 0; b < kLwBlocks; ++b) s += S.hnpart[b];
+  *out = s;
+  return 0;
+}
+
+// L2 norm of the FP64 residual rhs - L p.
+int lw_residual_l2(Solver &S, const double *p, double *out) {
+  int per[3];
+  solver_periodic(S.kind, per);
+  if (S.mixed)
+    HIPCHK(launch_residual_norm<float>(p, (const float *)S.rhs, S.npart, kLwBlocks, S.p.nx,
+                                       S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                       g_stream));
+  else
+    HIPCHK(launch_residual_norm<double>(p, (const double *)S.rhs, S.npart, kLwBlocks, S.p.nx,
+                                        S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                        g_stream));
+  HIPCHK(hipMemcpyAsync(S.hnpart, S.npart, 2 * kLwBlocks * sizeof(double),
+                        hipMemcpyDeviceToHost, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  double s = 0.0;
+  for (int b = 0; b < kLwBlocks; ++b) s += S.hnpart[2 * b + 1];
+  *out = sqrt(s);
+  return 0;
+}
+
+// Before the solve: keep p_prev, form r = rhs - L p_prev (float32).
+int solver_learning_prepare(Solver &S) {
+  CHK(lw_alloc(S));
+  const int n = S.n;
+  HIPCHK(hipMemcpyAsync(S.Pstart, S.P[S.cur], (size_t)n * 8, hipMemcpyDeviceToDevice,
+                        g_stream));
+  int per[3];
+  solver_periodic(S.kind, per);
+  // The residual kernel also zeroes its second output; Out is scratch here.
+  if (S.mixed)
+    HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.Rl, S.Out, S.p.nx,
+                                      S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                      g_stream));
+  else
+    HIPCHK(launch_residual_f32<double>(S.P[S.cur], (const double *)S.rhs, S.Rl, S.Out, S.p.nx,
+                                       S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                       g_stream));
+  if (S.cap_r)
+    HIPCHK(hipMemcpyAsync(S.cap_r, S.Rl, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));
+  if (S.cap_eprev)
+    HIPCHK(hipMemcpyAsync(S.cap_eprev, S.Eprev, (size_t)n * 4, hipMemcpyDeviceToDevice,
+                          g_stream));
+  return 0;
+}
+
+// Network guess p_prev + s_e * NN(r dx^2 / rms, e_prev / rms), accepted when
+// its L2 residual is below that of p_prev.
+int solver_warm_learned(Solver &S, int *used, double *warm) {
+  *used = 0;
+  warm[0] = warm[1] = 0.0;
+  if (S.hist < 1) return 0;
+  const int n = S.n;
+  double sr2 = 0.0, se2 = 0.0;
+  CHK(lw_sumsq(S, S.Rl, &sr2));
+  CHK(lw_sumsq(S, S.Eprev, &se2));
+  const double h2 = S.p.dx * S.p.dx;
+  const double rr = sqrt(sr2 / n) * h2, re = sqrt(se2 / n);
+  if (!(rr > 0.0) || !(re > 0.0)) return 0;
+  lw_features_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.Eprev, S.Feat, n,
+                                                            (float)(h2 / rr), (float)(1.0 / re));
+  HIPCHK(hipGetLastError());
+  CHK(o3d_nn_forward(S.nn, S.Feat, S.Out));
+  double fl = 0.0, ms = 0.0;
+  CHK(o3d_nn_last_timing(S.nn, &ms, &fl));
+  S.nn_ms += ms;
+  S.nn_flops += fl;
+  double *q = S.P[1 - S.cur];
+  lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Out, q, n, re);
+  HIPCHK(hipGetLastError());
+  CHK(lw_residual_l2(S, S.P[S.cur], &warm[0]));
+  CHK(lw_residual_l2(S, q, &warm[1]));
+  if (warm[1] < warm[0]) {
+    S.cur = 1 - S.cur;
+    *used = 1;
+  }
+  return 0;
+}
+
+// After the solve: e = p_new - p_prev becomes the next step's e_prev.
+int solver_learning_finish(Solver &S) {
+  const int n = S.n;
+  lw_increment_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n);
+  HIPCHK(hipGetLastError());
+  if (S.cap_e)
+    HIPCHK(hipMemcpyAsync(S.cap_e, S.Eprev, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));
+  S.hist++;
+  return 0;
+}
+
 template <typename T>
 int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   const o3d_params &p = S.p;
@@ -907,15 +1061,16 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   HIPCHK(launch_finalize_partials(S.part_div, S.tl.blocks(), 1, 0, S.fin + 0, g_stream));
   if (p.iles == 1)
     HIPCHK(launch_finalize_partials(S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5, g_stream));
-  if (S.cap_rhs) HIPCHK((launch_convert<float, T>(S.cap_rhs, (const T *)S.rhs, S.n, g_stream)));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
+  const bool learning = S.nn || S.cap_r || S.cap_eprev || S.cap_e;
   int warm = 0;
-  if (S.nn) CHK(o3d_nn_warm_start(S.nn, &S, itime, &warm, info->warm_dmax));
+  if (learning) CHK(solver_learning_prepare(S));
+  if (S.nn) CHK(solver_warm_learned(S, &warm, info->warm_dmax));
   if (S.mixed) CHK(solver_poisson_mixed(S, info));
   else CHK(solver_poisson_f64(S, info));
   info->warm_used = warm;
   HIPCHK(hipEventRecord(S.ev[5], g_stream));
-  if (S.cap_p) HIPCHK((launch_convert<float, double>(S.cap_p, S.P[S.cur], S.n, g_stream)));
+  if (learning) CHK(solver_learning_finish(S));
   CHK(solver_correct<T>(S));
   for (int c = 0; c < 3; ++c)
     HIPCHK(launch_finalize_partials(S.part_cor, S.tl.blocks(), 3, c, S.fin + 2 + c, g_stream));
@@ -1138,14 +1293,18 @@ extern "C" int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2) {
 extern "C" int o3d_solver_set_warm_model(void *handle, void *nn) {
   if (!handle) return fail("NULL handle");
   Solver &S = *(Solver *)handle;
+  if (nn) {
+    CHK(o3d_nn_resize(nn, S.p.nx, S.p.ny, S.p.nz));
+  }
   S.nn = nn;
   return 0;
 }
 
-extern "C" int o3d_solver_capture(void *handle, float *rhs_out, float *p_out) {
+extern "C" int o3d_solver_capture(void *handle, float *r_out, float *eprev_out, float *e_out) {
   if (!handle) return fail("NULL handle");
   Solver &S = *(Solver *)handle;
-  S.cap_rhs = rhs_out;
-  S.cap_p = p_out;
+  S.cap_r = r_out;
+  S.cap_eprev = eprev_out;
+  S.cap_e = e_out;
   return 0;
 }

@@ -18,8 +18,3 @@ hipStream_t current_stream();
 int set_error(const char *msg);
 }  // namespace o3d
 
-// Replace the solver's initial pressure by the learned guess when its first
-// red-black sweep moves the field less than the previous pressure's does.
-// used = 1 when accepted; warm_dmax[0] / [1] = first-sweep dmax of the
-// previous pressure / of the learned guess.
-extern "C" int o3d_nn_warm_start(void *nn, void *solver, int itime, int *used, double *warm_dmax);

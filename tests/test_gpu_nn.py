@@ -1,0 +1,145 @@
+"""The warm-start network on the GPU against its oracle, training, and the
+warm start inside the solver."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as N
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _params(seed=0):
+    L, head, n = N.layout()
+    rng = np.random.default_rng(seed)
+    P = np.zeros(n)
+    for li, (ci, co, _, wo, bo) in enumerate(L):
+        w = rng.uniform(-1, 1, (co, 27, ci)) * np.sqrt(6 / (27 * ci)) * 0.8
+        if li == 0:
+            w[:, :, 2:] = 0.0
+        P[wo:wo + co * 27 * ci] = w.reshape(-1)
+        P[bo:bo + co] = rng.standard_normal(co) * 0.05
+    P[head:head + 17] = rng.standard_normal(17) * 0.3
+    return P.astype(np.float32)
+
+
+def _net(torch, shape, P):
+    from osinco3d_amd.warmstart import Network
+    nz, ny, nx = shape
+    net = Network(nx, ny, nz)
+    net.set_params(P)
+    return net
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32)])
+def test_forward_matches_oracle(torch, shape):
+    print(f"[gpu] nn forward {shape}", flush=True)
+    P = _params()
+    rng = np.random.default_rng(3)
+    inp = rng.standard_normal((2,) + shape).astype(np.float32)
+    net = _net(torch, shape, P)
+    x = torch.from_numpy(inp).cuda().reshape(2, -1).contiguous()
+    out = torch.empty(x.shape[1], device="cuda")
+    net.forward(x, out)
+    got = out.cpu().numpy().reshape(shape)
+    exp = N.Net(P.astype(np.float64)).forward(inp.astype(np.float64))
+    err = np.abs(got - exp).max() / np.abs(exp).max()
+    print(f"[gpu] forward rel err {err:.2e}", flush=True)
+    assert err < 2e-2
+    assert np.corrcoef(got.ravel(), exp.ravel())[0, 1] > 0.9995
+
+
+def test_backward_matches_oracle(torch):
+    shape = (16, 16, 16)
+    P = _params(1)
+    rng = np.random.default_rng(4)
+    inp = rng.standard_normal((2,) + shape).astype(np.float32)
+    dout = rng.standard_normal(shape).astype(np.float32)
+    net = _net(torch, shape, P)
+    x = torch.from_numpy(inp).cuda().reshape(2, -1).contiguous()
+    out = torch.empty(x.shape[1], device="cuda")
+    net.zero_grad()
+    net.forward(x, out)
+    net.backward(torch.from_numpy(dout).cuda().reshape(-1).contiguous())
+    g = net.grads().astype(np.float64)
+    on = N.Net(P.astype(np.float64))
+    on.forward(inp.astype(np.float64))
+    go = on.backward(dout.astype(np.float64))
+    L, head, n = N.layout()
+    for li, (ci, co, _, wo, bo) in enumerate(L):
+        a, b = g[wo:wo + co * 27 * ci], go[wo:wo + co * 27 * ci]
+        if li == 0:
+            keep = np.zeros((co, 27, ci), bool)
+            keep[:, :, :2] = True
+            a, b = a[keep.ravel()], b[keep.ravel()]
+        rel = np.abs(a - b).max() / np.abs(b).max()
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        print(f"[gpu] layer {li} dW rel {rel:.2e} cos {cos:.6f}", flush=True)
+        assert rel < 5e-2 and cos > 0.999, li
+        bb = g[bo:bo + co] - go[bo:bo + co]
+        assert np.abs(bb).max() <= 5e-2 * np.abs(go[bo:bo + co]).max() + 1e-3
+    assert np.allclose(g[head:], go[head:], rtol=2e-2, atol=1e-3 * np.abs(go[head:]).max())
+
+
+def test_adam_matches_closed_form(torch):
+    shape = (8, 8, 16)
+    P = _params(2)
+    net = _net(torch, shape, P)
+    rng = np.random.default_rng(5)
+    x = torch.from_numpy(rng.standard_normal((2,) + shape).astype(np.float32)).cuda().reshape(2, -1)
+    tgt = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).cuda().reshape(-1)
+    out = torch.empty_like(tgt)
+    dout = torch.empty_like(tgt)
+    net.zero_grad()
+    net.forward(x.contiguous(), out)
+    net.mse(out, tgt, dout)
+    net.backward(dout)
+    g = net.grads().astype(np.float64)
+    net.adam(1e-3)
+    w1 = net.params().astype(np.float64)
+    exp, _, _ = N.adam(P.astype(np.float64), g, np.zeros_like(g), np.zeros_like(g), 1e-3, 0.9,
+                       0.999, 1e-8, 1)
+    assert np.abs(w1 - exp).max() < 1e-6
+
+
+def test_training_reduces_loss_and_warm_start_runs(torch):
+    """Train on a short captured TGV run at 32^3, then use the network as the
+    solver's warm start: every step still converges to eps, and the fields
+    match the run without it to a tolerance tied to eps."""
+    from osinco3d_amd import warmstart as W
+    from osinco3d_amd.config import tgv_config
+    from osinco3d_amd.solver import FP64, MIXED, Solver, taylor_green
+    cfg = tgv_config(32, 40, 3)
+    samples, iters = W.capture_run(cfg, 30, MIXED)
+    assert len(samples) >= 20
+    net = W.Network(32, 32, 32, seed=7)
+    before = W.relative_errors(net, samples)
+    hist = W.train(net, samples, epochs=4, lr=2e-3, batch=4)
+    after = W.relative_errors(net, samples)
+    print(f"[gpu] nn train mse {hist}, rel err before {before} after {after}", flush=True)
+    assert hist[-1] < hist[0]
+    a = Solver(cfg, MIXED)
+    b = Solver(cfg, MIXED)
+    a.upload(*taylor_green(cfg))
+    b.upload(*taylor_green(cfg))
+    net.attach(b)
+    used = 0
+    for it in range(1, 21):
+        ia = a.step(it)
+        ib = b.step(it)
+        assert ib["sor_reason"] == 0 and ia["sor_reason"] == 0
+        used += ib["warm_used"]
+        if it > 3:
+            assert ib["warm_dmax"][0] > 0
+    print(f"[gpu] warm start used on {used} of 20 steps", flush=True)
+    ga, gb = a.download(), b.download()
+    for c in range(3):
+        assert np.abs(ga[c] - gb[c]).max() < 1e-6
