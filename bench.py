@@ -23,8 +23,9 @@ launch stream over complete sweep batches inside the timed steps.
 cpu_baseline: the reference Fortran (oracle/_ref/o3d_ref_run, built from the
 reference sources at -O3, single-threaded) on one host core: a bounded sample
 of one 256^3 step with kmax capped, whose phases are timed; the per-step
-time is extrapolated with the measured per-iteration SOR cost and the
-FP64 form's mean iteration count.
+time is extrapolated with the measured per-sweep SOR cost and the number of
+lexicographic sweeps the reference needs (the FP64 form's red-black count
+times the lexicographic/red-black ratio measured on config 1).
 """
 from __future__ import annotations
 
@@ -152,7 +153,23 @@ def copy_bandwidth(nbytes=1 << 30, reps=20):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(n, kmax, iters_per_step):
+def lex_over_red_black():
+    """Sweeps the reference's lexicographic SOR needs per red-black sweep to
+    reach the same dmax < eps: the reference's config-1 run (TGV 64^3, 100
+    steps, committed fixture) against the FP64 path on the same 100 steps."""
+    import numpy as np
+    from osinco3d_amd.config import Config
+    from osinco3d_amd.solver import FP64, Solver
+    g = np.load(os.path.join(ROOT, "tests", "golden", "c1_tgv64.npz"), allow_pickle=False)
+    cfg = Config.from_namelist(bytes(g["params"]).decode())
+    s = Solver(cfg, FP64)
+    s.init_taylor_green()
+    rb = [s.step(it)["sor_exit_iter"] for it in range(1, cfg.itstop + 1)]
+    s.close()
+    return float(np.mean(g["iters"])) / float(np.mean(rb))
+
+
+def cpu_baseline(n, kmax, iters_per_step, ratio):
     """Reference Fortran, one core, one n^3 step with kmax capped."""
     from oracle import ref_run_path
     from osinco3d_amd.config import tgv_config
@@ -191,7 +208,8 @@ def cpu_baseline(n, kmax, iters_per_step):
     sweeps = min(iters, kmax)
     t_iter = (pression - div) / sweeps
     t_other = predict + div + correct + div + diag
-    t_step = t_other + iters_per_step * t_iter
+    lex_iters = iters_per_step * ratio
+    t_step = t_other + lex_iters * t_iter
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -208,7 +226,10 @@ def cpu_baseline(n, kmax, iters_per_step):
                    f"({wall:.1f} s wall); phases predict {predict:.2f} s, correct_pression "
                    f"{pression:.2f} s ({sweeps} sweeps, {t_iter:.3f} s/sweep), correct "
                    f"{correct:.2f} s, diagnostics {div + diag:.2f} s; step time = "
-                   f"{t_other:.2f} s + {iters_per_step:.1f} sweeps x {t_iter:.3f} s"),
+                   f"{t_other:.2f} s + {lex_iters:.1f} sweeps x {t_iter:.3f} s, where "
+                   f"{lex_iters:.1f} = the FP64 GPU path's {iters_per_step:.1f} red-black "
+                   f"sweeps x {ratio:.3f}, the reference's lexicographic/red-black sweep "
+                   f"ratio measured on config 1 (TGV 64^3, 100 steps)"),
         "seconds_per_step": t_step,
     }
 
@@ -258,7 +279,7 @@ def main():
     if not args.no_cpu:
         print("[bench] cpu baseline (reference Fortran, 1 core)", file=sys.stderr, flush=True)
         iters = (ref or head)["poisson_iters"]["mean"]
-        cpu = cpu_baseline(args.n, args.cpu_kmax, iters)
+        cpu = cpu_baseline(args.n, args.cpu_kmax, iters, lex_over_red_black())
     line = {
         "metric": METRIC,
         "value": head["steps_per_s"],

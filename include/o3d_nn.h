@@ -4,7 +4,8 @@
  * step's pressure, src/integration.f90:247 with pp in/out).  The network is a
  * 3D U-Net-lite: 4 levels with 16/32/64/128 channels, two 3x3x3 convolutions
  * per level and side, average pooling, nearest up-sampling, additive skips,
- * leaky-ReLU, a 1x1x1 FP32 head.  Weights and activations are BF16, products
+ * leaky-ReLU, a 1x1x1 FP32 head whose output is added to the second input
+ * plane (the network corrects that plane).  Weights and activations are BF16, products
  * accumulate in FP32 on MFMA (implicit-GEMM convolutions); FP32 master
  * weights are trained with Adam.  Convolutions pad periodically.
  *

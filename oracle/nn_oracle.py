@@ -4,7 +4,8 @@ Same architecture, parameter layout and rounding points as the HIP
 implementation (osinco3d_amd/csrc/o3d_nn.hip): BF16 activations and weights
 (round to nearest even), sums in float64 here (FP32 on the GPU), periodic
 padding, leaky-ReLU slope 0.1, average pooling, nearest up-sampling,
-additive skips before the activation, FP32 1x1x1 head.  It checks the GPU's
+additive skips before the activation, FP32 1x1x1 head added to the second
+input plane (the network predicts a correction to it).  It checks the GPU's
 forward pass, its backward pass (activation and weight gradients) and Adam.
 
 The network is the build's own (the reference has no learned component), so
@@ -151,7 +152,7 @@ class Net:
         wh = self.P[self.head:self.head + 16]
         bh = self.P[self.head + 16]
         self.a = a
-        return a["d0"] @ wh + bh
+        return a["d0"] @ wh + bh + inp[1]
 
     def backward(self, dout):
         """Gradient of sum(dout * out) w.r.t. the flat parameters."""

@@ -250,14 +250,16 @@ __global__ void __launch_bounds__(256) act_bwd_kernel(const float *__restrict__ 
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&db[c], sb[c]);
 }
 
-// out[v] = sum_c wh[c] d0[v][c] + bh
+// out[v] = sum_c wh[c] d0[v][c] + bh + id[v]: the network learns a
+// correction to its second input plane (the previous increment).
 __global__ void head_fwd_kernel(const bf16 *__restrict__ d0, const float *__restrict__ wh,
-                                float *__restrict__ out, int V, int C) {
+                                const float *__restrict__ id, float *__restrict__ out, int V,
+                                int C) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
   float s = wh[C];
   for (int c = 0; c < C; ++c) s += wh[c] * (float)d0[(size_t)v * C + c];
-  out[v] = s;
+  out[v] = s + id[v];
 }
 
 __global__ void __launch_bounds__(256) head_bwd_kernel(const float *__restrict__ dout,
@@ -478,7 +480,8 @@ int sync_weights(Net &n) {
   return 0;
 }
 
-// Deterministic He-style initialisation (splitmix64 -> uniform).
+// Deterministic He-style initialisation (splitmix64 -> uniform); the head
+// starts small so the untrained network returns about its identity input.
 void init_params(std::vector<float> &w, const Net &n, uint64_t seed) {
   uint64_t s = seed;
   auto next = [&]() {
@@ -500,7 +503,7 @@ void init_params(std::vector<float> &w, const Net &n, uint64_t seed) {
           w[c.woff + (size_t)o * 27 * c.ci + t * c.ci + ci] = (float)((2 * next() - 1) * bound);
         }
   }
-  for (int c = 0; c < 16; ++c) w[n.head_off + c] = (float)((2 * next() - 1) * 0.25);
+  for (int c = 0; c < 16; ++c) w[n.head_off + c] = (float)((2 * next() - 1) * 0.02);
 }
 
 int conv_fwd(Net &n, int li, const bf16 *x, const bf16 *skip, bf16 *y) {
@@ -581,8 +584,8 @@ int forward(Net &n, const float *in, float *out) {
                                                              n.nz[0], 32);
   NCHK(conv_fwd(n, 12, n.u0, n.e0, n.d0a));
   NCHK(conv_fwd(n, 13, n.d0a, nullptr, n.d0));
-  head_fwd_kernel<<<elem_blocks(n.V[0]), 256, 0, st()>>>(n.d0, n.W + n.head_off, out,
-                                                          (int)n.V[0], 16);
+  head_fwd_kernel<<<elem_blocks(n.V[0]), 256, 0, st()>>>(n.d0, n.W + n.head_off, in + n.V[0],
+                                                          out, (int)n.V[0], 16);
   NNCHK(hipGetLastError());
   NNCHK(hipEventRecord(n.ev[1], st()));
   return 0;

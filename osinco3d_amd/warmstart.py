@@ -3,7 +3,9 @@
 The network (include/o3d_nn.h) maps two planes, the residual of the previous
 pressure r = rhs - L p_prev (times dx^2) and the previous increment
 e_prev = p_prev - p_prev2, each divided by its RMS, to the increment
-e = p_new - p_prev divided by RMS(e_prev).  Training pairs come from the
+e = p_new - p_prev divided by RMS(e_prev); its output is a learned correction
+added to the second plane, so the untrained network starts from the linear
+extrapolation e = e_prev.  Training pairs come from the
 solver's own runs (o3d_solver_capture): every step of a run yields
 (r, e_prev, e) with p_new the SOR solution to eps.  The solver applies the
 same normalisation at inference time (o3d_solver_set_warm_model).
@@ -190,7 +192,7 @@ def relative_errors(net: Network, samples) -> Dict[str, float]:
             "previous": 1.0}
 
 
-def train(net: Network, samples, epochs: int = 3, lr: float = 1e-3, batch: int = 4,
+def train(net: Network, samples, epochs: int = 3, lr: float = 3e-4, batch: int = 4,
           seed: int = 0, log=None) -> List[float]:
     """Adam on the mean squared error, mini-batches of `batch` samples
     (gradients accumulate across the batch)."""
