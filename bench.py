@@ -10,7 +10,9 @@ driver's per-step diagnostics), with all fields resident in HBM.
 Each requested form is timed over the same steps:
   fp64   plain FP64 (the drop-in; bit-identical operators to the reference)
   mixed  FP32 stencils / AB update / SOR sweeps, FP64-refined Poisson
-  nn     mixed + learned warm start (needs a trained model file)
+  extrap mixed + warm start by linear extrapolation p_prev + (p_prev - p_prev2)
+  nn     mixed + learned warm start (the trained network of
+         osinco3d_amd/warmstart_weights.npy; skipped when it is missing)
 The JSON line's value is the steps/s of the headline form (the last of
 --forms); every form's figures are included under "forms".
 
@@ -45,6 +47,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TGV 256³ steps/s + Poisson ms & iters to eps=1e-8; plain FP64 vs mixed+NN"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_TFS = 2500.0
 
 
 def parse_args():
@@ -53,10 +56,10 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default="fp64,mixed,extrap,nn")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-kmax", type=int, default=10)
-    ap.add_argument("--model", default=os.path.join(ROOT, "gpurun_out", "warm_model.bin"))
+    ap.add_argument("--model", default=os.path.join(ROOT, "osinco3d_amd", "warmstart_weights.npy"))
     return ap.parse_args()
 
 
@@ -73,14 +76,17 @@ def run_form(form, cfg, steps, warmup, model_path):
     from osinco3d_amd import solver as S
 
     prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = S.WARM_LEARNED if form == "nn" else S.WARM_PREVIOUS
-    s = S.Solver(cfg, prec, warm)
+    warm = {"nn": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE}.get(form, S.WARM_PREVIOUS)
+    net = None
     if form == "nn":
         from osinco3d_amd import warmstart
-        net = warmstart.load_or_none(model_path)
+        net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
         if net is None:
-            s.close()
+            print(f"[bench] form nn skipped: no model at {model_path}", file=sys.stderr,
+                  flush=True)
             return None
+    s = S.Solver(cfg, prec, warm)
+    if net is not None:
         net.attach(s)
     s.init_taylor_green()
     for it in range(1, warmup + 1):
@@ -90,12 +96,18 @@ def run_form(form, cfg, steps, warmup, model_path):
     infos = []
     sor_ms = [0.0, 0.0]
     sor_sw = [0, 0]
+    nn_ms, nn_fl, nn_n = 0.0, 0.0, 0
     for k in range(1, steps + 1):
         infos.append(s.step(warmup + k))
         ms, sw = s.sor_timing()
         for q in range(2):
             sor_ms[q] += ms[q]
             sor_sw[q] += sw[q]
+        if net is not None:
+            a, b = net.last_timing()
+            nn_ms += a
+            nn_fl += b
+            nn_n += 1
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     t_end = (warmup + steps) * s.dt
@@ -128,8 +140,16 @@ def run_form(form, cfg, steps, warmup, model_path):
             gbs = bpc * n / (avg_ms * 1e-3) / 1e9
             out[name] = {"avg_launch_us": 1e3 * avg_ms, "launches_timed": sor_sw[q],
                          "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs}
-    if form == "nn":
+    if form in ("nn", "extrap"):
         out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)
+        out["warm_residual_l2"] = {"previous": statistics.mean(i["warm_dmax"][0] for i in infos),
+                                   "guess": statistics.mean(i["warm_dmax"][1] for i in infos)}
+    if net is not None and nn_n and nn_ms > 0:
+        tfs = nn_fl / (nn_ms * 1e-3) / 1e12
+        out["nn_inference"] = {"ms_per_step": nn_ms / nn_n, "flops": nn_fl / nn_n,
+                               "tflops": tfs, "mfma_util_bf16_dense": tfs / MFMA_BF16_TFS}
+    if net is not None:
+        net.close()
     return out
 
 

@@ -122,6 +122,7 @@ int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
 /* Warm start of the Poisson solve. */
 #define O3D_WARM_PREVIOUS 0 /* previous step's pressure (the reference's behaviour) */
 #define O3D_WARM_LEARNED 1  /* learned correction, previous pressure as fallback */
+#define O3D_WARM_EXTRAPOLATE 2 /* p_prev + (p_prev - p_prev2), previous pressure as fallback */
 
 typedef struct o3d_params {
   int nx, ny, nz;
@@ -135,7 +136,7 @@ typedef struct o3d_params {
   double omega, eps;
   int kmax, idyn;
   int precision;        /* O3D_FP64 or O3D_MIXED */
-  int warm_start;       /* O3D_WARM_PREVIOUS or O3D_WARM_LEARNED */
+  int warm_start;       /* O3D_WARM_PREVIOUS, O3D_WARM_LEARNED or O3D_WARM_EXTRAPOLATE */
   int inner_max;        /* O3D_MIXED: FP32 sweeps between FP64 refinements (0 = default) */
 } o3d_params;
 
@@ -152,7 +153,8 @@ typedef struct o3d_step_info {
   double omega;           /* omega after the solve */
   double poisson_ms;      /* device time of the Poisson solve */
   double step_ms;         /* device time of the whole step */
-  double warm_dmax[2];    /* first-sweep dmax: [0] previous pressure, [1] learned */
+  double warm_dmax[2];    /* L2 residual of the initial guess: [0] previous pressure,
+                             [1] the warm guess (0 when none was formed) */
   double divu_pred[6];    /* function_stats of div(u*) */
   double divu[6];         /* function_stats of div(u) */
   double vel_min[3], vel_max[3], vel_amax[3];

@@ -1008,29 +1008,37 @@ int solver_learning_prepare(Solver &S) {
   return 0;
 }
 
-// Network guess p_prev + s_e * NN(r dx^2 / rms, e_prev / rms), accepted when
-// its L2 residual is below that of p_prev.
-int solver_warm_learned(Solver &S, int *used, double *warm) {
+// Warm guess, accepted when its L2 residual is below that of p_prev:
+// p_prev + s_e * NN(r dx^2 / rms, e_prev / rms) with a network attached,
+// else (O3D_WARM_EXTRAPOLATE) the linear extrapolation p_prev + e_prev.
+int solver_warm_guess(Solver &S, int *used, double *warm) {
   *used = 0;
   warm[0] = warm[1] = 0.0;
   if (S.hist < 1) return 0;
   const int n = S.n;
-  double sr2 = 0.0, se2 = 0.0;
-  CHK(lw_sumsq(S, S.Rl, &sr2));
-  CHK(lw_sumsq(S, S.Eprev, &se2));
-  const double h2 = S.p.dx * S.p.dx;
-  const double rr = sqrt(sr2 / n) * h2, re = sqrt(se2 / n);
-  if (!(rr > 0.0) || !(re > 0.0)) return 0;
-  lw_features_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.Eprev, S.Feat, n,
-                                                            (float)(h2 / rr), (float)(1.0 / re));
-  HIPCHK(hipGetLastError());
-  CHK(o3d_nn_forward(S.nn, S.Feat, S.Out));
-  double fl = 0.0, ms = 0.0;
-  CHK(o3d_nn_last_timing(S.nn, &ms, &fl));
-  S.nn_ms += ms;
-  S.nn_flops += fl;
+  const float *inc = S.Eprev;
+  double scale = 1.0;
+  if (S.nn) {
+    double sr2 = 0.0, se2 = 0.0;
+    CHK(lw_sumsq(S, S.Rl, &sr2));
+    CHK(lw_sumsq(S, S.Eprev, &se2));
+    const double h2 = S.p.dx * S.p.dx;
+    const double rr = sqrt(sr2 / n) * h2, re = sqrt(se2 / n);
+    if (!(rr > 0.0) || !(re > 0.0)) return 0;
+    lw_features_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.Eprev, S.Feat, n,
+                                                              (float)(h2 / rr),
+                                                              (float)(1.0 / re));
+    HIPCHK(hipGetLastError());
+    CHK(o3d_nn_forward(S.nn, S.Feat, S.Out));
+    double fl = 0.0, ms = 0.0;
+    CHK(o3d_nn_last_timing(S.nn, &ms, &fl));
+    S.nn_ms += ms;
+    S.nn_flops += fl;
+    inc = S.Out;
+    scale = re;
+  }
   double *q = S.P[1 - S.cur];
-  lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Out, q, n, re);
+  lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], inc, q, n, scale);
   HIPCHK(hipGetLastError());
   CHK(lw_residual_l2(S, S.P[S.cur], &warm[0]));
   CHK(lw_residual_l2(S, q, &warm[1]));
@@ -1062,10 +1070,11 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   if (p.iles == 1)
     HIPCHK(launch_finalize_partials(S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5, g_stream));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
-  const bool learning = S.nn || S.cap_r || S.cap_eprev || S.cap_e;
+  const bool guess = S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE;
+  const bool learning = guess || S.cap_r || S.cap_eprev || S.cap_e;
   int warm = 0;
   if (learning) CHK(solver_learning_prepare(S));
-  if (S.nn) CHK(solver_warm_learned(S, &warm, info->warm_dmax));
+  if (guess) CHK(solver_warm_guess(S, &warm, info->warm_dmax));
   if (S.mixed) CHK(solver_poisson_mixed(S, info));
   else CHK(solver_poisson_f64(S, info));
   info->warm_used = warm;
@@ -1138,6 +1147,8 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
   }
   if (p->precision != O3D_FP64 && p->precision != O3D_MIXED)
     return fail("unknown precision form %d", p->precision);
+  if (p->warm_start < O3D_WARM_PREVIOUS || p->warm_start > O3D_WARM_EXTRAPOLATE)
+    return fail("unknown warm start %d", p->warm_start);
   const int kind = solver_from_bc(p->bc);
   if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");
   CHK(check_red_black(kind, p->nx, p->ny, p->nz));

@@ -20,6 +20,7 @@
 // old value, src/poisson.f90:100-101) is reduced with wavefront shuffles and
 // one 64-bit atomicMax per block on the IEEE bit pattern (dmax >= 0).
 #include <algorithm>
+#include <utility>
 
 #include "o3d_sor.h"
 
@@ -135,23 +136,43 @@ __device__ __forceinline__ T sor_gs(T w_, T e_, T s_, T n_, T b_, T t_, TR r, T 
   else return num * invA;
 }
 
-__device__ __forceinline__ int ring3(int s) { return (s + 6) % 3; }
-
 template <int K>
 struct IC { static constexpr int value = K; };
 
-// Planes of the old iterate and rhs are prefetched PD steps ahead into
-// registers, so each block keeps PD planes of loads in flight across the
-// barriers of the march.
-constexpr int PD = 3;
+template <int... K, typename F>
+__device__ __forceinline__ void unroll_steps(std::integer_sequence<int, K...>, F &&f) {
+  (f(IC<K>{}), ...);
+}
+
+// Planes handled per march step (PPS: one barrier pair per step) and the
+// prefetch depth in steps (PD): loads of the old iterate and rhs are issued
+// PD steps before their planes are published to LDS, so each block keeps
+// PD*PPS planes of loads in flight across the barriers.
+// Ring depths: phase 2 of a step reads old planes s-1..s+PPS and rhs planes
+// s-1..s+PPS-1, phase 3 reads red planes s-2..s+PPS-1.
+template <typename T>
+struct SorShape {
+  static constexpr int PPS = sizeof(T) == 4 ? 2 : 1;
+  static constexpr int PD = sizeof(T) == 4 ? 2 : 3;
+  static constexpr int NO = PPS + 2, NRD = PPS + 2, NRH = PPS + 1;
+};
+
+// Ring slot of plane q (q >= -2).
+template <int N>
+__device__ __forceinline__ int slot(int q) {
+  if constexpr ((N & (N - 1)) == 0) return q & (N - 1);
+  else return (q + 4 * N) % N;
+}
 
 template <typename T, typename TR>
 __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
-  // LDS rings: old iterate on R2 (3 planes), new red values on R1 (3 planes),
-  // rhs on R1 (2 planes).  Plane s lives in slot ring3(s) (rhs: s & 1).
-  __shared__ T oldr[3][R2N];
-  __shared__ T redr[3][R1N];
-  __shared__ TR rhsr[2][R1N];
+  constexpr int PPS = SorShape<T>::PPS;
+  constexpr int PD = SorShape<T>::PD;
+  constexpr int NO = SorShape<T>::NO, NRD = SorShape<T>::NRD, NRH = SorShape<T>::NRH;
+  // LDS rings: old iterate on R2, new red values and rhs on R1.
+  __shared__ T oldr[NO][R2N];
+  __shared__ T redr[NRD][R1N];
+  __shared__ TR rhsr[NRH][R1N];
   __shared__ double scr[SNW];
   SorState *st = a.st;
   if (st->done) return;
@@ -221,89 +242,105 @@ __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
 
   auto gz_of = [&](int zp) { return plane * sor_ghost(zp, nz, a.per[2]); };
 
-  T po[PD][2];
-  TR pr[PD][2];
+  // Prefetch registers: step k of the march publishes old planes
+  // s_k+1 .. s_k+PPS and rhs planes s_k .. s_k+PPS-1, s_k = z0-1 + k*PPS.
+  T po[PD][PPS][2];
+  TR pr[PD][PPS][2];
   {
     const int g2 = gz_of(z0 - 2), g1 = gz_of(z0 - 1);
-    oldr[ring3(z0 - 2)][tid] = a.pin[col2a + g2];
-    oldr[ring3(z0 - 1)][tid] = a.pin[col2a + g1];
+    oldr[slot<NO>(z0 - 2)][tid] = a.pin[col2a + g2];
+    oldr[slot<NO>(z0 - 1)][tid] = a.pin[col2a + g1];
     if (v2b) {
-      oldr[ring3(z0 - 2)][c2b] = a.pin[col2b + g2];
-      oldr[ring3(z0 - 1)][c2b] = a.pin[col2b + g1];
+      oldr[slot<NO>(z0 - 2)][c2b] = a.pin[col2b + g2];
+      oldr[slot<NO>(z0 - 1)][c2b] = a.pin[col2b + g1];
     }
 #pragma unroll
-    for (int k = 0; k < PD; ++k) {
-      const int go = gz_of(z0 + k), gr = gz_of(z0 - 1 + k);
-      po[k][0] = a.pin[col2a + go];
-      po[k][1] = a.pin[col2b + go];
-      pr[k][0] = a.rhs[col1a + gr];
-      pr[k][1] = a.rhs[col1b + gr];
-    }
+    for (int k = 0; k < PD; ++k)
+#pragma unroll
+      for (int q = 0; q < PPS; ++q) {
+        const int sk = z0 - 1 + k * PPS + q;
+        const int go = gz_of(sk + 1), gr = gz_of(sk);
+        po[k][q][0] = a.pin[col2a + go];
+        po[k][q][1] = a.pin[col2b + go];
+        pr[k][q][0] = a.rhs[col1a + gr];
+        pr[k][q][1] = a.rhs[col1b + gr];
+      }
   }
 
   double dmax = 0.0;
   auto step = [&](int s, auto K) {
     constexpr int k = decltype(K)::value;
-    // Phase 1: publish old(s+1) and rhs(s); prefetch old(s+1+PD), rhs(s+PD).
-    {
-      T *od = oldr[ring3(s + 1)];
-      TR *rd = rhsr[s & 1];
-      od[tid] = po[k][0];
-      rd[tid] = pr[k][0];
-      if (v2b) od[c2b] = po[k][1];
-      if (v1b) rd[c2b] = pr[k][1];
-      const int go = gz_of(s + 1 + PD), gr = gz_of(s + PD);
-      po[k][0] = a.pin[col2a + go];
-      po[k][1] = a.pin[col2b + go];
-      pr[k][0] = a.rhs[col1a + gr];
-      pr[k][1] = a.rhs[col1b + gr];
+    // Phase 1: publish old(s+1..s+PPS) and rhs(s..s+PPS-1); prefetch the
+    // planes PD steps ahead into the same registers.
+#pragma unroll
+    for (int q = 0; q < PPS; ++q) {
+      T *od = oldr[slot<NO>(s + 1 + q)];
+      TR *rd = rhsr[slot<NRH>(s + q)];
+      od[tid] = po[k][q][0];
+      rd[tid] = pr[k][q][0];
+      if (v2b) od[c2b] = po[k][q][1];
+      if (v1b) rd[c2b] = pr[k][q][1];
+      const int go = gz_of(s + 1 + q + PD * PPS), gr = gz_of(s + q + PD * PPS);
+      po[k][q][0] = a.pin[col2a + go];
+      po[k][q][1] = a.pin[col2b + go];
+      pr[k][q][0] = a.rhs[col1a + gr];
+      pr[k][q][1] = a.rhs[col1b + gr];
     }
     __syncthreads();
-    // Phase 2: red (colour 0) on plane s over R1 from old values only; keep
-    // this thread's old value and rhs of plane s-1 for the black pass.
-    if (red_lane) {
-      const int a1 = 2 * rk + ((rpar + s) & 1);
-      const int X = x0 - 1 + a1, Y = y0 - 1 + rb1;
-      const int i2 = (rb1 + 1) * R2W + a1 + 1;
-      const int i1 = rb1 * R1W + a1;
-      const T *o0 = oldr[ring3(s)];
-      const T *om_ = oldr[ring3(s - 1)];
-      const T *op = oldr[ring3(s + 1)];
-      const T pc = o0[i2];
-      const T pn = sor_gs<T, TR>(o0[i2 - 1], o0[i2 + 1], o0[i2 - R2W], o0[i2 + R2W], om_[i2],
-                                 op[i2], rhsr[s & 1][i1], ox, oy, oz, A, invA);
-      redr[ring3(s)][i1] = w1 * pc + w * pn;
-      if (a1 >= 1 && a1 <= SX && rb1 >= 1 && rb1 <= SY && X < nx && Y < ny && s >= z0 &&
-          s < z1)
-        dmax = fmax(dmax, (double)fabs(pn - pc));
-    }
-    const T bold = oldr[ring3(s - 1)][b2i];
-    const TR brhs = rhsr[(s - 1) & 1][b1i];
-    __syncthreads();
-    // Phase 3: black (colour 1) on plane s-1 over the tile; write plane s-1.
-    const int sb = s - 1;
-    if (sb >= z0 && bown) {
-      const T *rm = redr[ring3(sb - 1)];
-      const T *r0 = redr[ring3(sb)];
-      const T *rp = redr[ring3(s)];
-      T v;
-      if (((bpar + sb) & 1) != 0) {
-        const T pn = sor_gs<T, TR>(r0[b1i - 1], r0[b1i + 1], r0[b1i - R1W], r0[b1i + R1W],
-                                   rm[b1i], rp[b1i], brhs, ox, oy, oz, A, invA);
-        v = w1 * bold + w * pn;
-        dmax = fmax(dmax, (double)fabs(pn - bold));
-      } else {
-        v = r0[b1i];
+    // Phase 2: red (colour 0) on planes s..s+PPS-1 over R1 from old values
+    // only; keep this thread's old value and rhs of planes s-1..s+PPS-2 for
+    // the black pass.
+    T bold[PPS];
+    TR brhs[PPS];
+#pragma unroll
+    for (int q = 0; q < PPS; ++q) {
+      const int sr = s + q;
+      if (red_lane) {
+        const int a1 = 2 * rk + ((rpar + sr) & 1);
+        const int X = x0 - 1 + a1, Y = y0 - 1 + rb1;
+        const int i2 = (rb1 + 1) * R2W + a1 + 1;
+        const int i1 = rb1 * R1W + a1;
+        const T *o0 = oldr[slot<NO>(sr)];
+        const T *om_ = oldr[slot<NO>(sr - 1)];
+        const T *op = oldr[slot<NO>(sr + 1)];
+        const T pc = o0[i2];
+        const T pn = sor_gs<T, TR>(o0[i2 - 1], o0[i2 + 1], o0[i2 - R2W], o0[i2 + R2W], om_[i2],
+                                   op[i2], rhsr[slot<NRH>(sr)][i1], ox, oy, oz, A, invA);
+        redr[slot<NRD>(sr)][i1] = w1 * pc + w * pn;
+        if (a1 >= 1 && a1 <= SX && rb1 >= 1 && rb1 <= SY && X < nx && Y < ny && sr >= z0 &&
+            sr < z1)
+          dmax = fmax(dmax, (double)fabs(pn - pc));
       }
-      a.pout[bi + nx * bj + plane * sb] = v;
+      bold[q] = oldr[slot<NO>(sr - 1)][b2i];
+      brhs[q] = rhsr[slot<NRH>(sr - 1)][b1i];
+    }
+    __syncthreads();
+    // Phase 3: black (colour 1) on planes s-1..s+PPS-2 over the tile.
+#pragma unroll
+    for (int q = 0; q < PPS; ++q) {
+      const int sb = s - 1 + q;
+      if (sb >= z0 && sb < z1 && bown) {
+        const T *rm = redr[slot<NRD>(sb - 1)];
+        const T *r0 = redr[slot<NRD>(sb)];
+        const T *rp = redr[slot<NRD>(sb + 1)];
+        T v;
+        if (((bpar + sb) & 1) != 0) {
+          const T pn = sor_gs<T, TR>(r0[b1i - 1], r0[b1i + 1], r0[b1i - R1W], r0[b1i + R1W],
+                                     rm[b1i], rp[b1i], brhs[q], ox, oy, oz, A, invA);
+          v = w1 * bold[q] + w * pn;
+          dmax = fmax(dmax, (double)fabs(pn - bold[q]));
+        } else {
+          v = r0[b1i];
+        }
+        a.pout[bi + nx * bj + plane * sb] = v;
+      }
     }
   };
-  static_assert(PD == 3, "the march below is unrolled for PD == 3");
-  for (int s = z0 - 1; s <= z1; s += PD) {
-    step(s, IC<0>{});
-    if (s + 1 <= z1) step(s + 1, IC<1>{});
-    if (s + 2 <= z1) step(s + 2, IC<2>{});
-  }
+  for (int s = z0 - 1; s <= z1; s += PD * PPS)
+    unroll_steps(std::make_integer_sequence<int, PD>{}, [&](auto K) {
+      const int sk = s + decltype(K)::value * PPS;
+      if (decltype(K)::value == 0 || sk <= z1) step(sk, K);
+    });
   // Block max, one atomic per block.
   dmax = block_max<SNW>(dmax, scr);
   if (threadIdx.x == 0 && dmax != 0.0)

@@ -34,7 +34,11 @@ contains
          int(nz, c_int), omega, eps, int(kmax, c_int), int(idyn, c_int), int(kind, c_int), &
          it, dmax, reason), 'poisson')
     if (reason == 1) write(*, '(A29,E10.3)') " No converged at less than: ", eps
-    write(*, '(A19,I4,A1,E13.6)') ' Exit iteration : ', it, ',', dmax
+    if (kind == 2) then
+       write(*, '(A18,I4,A1,E13.6)') ' Exit iteration: ', it, ',', dmax
+    else
+       write(*, '(A19,I4,A1,E13.6)') ' Exit iteration : ', it, ',', dmax
+    end if
     if (idyn == 1) write(*,'(A23, F5.3)') ' With dynamic omega = ', omega
   end subroutine run_sor_x
 

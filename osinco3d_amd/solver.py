@@ -21,6 +21,7 @@ FP64 = 0
 MIXED = 1
 WARM_PREVIOUS = 0
 WARM_LEARNED = 1
+WARM_EXTRAPOLATE = 2
 
 SOR_REASONS = {0: "eps", 1: "stagnation", 2: "kmax"}
 

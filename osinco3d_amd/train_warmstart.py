@@ -1,0 +1,150 @@
+"""Train the warm-start network of the Poisson solve on the solver's own runs.
+
+    python -m osinco3d_amd.train_warmstart [--n 128] [--samples 2000]
+        [--epochs 4] [--out osinco3d_amd/warmstart_weights.npy] [--eval-n 256]
+
+Training pairs (inputs r, e_prev; target e, see warmstart.py) come from
+Taylor-Green runs at n^3 (periodic, AB3, CFL 0.25, SOR eps 1e-8, mixed
+precision form): Re 1600 over the vortex breakdown, Re 400, and Re 1600 with
+the axes permuted so the network does not learn one orientation.  Every
+sample stays resident in device memory (3 float32 planes of n^3 cells:
+24 MiB at 128^3, 48 GiB for 2000 samples).  The last tenth of each run is
+held out to report the relative error of the predicted increment against the
+linear extrapolation e = e_prev.
+
+Evaluation: the same solver steps with the previous-pressure, the
+extrapolated and the learned warm starts, reporting SOR sweeps per step.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import warmstart
+from .config import tgv_config
+from .solver import MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREVIOUS, Solver, taylor_green
+
+DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npy")
+
+
+def log(msg: str):
+    print(f"[train] {msg}", flush=True)
+
+
+def permuted(fields):
+    """Taylor-Green field with the axes permuted cyclically: v_x(x,y,z) =
+    u_z(y,z,x), v_y = u_x(y,z,x), v_z = u_y(y,z,x) (still divergence free);
+    arrays are (nz, ny, nx)."""
+    ux, uy, uz, pp = fields
+    t = lambda a: np.ascontiguousarray(np.transpose(a, (1, 2, 0)))
+    return t(uz), t(ux), t(uy), t(pp)
+
+
+def runs(n: int, total: int):
+    """(label, config, initial fields, steps) with about `total` steps."""
+    specs = [("tgv Re1600", 1600.0, False, 0.45), ("tgv Re400", 400.0, False, 0.25),
+             ("tgv Re1600 permuted", 1600.0, True, 0.30)]
+    out = []
+    for label, re, perm, frac in specs:
+        steps = max(8, int(round(total * frac)))
+        cfg = tgv_config(n, steps, 3, re=re)
+        init = taylor_green(cfg)
+        if perm:
+            init = permuted(init)
+        out.append((label, cfg, init, steps))
+    return out
+
+
+def collect(n: int, total: int):
+    train, held = [], []
+    for label, cfg, init, steps in runs(n, total):
+        t0 = time.perf_counter()
+        samples, iters = warmstart.capture_run(cfg, steps + 2, MIXED, init=init)
+        cut = int(len(samples) * 0.9)
+        train += samples[:cut]
+        held += samples[cut:]
+        log(f"{label}: {len(samples)} samples in {time.perf_counter() - t0:.1f} s, "
+            f"SOR sweeps/step mean {np.mean(iters):.1f}")
+    return train, held
+
+
+def sweeps_with(cfg, init, steps, warm, net=None, skip=3):
+    s = Solver(cfg, MIXED, warm)
+    if net is not None:
+        net.attach(s)
+    s.upload(*init)
+    its, used = [], []
+    for it in range(1, steps + 1):
+        info = s.step(it)
+        if it > skip:
+            its.append(info["sor_exit_iter"])
+            used.append(info["warm_used"])
+    s.close()
+    return float(np.mean(its)), float(np.mean(used))
+
+
+def evaluate(net, n: int, steps: int, re: float = 1600.0):
+    cfg = tgv_config(n, steps, 3, re=re)
+    init = taylor_green(cfg)
+    net.resize(n, n, n)
+    res = {}
+    for name, warm, nn in (("previous", WARM_PREVIOUS, None),
+                           ("extrapolation", WARM_EXTRAPOLATE, None),
+                           ("learned", WARM_LEARNED, net)):
+        res[name] = sweeps_with(cfg, init, steps, warm, nn)
+        log(f"eval {n}^3 Re{re:g}: {name}: {res[name][0]:.1f} sweeps/step "
+            f"(guess accepted {res[name][1]:.2f})")
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--init", default="", help="start from these weights (.npy)")
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--eval-n", type=int, default=256)
+    ap.add_argument("--eval-steps", type=int, default=16)
+    a = ap.parse_args(argv)
+
+    import torch
+    if not torch.cuda.is_available():
+        sys.exit("train_warmstart needs a HIP device")
+    train, held = collect(a.n, a.samples)
+    log(f"{len(train)} training and {len(held)} held-out samples at {a.n}^3")
+    net = warmstart.Network(a.n, a.n, a.n, seed=a.seed)
+    if a.init:
+        net.load(a.init)
+    log(f"network: {net.nparam} parameters")
+    before = warmstart.relative_errors(net, held)
+    log(f"held-out relative error before: {before}")
+    t0 = time.perf_counter()
+    for ep in range(a.epochs):
+        lr = a.lr if ep < a.epochs - 1 else a.lr * 0.3
+        hist = warmstart.train(net, train, epochs=1, lr=lr, batch=a.batch, seed=a.seed + ep,
+                               log=log, log_every=50)
+        ms, fl = net.last_timing()
+        err = warmstart.relative_errors(net, held)
+        log(f"epoch {ep}: lr {lr:.1e} train mse {hist[-1]:.4e}, held-out {err}, "
+            f"forward {ms:.2f} ms = {fl / (ms * 1e-3) / 1e12:.1f} TFLOP/s, "
+            f"elapsed {time.perf_counter() - t0:.0f} s")
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    net.save(a.out)
+    log(f"saved {a.out}")
+    del train, held
+    torch.cuda.empty_cache()
+    if a.eval_n > 0:
+        evaluate(net, a.eval_n, a.eval_steps)
+    net.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -193,9 +193,10 @@ def relative_errors(net: Network, samples) -> Dict[str, float]:
 
 
 def train(net: Network, samples, epochs: int = 3, lr: float = 3e-4, batch: int = 4,
-          seed: int = 0, log=None) -> List[float]:
+          seed: int = 0, log=None, log_every: int = 0) -> List[float]:
     """Adam on the mean squared error, mini-batches of `batch` samples
-    (gradients accumulate across the batch)."""
+    (gradients accumulate across the batch).  With `log`, reports each epoch
+    and, when log_every > 0, every log_every mini-batches."""
     import torch
     rng = np.random.default_rng(seed)
     out = torch.empty_like(samples[0][1])
@@ -214,6 +215,10 @@ def train(net: Network, samples, epochs: int = 3, lr: float = 3e-4, batch: int =
                 tot += net.mse(out, tgt, dout, scale=1.0 / len(idx))
                 net.backward(dout)
             net.adam(lr)
+            nb = b0 // batch + 1
+            if log and log_every > 0 and nb % log_every == 0:
+                log(f"epoch {ep}: {b0 + len(idx)}/{len(order)} samples, "
+                    f"mse {tot / (b0 + len(idx)):.4e} ({time.perf_counter() - t0:.1f} s)")
         hist.append(tot / len(order))
         if log:
             log(f"epoch {ep}: mse {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
