@@ -63,6 +63,25 @@ def parse_args():
     return ap.parse_args()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sor_kernel_tgv256_pmc.md")
+
+
+def measured_traffic(which):
+    """HBM bytes per SOR launch from the committed rocprofv3 --pmc summary
+    (lines "TRAFFIC f32 <bytes>" / "TRAFFIC f64 <bytes>": FETCH_SIZE +
+    WRITE_SIZE per dispatch, scaled by the same counters' reading for a copy
+    of known size), or None when there is no such summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                m = re.match(r"^TRAFFIC (f32|f64) ([0-9.eE+]+)", line.strip())
+                if m and m.group(1) == which:
+                    return float(m.group(2))
+    except OSError:
+        pass
+    return None
+
+
 def pct(v, q):
     v = sorted(v)
     if not v:
@@ -287,12 +306,15 @@ def main():
             r["speedup_vs_fp64"] = r["steps_per_s"] / ref["steps_per_s"]
             r["kinetic_energy_rel_err_vs_fp64"] = abs(r["kinetic_energy"] - ref["kinetic_energy"]) / abs(ref["kinetic_energy"])
             r["enstrophy_rel_err_vs_fp64"] = abs(r["enstrophy"] - ref["enstrophy"]) / abs(ref["enstrophy"])
-    dom = head.get("sor_f32") if head_form != "fp64" else head.get("sor_f64")
+    which = "f32" if head_form != "fp64" else "f64"
+    dom = head.get("sor_" + which)
     roofline = None
     if dom:
-        roofline = {"bound": "hbm", "kernel": "sor_kernel (red-black sweep)",
+        roofline = {"bound": "hbm", "kernel": f"sor_kernel (red-black iteration, {which})",
                     "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": dom["achieved_GBps"] / HBM_PEAK_GBS, "traffic": None,
+                    "frac": dom["achieved_GBps"] / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(which),
+                    "algorithmic_bytes": dom["algorithmic_bytes_per_launch"],
                     "frac_of_measured_copy": dom["achieved_GBps"] / copy_gbs,
                     "measured_copy_GBps": copy_gbs}
     cpu = None

@@ -56,74 +56,106 @@ struct ConvArgs {
   int accumulate;     // y += result (FP32 gradient buffers)
 };
 
-// One wave computes 16 consecutive voxels (linear order, so coarse levels
-// with nx < 16 work too) for 16*NT output channels.
-template <typename TO, int NT>
+// One wave computes 16*MV consecutive voxels (linear order, so coarse levels
+// with nx < 16 work too) for 16*NT output channels: MV x NT MFMA tiles share
+// each loaded A fragment across NT output tiles and each B fragment across MV
+// voxel tiles.
+template <typename TO, int NT, int MV>
 __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int V = a.nx * a.ny * a.nz;
-  const int vbase = (blockIdx.x * 4 + wv) * 16;
+  const int vbase = (blockIdx.x * 4 + wv) * 16 * MV;
   if (vbase >= V) return;  // no barriers in this kernel
   const int co0 = blockIdx.y * 16 * NT;
   const int g = lane >> 4, l16 = lane & 15;
-  f32x4 acc[NT];
+  f32x4 acc[MV][NT];
 #pragma unroll
-  for (int n = 0; n < NT; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int m = 0; m < MV; ++m)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ktot = 27 * a.ci;
   const int nch = a.kp >> 5;
-  // A-operand voxel of this lane.
-  const int vm = vbase + l16;
-  const bool vin = vm < V;
-  const int xm = vm % a.nx, ym = (vm / a.nx) % a.ny, zm = vm / (a.nx * a.ny);
+  // A-operand voxels of this lane, one per voxel tile.
+  int xm[MV], ym[MV], zm[MV];
+  bool vin[MV];
+#pragma unroll
+  for (int m = 0; m < MV; ++m) {
+    const int vm = vbase + 16 * m + l16;
+    vin[m] = vm < V;
+    xm[m] = vm % a.nx;
+    ym[m] = (vm / a.nx) % a.ny;
+    zm[m] = vm / (a.nx * a.ny);
+  }
   const bf16 *wrow[NT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) wrow[n] = a.w + (size_t)(co0 + 16 * n + l16) * a.kp;
   for (int kc = 0; kc < nch; ++kc) {
     const int k0 = (kc << 5) + (g << 3);
-    bf16x8 av;
-    if (k0 < ktot && vin) {
-      const int t = k0 >> a.lci;
-      const int c0 = k0 & (a.ci - 1);
-      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-      const int vx = wrapi(xm + dx, a.nx), vy = wrapi(ym + dy, a.ny), vz = wrapi(zm + dz, a.nz);
-      av = *(const bf16x8 *)(a.x + (size_t)(vx + a.nx * (vy + a.ny * vz)) * a.ci + c0);
-    } else {
+    bf16x8 bv[NT];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) av[q] = (bf16)0.f;
+    for (int n = 0; n < NT; ++n) bv[n] = *(const bf16x8 *)(wrow[n] + k0);
+    const bool kin = k0 < ktot;
+    const int t = k0 >> a.lci;
+    const int c0 = k0 & (a.ci - 1);
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+#pragma unroll
+    for (int m = 0; m < MV; ++m) {
+      bf16x8 av;
+      if (kin && vin[m]) {
+        const int vx = wrapi(xm[m] + dx, a.nx), vy = wrapi(ym[m] + dy, a.ny),
+                  vz = wrapi(zm[m] + dz, a.nz);
+        av = *(const bf16x8 *)(a.x + (size_t)(vx + a.nx * (vy + a.ny * vz)) * a.ci + c0);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) av[q] = (bf16)0.f;
+      }
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[m][n], 0, 0, 0);
     }
+  }
+#pragma unroll
+  for (int m = 0; m < MV; ++m)
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      const bf16x8 bv = *(const bf16x8 *)(wrow[n] + k0);
-      acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[n], 0, 0, 0);
-    }
-  }
+      const int c = co0 + 16 * n + l16;
+      const float b = a.bias ? a.bias[c] : 0.f;
 #pragma unroll
-  for (int n = 0; n < NT; ++n) {
-    const int c = co0 + 16 * n + l16;
-    const float b = a.bias ? a.bias[c] : 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int vox = vbase + 4 * g + i;
-      if (vox >= V) continue;
-      const size_t o = (size_t)vox * a.co + c;
-      float v = acc[n][i] + b;
-      if (a.skip) v += (float)a.skip[o];
-      if (a.act) v = act_f(v);
-      if (a.accumulate) v += (float)a.y[o];
-      a.y[o] = (TO)v;
+      for (int i = 0; i < 4; ++i) {
+        const int vox = vbase + 16 * m + 4 * g + i;
+        if (vox >= V) continue;
+        const size_t o = (size_t)vox * a.co + c;
+        float v = acc[m][n][i] + b;
+        if (a.skip) v += (float)a.skip[o];
+        if (a.act) v = act_f(v);
+        if (a.accumulate) v += (float)a.y[o];
+        a.y[o] = (TO)v;
+      }
     }
+}
+
+template <typename TO, int NT>
+hipError_t launch_conv_nt(const ConvArgs<TO> &a, int V, hipStream_t s) {
+  // Four voxel tiles per wave once the grid still has >= 2048 blocks.
+  constexpr int kMV = 4;
+  const int gy = a.co / (16 * NT);
+  if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 2048) {
+    dim3 grid((V + 64 * kMV - 1) / (64 * kMV), gy);
+    conv_kernel<TO, NT, kMV><<<grid, 256, 0, s>>>(a);
+  } else {
+    dim3 grid((V + 63) / 64, gy);
+    conv_kernel<TO, NT, 1><<<grid, 256, 0, s>>>(a);
   }
+  return hipGetLastError();
 }
 
 template <typename TO>
 hipError_t launch_conv(const ConvArgs<TO> &a, hipStream_t s) {
   const int V = a.nx * a.ny * a.nz;
   const int nt = a.co >= 64 ? 4 : (a.co >= 32 ? 2 : 1);
-  dim3 grid((V + 63) / 64, a.co / (16 * nt));
-  if (nt == 4) conv_kernel<TO, 4><<<grid, 256, 0, s>>>(a);
-  else if (nt == 2) conv_kernel<TO, 2><<<grid, 256, 0, s>>>(a);
-  else conv_kernel<TO, 1><<<grid, 256, 0, s>>>(a);
-  return hipGetLastError();
+  if (nt == 4) return launch_conv_nt<TO, 4>(a, V, s);
+  if (nt == 2) return launch_conv_nt<TO, 2>(a, V, s);
+  return launch_conv_nt<TO, 1>(a, V, s);
 }
 
 // Weight gradient: dW[co][tap*ci + c] = sum_v dz[v][co] * x[v + off(tap)][c],

@@ -1,0 +1,83 @@
+"""Taylor-Green vortex, Re 1600, 256^3, AB3, 1000 steps on the GPU.
+
+At this size the FP64 form is the oracle of the kinetic-energy and
+enstrophy series (a 1000-step run of the reference on one CPU core would
+take about 16 hours at its measured ~57 s per step); the FP64 form's
+operators are bit-identical to the reference (tested at small sizes) and its
+Poisson solve stops on the reference's rule.  The mixed-precision form and
+the warm-started forms must track the FP64 series within 1e-6 relative at
+every sample, taken every 25 steps as the reference's statistics are.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from osinco3d_amd.config import tgv_config
+from osinco3d_amd.solver import (FP64, MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREVIOUS,
+                                 Solver)
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS, EVERY = 256, 1000, 25
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "osinco3d_amd", "warmstart_weights.npy")
+
+
+def series(precision, warm=WARM_PREVIOUS, net=None):
+    cfg = tgv_config(N, STEPS, 3)
+    s = Solver(cfg, precision, warm)
+    if net is not None:
+        net.attach(s)
+    s.init_taylor_green()
+    ek, ens, iters = [], [], []
+    for it in range(1, STEPS + 1):
+        info = s.step(it)
+        assert info["sor_reason"] == 0 and not info["diverged"], (it, info["sor_reason"])
+        iters.append(info["sor_exit_iter"])
+        if it % EVERY == 0:
+            st = s.statistics(it * s.dt)
+            ek.append(st[1])
+            ens.append(st[4])
+        if it % 250 == 0:
+            print(f"[long] precision {precision} warm {warm}: step {it}, "
+                  f"E_k {st[1]:.10e}, iters {np.mean(iters[-250:]):.1f}", flush=True)
+    s.close()
+    return np.array(ek), np.array(ens), np.array(iters)
+
+
+@pytest.fixture(scope="module")
+def fp64_series():
+    return series(FP64)
+
+
+def _check(fp64_series, other, label):
+    ek0, en0, it0 = fp64_series
+    ek, en, it = other
+    rel_ek = np.abs(ek - ek0) / np.abs(ek0)
+    rel_en = np.abs(en - en0) / np.abs(en0)
+    print(f"[long] {label}: max rel E_k {rel_ek.max():.2e}, enstrophy {rel_en.max():.2e}; "
+          f"SOR iterations/step {it.mean():.1f} (FP64 {it0.mean():.1f})", flush=True)
+    assert rel_ek.max() <= 1e-6
+    assert rel_en.max() <= 1e-6
+
+
+def test_mixed_tracks_fp64(fp64_series):
+    _check(fp64_series, series(MIXED), "mixed")
+
+
+def test_extrapolated_warm_start_tracks_fp64(fp64_series):
+    _check(fp64_series, series(MIXED, WARM_EXTRAPOLATE), "mixed + extrapolation")
+
+
+def test_learned_warm_start_tracks_fp64(fp64_series):
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained warm-start weights in the package")
+    from osinco3d_amd import warmstart
+    net = warmstart.load_or_none(WEIGHTS, (N, N, N))
+    try:
+        _check(fp64_series, series(MIXED, WARM_LEARNED, net), "mixed + learned")
+    finally:
+        net.close()
