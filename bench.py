@@ -11,6 +11,7 @@ Each requested form is timed over the same steps:
   fp64   plain FP64 (the drop-in; bit-identical operators to the reference)
   mixed  FP32 stencils / AB update / SOR sweeps, FP64-refined Poisson
   extrap mixed + warm start by linear extrapolation p_prev + (p_prev - p_prev2)
+  extrap2 mixed + quadratic extrapolation p_prev + 2 e_prev - e_prev2
   nn     mixed + learned warm start (the trained network of
          osinco3d_amd/warmstart_weights.npy; skipped when it is missing)
 The JSON line's value is the steps/s of the headline form (the last of
@@ -95,7 +96,8 @@ def run_form(form, cfg, steps, warmup, model_path):
     from osinco3d_amd import solver as S
 
     prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = {"nn": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE}.get(form, S.WARM_PREVIOUS)
+    warm = {"nn": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE,
+            "extrap2": S.WARM_EXTRAPOLATE2}.get(form, S.WARM_PREVIOUS)
     net = None
     if form == "nn":
         from osinco3d_amd import warmstart
@@ -159,7 +161,7 @@ def run_form(form, cfg, steps, warmup, model_path):
             gbs = bpc * n / (avg_ms * 1e-3) / 1e9
             out[name] = {"avg_launch_us": 1e3 * avg_ms, "launches_timed": sor_sw[q],
                          "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs}
-    if form in ("nn", "extrap"):
+    if form in ("nn", "extrap", "extrap2"):
         out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)
         out["warm_residual_l2"] = {"previous": statistics.mean(i["warm_dmax"][0] for i in infos),
                                    "guess": statistics.mean(i["warm_dmax"][1] for i in infos)}

@@ -123,6 +123,7 @@ int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
 #define O3D_WARM_PREVIOUS 0 /* previous step's pressure (the reference's behaviour) */
 #define O3D_WARM_LEARNED 1  /* learned correction, previous pressure as fallback */
 #define O3D_WARM_EXTRAPOLATE 2 /* p_prev + (p_prev - p_prev2), previous pressure as fallback */
+#define O3D_WARM_EXTRAPOLATE2 3 /* quadratic: p_prev + 2 e_prev - e_prev2, same fallback */
 
 typedef struct o3d_params {
   int nx, ny, nz;
@@ -136,7 +137,7 @@ typedef struct o3d_params {
   double omega, eps;
   int kmax, idyn;
   int precision;        /* O3D_FP64 or O3D_MIXED */
-  int warm_start;       /* O3D_WARM_PREVIOUS, O3D_WARM_LEARNED or O3D_WARM_EXTRAPOLATE */
+  int warm_start;       /* O3D_WARM_PREVIOUS, _LEARNED, _EXTRAPOLATE or _EXTRAPOLATE2 */
   int inner_max;        /* O3D_MIXED: FP32 sweeps between FP64 refinements (0 = default) */
 } o3d_params;
 
@@ -182,18 +183,36 @@ void *o3d_solver_field(void *handle, int which);
  * them, per precision (index 0 FP64, 1 FP32). */
 int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2);
 /* Attach the learned warm start (an o3d_nn network, include/o3d_nn.h), or
- * NULL to detach.  Each step then forms, before the Poisson solve, the
- * residual r = rhs - L p_prev of the previous pressure and the previous
- * step's increment e_prev = p_prev - p_prev2, feeds the network the two
- * planes (r dx^2 and e_prev, each divided by its RMS), and offers
- * p_prev + RMS(e_prev) * output as the initial guess; the previous pressure
- * stays the initial guess unless the guess has the smaller L2 residual. */
+ * NULL to detach; the network is re-sized to the grid coarsened by
+ * O3D_WARM_COARSEN (every axis must be a multiple of 8 * O3D_WARM_COARSEN).
+ * Each step then forms, before the Poisson solve, the linear extrapolation
+ * p_prev + e_prev of the pressure (e_prev = p_prev - p_prev2, the previous
+ * step's increment) and its residual r = rhs - L (p_prev + e_prev), feeds the
+ * network their block averages (o3d_warm_features), and offers
+ * p_prev + e_prev + RMS(e_prev) * trilinear(output - input plane 1) as the
+ * initial guess; the previous pressure stays the initial guess unless the
+ * guess has the smaller L2 residual. */
 int o3d_solver_set_warm_model(void *handle, void *nn);
 /* Training data capture: when set, every step writes float32 planes of n
- * cells each: r_out = rhs - L p_prev (before the solve), eprev_out = the
- * previous step's pressure increment, e_out = this step's increment
- * p_new - p_prev.  NULL pointers disable the capture. */
+ * cells each: r_out = rhs - L (p_prev + e_prev) before the solve (rhs - L p_prev
+ * while no increment exists), eprev_out = the previous step's pressure
+ * increment e_prev, e_out = this step's increment p_new - p_prev.  NULL
+ * pointers disable the capture. */
 int o3d_solver_capture(void *handle, float *r_out, float *eprev_out, float *e_out);
+
+/* Coarsening factor of the learned warm start: the network runs on block
+ * averages over O3D_WARM_COARSEN^3 cells, and its correction is interpolated
+ * back trilinearly. */
+#define O3D_WARM_COARSEN 4
+/* The network's input and target for one captured step, as the solver forms
+ * them (device float32 arrays, n = nx*ny*nz fine cells, nc = n / 64 coarse
+ * cells, F = O3D_WARM_COARSEN): inp (2 nc) = block averages of
+ * r dx^2 F^2 / RMS(e_prev) and of e_prev / RMS(e_prev); tgt (nc, may be NULL,
+ * then e may be NULL) = block averages of e / RMS(e_prev); scale =
+ * RMS(e_prev).  r, e_prev, e as o3d_solver_capture writes them.  Returns 1,
+ * with nothing written, when RMS(e_prev) vanishes. */
+int o3d_warm_features(const float *r, const float *eprev, const float *e, int nx, int ny, int nz,
+                      double dx, float *inp, float *tgt, double *scale);
 
 #ifdef __cplusplus
 }

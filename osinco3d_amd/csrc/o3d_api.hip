@@ -693,9 +693,11 @@ struct Solver {
   // Learned warm start / capture state.
   double *Pstart = nullptr;    // pressure at the start of the solve (p_prev)
   float *Eprev = nullptr;      // previous step's increment
+  float *Eprev2 = nullptr;     // the one before
   float *Rl = nullptr;         // residual of p_prev (float32)
-  float *Feat = nullptr;       // network input, 2 planes
-  float *Out = nullptr;        // network output
+  float *Feat = nullptr;       // network input, 2 coarse planes
+  float *Out = nullptr;        // fine-grid scratch
+  float *Outc = nullptr;       // network output (coarse)
   double *npart = nullptr;     // reduction partials
   double *hnpart = nullptr;    // pinned copy
   int hist = 0;                // steps with an increment available
@@ -911,16 +913,81 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
 }
 
 // ---- learned warm start and training capture ------------------------------
-__global__ void lw_features_kernel(const float *r, const float *e, float *f, int n, float sr,
-                                   float se) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  f[i] = r[i] * sr;
-  f[n + i] = e[i] * se;
+// The network works on a grid O3D_WARM_COARSEN times coarser per axis: its
+// inputs are block averages of the fine planes, and its correction to the
+// extrapolation is interpolated back trilinearly, so the correction is smooth.
+// (At the near-optimal omega every SOR error mode decays at about the same
+// rate, and the dmax stop rule weighs high-frequency error most, so a guess
+// carrying BF16-level noise at the fine scale would cost sweeps.)
+__global__ void lw_pool_kernel(const float *__restrict__ in, float *__restrict__ out, int nx,
+                               int ny, int nz, int F, float scale) {
+  const int mx = nx / F, my = ny / F, mz = nz / F;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= mx * my * mz) return;
+  const int i0 = (I % mx) * F, j0 = ((I / mx) % my) * F, k0 = (I / (mx * my)) * F;
+  float s = 0.f;
+  for (int k = 0; k < F; ++k)
+    for (int j = 0; j < F; ++j) {
+      const float *row = in + (size_t)i0 + (size_t)nx * ((j0 + j) + (size_t)ny * (k0 + k));
+      for (int i = 0; i < F; ++i) s += row[i];
+    }
+  out[I] = s * (scale / (float)(F * F * F));
 }
+
+// Position of fine node i on the coarse grid of cell averages: lower cell and
+// weight of the upper one; periodic axes wrap, others clamp.
+__device__ __forceinline__ void lw_axis(int i, int F, int m, int per, int &c0, int &c1, float &w) {
+  const float X = ((float)i - 0.5f * (float)(F - 1)) / (float)F;
+  int I0 = (int)floorf(X);
+  w = X - (float)I0;
+  int I1 = I0 + 1;
+  if (per) {
+    I0 = I0 < 0 ? I0 + m : I0;
+    I1 = I1 >= m ? I1 - m : I1;
+  } else {
+    if (I0 < 0) { I0 = 0; w = 0.f; }
+    if (I1 >= m) { I1 = m - 1; w = I0 == m - 1 ? 0.f : w; }
+  }
+  c0 = I0;
+  c1 = I1;
+}
+
+// q = p + e_prev + s * trilinear(out - in1), out and in1 on the coarse grid.
+__global__ void lw_guess_coarse_kernel(const double *__restrict__ p, const float *__restrict__ ep,
+                                       const float *__restrict__ out,
+                                       const float *__restrict__ in1, double *__restrict__ q,
+                                       int nx, int ny, int nz, int F, int px, int py, int pz,
+                                       double s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= nx) return;
+  const int mx = nx / F, my = ny / F, mz = nz / F;
+  int x0, x1, y0, y1, z0, z1;
+  float wx, wy, wz;
+  lw_axis(i, F, mx, px, x0, x1, wx);
+  lw_axis(j, F, my, py, y0, y1, wy);
+  lw_axis(k, F, mz, pz, z0, z1, wz);
+  auto c = [&](int a, int b, int d) {
+    const size_t o = (size_t)a + (size_t)mx * (b + (size_t)my * d);
+    return out[o] - in1[o];
+  };
+  const float v =
+      (1.f - wz) * ((1.f - wy) * ((1.f - wx) * c(x0, y0, z0) + wx * c(x1, y0, z0)) +
+                    wy * ((1.f - wx) * c(x0, y1, z0) + wx * c(x1, y1, z0))) +
+      wz * ((1.f - wy) * ((1.f - wx) * c(x0, y0, z1) + wx * c(x1, y0, z1)) +
+            wy * ((1.f - wx) * c(x0, y1, z1) + wx * c(x1, y1, z1)));
+  const size_t o = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+  q[o] = p[o] + (double)ep[o] + s * (double)v;
+}
+
 __global__ void lw_guess_kernel(const double *p, const float *o, double *q, int n, double s) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) q[i] = p[i] + s * (double)o[i];
+}
+__global__ void lw_guess2_kernel(const double *p, const float *e1, const float *e2, double *q,
+                                 int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = p[i] + (2.0 * (double)e1[i] - (double)e2[i]);
 }
 __global__ void lw_increment_kernel(const double *p, const double *p0, float *e, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -937,29 +1004,77 @@ __global__ void __launch_bounds__(256) lw_sumsq_kernel(const float *a, int n, do
 
 constexpr int kLwBlocks = 512;
 
+// Partial-sum buffers of the warm-start reductions.
+struct LwSums {
+  double *part = nullptr;   // device, 2 * kLwBlocks
+  double *hpart = nullptr;  // pinned host copy
+};
+
+int lw_sums_alloc(LwSums &w) {
+  if (w.part) return 0;
+  HIPCHK(hipMalloc((void **)&w.part, 2 * kLwBlocks * sizeof(double)));
+  HIPCHK(hipHostMalloc((void **)&w.hpart, 2 * kLwBlocks * sizeof(double)));
+  return 0;
+}
+
+int lw_sumsq(LwSums &w, const float *a, int n, double *out) {
+  lw_sumsq_kernel<<<kLwBlocks, 256, 0, g_stream>>>(a, n, w.part);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(w.hpart, w.part, kLwBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                        g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  double s = 0.0;
+  for (int b = 0; b < kLwBlocks; ++b) s += w.hpart[b];
+  *out = s;
+  return 0;
+}
+
+// Network input (2 coarse planes: r dx^2 F^2 / RMS(e_prev) and
+// e_prev / RMS(e_prev), F the coarsening factor, so that the first plane is the
+// coarse grid's dimensionless residual in the target's units) and, when e is
+// given, the target (the coarse e / RMS(e_prev)).  Returns 1 (nothing
+// written) when RMS(e_prev) vanishes.
+int lw_features(LwSums &w, const float *r, const float *ep, const float *e, int nx, int ny,
+                int nz, double h2, float *inp, float *tgt, double *scale) {
+  const int n = nx * ny * nz;
+  const int F = O3D_WARM_COARSEN;
+  const int nc = n / (F * F * F);
+  double se2 = 0.0;
+  CHK(lw_sumsq(w, ep, n, &se2));
+  const double re = sqrt(se2 / n);
+  *scale = re;
+  if (!(re > 0.0)) return 1;
+  const int blocks = (nc + 255) / 256;
+  lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(r, inp, nx, ny, nz, F,
+                                               (float)(h2 * F * F / re));
+  lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(ep, inp + nc, nx, ny, nz, F, (float)(1.0 / re));
+  if (e && tgt)
+    lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(e, tgt, nx, ny, nz, F, (float)(1.0 / re));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int lw_alloc(Solver &S) {
   if (S.Pstart) return 0;
   const size_t n = S.n;
+  const size_t nc = n / (O3D_WARM_COARSEN * O3D_WARM_COARSEN * O3D_WARM_COARSEN);
   CHK(S.alloc(&S.Pstart, n));
   CHK(S.alloc(&S.Eprev, n));
+  CHK(S.alloc(&S.Eprev2, n));
   CHK(S.alloc(&S.Rl, n));
-  CHK(S.alloc(&S.Feat, 2 * n));
   CHK(S.alloc(&S.Out, n));
+  CHK(S.alloc(&S.Feat, 2 * nc));
+  CHK(S.alloc(&S.Outc, nc));
   CHK(S.alloc(&S.npart, 2 * kLwBlocks));
   HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
   return 0;
 }
 
-int lw_sumsq(Solver &S, const float *a, double *out) {
-  lw_sumsq_kernel<<<kLwBlocks, 256, 0, g_stream>>>(a, S.n, S.npart);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(S.hnpart, S.npart, kLwBlocks * sizeof(double), hipMemcpyDeviceToHost,
-                        g_stream));
-  HIPCHK(hipStreamSynchronize(g_stream));
-  double s = 0.0;
-  for (int b = 0; b < kLwBlocks; ++b) s += S.hnpart[b];
-  *out = s;
-  return 0;
+LwSums lw_of(Solver &S) {
+  LwSums w;
+  w.part = S.npart;
+  w.hpart = S.hnpart;
+  return w;
 }
 
 // L2 norm of the FP64 residual rhs - L p.
@@ -983,22 +1098,31 @@ int lw_residual_l2(Solver &S, const double *p, double *out) {
   return 0;
 }
 
-// Before the solve: keep p_prev, form r = rhs - L p_prev (float32).
+// Before the solve: keep p_prev; with an increment available, form the
+// extrapolated pressure p_prev + e_prev (in the spare pressure buffer) and its
+// residual r = rhs - L (p_prev + e_prev) in float32; on the first step r is
+// the residual of p_prev.
 int solver_learning_prepare(Solver &S) {
   CHK(lw_alloc(S));
   const int n = S.n;
   HIPCHK(hipMemcpyAsync(S.Pstart, S.P[S.cur], (size_t)n * 8, hipMemcpyDeviceToDevice,
                         g_stream));
+  const double *base = S.P[S.cur];
+  if (S.hist >= 1) {
+    lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.P[1 - S.cur],
+                                                           n, 1.0);
+    HIPCHK(hipGetLastError());
+    base = S.P[1 - S.cur];
+  }
   int per[3];
   solver_periodic(S.kind, per);
   // The residual kernel also zeroes its second output; Out is scratch here.
   if (S.mixed)
-    HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.Rl, S.Out, S.p.nx,
-                                      S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
-                                      g_stream));
+    HIPCHK(launch_residual_f32<float>(base, (const float *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
+                                      S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
   else
-    HIPCHK(launch_residual_f32<double>(S.P[S.cur], (const double *)S.rhs, S.Rl, S.Out, S.p.nx,
-                                       S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+    HIPCHK(launch_residual_f32<double>(base, (const double *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
+                                       S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
                                        g_stream));
   if (S.cap_r)
     HIPCHK(hipMemcpyAsync(S.cap_r, S.Rl, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));
@@ -1009,36 +1133,43 @@ int solver_learning_prepare(Solver &S) {
 }
 
 // Warm guess, accepted when its L2 residual is below that of p_prev:
-// p_prev + s_e * NN(r dx^2 / rms, e_prev / rms) with a network attached,
-// else (O3D_WARM_EXTRAPOLATE) the linear extrapolation p_prev + e_prev.
+//  * a network attached: p_prev + e_prev + RMS(e_prev) * trilinear(NN(features)
+//    - features[1]), the network's coarse-grid correction of the linear
+//    extrapolation, from the extrapolation's residual and e_prev;
+//  * O3D_WARM_EXTRAPOLATE: p_prev + e_prev;
+//  * O3D_WARM_EXTRAPOLATE2: p_prev + 2 e_prev - e_prev2 (linear until two
+//    increments exist).
 int solver_warm_guess(Solver &S, int *used, double *warm) {
   *used = 0;
   warm[0] = warm[1] = 0.0;
   if (S.hist < 1) return 0;
   const int n = S.n;
-  const float *inc = S.Eprev;
-  double scale = 1.0;
+  double *q = S.P[1 - S.cur];
   if (S.nn) {
-    double sr2 = 0.0, se2 = 0.0;
-    CHK(lw_sumsq(S, S.Rl, &sr2));
-    CHK(lw_sumsq(S, S.Eprev, &se2));
+    LwSums w = lw_of(S);
+    double re = 0.0;
     const double h2 = S.p.dx * S.p.dx;
-    const double rr = sqrt(sr2 / n) * h2, re = sqrt(se2 / n);
-    if (!(rr > 0.0) || !(re > 0.0)) return 0;
-    lw_features_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.Eprev, S.Feat, n,
-                                                              (float)(h2 / rr),
-                                                              (float)(1.0 / re));
-    HIPCHK(hipGetLastError());
-    CHK(o3d_nn_forward(S.nn, S.Feat, S.Out));
+    const int rc = lw_features(w, S.Rl, S.Eprev, nullptr, S.p.nx, S.p.ny, S.p.nz, h2, S.Feat,
+                               nullptr, &re);
+    if (rc < 0) return rc;
+    if (rc == 1) return 0;
+    CHK(o3d_nn_forward(S.nn, S.Feat, S.Outc));
     double fl = 0.0, ms = 0.0;
     CHK(o3d_nn_last_timing(S.nn, &ms, &fl));
     S.nn_ms += ms;
     S.nn_flops += fl;
-    inc = S.Out;
-    scale = re;
+    const int F = O3D_WARM_COARSEN;
+    const int nc = n / (F * F * F);
+    int per[3];
+    solver_periodic(S.kind, per);
+    dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
+    lw_guess_coarse_kernel<<<grid, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Outc,
+                                                       S.Feat + nc, q, S.p.nx, S.p.ny, S.p.nz,
+                                                       F, per[0], per[1], per[2], re);
+  } else if (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2) {
+    lw_guess2_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Eprev2, q, n);
   }
-  double *q = S.P[1 - S.cur];
-  lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], inc, q, n, scale);
+  // Otherwise q already holds p_prev + e_prev (solver_learning_prepare).
   HIPCHK(hipGetLastError());
   CHK(lw_residual_l2(S, S.P[S.cur], &warm[0]));
   CHK(lw_residual_l2(S, q, &warm[1]));
@@ -1052,6 +1183,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
 // After the solve: e = p_new - p_prev becomes the next step's e_prev.
 int solver_learning_finish(Solver &S) {
   const int n = S.n;
+  std::swap(S.Eprev, S.Eprev2);
   lw_increment_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n);
   HIPCHK(hipGetLastError());
   if (S.cap_e)
@@ -1070,7 +1202,8 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   if (p.iles == 1)
     HIPCHK(launch_finalize_partials(S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5, g_stream));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
-  const bool guess = S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE;
+  const bool guess =
+      S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE || p.warm_start == O3D_WARM_EXTRAPOLATE2;
   const bool learning = guess || S.cap_r || S.cap_eprev || S.cap_e;
   int warm = 0;
   if (learning) CHK(solver_learning_prepare(S));
@@ -1147,7 +1280,7 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
   }
   if (p->precision != O3D_FP64 && p->precision != O3D_MIXED)
     return fail("unknown precision form %d", p->precision);
-  if (p->warm_start < O3D_WARM_PREVIOUS || p->warm_start > O3D_WARM_EXTRAPOLATE)
+  if (p->warm_start < O3D_WARM_PREVIOUS || p->warm_start > O3D_WARM_EXTRAPOLATE2)
     return fail("unknown warm start %d", p->warm_start);
   const int kind = solver_from_bc(p->bc);
   if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");
@@ -1305,10 +1438,32 @@ extern "C" int o3d_solver_set_warm_model(void *handle, void *nn) {
   if (!handle) return fail("NULL handle");
   Solver &S = *(Solver *)handle;
   if (nn) {
-    CHK(o3d_nn_resize(nn, S.p.nx, S.p.ny, S.p.nz));
+    const int F = O3D_WARM_COARSEN;
+    if (S.p.nx % (8 * F) || S.p.ny % (8 * F) || S.p.nz % (8 * F))
+      return fail("learned warm start: every grid axis must be a multiple of %d (got %dx%dx%d)",
+                  8 * F, S.p.nx, S.p.ny, S.p.nz);
+    CHK(o3d_nn_resize(nn, S.p.nx / F, S.p.ny / F, S.p.nz / F));
   }
   S.nn = nn;
   return 0;
+}
+
+namespace {
+LwSums g_lw;  // reduction buffers of o3d_warm_features
+}
+
+extern "C" int o3d_warm_features(const float *r, const float *eprev, const float *e, int nx,
+                                 int ny, int nz, double dx, float *inp, float *tgt,
+                                 double *scale) {
+  const int F = O3D_WARM_COARSEN;
+  if (!r || !eprev || !inp || !scale) return fail("NULL argument");
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nx % F || ny % F || nz % F)
+    return fail("grid %dx%dx%d: every axis must be a positive multiple of %d", nx, ny, nz, F);
+  CHK(lw_sums_alloc(g_lw));
+  const int rc = lw_features(g_lw, r, eprev, e, nx, ny, nz, dx * dx, inp, tgt, scale);
+  if (rc < 0) return rc;
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return rc;
 }
 
 extern "C" int o3d_solver_capture(void *handle, float *r_out, float *eprev_out, float *e_out) {

@@ -7,8 +7,9 @@ Training pairs (inputs r, e_prev; target e, see warmstart.py) come from
 Taylor-Green runs at n^3 (periodic, AB3, CFL 0.25, SOR eps 1e-8, mixed
 precision form): Re 1600 over the vortex breakdown, Re 400, and Re 1600 with
 the axes permuted so the network does not learn one orientation.  Every
-sample stays resident in device memory (3 float32 planes of n^3 cells:
-24 MiB at 128^3, 48 GiB for 2000 samples).  The last tenth of each run is
+sample stays resident in device memory: the network's input and target, 3
+float32 planes on the grid coarsened by capi.WARM_COARSEN (384 KiB per
+sample at 128^3).  The last tenth of each run is
 held out to report the relative error of the predicted increment against the
 linear extrapolation e = e_prev.
 
@@ -24,9 +25,10 @@ import time
 
 import numpy as np
 
-from . import warmstart
+from . import capi, warmstart
 from .config import tgv_config
-from .solver import MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREVIOUS, Solver, taylor_green
+from .solver import (MIXED, WARM_EXTRAPOLATE, WARM_EXTRAPOLATE2, WARM_LEARNED, WARM_PREVIOUS,
+                     Solver, taylor_green)
 
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npy")
 
@@ -90,10 +92,10 @@ def sweeps_with(cfg, init, steps, warm, net=None, skip=3):
 def evaluate(net, n: int, steps: int, re: float = 1600.0):
     cfg = tgv_config(n, steps, 3, re=re)
     init = taylor_green(cfg)
-    net.resize(n, n, n)
     res = {}
     for name, warm, nn in (("previous", WARM_PREVIOUS, None),
                            ("extrapolation", WARM_EXTRAPOLATE, None),
+                           ("quadratic extrapolation", WARM_EXTRAPOLATE2, None),
                            ("learned", WARM_LEARNED, net)):
         res[name] = sweeps_with(cfg, init, steps, warm, nn)
         log(f"eval {n}^3 Re{re:g}: {name}: {res[name][0]:.1f} sweeps/step "
@@ -120,7 +122,8 @@ def main(argv=None):
         sys.exit("train_warmstart needs a HIP device")
     train, held = collect(a.n, a.samples)
     log(f"{len(train)} training and {len(held)} held-out samples at {a.n}^3")
-    net = warmstart.Network(a.n, a.n, a.n, seed=a.seed)
+    F = capi.WARM_COARSEN
+    net = warmstart.Network(a.n // F, a.n // F, a.n // F, seed=a.seed)
     if a.init:
         net.load(a.init)
     log(f"network: {net.nparam} parameters")

@@ -5,10 +5,12 @@ pressure r = rhs - L p_prev (times dx^2) and the previous increment
 e_prev = p_prev - p_prev2, each divided by its RMS, to the increment
 e = p_new - p_prev divided by RMS(e_prev); its output is a learned correction
 added to the second plane, so the untrained network starts from the linear
-extrapolation e = e_prev.  Training pairs come from the
-solver's own runs (o3d_solver_capture): every step of a run yields
-(r, e_prev, e) with p_new the SOR solution to eps.  The solver applies the
-same normalisation at inference time (o3d_solver_set_warm_model).
+extrapolation e = e_prev.  All three planes are block averages over
+WARM_COARSEN^3 cells: the network runs on the coarse grid and the solver
+interpolates its correction back.  Training pairs come from the solver's own
+runs (o3d_solver_capture): every step of a run yields (r, e_prev, e) with
+p_new the SOR solution to eps, reduced to the network's input and target by
+the same library call the solver uses (o3d_warm_features).
 
 Device memory is plumbing through torch tensors; all compute is libo3d.so.
 """
@@ -123,26 +125,36 @@ class Network:
 
 
 def load_or_none(path: str, shape=(256, 256, 256)) -> Optional[Network]:
+    """The trained network for a solver grid `shape` (nx, ny, nz), or None
+    when `path` does not exist.  The network itself runs on the grid
+    coarsened by capi.WARM_COARSEN."""
     if not os.path.exists(path):
         return None
-    net = Network(*shape)
+    F = capi.WARM_COARSEN
+    net = Network(*[s // F for s in shape])
     net.load(path)
     return net
 
 
 # ------------------------------------------------------------- training data
 def normalize(r, eprev, e, dx: float):
-    """Network input and target with the solver's normalisation
-    (o3d_api.hip, solver_warm_learned): returns (inp [2, V], target [V],
-    scale) or None when e_prev vanishes."""
+    """Network input and target as the solver forms them (o3d_warm_features):
+    returns (inp [2, nc], target [nc], scale) on the coarse grid, or None
+    when e_prev vanishes.  r, eprev, e: float32 device planes of shape
+    (nz, ny, nx) or flat with `dims` set."""
     import torch
-    rr = torch.sqrt(torch.mean(r.double() ** 2)).item() * dx * dx
-    re = torch.sqrt(torch.mean(eprev.double() ** 2)).item()
-    if not (rr > 0 and re > 0):
+    nz, ny, nx = r.shape
+    F = capi.WARM_COARSEN
+    nc = (nx // F) * (ny // F) * (nz // F)
+    inp = torch.empty(2, nc, dtype=torch.float32, device=r.device)
+    tgt = torch.empty(nc, dtype=torch.float32, device=r.device)
+    scale = C.c_double(0.0)
+    rc = capi.check(capi.load().o3d_warm_features(
+        capi.ptr(r), capi.ptr(eprev), capi.ptr(e), nx, ny, nz, C.c_double(dx), capi.ptr(inp),
+        capi.ptr(tgt), C.byref(scale)))
+    if rc == 1:
         return None
-    inp = torch.stack([r * (dx * dx / rr), eprev / re]).float().contiguous()
-    tgt = (e / re).float().contiguous()
-    return inp.reshape(2, -1), tgt.reshape(-1), re
+    return inp, tgt, scale.value
 
 
 def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 2,
@@ -155,8 +167,7 @@ def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 
         s.upload(*taylor_green(cfg))
     else:
         s.upload(*init)
-    n = cfg.nx * cfg.ny * cfg.nz
-    r = torch.empty(n, dtype=torch.float32, device="cuda")
+    r = torch.empty((cfg.nz, cfg.ny, cfg.nx), dtype=torch.float32, device="cuda")
     ep = torch.empty_like(r)
     e = torch.empty_like(r)
     capi.check(capi.load().o3d_solver_capture(s.handle, capi.ptr(r), capi.ptr(ep), capi.ptr(e)))
@@ -170,7 +181,7 @@ def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 
             z = normalize(r, ep, e, dx)
             if z is not None:
                 inp, tgt, _ = z
-                samples.append((inp.clone(), tgt.clone()))
+                samples.append((inp, tgt))
     capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
     s.close()
     return samples, iters

@@ -111,16 +111,18 @@ def test_adam_matches_closed_form(torch):
 
 
 def test_training_reduces_loss_and_warm_start_runs(torch):
-    """Train on a short captured TGV run at 32^3, then use the network as the
-    solver's warm start: every step still converges to eps, and the fields
-    match the run without it to a tolerance tied to eps."""
+    """Train on a short captured TGV run at 64^3 (the network runs on the
+    16^3 coarse grid), then use the network as the solver's warm start: every
+    step still converges to eps, and the fields match the run without it to a
+    tolerance tied to eps."""
     from osinco3d_amd import warmstart as W
     from osinco3d_amd.config import tgv_config
     from osinco3d_amd.solver import FP64, MIXED, Solver, taylor_green
-    cfg = tgv_config(32, 40, 3)
+    cfg = tgv_config(64, 40, 3)
     samples, iters = W.capture_run(cfg, 30, MIXED)
     assert len(samples) >= 20
-    net = W.Network(32, 32, 32, seed=7)
+    assert tuple(samples[0][0].shape) == (2, 16 ** 3)
+    net = W.Network(16, 16, 16, seed=7)
     before = W.relative_errors(net, samples)
     hist = W.train(net, samples, epochs=4, lr=3e-4, batch=4)
     after = W.relative_errors(net, samples)
@@ -144,3 +146,32 @@ def test_training_reduces_loss_and_warm_start_runs(torch):
     ga, gb = a.download(), b.download()
     for c in range(3):
         assert np.abs(ga[c] - gb[c]).max() < 1e-6
+
+
+def test_warm_features_are_block_averages(torch):
+    """o3d_warm_features: the network's input and target are block averages
+    over WARM_COARSEN^3 cells of r dx^2 / RMS(r dx^2), e_prev / RMS(e_prev) and
+    e / RMS(e_prev), with the RMS over the fine grid in FP64."""
+    from osinco3d_amd import capi
+    from osinco3d_amd.warmstart import normalize
+    rng = np.random.default_rng(5)
+    sh = (16, 24, 32)                      # (nz, ny, nx)
+    r, ep, e = [rng.standard_normal(sh).astype(np.float32) for _ in range(3)]
+    dx = 0.1
+    inp, tgt, scale = normalize(*[torch.from_numpy(a).cuda() for a in (r, ep, e)], dx)
+    F = capi.WARM_COARSEN
+
+    def pool(a):
+        a = a.astype(np.float64)
+        return a.reshape(sh[0] // F, F, sh[1] // F, F, sh[2] // F, F).mean(axis=(1, 3, 5))
+
+    rr = np.sqrt(np.mean(r.astype(np.float64) ** 2)) * dx * dx
+    re = np.sqrt(np.mean(ep.astype(np.float64) ** 2))
+    assert abs(scale - re) <= 1e-12 * re
+    got = inp.cpu().numpy()
+    np.testing.assert_allclose(got[0], (pool(r) * dx * dx / rr).reshape(-1), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(got[1], (pool(ep) / re).reshape(-1), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(tgt.cpu().numpy(), (pool(e) / re).reshape(-1), rtol=1e-5,
+                               atol=1e-6)
+    z = torch.zeros(sh, dtype=torch.float32, device="cuda")
+    assert normalize(z, z, z, dx) is None
