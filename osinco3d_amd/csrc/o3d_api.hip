@@ -479,7 +479,7 @@ int poisson_dev(double *pp, const double *rhs, double dx, double dy, double dz, 
   a.log_len = 0;
   a.nx = nx; a.ny = ny; a.nz = nz;
   solver_periodic(solver, a.per);
-  a.tl = make_sor_tiling(nx, ny, nz, sor_slots<double, double>());
+  a.tl = sor_tiling<double, double>(nx, ny, nz);
   const SorConst c = sor_const(dx, dy, dz);
   a.ox = c.ox; a.oy = c.oy; a.oz = c.oz; a.A = c.A;
   a.eps = eps;
@@ -811,7 +811,7 @@ void sor_args_common(Solver &S, SorArgs<T, TR> &a, const TR *rhs, SorState *st, 
   a.log_len = 0;
   a.nx = S.p.nx; a.ny = S.p.ny; a.nz = S.p.nz;
   solver_periodic(S.kind, a.per);
-  a.tl = make_sor_tiling(S.p.nx, S.p.ny, S.p.nz, sor_slots<T, TR>());
+  a.tl = sor_tiling<T, TR>(S.p.nx, S.p.ny, S.p.nz);
   a.ox = S.sc.ox; a.oy = S.sc.oy; a.oz = S.sc.oz; a.A = S.sc.A;
   a.eps = S.p.eps;
   a.factor = S.kind == SOR_0011 ? 1.01 : 1.05;

@@ -36,10 +36,13 @@ struct SorTiling {
   int ntx, nty, ntz, zc;
   int blocks() const { return ntx * nty * ntz; }
 };
-SorTiling make_sor_tiling(int nx, int ny, int nz, int slots);
+SorTiling make_sor_tiling(int nx, int ny, int nz, int slots, int zmult);
 // Concurrent blocks of the sweep kernel on this device (occupancy x CUs).
 template <typename T, typename TR>
 int sor_slots();
+// The tiling sor_kernel<T, TR> needs: z chunks a multiple of its ring depth.
+template <typename T, typename TR>
+SorTiling sor_tiling(int nx, int ny, int nz);
 
 template <typename T, typename TR>
 struct SorArgs {

@@ -39,7 +39,7 @@ static_assert(R2N <= 2 * SNT && R1N <= 2 * SNT, "two cells per thread at most");
 // grid should fill the machine in whole rounds of `slots` concurrent blocks
 // (a partial last round leaves CUs idle).  Pick the chunk count with the
 // least rounds * (zc + 3).
-SorTiling make_sor_tiling(int nx, int ny, int nz, int slots) {
+SorTiling make_sor_tiling(int nx, int ny, int nz, int slots, int zmult) {
   SorTiling t;
   t.ntx = (nx + SX - 1) / SX;
   t.nty = (ny + SY - 1) / SY;
@@ -48,7 +48,7 @@ SorTiling make_sor_tiling(int nx, int ny, int nz, int slots) {
   double best = 1e30;
   int best_ntz = 1;
   for (int ntz = 1; ntz <= std::max(1, nz / 8); ++ntz) {
-    const int zc = (nz + ntz - 1) / ntz;
+    const int zc = ((nz + ntz - 1) / ntz + zmult - 1) / zmult * zmult;
     const int n_t = (nz + zc - 1) / zc;
     const int blocks = layer * n_t;
     const int rounds = (blocks + slots - 1) / slots;
@@ -57,7 +57,7 @@ SorTiling make_sor_tiling(int nx, int ny, int nz, int slots) {
     const double cost = (double)rounds * (zc + 3.0) * slots / ((double)nz * layer);
     if (cost < best - 1e-9) { best = cost; best_ntz = ntz; }
   }
-  t.zc = (nz + best_ntz - 1) / best_ntz;
+  t.zc = ((nz + best_ntz - 1) / best_ntz + zmult - 1) / zmult * zmult;
   t.ntz = (nz + t.zc - 1) / t.zc;
   return t;
 }
@@ -80,6 +80,14 @@ int sor_slots() {
     slots = std::max(1, per_cu * cus);
   }
   return slots;
+}
+
+template <typename T>
+struct SorShape;
+
+template <typename T, typename TR>
+SorTiling sor_tiling(int nx, int ny, int nz) {
+  return make_sor_tiling(nx, ny, nz, sor_slots<T, TR>(), SorShape<T>::NO);
 }
 
 // Neighbour index for the SOR rules: periodic wrap, or the free-slip mirror
@@ -149,12 +157,18 @@ __device__ __forceinline__ void unroll_steps(std::integer_sequence<int, K...>, F
 // PD steps before their planes are published to LDS, so each block keeps
 // PD*PPS planes of loads in flight across the barriers.
 // Ring depths: phase 2 of a step reads old planes s-1..s+PPS and rhs planes
-// s-1..s+PPS-1, phase 3 reads red planes s-2..s+PPS-1.
+// s-1..s+PPS-1, phase 3 reads red planes s-2..s+PPS-1.  Each ring's depth
+// divides the planes per loop iteration (PD*PPS), so a plane's slot depends
+// only on its position inside the unrolled iteration: slots are loop
+// invariants, not per-step arithmetic.
 template <typename T>
 struct SorShape {
   static constexpr int PPS = sizeof(T) == 4 ? 2 : 1;
   static constexpr int PD = sizeof(T) == 4 ? 2 : 3;
-  static constexpr int NO = PPS + 2, NRD = PPS + 2, NRH = PPS + 1;
+  static constexpr int NO = sizeof(T) == 4 ? 4 : 3;
+  static constexpr int NRD = NO, NRH = NO;
+  static_assert(NO >= PPS + 2 && NRH >= PPS + 1, "ring too shallow");
+  static_assert((PD * PPS) % NO == 0, "ring depth must divide the planes per iteration");
 };
 
 // Ring slot of plane q (q >= -2).
@@ -164,15 +178,37 @@ __device__ __forceinline__ int slot(int q) {
   else return (q + 4 * N) % N;
 }
 
+// Raw buffer loads: resource in SGPRs, a 32-bit per-lane column offset and a
+// uniform plane offset, so the march needs no per-lane 64-bit address
+// arithmetic.  Word 3 of the gfx9 buffer descriptor: raw 32-bit data.
+constexpr int kBufWord3 = 0x00020000;
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *p, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, n * (int)sizeof(T), kBufWord3);
+}
+
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 template <typename T, typename TR>
 __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
   constexpr int PPS = SorShape<T>::PPS;
   constexpr int PD = SorShape<T>::PD;
   constexpr int NO = SorShape<T>::NO, NRD = SorShape<T>::NRD, NRH = SorShape<T>::NRH;
-  // LDS rings: old iterate on R2, new red values and rhs on R1.
-  __shared__ T oldr[NO][R2N];
+  // LDS rings: old iterate on R2, new red values and rhs on R1.  In FP32 the
+  // old and rhs planes are padded to 2*SNT entries so every thread stores
+  // its second cell unconditionally (no divergent branch per plane).
+  constexpr bool PAD = false;
+  constexpr int R2S = PAD ? 2 * SNT : R2N, R1S = PAD ? 2 * SNT : R1N;
+  __shared__ T oldr[NO][R2S];
   __shared__ T redr[NRD][R1N];
-  __shared__ TR rhsr[NRH][R1N];
+  __shared__ TR rhsr[NRH][R1S];
   __shared__ double scr[SNW];
   SorState *st = a.st;
   if (st->done) return;
@@ -240,51 +276,73 @@ __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
   const int b2i = (ty + 2) * R2W + tx + 2;
   const int bpar = (bi + bj) & 1;
 
-  auto gz_of = [&](int zp) { return plane * sor_ghost(zp, nz, a.per[2]); };
-
+  // Byte offset of plane zp >= 0 (the march never goes below z0 - 2; only
+  // the upper ghost rule can apply inside the loop), for old and rhs.
+  const int perz = a.per[2];
+  auto zq = [&](int zp) {
+    const int hi = zp > nz + 1 ? nz - 1 : (perz ? zp - nz : 2 * (nz - 1) - zp);
+    return (unsigned)(plane * (zp < nz ? zp : hi));
+  };
+  auto zq_lo = [&](int zp) { return (unsigned)(plane * sor_ghost(zp, nz, perz)); };
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(a.pin, plane * nz);
+  const __amdgpu_buffer_rsrc_t rrh = buf_rsrc(a.rhs, plane * nz);
+  const unsigned o2a = (unsigned)col2a * sizeof(T), o2b = (unsigned)col2b * sizeof(T);
+  const unsigned o1a = (unsigned)col1a * sizeof(TR), o1b = (unsigned)col1b * sizeof(TR);
+  auto ld_old = [&](unsigned o, unsigned zoff) { return bload<T>(rin, o, zoff * sizeof(T)); };
+  auto ld_rhs = [&](unsigned o, unsigned zoff) { return bload<TR>(rrh, o, zoff * sizeof(TR)); };
+  
   // Prefetch registers: step k of the march publishes old planes
   // s_k+1 .. s_k+PPS and rhs planes s_k .. s_k+PPS-1, s_k = z0-1 + k*PPS.
   T po[PD][PPS][2];
   TR pr[PD][PPS][2];
   {
-    const int g2 = gz_of(z0 - 2), g1 = gz_of(z0 - 1);
-    oldr[slot<NO>(z0 - 2)][tid] = a.pin[col2a + g2];
-    oldr[slot<NO>(z0 - 1)][tid] = a.pin[col2a + g1];
+    const unsigned g2 = zq_lo(z0 - 2), g1 = zq_lo(z0 - 1);
+    oldr[slot<NO>(-2)][tid] = ld_old(o2a, g2);
+    oldr[slot<NO>(-1)][tid] = ld_old(o2a, g1);
     if (v2b) {
-      oldr[slot<NO>(z0 - 2)][c2b] = a.pin[col2b + g2];
-      oldr[slot<NO>(z0 - 1)][c2b] = a.pin[col2b + g1];
+      oldr[slot<NO>(-2)][c2b] = ld_old(o2b, g2);
+      oldr[slot<NO>(-1)][c2b] = ld_old(o2b, g1);
     }
 #pragma unroll
     for (int k = 0; k < PD; ++k)
 #pragma unroll
       for (int q = 0; q < PPS; ++q) {
         const int sk = z0 - 1 + k * PPS + q;
-        const int go = gz_of(sk + 1), gr = gz_of(sk);
-        po[k][q][0] = a.pin[col2a + go];
-        po[k][q][1] = a.pin[col2b + go];
-        pr[k][q][0] = a.rhs[col1a + gr];
-        pr[k][q][1] = a.rhs[col1b + gr];
+        const unsigned go = zq_lo(sk + 1), gr = zq_lo(sk);
+        po[k][q][0] = ld_old(o2a, go);
+        po[k][q][1] = ld_old(o2b, go);
+        pr[k][q][0] = ld_rhs(o1a, gr);
+        pr[k][q][1] = ld_rhs(o1b, gr);
       }
   }
 
-  double dmax = 0.0;
+  // Largest un-relaxed update, in the sweep's own precision (exact: the
+  // maximum of |pn - pc| values that are themselves values of type T).
+  T dmax = T(0);
+  auto upd = [&](T d) { dmax = d > dmax ? d : dmax; };
   auto step = [&](int s, auto K) {
     constexpr int k = decltype(K)::value;
+    // Slots of planes s + j: s = z0 - 1 + k*PPS + (a multiple of every
+    // depth) and z0 is a multiple of every depth (sor_tiling), so the slots
+    // are compile-time constants.
+    auto so = [&](int j) { return slot<NO>(-1 + k * PPS + j); };
+    auto sd = [&](int j) { return slot<NRD>(-1 + k * PPS + j); };
+    auto sh = [&](int j) { return slot<NRH>(-1 + k * PPS + j); };
     // Phase 1: publish old(s+1..s+PPS) and rhs(s..s+PPS-1); prefetch the
     // planes PD steps ahead into the same registers.
 #pragma unroll
     for (int q = 0; q < PPS; ++q) {
-      T *od = oldr[slot<NO>(s + 1 + q)];
-      TR *rd = rhsr[slot<NRH>(s + q)];
+      T *od = oldr[so(1 + q)];
+      TR *rd = rhsr[sh(q)];
       od[tid] = po[k][q][0];
       rd[tid] = pr[k][q][0];
-      if (v2b) od[c2b] = po[k][q][1];
-      if (v1b) rd[c2b] = pr[k][q][1];
-      const int go = gz_of(s + 1 + q + PD * PPS), gr = gz_of(s + q + PD * PPS);
-      po[k][q][0] = a.pin[col2a + go];
-      po[k][q][1] = a.pin[col2b + go];
-      pr[k][q][0] = a.rhs[col1a + gr];
-      pr[k][q][1] = a.rhs[col1b + gr];
+      if (PAD || v2b) od[c2b] = po[k][q][1];
+      if (PAD || v1b) rd[c2b] = pr[k][q][1];
+      const unsigned go = zq(s + 1 + q + PD * PPS), gr = zq(s + q + PD * PPS);
+      po[k][q][0] = ld_old(o2a, go);
+      po[k][q][1] = ld_old(o2b, go);
+      pr[k][q][0] = ld_rhs(o1a, gr);
+      pr[k][q][1] = ld_rhs(o1b, gr);
     }
     __syncthreads();
     // Phase 2: red (colour 0) on planes s..s+PPS-1 over R1 from old values
@@ -300,39 +358,37 @@ __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
         const int X = x0 - 1 + a1, Y = y0 - 1 + rb1;
         const int i2 = (rb1 + 1) * R2W + a1 + 1;
         const int i1 = rb1 * R1W + a1;
-        const T *o0 = oldr[slot<NO>(sr)];
-        const T *om_ = oldr[slot<NO>(sr - 1)];
-        const T *op = oldr[slot<NO>(sr + 1)];
+        const T *o0 = oldr[so(q)];
+        const T *om_ = oldr[so(q - 1)];
+        const T *op = oldr[so(q + 1)];
         const T pc = o0[i2];
         const T pn = sor_gs<T, TR>(o0[i2 - 1], o0[i2 + 1], o0[i2 - R2W], o0[i2 + R2W], om_[i2],
-                                   op[i2], rhsr[slot<NRH>(sr)][i1], ox, oy, oz, A, invA);
-        redr[slot<NRD>(sr)][i1] = w1 * pc + w * pn;
-        if (a1 >= 1 && a1 <= SX && rb1 >= 1 && rb1 <= SY && X < nx && Y < ny && sr >= z0 &&
-            sr < z1)
-          dmax = fmax(dmax, (double)fabs(pn - pc));
+                                   op[i2], rhsr[sh(q)][i1], ox, oy, oz, A, invA);
+        redr[sd(q)][i1] = w1 * pc + w * pn;
+        const bool own = a1 >= 1 && a1 <= SX && rb1 >= 1 && rb1 <= SY && X < nx && Y < ny;
+        if (sr >= z0 && sr < z1) upd(own ? T(fabs(pn - pc)) : T(0));
       }
-      bold[q] = oldr[slot<NO>(sr - 1)][b2i];
-      brhs[q] = rhsr[slot<NRH>(sr - 1)][b1i];
+      bold[q] = oldr[so(q - 1)][b2i];
+      brhs[q] = rhsr[sh(q - 1)][b1i];
     }
     __syncthreads();
     // Phase 3: black (colour 1) on planes s-1..s+PPS-2 over the tile.
 #pragma unroll
     for (int q = 0; q < PPS; ++q) {
       const int sb = s - 1 + q;
-      if (sb >= z0 && sb < z1 && bown) {
-        const T *rm = redr[slot<NRD>(sb - 1)];
-        const T *r0 = redr[slot<NRD>(sb)];
-        const T *rp = redr[slot<NRD>(sb + 1)];
-        T v;
-        if (((bpar + sb) & 1) != 0) {
-          const T pn = sor_gs<T, TR>(r0[b1i - 1], r0[b1i + 1], r0[b1i - R1W], r0[b1i + R1W],
-                                     rm[b1i], rp[b1i], brhs[q], ox, oy, oz, A, invA);
-          v = w1 * bold[q] + w * pn;
-          dmax = fmax(dmax, (double)fabs(pn - bold[q]));
-        } else {
-          v = r0[b1i];
-        }
-        a.pout[bi + nx * bj + plane * sb] = v;
+      if (sb >= z0 && sb < z1) {
+        // Every lane evaluates the black update (branch-free); red lanes
+        // keep their new red value.
+        const T *rm = redr[sd(q - 2)];
+        const T *r0 = redr[sd(q - 1)];
+        const T *rp = redr[sd(q)];
+        const T c = r0[b1i];
+        const T pn = sor_gs<T, TR>(r0[b1i - 1], r0[b1i + 1], r0[b1i - R1W], r0[b1i + R1W],
+                                   rm[b1i], rp[b1i], brhs[q], ox, oy, oz, A, invA);
+        const bool blk = ((bpar + sb) & 1) != 0;
+        const T v = blk ? w1 * bold[q] + w * pn : c;
+        upd(blk && bown ? T(fabs(pn - bold[q])) : T(0));
+        if (bown) (a.pout + (size_t)plane * sb)[(unsigned)(bi + nx * bj)] = v;
       }
     }
   };
@@ -342,13 +398,18 @@ __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
       if (decltype(K)::value == 0 || sk <= z1) step(sk, K);
     });
   // Block max, one atomic per block.
-  dmax = block_max<SNW>(dmax, scr);
-  if (threadIdx.x == 0 && dmax != 0.0)
-    atomicMax(&st->dmax_bits[it & 3], (unsigned long long)__double_as_longlong(dmax));
+  const double dm = block_max<SNW>((double)dmax, scr);
+  if (threadIdx.x == 0 && dm != 0.0)
+    atomicMax(&st->dmax_bits[it & 3], (unsigned long long)__double_as_longlong(dm));
 }
 
 template <typename T, typename TR>
 hipError_t launch_sor_iteration(const SorArgs<T, TR> &a, hipStream_t s) {
+  // The kernel's ring slots assume z chunks that are a multiple of the ring
+  // depth, and its buffer descriptors address at most INT_MAX bytes.
+  if (a.tl.zc % SorShape<T>::NO != 0 ||
+      (long long)a.nx * a.ny * a.nz * (long long)sizeof(T) > 0x7fffffffLL)
+    return hipErrorInvalidValue;
   sor_kernel<T, TR><<<dim3(a.tl.blocks()), dim3(SNT), 0, s>>>(a);
   return hipGetLastError();
 }
@@ -497,6 +558,9 @@ hipError_t launch_fill_f32(float *dst, float v, int n, hipStream_t s) {
 template int sor_slots<double, double>();
 template int sor_slots<double, float>();
 template int sor_slots<float, float>();
+template SorTiling sor_tiling<double, double>(int, int, int);
+template SorTiling sor_tiling<double, float>(int, int, int);
+template SorTiling sor_tiling<float, float>(int, int, int);
 template hipError_t launch_sor_iteration<double, double>(const SorArgs<double, double> &,
                                                          hipStream_t);
 template hipError_t launch_sor_iteration<double, float>(const SorArgs<double, float> &,
