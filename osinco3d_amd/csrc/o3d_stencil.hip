@@ -220,9 +220,11 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
   FieldPartial nut_p;
   partial_init(nut_p);
 
-  T q[3][7];
+  // Column queue of planes k-3..k+3 (slots 0..6) plus slot 7, the plane
+  // k+4 whose load is issued one step before it is needed.
+  T q[3][8];
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
+  for (int s = 0; s < 7; ++s) {
     bool fz;
     const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
     q[0][s] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
@@ -234,11 +236,12 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
 
   for (int k = P.z0; k < P.z1; ++k) {
     {
+      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
       bool fz;
-      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
-      q[0][6] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
-      q[1][6] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
-      q[2][6] = load_ghost(A.u[2], col + plane * kk, fz);
+      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
+      q[0][7] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+      q[1][7] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+      q[2][7] = load_ghost(A.u[2], col + plane * kk, fz);
     }
     const int pk = plane * k;
     T hx[3], hy[3];
@@ -262,6 +265,13 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
       const int r = ty + 3, cc = tx + 3;
+      // History loads issued before the stencil arithmetic that hides them.
+      T h2[3], h3[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (NH >= 2) h2[c] = A.f2[c][idx];
+        if (NH >= 3) h3[c] = A.f3[c][idx];
+      }
       T g[3][3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -299,8 +309,8 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
         // src/integration.f90:129-134 (and :149-154, :169-174)
         const T F = nu_eff * (lx + ly + lz) - (ux * g[c][0] + uy * g[c][1] + uz * g[c][2]);
         T un = q[c][3] + A.adu * F;
-        if (NH >= 2) un = un + A.bdu * A.f2[c][idx];
-        if (NH >= 3) un = un + A.cdu * A.f3[c][idx];
+        if (NH >= 2) un = un + A.bdu * h2[c];
+        if (NH >= 3) un = un + A.cdu * h3[c];
         A.f1[c][idx] = F;
         A.up[c][idx] = un;
       }
@@ -309,7 +319,7 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
-      for (int s = 0; s < 6; ++s) q[c][s] = q[c][s + 1];
+      for (int s = 0; s < 7; ++s) q[c][s] = q[c][s + 1];
     }
   }
   if (LES) partial_store(nut_p, A.nut_part + blockIdx.x, sc);

@@ -176,12 +176,15 @@ def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 
     iters = []
     for it in range(1, steps + 1):
         info = s.step(it)
+        if info["diverged"] or info["has_nan"] or info["sor_reason"] == 2:
+            break   # the run left the regime the warm start is meant for
         iters.append(info["sor_exit_iter"])
         if it > skip and (it - skip) % keep_every == 0:
             z = normalize(r, ep, e, dx)
             if z is not None:
                 inp, tgt, _ = z
-                samples.append((inp, tgt))
+                if bool(torch.isfinite(inp).all()) and bool(torch.isfinite(tgt).all()):
+                    samples.append((inp, tgt))
     capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
     s.close()
     return samples, iters

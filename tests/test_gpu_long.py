@@ -35,7 +35,9 @@ def series(precision, warm=WARM_PREVIOUS, net=None):
     ek, ens, iters = [], [], []
     for it in range(1, STEPS + 1):
         info = s.step(it)
-        assert info["sor_reason"] == 0 and not info["diverged"], (it, info["sor_reason"])
+        # eps reached, or the reference's stagnation exit (it prints "No
+        # converged" and the run goes on, src/poisson.f90:111-114); never kmax.
+        assert info["sor_reason"] in (0, 1) and not info["diverged"], (it, info["sor_reason"])
         iters.append(info["sor_exit_iter"])
         if it % EVERY == 0:
             st = s.statistics(it * s.dt)

@@ -150,8 +150,9 @@ def test_training_reduces_loss_and_warm_start_runs(torch):
 
 def test_warm_features_are_block_averages(torch):
     """o3d_warm_features: the network's input and target are block averages
-    over WARM_COARSEN^3 cells of r dx^2 / RMS(r dx^2), e_prev / RMS(e_prev) and
-    e / RMS(e_prev), with the RMS over the fine grid in FP64."""
+    over F^3 cells (F = WARM_COARSEN) of r dx^2 F^2 / RMS(e_prev),
+    e_prev / RMS(e_prev) and e / RMS(e_prev), with the RMS over the fine grid
+    in FP64."""
     from osinco3d_amd import capi
     from osinco3d_amd.warmstart import normalize
     rng = np.random.default_rng(5)
@@ -165,11 +166,11 @@ def test_warm_features_are_block_averages(torch):
         a = a.astype(np.float64)
         return a.reshape(sh[0] // F, F, sh[1] // F, F, sh[2] // F, F).mean(axis=(1, 3, 5))
 
-    rr = np.sqrt(np.mean(r.astype(np.float64) ** 2)) * dx * dx
     re = np.sqrt(np.mean(ep.astype(np.float64) ** 2))
     assert abs(scale - re) <= 1e-12 * re
     got = inp.cpu().numpy()
-    np.testing.assert_allclose(got[0], (pool(r) * dx * dx / rr).reshape(-1), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(got[0], (pool(r) * dx * dx * F * F / re).reshape(-1), rtol=1e-5,
+                               atol=1e-6)
     np.testing.assert_allclose(got[1], (pool(ep) / re).reshape(-1), rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(tgt.cpu().numpy(), (pool(e) / re).reshape(-1), rtol=1e-5,
                                atol=1e-6)
