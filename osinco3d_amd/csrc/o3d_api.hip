@@ -698,10 +698,11 @@ struct Solver {
   float *Feat = nullptr;       // network input, 2 coarse planes
   float *Out = nullptr;        // fine-grid scratch
   float *Outc = nullptr;       // network output (coarse)
-  double *npart = nullptr;     // reduction partials
+  double *npart = nullptr;     // reduction partials (two residual norms)
   double *hnpart = nullptr;    // pinned copy
+  double *epart = nullptr;     // sum-of-squares partials of e_prev
+  double *hepart = nullptr;    // pinned copy, valid after the step's final sync
   int hist = 0;                // steps with an increment available
-  double nn_ms = 0.0, nn_flops = 0.0;
   std::vector<void *> allocs;
 
   size_t esz() const { return mixed ? 4 : 8; }
@@ -728,6 +729,7 @@ struct Solver {
     if (hfin) (void)hipHostFree(hfin);
     if (hst) (void)hipHostFree(hst);
     if (hnpart) (void)hipHostFree(hnpart);
+    if (hepart) (void)hipHostFree(hepart);
   }
 };
 
@@ -1035,12 +1037,13 @@ int lw_sumsq(LwSums &w, const float *a, int n, double *out) {
 // given, the target (the coarse e / RMS(e_prev)).  Returns 1 (nothing
 // written) when RMS(e_prev) vanishes.
 int lw_features(LwSums &w, const float *r, const float *ep, const float *e, int nx, int ny,
-                int nz, double h2, float *inp, float *tgt, double *scale) {
+                int nz, double h2, float *inp, float *tgt, double *scale,
+                double se2_known = -1.0) {
   const int n = nx * ny * nz;
   const int F = O3D_WARM_COARSEN;
   const int nc = n / (F * F * F);
-  double se2 = 0.0;
-  CHK(lw_sumsq(w, ep, n, &se2));
+  double se2 = se2_known;
+  if (se2 < 0.0) CHK(lw_sumsq(w, ep, n, &se2));
   const double re = sqrt(se2 / n);
   *scale = re;
   if (!(re > 0.0)) return 1;
@@ -1065,8 +1068,10 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.Out, n));
   CHK(S.alloc(&S.Feat, 2 * nc));
   CHK(S.alloc(&S.Outc, nc));
-  CHK(S.alloc(&S.npart, 2 * kLwBlocks));
-  HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
+  CHK(S.alloc(&S.npart, 4 * kLwBlocks));
+  HIPCHK(hipHostMalloc((void **)&S.hnpart, 4 * kLwBlocks * sizeof(double)));
+  CHK(S.alloc(&S.epart, kLwBlocks));
+  HIPCHK(hipHostMalloc((void **)&S.hepart, kLwBlocks * sizeof(double)));
   return 0;
 }
 
@@ -1077,24 +1082,30 @@ LwSums lw_of(Solver &S) {
   return w;
 }
 
-// L2 norm of the FP64 residual rhs - L p.
-int lw_residual_l2(Solver &S, const double *p, double *out) {
+// L2 norms of the FP64 residuals rhs - L p of two fields, one host sync.
+int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *out) {
   int per[3];
   solver_periodic(S.kind, per);
-  if (S.mixed)
-    HIPCHK(launch_residual_norm<float>(p, (const float *)S.rhs, S.npart, kLwBlocks, S.p.nx,
-                                       S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
-                                       g_stream));
-  else
-    HIPCHK(launch_residual_norm<double>(p, (const double *)S.rhs, S.npart, kLwBlocks, S.p.nx,
-                                        S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
-                                        g_stream));
-  HIPCHK(hipMemcpyAsync(S.hnpart, S.npart, 2 * kLwBlocks * sizeof(double),
+  const double *pp[2] = {p0, p1};
+  for (int q = 0; q < 2; ++q) {
+    double *part = S.npart + 2 * kLwBlocks * q;
+    if (S.mixed)
+      HIPCHK(launch_residual_norm<float>(pp[q], (const float *)S.rhs, part, kLwBlocks, S.p.nx,
+                                         S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz,
+                                         S.sc.A, g_stream));
+    else
+      HIPCHK(launch_residual_norm<double>(pp[q], (const double *)S.rhs, part, kLwBlocks,
+                                          S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy,
+                                          S.sc.oz, S.sc.A, g_stream));
+  }
+  HIPCHK(hipMemcpyAsync(S.hnpart, S.npart, 4 * kLwBlocks * sizeof(double),
                         hipMemcpyDeviceToHost, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
-  double s = 0.0;
-  for (int b = 0; b < kLwBlocks; ++b) s += S.hnpart[2 * b + 1];
-  *out = sqrt(s);
+  for (int q = 0; q < 2; ++q) {
+    double s = 0.0;
+    for (int b = 0; b < kLwBlocks; ++b) s += S.hnpart[2 * kLwBlocks * q + 2 * b + 1];
+    out[q] = sqrt(s);
+  }
   return 0;
 }
 
@@ -1149,15 +1160,15 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     LwSums w = lw_of(S);
     double re = 0.0;
     const double h2 = S.p.dx * S.p.dx;
+    // Sum of squares of e_prev: reduced when e_prev was formed, read back by
+    // the previous step's final synchronisation.
+    double se2 = 0.0;
+    for (int b = 0; b < kLwBlocks; ++b) se2 += S.hepart[b];
     const int rc = lw_features(w, S.Rl, S.Eprev, nullptr, S.p.nx, S.p.ny, S.p.nz, h2, S.Feat,
-                               nullptr, &re);
+                               nullptr, &re, se2);
     if (rc < 0) return rc;
     if (rc == 1) return 0;
     CHK(o3d_nn_forward(S.nn, S.Feat, S.Outc));
-    double fl = 0.0, ms = 0.0;
-    CHK(o3d_nn_last_timing(S.nn, &ms, &fl));
-    S.nn_ms += ms;
-    S.nn_flops += fl;
     const int F = O3D_WARM_COARSEN;
     const int nc = n / (F * F * F);
     int per[3];
@@ -1171,8 +1182,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
   }
   // Otherwise q already holds p_prev + e_prev (solver_learning_prepare).
   HIPCHK(hipGetLastError());
-  CHK(lw_residual_l2(S, S.P[S.cur], &warm[0]));
-  CHK(lw_residual_l2(S, q, &warm[1]));
+  CHK(lw_residual_l2_pair(S, S.P[S.cur], q, warm));
   if (warm[1] < warm[0]) {
     S.cur = 1 - S.cur;
     *used = 1;
@@ -1185,7 +1195,10 @@ int solver_learning_finish(Solver &S) {
   const int n = S.n;
   std::swap(S.Eprev, S.Eprev2);
   lw_increment_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n);
+  lw_sumsq_kernel<<<kLwBlocks, 256, 0, g_stream>>>(S.Eprev, n, S.epart);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(S.hepart, S.epart, kLwBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                        g_stream));
   if (S.cap_e)
     HIPCHK(hipMemcpyAsync(S.cap_e, S.Eprev, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));
   S.hist++;
