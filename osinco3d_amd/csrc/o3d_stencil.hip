@@ -77,9 +77,18 @@ __device__ __forceinline__ T d1(T m3, T m2, T m1, T p1, T p2, T p3, const T *c) 
   return c[0] * (p3 - m3) - c[1] * (p2 - m2) + c[2] * (p1 - m1);
 }
 // 4th-order second derivative: -a*(f-2 + f+2) + b*(f-1 + f+1) - c*f.
+// FP64 keeps the reference's expression (bit-identical results).  In FP32
+// the rounded coefficients no longer satisfy c = 2b - 2a exactly, so that
+// form would give a constant field a second derivative of ~1e-7 c f, a
+// bias that accumulates over thousands of steps; the FP32 form uses the
+// equivalent differences -a((f-2 - f) + (f+2 - f)) + b((f-1 - f) + (f+1 - f)),
+// which vanish exactly on constants and do not cancel catastrophically.
 template <typename T>
 __device__ __forceinline__ T d2(T m2, T m1, T z, T p1, T p2, const T *c) {
-  return -c[0] * (m2 + p2) + c[1] * (m1 + p1) - c[2] * z;
+  if constexpr (sizeof(T) == 4)
+    return -c[0] * ((m2 - z) + (p2 - z)) + c[1] * ((m1 - z) + (p1 - z));
+  else
+    return -c[0] * (m2 + p2) + c[1] * (m1 + p1) - c[2] * z;
 }
 
 // ------------------------------------------------------------ generic D1/D2

@@ -108,7 +108,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--samples", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default="", help="start from these weights (.npy)")
@@ -129,6 +129,9 @@ def main(argv=None):
     log(f"network: {net.nparam} parameters")
     before = warmstart.relative_errors(net, held)
     log(f"held-out relative error before: {before}")
+    # Model selection on the held-out error: the saved weights are those of
+    # the best epoch (the initial, extrapolation-like network included).
+    best_err, best_params, best_ep = before["nn"], net.params(), -1
     t0 = time.perf_counter()
     for ep in range(a.epochs):
         lr = a.lr if ep < a.epochs - 1 else a.lr * 0.3
@@ -136,12 +139,16 @@ def main(argv=None):
                                log=log, log_every=50)
         ms, fl = net.last_timing()
         err = warmstart.relative_errors(net, held)
+        if np.isfinite(err["nn"]) and err["nn"] < best_err:
+            best_err, best_params, best_ep = err["nn"], net.params(), ep
         log(f"epoch {ep}: lr {lr:.1e} train mse {hist[-1]:.4e}, held-out {err}, "
             f"forward {ms:.2f} ms = {fl / (ms * 1e-3) / 1e12:.1f} TFLOP/s, "
             f"elapsed {time.perf_counter() - t0:.0f} s")
+    net.set_params(best_params)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     net.save(a.out)
-    log(f"saved {a.out}")
+    log(f"saved {a.out} (epoch {best_ep}, held-out relative error {best_err:.4f}, "
+        f"extrapolation {before['extrapolation']:.4f})")
     del train, held
     torch.cuda.empty_cache()
     if a.eval_n > 0:
