@@ -1,17 +1,19 @@
 """Train the warm-start network of the Poisson solve on the solver's own runs.
 
     python -m osinco3d_amd.train_warmstart [--n 128] [--samples 2000]
-        [--epochs 4] [--out osinco3d_amd/warmstart_weights.npy] [--eval-n 256]
+        [--epochs 12] [--out osinco3d_amd/warmstart_weights.npy] [--eval-n 256]
 
-Training pairs (inputs r, e_prev; target e, see warmstart.py) come from
+Training pairs (inputs: the residual of the extrapolated pressure and e_prev;
+target: the increment e; see warmstart.py) come from
 Taylor-Green runs at n^3 (periodic, AB3, CFL 0.25, SOR eps 1e-8, mixed
 precision form): Re 1600 over the vortex breakdown, Re 400, and Re 1600 with
 the axes permuted so the network does not learn one orientation.  Every
 sample stays resident in device memory: the network's input and target, 3
 float32 planes on the grid coarsened by capi.WARM_COARSEN (384 KiB per
-sample at 128^3).  The last tenth of each run is
-held out to report the relative error of the predicted increment against the
-linear extrapolation e = e_prev.
+sample at 128^3).  Every tenth step of each run is held out to report the
+relative error of the predicted increment against the linear extrapolation
+e = e_prev, and the saved weights are those of the epoch with the smallest
+held-out error.
 
 Evaluation: the same solver steps with the previous-pressure, the
 extrapolated and the learned warm starts, reporting SOR sweeps per step.
@@ -66,9 +68,8 @@ def collect(n: int, total: int):
     for label, cfg, init, steps in runs(n, total):
         t0 = time.perf_counter()
         samples, iters = warmstart.capture_run(cfg, steps + 2, MIXED, init=init)
-        cut = int(len(samples) * 0.9)
-        train += samples[:cut]
-        held += samples[cut:]
+        train += [x for i, x in enumerate(samples) if i % 10 != 9]
+        held += samples[9::10]
         log(f"{label}: {len(samples)} samples in {time.perf_counter() - t0:.1f} s, "
             f"SOR sweeps/step mean {np.mean(iters):.1f}")
     return train, held
@@ -107,7 +108,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
