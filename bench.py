@@ -18,10 +18,11 @@ The JSON line's value is the steps/s of the headline form (the last of
 --forms); every form's figures are included under "forms".
 
 roofline: the dominant kernel of the headline form is the red-black SOR
-sweep.  Its algorithmic bytes per launch are (read p, read rhs, write p) =
+kernel.  Its algorithmic bytes per launch are (read p, read rhs, write p) =
 3 values per cell: 24 B/cell in FP64, 12 B/cell in FP32, times the cell
-count.  Its average launch duration is measured with HIP events on the
-launch stream over complete sweep batches inside the timed steps.
+count; the FP32 kernel runs two iterations per launch for that traffic.  Its
+average launch duration is measured with HIP events on the launch stream
+over complete batches of launches inside the timed steps.
 
 cpu_baseline: the reference Fortran (oracle/_ref/o3d_ref_run, built from the
 reference sources at -O3, single-threaded) on one host core: a bounded sample
@@ -117,13 +118,15 @@ def run_form(form, cfg, steps, warmup, model_path):
     infos = []
     sor_ms = [0.0, 0.0]
     sor_sw = [0, 0]
+    sor_ln = [0, 0]
     nn_ms, nn_fl, nn_n = 0.0, 0.0, 0
     for k in range(1, steps + 1):
         infos.append(s.step(warmup + k))
-        ms, sw = s.sor_timing()
+        ms, sw, ln = s.sor_timing(launches=True)
         for q in range(2):
             sor_ms[q] += ms[q]
             sor_sw[q] += sw[q]
+            sor_ln[q] += ln[q]
         if net is not None:
             a, b = net.last_timing()
             nn_ms += a
@@ -154,12 +157,17 @@ def run_form(form, cfg, steps, warmup, model_path):
         "enstrophy": st[4],
         "t_end": t_end,
     }
-    # SOR sweep kernel: average launch duration from events over full batches.
+    # SOR kernel: average launch duration from events over complete batches.
+    # One launch reads the iterate and the rhs and writes the new iterate
+    # once (24 B/cell FP64, 12 B/cell FP32), whether it runs one iteration
+    # (sor_kernel) or two (the FP32 pair kernel).
     for q, name, bpc in ((0, "sor_f64", 24), (1, "sor_f32", 12)):
-        if sor_sw[q] > 0:
-            avg_ms = sor_ms[q] / sor_sw[q]
+        if sor_ln[q] > 0:
+            avg_ms = sor_ms[q] / sor_ln[q]
             gbs = bpc * n / (avg_ms * 1e-3) / 1e9
-            out[name] = {"avg_launch_us": 1e3 * avg_ms, "launches_timed": sor_sw[q],
+            out[name] = {"avg_launch_us": 1e3 * avg_ms, "launches_timed": sor_ln[q],
+                         "iterations_per_launch": sor_sw[q] / sor_ln[q],
+                         "us_per_iteration": 1e3 * sor_ms[q] / sor_sw[q],
                          "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs}
     if form in ("nn", "extrap", "extrap2"):
         out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)

@@ -85,6 +85,15 @@ int o3d_poisson(double *pp, const double *rhs, double dx, double dy, double dz, 
                 int nz, double *omega, double eps, int kmax, int idyn, int solver,
                 int *exit_iter, double *dmax, int *reason);
 
+/* The red-black SOR of o3d_poisson in FP32 (the sweeps of the mixed form),
+ * same rules and outputs.  pair = 1 runs two iterations per kernel launch
+ * (every axis >= 8): the iterates are bit-identical to pair = 0, the stop
+ * rules are checked after every second iteration.  No counterpart in the
+ * reference (its solver is FP64). */
+int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz, int nx, int ny,
+                    int nz, double *omega, double eps, int kmax, int idyn, int solver, int pair,
+                    int *exit_iter, double *dmax, int *reason);
+
 /* correct_pression, src/integration.f90:199-255 (SOR branch): divergence of
  * the predicted velocity (odd variants), rhs = div/dt, then the Poisson solver
  * that schemes() binds for bc (src/initialization.f90:283-301). */
@@ -179,9 +188,11 @@ int o3d_solver_statistics(void *handle, double t, double *out17);
  * (p is float64 in both forms). */
 void *o3d_solver_field(void *handle, int which);
 /* Time of SOR sweeps measured with HIP events on the launch stream over the
- * last step: total ms of complete sweep batches and the number of sweeps in
- * them, per precision (index 0 FP64, 1 FP32). */
-int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2);
+ * last step: total ms of complete sweep batches, the number of sweeps
+ * (iterations) and of kernel launches in them, per precision (index 0 FP64,
+ * 1 FP32; the FP32 correction sweeps run two iterations per launch when
+ * every axis has at least 8 nodes).  launches2 may be NULL. */
+int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches2);
 /* Attach the learned warm start (an o3d_nn network, include/o3d_nn.h), or
  * NULL to detach; the network is re-sized to the grid coarsened by
  * O3D_WARM_COARSEN (every axis must be a multiple of 8 * O3D_WARM_COARSEN).

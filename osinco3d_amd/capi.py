@@ -73,7 +73,7 @@ class O3DStepInfo(C.Structure):
 EXPORTS = [
     "o3d_init", "o3d_set_stream", "o3d_last_error", "o3d_sync", "o3d_version",
     "o3d_der", "o3d_der_f32", "o3d_divergence", "o3d_nu_t", "o3d_predict_velocity",
-    "o3d_poisson", "o3d_correct_pression", "o3d_correct_velocity", "o3d_function_stats",
+    "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity", "o3d_function_stats",
     "o3d_statistics", "o3d_solver_create", "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
@@ -197,6 +197,20 @@ def poisson(pp, rhs, d, omega, eps, kmax, idyn, solver):
     check(load().o3d_poisson(ptr(pp), ptr(rhs), *map(_d, d), nx, ny, nz, C.byref(om), _d(eps),
                              int(kmax), int(idyn), int(code), C.byref(it), C.byref(dm),
                              C.byref(rs)))
+    return it.value, dm.value, rs.value, om.value
+
+
+def poisson_f32(pp, rhs, d, omega, eps, kmax, idyn, solver, pair=False):
+    """FP32 red-black SOR; returns (exit_iter, dmax, reason, omega)."""
+    nx, ny, nz = _dims(pp)
+    om = C.c_double(omega)
+    it = C.c_int(0)
+    dm = C.c_double(0)
+    rs = C.c_int(0)
+    code = {"0000": 0, "0011": 1, "111111": 2}.get(solver, solver)
+    check(load().o3d_poisson_f32(ptr(pp), ptr(rhs), *map(_d, d), nx, ny, nz, C.byref(om),
+                                 _d(eps), int(kmax), int(idyn), int(code), int(bool(pair)),
+                                 C.byref(it), C.byref(dm), C.byref(rs)))
     return it.value, dm.value, rs.value, om.value
 
 

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -193,32 +194,49 @@ SorConst sor_const(double dx, double dy, double dz) {
 }
 
 // Batched launcher of red-black iterations with device-side stop control.
-// Buffers: buf[0] holds the start iterate; iteration it reads buf[(it-1)&1].
+// Buffers: buf[0] holds the start iterate; launch L reads buf[L & 1] and
+// writes buf[(L + 1) & 1].  With `pair` (FP32 only) each launch runs two
+// iterations (launch_sor2_iteration).
 template <typename T, typename TR>
 struct SorRun {
   SorArgs<T, TR> a;
   SorState *st = nullptr;    // device
   SorState *hst = nullptr;   // pinned host copy
   T *buf[2] = {nullptr, nullptr};
-  // Timing of complete batches (all launches did a sweep).
+  bool pair = false;
+  // Timing of complete batches (all launches did their iterations).
   double ms_full = 0.0;
   int sweeps_full = 0;
+  int launches_full = 0;
   std::vector<hipEvent_t> *events = nullptr;
+
+  hipError_t launch() {
+    if constexpr (std::is_same<T, float>::value && std::is_same<TR, float>::value) {
+      if (pair) return launch_sor2_iteration(a, g_stream);
+    }
+    return launch_sor_iteration<T, TR>(a, g_stream);
+  }
+
+  // Buffer holding the result after `sweeps` iterations.
+  T *result(int sweeps) const { return buf[(pair ? sweeps / 2 : sweeps) & 1]; }
 
   int run(double omega, int first_batch, int *sweeps_out) {
     HIPCHK(launch_sor_init(st, omega, 0, g_stream));
-    int it = 1;
-    int batch = std::max(1, first_batch);
-    int done_before = 0;
+    const int step = pair ? 2 : 1;
+    int it = 1, L = 0;
+    int batch = std::max(1, first_batch / step);
+    int done_before = 0, launches_before = 0;
     while (true) {
       hipEvent_t e0 = (*events)[0], e1 = (*events)[1];
       HIPCHK(hipEventRecord(e0, g_stream));
-      for (int b = 0; b < batch; ++b, ++it) {
+      for (int b = 0; b < batch; ++b) {
         a.it = it;
-        a.pin = buf[(it - 1) & 1];
-        a.pout = buf[it & 1];
-        HIPCHK(launch_sor_iteration<T, TR>(a, g_stream));
-        if (it > a.kmax) { ++it; break; }
+        a.pin = buf[L & 1];
+        a.pout = buf[(L + 1) & 1];
+        HIPCHK(launch());
+        ++L;
+        it += step;
+        if (it - step > a.kmax) break;  // that launch closes the state
       }
       HIPCHK(hipEventRecord(e1, g_stream));
       HIPCHK(hipMemcpyAsync(hst, st, sizeof(SorState), hipMemcpyDeviceToHost, g_stream));
@@ -229,9 +247,11 @@ struct SorRun {
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         ms_full += ms;
         sweeps_full += launched - done_before;
+        launches_full += L - launches_before;
         done_before = launched;
+        launches_before = L;
         if (launched > a.kmax) return fail("SOR state did not close after kmax");
-        batch = 8;
+        batch = std::max(1, 8 / step);
         continue;
       }
       break;
@@ -455,31 +475,33 @@ extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz
 }
 
 namespace {
-int poisson_dev(double *pp, const double *rhs, double dx, double dy, double dz, int nx, int ny,
-                int nz, double *omega, double eps, int kmax, int idyn, int solver, int *exit_iter,
-                double *dmax, int *reason, Staged &s) {
+template <typename T>
+int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, int ny, int nz,
+                double *omega, double eps, int kmax, int idyn, int solver, int *exit_iter,
+                double *dmax, int *reason, Staged &s, bool pair = false) {
   if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
   CHK(check_red_black(solver, nx, ny, nz));
   if (kmax < 1) return fail("kmax must be >= 1");
   const int n = nx * ny * nz;
-  SorRun<double, double> r;
+  SorRun<T, T> r;
+  r.pair = pair;
   CHK(s.scratch(1, &r.st));
   HIPCHK(hipHostMalloc((void **)&r.hst, sizeof(SorState)));
   std::vector<hipEvent_t> ev(2);
   for (auto &e : ev) HIPCHK(hipEventCreate(&e));
   r.events = &ev;
-  double *other;
+  T *other;
   CHK(s.scratch(n, &other));
   r.buf[0] = pp;
   r.buf[1] = other;
-  SorArgs<double, double> &a = r.a;
+  SorArgs<T, T> &a = r.a;
   a.rhs = rhs;
   a.st = r.st;
   a.log = nullptr;
   a.log_len = 0;
   a.nx = nx; a.ny = ny; a.nz = nz;
   solver_periodic(solver, a.per);
-  a.tl = sor_tiling<double, double>(nx, ny, nz);
+  a.tl = pair ? sor2_tiling(nx, ny, nz) : sor_tiling<T, T>(nx, ny, nz);
   const SorConst c = sor_const(dx, dy, dz);
   a.ox = c.ox; a.oy = c.oy; a.oz = c.oz; a.A = c.A;
   a.eps = eps;
@@ -488,7 +510,8 @@ int poisson_dev(double *pp, const double *rhs, double dx, double dy, double dz, 
   a.idyn = idyn;
   int sweeps = 0;
   int rc = r.run(*omega, 16, &sweeps);
-  if (rc == 0 && (sweeps & 1)) rc = launch_copy<double>(pp, other, n, g_stream) == hipSuccess ? 0 : fail("copy failed");
+  if (rc == 0 && r.result(sweeps) != pp)
+    rc = launch_copy<T>(pp, other, n, g_stream) == hipSuccess ? 0 : fail("copy failed");
   if (rc == 0) {
     *omega = r.hst->omega_final;
     if (exit_iter) *exit_iter = r.hst->exit_iter;
@@ -515,6 +538,26 @@ extern "C" int o3d_poisson(double *pp, const double *rhs, double dx, double dy, 
   CHK(s.in(rhs, n, &r));
   CHK(poisson_dev(p, r, dx, dy, dz, nx, ny, nz, omega, eps, kmax, idyn, solver, exit_iter, dmax,
                   reason, s));
+  return s.finish();
+}
+
+extern "C" int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz,
+                               int nx, int ny, int nz, double *omega, double eps, int kmax,
+                               int idyn, int solver, int pair, int *exit_iter, double *dmax,
+                               int *reason) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  if (!omega) return fail("omega is NULL");
+  if (pair && !sor2_supported(nx, ny, nz))
+    return fail("two iterations per launch need every axis >= 8 (grid %dx%dx%d)", nx, ny, nz);
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  float *p;
+  const float *r;
+  CHK(s.out(pp, n, &p, true));
+  CHK(s.in(rhs, n, &r));
+  CHK(poisson_dev<float>(p, r, dx, dy, dz, nx, ny, nz, omega, eps, kmax, idyn, solver, exit_iter,
+                         dmax, reason, s, pair != 0));
   return s.finish();
 }
 
@@ -688,6 +731,7 @@ struct Solver {
   int last_sweeps[2] = {0, 0};
   double sor_ms[2] = {0, 0};
   int sor_sweeps[2] = {0, 0};
+  int sor_launches[2] = {0, 0};
   void *nn = nullptr;          // learned warm start
   float *cap_r = nullptr, *cap_eprev = nullptr, *cap_e = nullptr;
   // Learned warm start / capture state.
@@ -838,6 +882,7 @@ int solver_poisson_f64(Solver &S, o3d_step_info *info) {
   S.omega = S.hst->omega_final;
   S.sor_ms[0] += r.ms_full;
   S.sor_sweeps[0] += r.sweeps_full;
+  S.sor_launches[0] += r.launches_full;
   info->sor_exit_iter = S.hst->exit_iter;
   info->sor_reason = S.hst->reason;
   info->sweeps_f64 = sweeps;
@@ -871,6 +916,8 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     r.buf[1] = S.E[1];
     const int kin = std::min(inner_max, p.kmax - total);
     sor_args_common<float, float>(S, r.a, S.R, S.st, kin);
+    r.pair = sor2_supported(p.nx, p.ny, p.nz);
+    if (r.pair) r.a.tl = sor2_tiling(p.nx, p.ny, p.nz);
     int sweeps = 0;
     const int first = cycles == 0 && S.last_sweeps[1] > 0
                           ? std::min(kin, std::max(4, S.last_sweeps[1] - 2))
@@ -878,12 +925,13 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     CHK(r.run(S.omega, first, &sweeps));
     S.sor_ms[1] += r.ms_full;
     S.sor_sweeps[1] += r.sweeps_full;
+    S.sor_launches[1] += r.launches_full;
     const int rin = S.hst->reason;
     S.omega = S.hst->omega_final;
     total += sweeps;
     sw32 += sweeps;
     ++cycles;
-    HIPCHK(launch_fold(S.P[S.cur], S.E[sweeps & 1], n, g_stream));
+    HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
     if (rin == 2 && total < p.kmax) continue;  // cycle cap: refresh the residual
     // FP64 judge sweep (applied, like the reference's last sweep).
     SorRun<double, float> j;
@@ -1410,6 +1458,7 @@ extern "C" int o3d_solver_step(void *handle, int itime, o3d_step_info *info) {
   memset(info, 0, sizeof(*info));
   S.sor_ms[0] = S.sor_ms[1] = 0.0;
   S.sor_sweeps[0] = S.sor_sweeps[1] = 0;
+  S.sor_launches[0] = S.sor_launches[1] = 0;
   return S.mixed ? solver_step_t<float>(S, itime, info) : solver_step_t<double>(S, itime, info);
 }
 
@@ -1440,10 +1489,14 @@ extern "C" void *o3d_solver_field(void *handle, int which) {
   }
 }
 
-extern "C" int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2) {
+extern "C" int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches2) {
   if (!handle || !ms2 || !sweeps2) return fail("NULL argument");
   Solver &S = *(Solver *)handle;
-  for (int q = 0; q < 2; ++q) { ms2[q] = S.sor_ms[q]; sweeps2[q] = S.sor_sweeps[q]; }
+  for (int q = 0; q < 2; ++q) {
+    ms2[q] = S.sor_ms[q];
+    sweeps2[q] = S.sor_sweeps[q];
+    if (launches2) launches2[q] = S.sor_launches[q];
+  }
   return 0;
 }
 

@@ -11,8 +11,8 @@ namespace o3d {
 // stop rule has fired (src/poisson.f90:110-114), so the host can queue many
 // launches without reading anything back.
 struct SorState {
-  unsigned long long dmax_bits[4];  // IEEE bits of dmax of iteration it, ring by it & 3
-  double omega[4];                  // omega used by iteration it, ring by it & 3
+  unsigned long long dmax_bits[8];  // IEEE bits of dmax of iteration it, ring by it & 7
+  double omega[8];                  // omega used by iteration it, ring by it & 7
   double omega_final;               // omega after the solve (persists, idyn = 1)
   double dmax_final;                // dmax of the last sweep
   int done;                         // stop rule fired
@@ -62,6 +62,13 @@ struct SorArgs {
 
 template <typename T, typename TR>
 hipError_t launch_sor_iteration(const SorArgs<T, TR> &a, hipStream_t s);
+// Two red-black iterations (a.it and a.it + 1) in one launch, FP32 only:
+// bit-identical to two launch_sor_iteration calls, except that the stop
+// rules (and the idyn omega update) are checked once per pair.  Needs every
+// axis >= 8 and its own tiling (sor2_tiling).
+hipError_t launch_sor2_iteration(const SorArgs<float, float> &a, hipStream_t s);
+SorTiling sor2_tiling(int nx, int ny, int nz);
+bool sor2_supported(int nx, int ny, int nz);
 hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s);
 
 // Mixed precision helpers.  r = rhs - L p (FP64 arithmetic) stored in FP32,

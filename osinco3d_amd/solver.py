@@ -134,8 +134,13 @@ class Solver:
     def field_ptr(self, which: int) -> int:
         return int(self.lib.o3d_solver_field(self.handle, int(which)) or 0)
 
-    def sor_timing(self):
+    def sor_timing(self, launches: bool = False):
+        """(ms, sweeps) of the last step's complete SOR batches per precision
+        [FP64, FP32]; with launches=True also the kernel launch counts."""
         ms = (C.c_double * 2)()
         sw = (C.c_int * 2)()
-        capi.check(self.lib.o3d_solver_sor_timing(self.handle, ms, sw))
+        ln = (C.c_int * 2)()
+        capi.check(self.lib.o3d_solver_sor_timing(self.handle, ms, sw, ln))
+        if launches:
+            return list(ms), list(sw), list(ln)
         return list(ms), list(sw)
