@@ -916,8 +916,9 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     r.buf[1] = S.E[1];
     const int kin = std::min(inner_max, p.kmax - total);
     sor_args_common<float, float>(S, r.a, S.R, S.st, kin);
-    r.pair = sor2_supported(p.nx, p.ny, p.nz);
-    if (r.pair) r.a.tl = sor2_tiling(p.nx, p.ny, p.nz);
+    // Two iterations per launch (sor2_kernel) measured slower than single
+    // launches at 256^3 (56 vs 47.5 us per iteration): single launches.
+    r.pair = false;
     int sweeps = 0;
     const int first = cycles == 0 && S.last_sweeps[1] > 0
                           ? std::min(kin, std::max(4, S.last_sweeps[1] - 2))

@@ -377,24 +377,37 @@ __global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
 
   FieldPartial st;
   partial_init(st);
-  T qw[7];
+  // w column queue (planes k-3..k+3) plus the plane k+4 in flight, and the
+  // u / v values of plane k+1, loaded one step before they are staged.
+  T qw[8];
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
+  for (int s = 0; s < 7; ++s) {
     bool fz;
     const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
     qw[s] = load_ghost(A.f[2], col + plane * kk, odd && fz);
   }
+  T nu, nv, nhx = T(0), nhy = T(0);
+  auto load_uv = [&](int kp) {
+    const int pk = plane * min(kp, nz - 1);  // the plane after the last is not used
+    nu = load_ghost(A.f[0], col + pk, odd && P.fi);
+    nv = load_ghost(A.f[1], col + pk, odd && P.fj);
+    if (hx_col >= 0) nhx = load_ghost(A.f[0], hx_idx + pk, odd && hx_f);
+    if (hy_row >= 0) nhy = load_ghost(A.f[1], hy_idx + pk, odd && hy_f);
+  };
+  load_uv(P.z0);
   for (int k = P.z0; k < P.z1; ++k) {
-    {
-      bool fz;
-      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
-      qw[6] = load_ghost(A.f[2], col + plane * kk, odd && fz);
-    }
     const int pk = plane * k;
-    su[ty][tx + 3] = load_ghost(A.f[0], col + pk, odd && P.fi);
-    sv[ty + 3][tx] = load_ghost(A.f[1], col + pk, odd && P.fj);
-    if (hx_col >= 0) su[ty][hx_col] = load_ghost(A.f[0], hx_idx + pk, odd && hx_f);
-    if (hy_row >= 0) sv[hy_row][tx] = load_ghost(A.f[1], hy_idx + pk, odd && hy_f);
+    su[ty][tx + 3] = nu;
+    sv[ty + 3][tx] = nv;
+    if (hx_col >= 0) su[ty][hx_col] = nhx;
+    if (hy_row >= 0) sv[hy_row][tx] = nhy;
+    {
+      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
+      bool fz;
+      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
+      qw[7] = load_ghost(A.f[2], col + plane * kk, odd && fz);
+    }
+    load_uv(k + 1);
     __syncthreads();
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
@@ -413,7 +426,7 @@ __global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 6; ++s) qw[s] = qw[s + 1];
+    for (int s = 0; s < 7; ++s) qw[s] = qw[s + 1];
   }
   partial_store(st, A.part + blockIdx.x, sc);
 }
@@ -453,27 +466,42 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
 
   FieldPartial st[3];
   partial_init(st[0]); partial_init(st[1]); partial_init(st[2]);
-  TP q[7];
+  // p column queue (planes k-3..k+3) plus plane k+4 in flight; the halo
+  // values of plane k+1 are loaded one step before they are staged.
+  TP q[8];
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
+  for (int s = 0; s < 7; ++s) {
     bool fz;
     const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
     q[s] = A.p[col + plane * kk];
   }
+  TP nhx = TP(0), nhy = TP(0);
+  auto load_halo = [&](int kp) {
+    const int pk = plane * min(kp, nz - 1);  // the plane after the last is not used
+    if (hx_col >= 0) nhx = A.p[hx_idx + pk];
+    if (hy_row >= 0) nhy = A.p[hy_idx + pk];
+  };
+  load_halo(P.z0);
   for (int k = P.z0; k < P.z1; ++k) {
-    {
-      bool fz;
-      const int kk = ghost_index(k + 3, nz, A.bc[2], fz);
-      q[6] = A.p[col + plane * kk];
-    }
     const int pk = plane * k;
     pl[ty + 3][tx + 3] = q[3];
-    if (hx_col >= 0) pl[ty + 3][hx_col] = A.p[hx_idx + pk];
-    if (hy_row >= 0) pl[hy_row][tx + 3] = A.p[hy_idx + pk];
+    if (hx_col >= 0) pl[ty + 3][hx_col] = nhx;
+    if (hy_row >= 0) pl[hy_row][tx + 3] = nhy;
+    {
+      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
+      bool fz;
+      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
+      q[7] = A.p[col + plane * kk];
+    }
+    load_halo(k + 1);
     __syncthreads();
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
       const int cc = tx + 3, r = ty + 3;
+      // u* loads issued before the derivative arithmetic that hides them.
+      T upv[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) upv[c] = A.up[c][idx];
       TP g[3];
       g[0] = d1<TP>(pl[r][cc - 3], pl[r][cc - 2], pl[r][cc - 1], pl[r][cc + 1], pl[r][cc + 2],
                     pl[r][cc + 3], A.d1[0]);
@@ -485,14 +513,14 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
       if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) g[2] = TP(0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T un = T(TP(A.up[c][idx]) - A.dt * g[c]);
+        const T un = T(TP(upv[c]) - A.dt * g[c]);
         A.u[c][idx] = un;
         partial_add(st[c], (double)un, idx);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 6; ++s) q[s] = q[s + 1];
+    for (int s = 0; s < 7; ++s) q[s] = q[s + 1];
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) partial_store(st[c], A.part + 3 * blockIdx.x + c, sc);
