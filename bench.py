@@ -59,6 +59,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--forms", default="fp64,mixed,extrap,nn")
+    ap.add_argument("--workload", choices=("tgv", "mixing"), default="tgv",
+                    help="tgv: the headline Taylor-Green case; mixing: the 384x256x128 LES "
+                         "temporal mixing layer (free-slip y, AB2), no CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-kmax", type=int, default=10)
     ap.add_argument("--model", default=os.path.join(ROOT, "osinco3d_amd", "warmstart_weights.npy"))
@@ -92,7 +95,7 @@ def pct(v, q):
     return v[k]
 
 
-def run_form(form, cfg, steps, warmup, model_path):
+def run_form(form, cfg, steps, warmup, model_path, init=None):
     import torch
     from osinco3d_amd import solver as S
 
@@ -110,7 +113,10 @@ def run_form(form, cfg, steps, warmup, model_path):
     s = S.Solver(cfg, prec, warm)
     if net is not None:
         net.attach(s)
-    s.init_taylor_green()
+    if init is None:
+        s.init_taylor_green()
+    else:
+        s.upload(*init)
     for it in range(1, warmup + 1):
         s.step(it)
     torch.cuda.synchronize()
@@ -294,16 +300,29 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
-    from osinco3d_amd.config import tgv_config
+    from osinco3d_amd.config import mixing_layer_config, tgv_config
 
-    cfg = tgv_config(args.n, 1000, 3)
+    init = None
+    if args.workload == "mixing":
+        from osinco3d_amd.solver import mixing_layer
+        cfg = mixing_layer_config()
+        init = mixing_layer(cfg)
+        args.no_cpu = True
+        workload = ("Temporal mixing layer, 384x256x128 (60x40x20), free-slip y, Smagorinsky LES "
+                    "cs=0.17, Re=3000, AB2, CFL 0.25, SOR eps=1e-8 (solver 0011)")
+        data = "synthetic (tanh profile + SplitMix64 seed 1234 uniform noise, 0.05 U0)"
+    else:
+        cfg = tgv_config(args.n, 1000, 3)
+        workload = (f"Taylor-Green vortex Re=1600, {args.n}^3 periodic, AB3, CFL 0.25, "
+                    f"SOR eps=1e-8")
+        data = "synthetic (analytic Taylor-Green initial field)"
     copy_gbs = copy_bandwidth()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     results = {}
     for form in forms:
-        print(f"[bench] form {form}: {args.n}^3, warmup {args.warmup}, steps {args.steps}",
-              file=sys.stderr, flush=True)
-        r = run_form(form, cfg, args.steps, args.warmup, args.model)
+        print(f"[bench] form {form}: {cfg.nx}x{cfg.ny}x{cfg.nz}, warmup {args.warmup}, "
+              f"steps {args.steps}", file=sys.stderr, flush=True)
+        r = run_form(form, cfg, args.steps, args.warmup, args.model, init)
         if r is not None:
             results[form] = r
             print(f"[bench] form {form}: {r['steps_per_s']:.2f} steps/s, "
@@ -344,9 +363,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head_form == "fp64" else "f32+f64",
-        "data": "synthetic (analytic Taylor-Green initial field)",
-        "config": {"workload": f"Taylor-Green vortex Re=1600, {args.n}^3 periodic, AB3, "
-                               f"CFL 0.25, SOR eps=1e-8", "grid": [args.n] * 3,
+        "data": data,
+        "config": {"workload": workload, "grid": [cfg.nx, cfg.ny, cfg.nz],
                    "form": head_form, "parallelism": "none (1 GPU)"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -54,6 +54,34 @@ def taylor_green(cfg: Config) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.nd
             np.ascontiguousarray(pp))
 
 
+def splitmix64_uniform(n: int, seed: int) -> np.ndarray:
+    """n uniform doubles in [0, 1) from the SplitMix64 sequence of `seed`
+    (state seed + k * 0x9E3779B97F4A7C15, k = 1..n; top 53 bits)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(1, n + 1, dtype=np.uint64) *
+                               np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def mixing_layer(cfg: Config, seed: int = 1234, amplitude: float = 0.05):
+    """Temporal mixing layer: u = 0.5 U0 tanh(2 y / l0) plus uniform noise of
+    amplitude `amplitude` U0 on all three components (SplitMix64, `seed`),
+    p = 0.  Written to fields.bin for the reference (typesim = 0)."""
+    x, y, z = grid(cfg)
+    sh = (cfg.nz, cfg.ny, cfg.nx)
+    n = cfg.nx * cfg.ny * cfg.nz
+    noise = splitmix64_uniform(3 * n, seed).reshape(3, *sh)
+    a = amplitude * cfg.u0
+    ux = 0.5 * cfg.u0 * np.tanh(2.0 * y / cfg.l0)[None, :, None] + a * (2.0 * noise[0] - 1.0)
+    uy = a * (2.0 * noise[1] - 1.0)
+    uz = a * (2.0 * noise[2] - 1.0)
+    return (np.ascontiguousarray(ux), np.ascontiguousarray(uy), np.ascontiguousarray(uz),
+            np.zeros(sh))
+
+
 def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
                 inner_max: int = 0) -> capi.O3DParams:
     p = capi.O3DParams()
