@@ -1040,9 +1040,18 @@ __global__ void lw_guess2_kernel(const double *p, const float *e1, const float *
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) q[i] = p[i] + (2.0 * (double)e1[i] - (double)e2[i]);
 }
-__global__ void lw_increment_kernel(const double *p, const double *p0, float *e, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) e[i] = (float)(p[i] - p0[i]);
+// e = p - p0 (float32) and the per-block sum of squares of e.
+__global__ void __launch_bounds__(256) lw_increment_kernel(const double *p, const double *p0,
+                                                           float *e, int n, double *part) {
+  __shared__ double sc[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float v = (float)(p[i] - p0[i]);
+    e[i] = v;
+    s += (double)v * v;
+  }
+  s = block_sum<4>(s, sc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 __global__ void __launch_bounds__(256) lw_sumsq_kernel(const float *a, int n, double *part) {
   __shared__ double sc[4];
@@ -1243,8 +1252,8 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
 int solver_learning_finish(Solver &S) {
   const int n = S.n;
   std::swap(S.Eprev, S.Eprev2);
-  lw_increment_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n);
-  lw_sumsq_kernel<<<kLwBlocks, 256, 0, g_stream>>>(S.Eprev, n, S.epart);
+  lw_increment_kernel<<<kLwBlocks, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n,
+                                                      S.epart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(S.hepart, S.epart, kLwBlocks * sizeof(double), hipMemcpyDeviceToHost,
                         g_stream));

@@ -54,6 +54,11 @@ struct ConvArgs {
   int lci, ci, co, kp;
   int act;            // 1: leaky ReLU
   int accumulate;     // y += result (FP32 gradient buffers)
+  // Split-K (coarse grids): grid.z = nsplit slices of the K chunks; each
+  // writes raw FP32 sums to part[slice][V][co] and conv_reduce_kernel applies
+  // bias, skip, activation and accumulation.  part = null: one slice.
+  float *part = nullptr;
+  int nsplit = 1;
 };
 
 // One wave computes 16*MV consecutive voxels (linear order, so coarse levels
@@ -74,7 +79,9 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
 #pragma unroll
     for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ktot = 27 * a.ci;
-  const int nch = a.kp >> 5;
+  const int nchunk = a.kp >> 5;
+  const int kc0 = (int)blockIdx.z * nchunk / a.nsplit;
+  const int kc1 = ((int)blockIdx.z + 1) * nchunk / a.nsplit;
   // A-operand voxels of this lane, one per voxel tile.
   int xm[MV], ym[MV], zm[MV];
   bool vin[MV];
@@ -89,7 +96,7 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
   const bf16 *wrow[NT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) wrow[n] = a.w + (size_t)(co0 + 16 * n + l16) * a.kp;
-  for (int kc = 0; kc < nch; ++kc) {
+  for (int kc = kc0; kc < kc1; ++kc) {
     const int k0 = (kc << 5) + (g << 3);
     bf16x8 bv[NT];
 #pragma unroll
@@ -114,6 +121,19 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
         acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[m][n], 0, 0, 0);
     }
   }
+  if (a.part) {
+    float *pp = a.part + (size_t)blockIdx.z * V * a.co;
+#pragma unroll
+    for (int m = 0; m < MV; ++m)
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int vox = vbase + 16 * m + 4 * g + i;
+          if (vox < V) pp[(size_t)vox * a.co + co0 + 16 * n + l16] = acc[m][n][i];
+        }
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < MV; ++m)
 #pragma unroll
@@ -134,12 +154,50 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
     }
 }
 
+// y[v][c] (+)= act(sum_s part[s][v][c] + bias[c] + skip[v][c]).
+template <typename TO>
+__global__ void __launch_bounds__(256) conv_reduce_kernel(const ConvArgs<TO> a, int V) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nel = (size_t)V * a.co;
+  if (e >= nel) return;
+  const int c = (int)(e % a.co);
+  float v = 0.f;
+  for (int q = 0; q < a.nsplit; ++q) v += a.part[(size_t)q * nel + e];
+  if (a.bias) v += a.bias[c];
+  if (a.skip) v += (float)a.skip[e];
+  if (a.act) v = act_f(v);
+  if (a.accumulate) v += (float)a.y[e];
+  a.y[e] = (TO)v;
+}
+
+// Blocks a convolution launches without voxel-tile blocking or split-K.
+inline int conv_blocks(int V, int co) {
+  const int nt = co >= 64 ? 4 : (co >= 32 ? 2 : 1);
+  return ((V + 63) / 64) * (co / (16 * nt));
+}
+
+// K slices for a coarse grid: enough blocks to fill the machine (>= 2048),
+// at most one slice per 32-wide K chunk, at most 27.
+inline int conv_splits(int V, int co, int kp) {
+  const int b = conv_blocks(V, co);
+  if (b >= 1024) return 1;
+  return std::max(1, std::min({(2048 + b - 1) / b, kp / 32, 27}));
+}
+
 template <typename TO, int NT>
-hipError_t launch_conv_nt(const ConvArgs<TO> &a, int V, hipStream_t s) {
-  // Four voxel tiles per wave once the grid still has >= 2048 blocks.
+hipError_t launch_conv_nt(ConvArgs<TO> a, int V, float *ws, size_t ws_n, hipStream_t s) {
+  // Four voxel tiles per wave once the grid still has >= 2048 blocks; on
+  // coarse grids, split K instead.
   constexpr int kMV = 4;
   const int gy = a.co / (16 * NT);
-  if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 2048) {
+  const int split = ws ? conv_splits(V, a.co, a.kp) : 1;
+  if (split > 1 && (size_t)split * V * a.co <= ws_n) {
+    a.part = ws;
+    a.nsplit = split;
+    dim3 grid((V + 63) / 64, gy, split);
+    conv_kernel<TO, NT, 1><<<grid, 256, 0, s>>>(a);
+    conv_reduce_kernel<TO><<<(int)(((size_t)V * a.co + 255) / 256), 256, 0, s>>>(a, V);
+  } else if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 2048) {
     dim3 grid((V + 64 * kMV - 1) / (64 * kMV), gy);
     conv_kernel<TO, NT, kMV><<<grid, 256, 0, s>>>(a);
   } else {
@@ -150,12 +208,12 @@ hipError_t launch_conv_nt(const ConvArgs<TO> &a, int V, hipStream_t s) {
 }
 
 template <typename TO>
-hipError_t launch_conv(const ConvArgs<TO> &a, hipStream_t s) {
+hipError_t launch_conv(const ConvArgs<TO> &a, float *ws, size_t ws_n, hipStream_t s) {
   const int V = a.nx * a.ny * a.nz;
   const int nt = a.co >= 64 ? 4 : (a.co >= 32 ? 2 : 1);
-  if (nt == 4) return launch_conv_nt<TO, 4>(a, V, s);
-  if (nt == 2) return launch_conv_nt<TO, 2>(a, V, s);
-  return launch_conv_nt<TO, 1>(a, V, s);
+  if (nt == 4) return launch_conv_nt<TO, 4>(a, V, ws, ws_n, s);
+  if (nt == 2) return launch_conv_nt<TO, 2>(a, V, ws, ws_n, s);
+  return launch_conv_nt<TO, 1>(a, V, ws, ws_n, s);
 }
 
 // Weight gradient: dW[co][tap*ci + c] = sum_v dz[v][co] * x[v + off(tap)][c],
@@ -393,6 +451,8 @@ struct Net {
   size_t gbytes = 0;
   float *part = nullptr;   // wgrad partial sums
   size_t part_n = 0;
+  float *ws = nullptr;     // split-K partial sums of coarse-grid convolutions
+  size_t ws_n = 0;
   double *mse_part = nullptr;
   double *hpart = nullptr;
   void *act_arena = nullptr;
@@ -405,6 +465,7 @@ struct Net {
     if (act_arena) (void)hipFree(act_arena);
     if (gbuf) (void)hipFree(gbuf);
     if (part) (void)hipFree(part);
+    if (ws) (void)hipFree(ws);
     if (hpart) (void)hipHostFree(hpart);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
@@ -497,6 +558,16 @@ int alloc_activations(Net &n, int nx, int ny, int nz) {
   const size_t slabs = std::max<size_t>(1, std::min<size_t>(512, n.V[0] / 4096));
   n.part_n = maxe * slabs;
   NNCHK(hipMalloc((void **)&n.part, n.part_n * 4));
+  // Split-K workspace: the largest slices * V * co over forward and
+  // data-gradient convolutions.
+  if (n.ws) { (void)hipFree(n.ws); n.ws = nullptr; }
+  n.ws_n = 0;
+  for (const Conv &c : n.conv) {
+    const int V = (int)n.V[c.level];
+    n.ws_n = std::max(n.ws_n, (size_t)conv_splits(V, c.co, c.kpf) * V * c.co);
+    n.ws_n = std::max(n.ws_n, (size_t)conv_splits(V, c.ci, c.kpb) * V * c.ci);
+  }
+  if (n.ws_n) NNCHK(hipMalloc((void **)&n.ws, n.ws_n * 4));
   return 0;
 }
 
@@ -545,7 +616,7 @@ int conv_fwd(Net &n, int li, const bf16 *x, const bf16 *skip, bf16 *y) {
   a.nx = n.nx[c.level]; a.ny = n.ny[c.level]; a.nz = n.nz[c.level];
   a.lci = ilog2(c.ci); a.ci = c.ci; a.co = c.co; a.kp = c.kpf;
   a.act = 1; a.accumulate = 0;
-  NNCHK(launch_conv<bf16>(a, st()));
+  NNCHK(launch_conv<bf16>(a, n.ws, n.ws_n, st()));
   n.last_flops += 54.0 * c.ci * c.co * (double)n.V[c.level];
   return 0;
 }
@@ -579,7 +650,7 @@ int conv_bwd(Net &n, int li, const float *gy, const bf16 *y, const bf16 *x, floa
     a.nx = n.nx[L]; a.ny = n.ny[L]; a.nz = n.nz[L];
     a.lci = ilog2(c.co); a.ci = c.co; a.co = c.ci; a.kp = c.kpb;
     a.act = 0; a.accumulate = 1;
-    NNCHK(launch_conv<float>(a, st()));
+    NNCHK(launch_conv<float>(a, n.ws, n.ws_n, st()));
   }
   return 0;
 }
