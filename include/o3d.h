@@ -122,6 +122,20 @@ int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
                    double dx, double dy, double dz, double re, double t, const int *bc,
                    double *out17);
 
+/* rotational(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, nz),
+ * src/differential_operators.f90:40-76: the curl from the p (even) D1
+ * variants.  One fused pass over (ux, uy, uz); bc as for o3d_divergence. */
+int o3d_rotational(double *rotx, double *roty, double *rotz, const double *ux, const double *uy,
+                   const double *uz, double dx, double dy, double dz, int nx, int ny, int nz,
+                   const int *bc);
+
+/* calculate_Q_criterion(Q, ux, uy, uz, dx, dy, dz, nx, ny, nz),
+ * src/differential_operators.f90:78-108: Q = -1/2 (ux_x^2 + uy_y^2 + uz_z^2)
+ * - ux_y uy_x - ux_z uz_x - uy_z uz_y, diagonal terms from the i (odd)
+ * variants, the others from the p variants, in the reference's order. */
+int o3d_q_criterion(double *q, const double *ux, const double *uy, const double *uz, double dx,
+                    double dy, double dz, int nx, int ny, int nz, const int *bc);
+
 /* ------------------------------------------------- device-resident solver */
 
 /* Precision forms. */

@@ -124,6 +124,41 @@ static void grad9(double **g, const double *ux, const double *uy,
             orc_der1(g[3 * c + a], u[c], nx, ny, nz, a, kind_of(bc, a, a == c), h[a]);
 }
 
+/* rotational, src/differential_operators.f90:40-76: curl from the p (even)
+ * variants (the off-diagonal entries of grad9). */
+void orc_rotational(double *rx, double *ry, double *rz, const double *ux,
+                    const double *uy, const double *uz, double dx, double dy,
+                    double dz, int nx, int ny, int nz, const int *bc) {
+    const size_t N = (size_t)nx * ny * nz;
+    double *g[9];
+    for (int q = 0; q < 9; ++q) g[q] = malloc(N * sizeof(double));
+    grad9(g, ux, uy, uz, dx, dy, dz, nx, ny, nz, bc);
+    for (size_t q = 0; q < N; ++q) {
+        rx[q] = g[7][q] - g[5][q]; /* duzdy - duydz (:64-66) */
+        ry[q] = g[2][q] - g[6][q]; /* duxdz - duzdx (:68-70) */
+        rz[q] = g[3][q] - g[1][q]; /* duydx - duxdy (:72-74) */
+    }
+    for (int q = 0; q < 9; ++q) free(g[q]);
+}
+
+/* calculate_Q_criterion, src/differential_operators.f90:78-108: diagonal
+ * terms from the i (odd) variants, the others from the p variants;
+ * expression order of :103-104. */
+void orc_q_criterion(double *Q, const double *ux, const double *uy,
+                     const double *uz, double dx, double dy, double dz, int nx,
+                     int ny, int nz, const int *bc) {
+    const size_t N = (size_t)nx * ny * nz;
+    double *g[9];
+    for (int q = 0; q < 9; ++q) g[q] = malloc(N * sizeof(double));
+    grad9(g, ux, uy, uz, dx, dy, dz, nx, ny, nz, bc);
+    for (size_t q = 0; q < N; ++q) {
+        const double a = g[0][q], b = g[4][q], c = g[8][q];
+        Q[q] = -(0.5 * (a * a + b * b + c * c)) - g[1][q] * g[3][q] -
+               g[2][q] * g[6][q] - g[5][q] * g[7][q];
+    }
+    for (int q = 0; q < 9; ++q) free(g[q]);
+}
+
 static void lap9(double **g, const double *ux, const double *uy,
                  const double *uz, double dx, double dy, double dz, int nx,
                  int ny, int nz, const int *bc) {

@@ -59,4 +59,10 @@ void orc_residual(double *r, const double *pp, const double *rhs, double dx,
 #ifdef __cplusplus
 }
 #endif
+void orc_rotational(double *rx, double *ry, double *rz, const double *ux,
+                    const double *uy, const double *uz, double dx, double dy,
+                    double dz, int nx, int ny, int nz, const int *bc);
+void orc_q_criterion(double *Q, const double *ux, const double *uy,
+                     const double *uz, double dx, double dy, double dz, int nx,
+                     int ny, int nz, const int *bc);
 #endif

@@ -61,6 +61,18 @@ class Oracle:
                                 *_dims(fx), _bc(bc), int(odd))
         return out
 
+    def rotational(self, ux, uy, uz, d, bc):
+        out = [np.empty_like(ux) for _ in range(3)]
+        self.lib.orc_rotational(*[_p(o) for o in out], _p(ux), _p(uy), _p(uz),
+                                *[C.c_double(v) for v in d], *_dims(ux), _bc(bc))
+        return out
+
+    def q_criterion(self, ux, uy, uz, d, bc):
+        out = np.empty_like(ux)
+        self.lib.orc_q_criterion(_p(out), _p(ux), _p(uy), _p(uz), *[C.c_double(v) for v in d],
+                                 *_dims(ux), _bc(bc))
+        return out
+
     def nu_t(self, ux, uy, uz, d, cs, delta, bc):
         out = np.empty_like(ux)
         self.lib.orc_nu_t(_p(out), _p(ux), _p(uy), _p(uz), *[C.c_double(v) for v in d],
@@ -146,6 +158,18 @@ class Reference:
         out = np.empty_like(fx)
         self.lib.ref_divergence(_p(out), _p(fx), _p(fy), _p(fz), *[C.c_double(v) for v in d],
                                 *_dims(fx), int(odd))
+        return out
+
+    def rotational(self, ux, uy, uz, d):
+        out = [np.empty_like(ux) for _ in range(3)]
+        self.lib.ref_rotational(*[_p(o) for o in out], _p(ux), _p(uy), _p(uz),
+                                *[C.c_double(v) for v in d], *_dims(ux))
+        return out
+
+    def q_criterion(self, ux, uy, uz, d):
+        out = np.empty_like(ux)
+        self.lib.ref_q_criterion(_p(out), _p(ux), _p(uy), _p(uz), *[C.c_double(v) for v in d],
+                                 *_dims(ux))
         return out
 
     def nu_t(self, ux, uy, uz, d, cs, delta):

@@ -76,6 +76,24 @@ contains
     call divergence(divf, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3, odd)
   end subroutine ref_divergence
 
+  subroutine ref_rotational(rx, ry, rz, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3) &
+       bind(C, name="ref_rotational")
+    integer(c_int), value :: n1, n2, n3
+    real(c_double), value :: ddx, ddy, ddz
+    real(c_double), intent(in) :: fx(n1,n2,n3), fy(n1,n2,n3), fz(n1,n2,n3)
+    real(c_double), intent(out) :: rx(n1,n2,n3), ry(n1,n2,n3), rz(n1,n2,n3)
+    call rotational(rx, ry, rz, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3)
+  end subroutine ref_rotational
+
+  subroutine ref_q_criterion(q, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3) &
+       bind(C, name="ref_q_criterion")
+    integer(c_int), value :: n1, n2, n3
+    real(c_double), value :: ddx, ddy, ddz
+    real(c_double), intent(in) :: fx(n1,n2,n3), fy(n1,n2,n3), fz(n1,n2,n3)
+    real(c_double), intent(out) :: q(n1,n2,n3)
+    call calculate_Q_criterion(q, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3)
+  end subroutine ref_q_criterion
+
   subroutine ref_nu_t(nut, u, v, w, ddx, ddy, ddz, c_s, dlt, n1, n2, n3) &
        bind(C, name="ref_nu_t")
     integer(c_int), value :: n1, n2, n3

@@ -73,8 +73,10 @@ class O3DStepInfo(C.Structure):
 EXPORTS = [
     "o3d_init", "o3d_set_stream", "o3d_last_error", "o3d_sync", "o3d_version",
     "o3d_der", "o3d_der_f32", "o3d_divergence", "o3d_nu_t", "o3d_predict_velocity",
-    "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity", "o3d_function_stats",
-    "o3d_statistics", "o3d_solver_create", "o3d_solver_destroy", "o3d_solver_upload",
+    "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity",
+    "o3d_function_stats",
+    "o3d_statistics", "o3d_rotational", "o3d_q_criterion", "o3d_solver_create",
+    "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
     "o3d_warm_features",
@@ -242,6 +244,25 @@ def function_stats(f):
     st = np.zeros(6)
     check(load().o3d_function_stats(ptr(f), nx, ny, nz, st.ctypes.data_as(_dp)))
     return st
+
+
+def rotational(ux, uy, uz, d, bc):
+    """curl u (src/differential_operators.f90:40-76): numpy arrays in,
+    (rotx, roty, rotz) out."""
+    nx, ny, nz = _dims(ux)
+    out = [np.empty_like(ux) for _ in range(3)]
+    check(load().o3d_rotational(*[ptr(o) for o in out], ptr(ux), ptr(uy), ptr(uz), *map(_d, d),
+                                nx, ny, nz, _bc(bc)))
+    return out
+
+
+def q_criterion(ux, uy, uz, d, bc):
+    """Q criterion (src/differential_operators.f90:78-108)."""
+    nx, ny, nz = _dims(ux)
+    q = np.empty_like(ux)
+    check(load().o3d_q_criterion(ptr(q), ptr(ux), ptr(uy), ptr(uz), *map(_d, d), nx, ny, nz,
+                                 _bc(bc)))
+    return q
 
 
 def statistics(ux, uy, uz, d, re, t, bc):

@@ -705,6 +705,47 @@ extern "C" int o3d_statistics(const double *ux, const double *uy, const double *
   return statistics_dev<double>(u, nx, ny, nz, dx, dy, dz, re, t, bc, out17, s);
 }
 
+namespace {
+int diag_dev(int mode, double *out0, double *out1, double *out2, const double *ux,
+             const double *uy, const double *uz, double dx, double dy, double dz, int nx, int ny,
+             int nz, const int *bc) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  DiagArgs<double> a;
+  CHK(s.in(ux, n, &a.u[0]));
+  CHK(s.in(uy, n, &a.u[1]));
+  CHK(s.in(uz, n, &a.u[2]));
+  double *o[3] = {out0, out1, out2};
+  for (int c = 0; c < (mode == 0 ? 3 : 1); ++c) {
+    if (!o[c]) return fail("NULL output");
+    CHK(s.out(o[c], n, &a.out[c], false));
+  }
+  a.mode = mode;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.tl = make_tiling(nx, ny, nz);
+  const double h[3] = {dx, dy, dz};
+  fill_coef<double>(a.d1, h, false);
+  HIPCHK(launch_diag<double>(a, g_stream));
+  return s.finish();
+}
+}  // namespace
+
+extern "C" int o3d_rotational(double *rotx, double *roty, double *rotz, const double *ux,
+                              const double *uy, const double *uz, double dx, double dy,
+                              double dz, int nx, int ny, int nz, const int *bc) {
+  return diag_dev(0, rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, nz, bc);
+}
+
+extern "C" int o3d_q_criterion(double *q, const double *ux, const double *uy, const double *uz,
+                               double dx, double dy, double dz, int nx, int ny, int nz,
+                               const int *bc) {
+  return diag_dev(1, q, nullptr, nullptr, ux, uy, uz, dx, dy, dz, nx, ny, nz, bc);
+}
+
 // ====================================================== device-resident solver
 namespace {
 

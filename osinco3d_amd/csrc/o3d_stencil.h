@@ -76,8 +76,20 @@ struct StatsArgs {
 };
 
 template <typename T>
+struct DiagArgs {
+  const T *u[3];
+  T *out[3];     // mode 0: curl components; mode 1: out[0] = Q
+  int mode;
+  int nx, ny, nz, bc[3];
+  Tiling tl;
+  T d1[3][3];
+};
+
+template <typename T>
 hipError_t launch_der(int which, T *df, const T *f, int nx, int ny, int nz, double h,
                       hipStream_t s);
+template <typename T>
+hipError_t launch_diag(const DiagArgs<T> &a, hipStream_t s);
 template <typename T>
 hipError_t launch_predict(const PredictArgs<T> &a, int nh, bool les, hipStream_t s);
 template <typename T>

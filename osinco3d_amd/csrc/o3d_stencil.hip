@@ -648,6 +648,109 @@ __global__ void __launch_bounds__(kTX *kTY) stats_kernel(const StatsArgs<T> A) {
   }
 }
 
+// ------------------------------------------------------- vortical diagnostics
+// rotational (mode 0: out[0..2] = curl u, src/differential_operators.f90:40-76)
+// and calculate_Q_criterion (mode 1: out[0] = Q, :78-108) in one pass over
+// (u, v, w).  The nine first derivatives carry the parity of the reference's
+// variant choice (u_c odd along axis c: derxi/deryi/derzi on the diagonal,
+// the p variants with their wall zeros across), as in the predict kernel, and
+// each output keeps the reference's operation order.
+template <typename T>
+__global__ void __launch_bounds__(kTX *kTY) diag_kernel(const DiagArgs<T> A) {
+  __shared__ T pl[3][kTY + 6][kTX + 6];
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const TilePos P = tile_pos(A.tl, nx, ny, nz, A.bc);
+  const int plane = nx * ny;
+  const int col = P.gi + nx * P.gj;
+  const int tx = P.tx, ty = P.ty;
+  const bool own_neg[2] = {P.fi, P.fj};
+  int hx_col = -1, hx_g = 0;
+  bool hx_f = false;
+  if (tx < 3) { hx_col = tx; hx_g = ghost_safe(P.x0 + tx - 3, nx, A.bc[0], hx_f); }
+  else if (tx >= kTX - 3) { hx_col = tx + 6; hx_g = ghost_safe(P.x0 + tx + 3, nx, A.bc[0], hx_f); }
+  int hy_row = -1, hy_g = 0;
+  bool hy_f = false;
+  if (ty < 3) { hy_row = ty; hy_g = ghost_safe(P.y0 + ty - 3, ny, A.bc[1], hy_f); }
+  else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
+  const int hx_idx = hx_g + nx * P.gj;
+  const int hy_idx = P.gi + nx * hy_g;
+  const bool edge_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  const bool edge_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  T q[3][8];
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    bool fz;
+    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
+    q[0][s] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+    q[1][s] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+    q[2][s] = load_ghost(A.u[2], col + plane * kk, fz);
+  }
+  for (int k = P.z0; k < P.z1; ++k) {
+    {
+      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
+      bool fz;
+      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
+      q[0][7] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
+      q[1][7] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
+      q[2][7] = load_ghost(A.u[2], col + plane * kk, fz);
+    }
+    const int pk = plane * k;
+    T hx[3], hy[3];
+    if (hx_col >= 0) {
+      hx[0] = load_ghost(A.u[0], hx_idx + pk, hx_f);
+      hx[1] = load_ghost(A.u[1], hx_idx + pk, own_neg[1]);
+      hx[2] = A.u[2][hx_idx + pk];
+    }
+    if (hy_row >= 0) {
+      hy[0] = load_ghost(A.u[0], hy_idx + pk, own_neg[0]);
+      hy[1] = load_ghost(A.u[1], hy_idx + pk, hy_f);
+      hy[2] = A.u[2][hy_idx + pk];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pl[c][ty + 3][tx + 3] = q[c][3];
+      if (hx_col >= 0) pl[c][ty + 3][hx_col] = hx[c];
+      if (hy_row >= 0) pl[c][hy_row][tx + 3] = hy[c];
+    }
+    __syncthreads();
+    if (P.valid) {
+      const int idx = P.i + nx * P.j + pk;
+      const int r = ty + 3, cc = tx + 3;
+      T g[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        g[c][0] = d1<T>(pl[c][r][cc - 3], pl[c][r][cc - 2], pl[c][r][cc - 1], pl[c][r][cc + 1],
+                        pl[c][r][cc + 2], pl[c][r][cc + 3], A.d1[0]);
+        g[c][1] = d1<T>(pl[c][r - 3][cc], pl[c][r - 2][cc], pl[c][r - 1][cc], pl[c][r + 1][cc],
+                        pl[c][r + 2][cc], pl[c][r + 3][cc], A.d1[1]);
+        g[c][2] = d1<T>(q[c][0], q[c][1], q[c][2], q[c][4], q[c][5], q[c][6], A.d1[2]);
+      }
+      if (edge_x) { g[1][0] = T(0); g[2][0] = T(0); }
+      if (edge_y) { g[0][1] = T(0); g[2][1] = T(0); }
+      if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) { g[0][2] = T(0); g[1][2] = T(0); }
+      if (A.mode == 0) {
+        A.out[0][idx] = g[2][1] - g[1][2];  // duz/dy - duy/dz
+        A.out[1][idx] = g[0][2] - g[2][0];  // dux/dz - duz/dx
+        A.out[2][idx] = g[1][0] - g[0][1];  // duy/dx - dux/dy
+      } else {
+        A.out[0][idx] = -(T(0.5) * (g[0][0] * g[0][0] + g[1][1] * g[1][1] + g[2][2] * g[2][2])) -
+                        g[0][1] * g[1][0] - g[0][2] * g[2][0] - g[1][2] * g[2][1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int s = 0; s < 7; ++s) q[c][s] = q[c][s + 1];
+  }
+}
+
+template <typename T>
+hipError_t launch_diag(const DiagArgs<T> &a, hipStream_t s) {
+  diag_kernel<T><<<dim3(a.tl.blocks()), dim3(kTX * kTY), 0, s>>>(a);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_statistics(const StatsArgs<T> &a, hipStream_t s) {
   stats_kernel<T><<<dim3(a.tl.blocks()), dim3(kTX * kTY), 0, s>>>(a);
@@ -742,6 +845,7 @@ template hipError_t launch_correct<double, double>(const CorrectArgs<double, dou
 template hipError_t launch_correct<float, double>(const CorrectArgs<float, double> &,
                                                   hipStream_t);
 template hipError_t launch_statistics<double>(const StatsArgs<double> &, hipStream_t);
+template hipError_t launch_diag<double>(const DiagArgs<double> &, hipStream_t);
 template hipError_t launch_statistics<float>(const StatsArgs<float> &, hipStream_t);
 template hipError_t launch_field_stats<double>(const double *, int, FieldPartial *, int,
                                                hipStream_t);

@@ -1,16 +1,17 @@
 ! Drop-in module `diffoper` (reference: src/differential_operators.f90).
 !
-! divergence runs the fused HIP divergence kernel (o3d_divergence); the
-! diagnostic rotational and calculate_Q_criterion (called every nfre steps by
-! the driver) combine derivatives from the drop-in `derivation` pointers on
-! the host, as the reference does.  Boundary types come from the reference's
-! initialization module, whose schemes() the driver calls unchanged.
+! divergence, rotational and calculate_Q_criterion each run one fused HIP
+! kernel (o3d_divergence, o3d_rotational, o3d_q_criterion) that computes all
+! the first derivatives it needs from one read of the fields, with the
+! derivative variants the reference's routines select.  Boundary types come
+! from the reference's initialization module, whose schemes() the driver
+! calls unchanged.
 module diffoper
   use initialization
   use derivation
   use o3d_capi
   implicit none
-  private :: div_x
+  private :: div_x, rot_x, q_x
 
 contains
 
@@ -41,19 +42,19 @@ contains
     real(kind=8), intent(in) :: dx, dy, dz
     real(kind=8), intent(out) :: rotx(:,:,:), roty(:,:,:), rotz(:,:,:)
     integer, intent(in) :: nx, ny, nz
-    real(kind=8), allocatable :: a(:,:,:), b(:,:,:)
-    allocate(a(nx, ny, nz), b(nx, ny, nz))
-    call deryp(a, uz, dy)
-    call derzp(b, uy, dz)
-    rotx = a - b
-    call derzp(a, ux, dz)
-    call derxp(b, uz, dx)
-    roty = a - b
-    call derxp(a, uy, dx)
-    call deryp(b, ux, dy)
-    rotz = a - b
-    deallocate(a, b)
+    call rot_x(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, nz)
   end subroutine rotational
+
+  subroutine rot_x(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(kind=8), intent(in) :: ux(nx, ny, nz), uy(nx, ny, nz), uz(nx, ny, nz)
+    real(kind=8), intent(in) :: dx, dy, dz
+    real(kind=8), intent(out) :: rotx(nx, ny, nz), roty(nx, ny, nz), rotz(nx, ny, nz)
+    integer(c_int) :: bc(3)
+    bc = [int(nbcx1, c_int), int(nbcy1, c_int), int(nbcz1, c_int)]
+    call o3d_check(o3d_rotational(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, int(nx, c_int), &
+         int(ny, c_int), int(nz, c_int), bc), 'rotational')
+  end subroutine rot_x
 
   !> Q = -1/2 (ux_x^2 + uy_y^2 + uz_z^2) - ux_y uy_x - ux_z uz_x - uy_z uz_y
   !> (src/differential_operators.f90:79-108).
@@ -62,20 +63,18 @@ contains
     real(kind=8), intent(in) :: ux(:,:,:), uy(:,:,:), uz(:,:,:)
     real(kind=8), intent(in) :: dx, dy, dz
     real(kind=8), intent(out) :: Q(:,:,:)
-    real(kind=8), allocatable :: g(:,:,:,:)
-    allocate(g(nx, ny, nz, 9))
-    call derxi(g(:,:,:,1), ux, dx)
-    call deryi(g(:,:,:,2), uy, dy)
-    call derzi(g(:,:,:,3), uz, dz)
-    call deryp(g(:,:,:,4), ux, dy)
-    call derzp(g(:,:,:,5), ux, dz)
-    call derxp(g(:,:,:,6), uy, dx)
-    call derzp(g(:,:,:,7), uy, dz)
-    call derxp(g(:,:,:,8), uz, dx)
-    call deryp(g(:,:,:,9), uz, dy)
-    Q = - 0.5d0 * (g(:,:,:,1) ** 2 + g(:,:,:,2) ** 2 + g(:,:,:,3) ** 2) - &
-         g(:,:,:,4) * g(:,:,:,6) - g(:,:,:,5) * g(:,:,:,8) - g(:,:,:,7) * g(:,:,:,9)
-    deallocate(g)
+    call q_x(Q, ux, uy, uz, dx, dy, dz, nx, ny, nz)
   end subroutine calculate_Q_criterion
+
+  subroutine q_x(Q, ux, uy, uz, dx, dy, dz, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(kind=8), intent(in) :: ux(nx, ny, nz), uy(nx, ny, nz), uz(nx, ny, nz)
+    real(kind=8), intent(in) :: dx, dy, dz
+    real(kind=8), intent(out) :: Q(nx, ny, nz)
+    integer(c_int) :: bc(3)
+    bc = [int(nbcx1, c_int), int(nbcy1, c_int), int(nbcz1, c_int)]
+    call o3d_check(o3d_q_criterion(Q, ux, uy, uz, dx, dy, dz, int(nx, c_int), int(ny, c_int), &
+         int(nz, c_int), bc), 'calculate_Q_criterion')
+  end subroutine q_x
 
 end module diffoper

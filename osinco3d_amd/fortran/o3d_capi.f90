@@ -26,6 +26,26 @@ module o3d_capi
        real(c_double) :: stats6(6)
      end function o3d_divergence
 
+     integer(c_int) function o3d_rotational(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, &
+          nz, bc) bind(C, name="o3d_rotational")
+       import :: c_int, c_double
+       real(c_double) :: rotx(*), roty(*), rotz(*)
+       real(c_double), intent(in) :: ux(*), uy(*), uz(*)
+       real(c_double), value :: dx, dy, dz
+       integer(c_int), value :: nx, ny, nz
+       integer(c_int), intent(in) :: bc(3)
+     end function o3d_rotational
+
+     integer(c_int) function o3d_q_criterion(q, ux, uy, uz, dx, dy, dz, nx, ny, nz, bc) &
+          bind(C, name="o3d_q_criterion")
+       import :: c_int, c_double
+       real(c_double) :: q(*)
+       real(c_double), intent(in) :: ux(*), uy(*), uz(*)
+       real(c_double), value :: dx, dy, dz
+       integer(c_int), value :: nx, ny, nz
+       integer(c_int), intent(in) :: bc(3)
+     end function o3d_q_criterion
+
      integer(c_int) function o3d_nu_t(nu_t, ux, uy, uz, dx, dy, dz, cs, delta, nx, ny, nz, &
           bc, stats6) bind(C, name="o3d_nu_t")
        import :: c_int, c_double

@@ -288,3 +288,13 @@ def test_sor_f32_converges_to_fp64_solution(orc):
     # eps reached, or the stagnation rule at FP32's rounding floor; never kmax.
     assert rs in (0, 1) and dm < 1e-4
     assert np.abs((p - p.mean()) - (q - q.mean())).max() <= 1e-3
+
+
+@pytest.mark.parametrize("bc", BCS)
+def test_vortical_diagnostics(orc, bc):
+    """rotational and calculate_Q_criterion in one fused pass, bit for bit."""
+    u = rand_fields((10, 12, 70), 3, seed=43)
+    d = (0.11, 0.13, 0.17)
+    for a, b in zip(capi.rotational(*u, d, bc), orc.rotational(*u, d, bc)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(capi.q_criterion(*u, d, bc), orc.q_criterion(*u, d, bc))

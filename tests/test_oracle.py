@@ -300,3 +300,37 @@ def test_oracle_equals_reference_live(orc, ref, bc):
     ref.set_bc(bc)
     assert np.array_equal(orc.divergence(*u, d, bc, 1), ref.divergence(*u, d, 1))
     assert np.array_equal(orc.nu_t(*u, d, 0.2, 0.3, bc), ref.nu_t(*u, d, 0.2, 0.3))
+    for a, b in zip(orc.rotational(*u, d, bc), ref.rotational(*u, d)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(orc.q_criterion(*u, d, bc), ref.q_criterion(*u, d))
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1)])
+def test_vortical_diagnostics_closed_form(orc, bc):
+    """Taylor-Green field (u = sin x cos y cos z, v = -cos x sin y cos z,
+    w = 0): with the modified wavenumber k1 of the 6th-order D1 the discrete
+    curl is (-k1 cos x sin y sin z, -k1 sin x cos y sin z,
+    2 k1 sin x sin y cos z) and Q = k1^2 cos^2 z (sin^2 x sin^2 y -
+    cos^2 x cos^2 y), exactly (periodic box sampled on 2 pi periods, or
+    free-slip [0, pi]^3 where the parity ghosts reproduce the field)."""
+    n = 16
+    if bc[0] == 0:
+        h = 2 * math.pi / n
+    else:
+        h = math.pi / (n - 1)
+    x = np.arange(n) * h
+    X = x[None, None, :]
+    Y = x[None, :, None]
+    Z = x[:, None, None]
+    ux = np.ascontiguousarray(np.sin(X) * np.cos(Y) * np.cos(Z))
+    uy = np.ascontiguousarray(-np.cos(X) * np.sin(Y) * np.cos(Z))
+    uz = np.zeros_like(ux)
+    k1 = _mw1(h, h)
+    rx, ry, rz = orc.rotational(ux, uy, uz, (h, h, h), bc)
+    assert np.abs(rx + k1 * np.cos(X) * np.sin(Y) * np.sin(Z)).max() < 1e-12
+    assert np.abs(ry + k1 * np.sin(X) * np.cos(Y) * np.sin(Z)).max() < 1e-12
+    assert np.abs(rz - 2 * k1 * np.sin(X) * np.sin(Y) * np.cos(Z)).max() < 1e-12
+    q = orc.q_criterion(ux, uy, uz, (h, h, h), bc)
+    qe = k1 ** 2 * np.cos(Z) ** 2 * (np.sin(X) ** 2 * np.sin(Y) ** 2 -
+                                     np.cos(X) ** 2 * np.cos(Y) ** 2)
+    assert np.abs(q - qe).max() < 1e-12
