@@ -207,6 +207,15 @@ void *o3d_solver_field(void *handle, int which);
  * 1 FP32; the FP32 correction sweeps run two iterations per launch when
  * every axis has at least 8 nodes).  launches2 may be NULL. */
 int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches2);
+/* Adams-Bashforth history for checkpoint / restart (the reference has none:
+ * it restarts with an Euler step).  hist: host array of 9 n elements, float64
+ * in O3D_FP64 and float32 in O3D_MIXED, ordered by age (0 = the newest
+ * right-hand side F, as fux(:,:,:,1) after predict_velocity) then component
+ * (x, y, z); state2 = {history depth (predict calls so far), omega}.
+ * write = 0 reads the solver's history into hist / state2, write = 1 sets it.
+ * With the fields (o3d_solver_upload) this restores a run exactly; the
+ * learned / extrapolated warm start re-primes itself after a restart. */
+int o3d_solver_history(void *handle, int write, void *hist, double *state2);
 /* Attach the learned warm start (an o3d_nn network, include/o3d_nn.h), or
  * NULL to detach; the network is re-sized to the grid coarsened by
  * O3D_WARM_COARSEN (every axis must be a multiple of 8 * O3D_WARM_COARSEN).

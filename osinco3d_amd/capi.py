@@ -79,7 +79,7 @@ EXPORTS = [
     "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
-    "o3d_warm_features",
+    "o3d_warm_features", "o3d_solver_history",
 ]
 NN_EXPORTS = ["o3d_nn_create", "o3d_nn_destroy", "o3d_nn_resize", "o3d_nn_num_params",
               "o3d_nn_get_params", "o3d_nn_set_params", "o3d_nn_get_grads", "o3d_nn_zero_grad",
@@ -103,7 +103,8 @@ def load(path: str = LIB_PATH):
     lib.o3d_solver_field.argtypes = [_vp, C.c_int]
     for name in ("o3d_solver_destroy", "o3d_solver_upload", "o3d_solver_download",
                  "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_sor_timing",
-                 "o3d_solver_set_warm_model", "o3d_solver_capture", "o3d_warm_features"):
+                 "o3d_solver_set_warm_model", "o3d_solver_capture", "o3d_warm_features",
+                 "o3d_solver_history"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

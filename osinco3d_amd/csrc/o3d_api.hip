@@ -1540,6 +1540,35 @@ extern "C" void *o3d_solver_field(void *handle, int which) {
   }
 }
 
+extern "C" int o3d_solver_history(void *handle, int write, void *hist, double *state2) {
+  if (!handle || !hist || !state2) return fail("NULL argument");
+  Solver &S = *(Solver *)handle;
+  const size_t bytes = (size_t)S.n * S.esz();
+  if (write) {
+    const double r = state2[0];
+    if (!(r >= 0.0) || r > 1e9 || r != (double)(long long)r)
+      return fail("history depth %g is not a count", r);
+    S.ring = (int)r;
+    S.omega = state2[1];
+  }
+  for (int age = 0; age < 3; ++age) {
+    const int slot = ((S.ring - 1 - age) % 3 + 3) % 3;
+    for (int c = 0; c < 3; ++c) {
+      char *h = (char *)hist + (size_t)(3 * age + c) * bytes;
+      if (write)
+        HIPCHK(hipMemcpyAsync(S.F[slot][c], h, bytes, hipMemcpyHostToDevice, g_stream));
+      else
+        HIPCHK(hipMemcpyAsync(h, S.F[slot][c], bytes, hipMemcpyDeviceToHost, g_stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(g_stream));
+  if (!write) {
+    state2[0] = S.ring;
+    state2[1] = S.omega;
+  }
+  return 0;
+}
+
 extern "C" int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches2) {
   if (!handle || !ms2 || !sweeps2) return fail("NULL argument");
   Solver &S = *(Solver *)handle;

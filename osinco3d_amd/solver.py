@@ -116,6 +116,7 @@ class Solver:
     def __init__(self, cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
                  inner_max: int = 0):
         self.cfg = cfg
+        self.precision = precision
         self.lib = capi.load()
         self.params = make_params(cfg, precision, warm, inner_max)
         self.handle = C.c_void_p(0)

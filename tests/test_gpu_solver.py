@@ -182,3 +182,34 @@ def test_config1_tgv64_matches_reference():
     assert np.mean(iters) <= ref_iters.mean()
     assert np.all(np.array(iters) <= 1.1 * ref_iters)
     s.close()
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_checkpoint_restart_continues_bit_for_bit(tmp_path, precision):
+    """fields.bin + the AB-history sidecar restore a run exactly: 4 steps,
+    checkpoint, a fresh solver restored from it, 4 more steps == 8 steps in
+    one go (AB3 uses the two previous right-hand sides after the restart)."""
+    from osinco3d_amd import checkpoint
+    cfg = tgv_config(16, 8, 3)
+    a = Solver(cfg, precision)
+    a.init_taylor_green()
+    for it in range(1, 9):
+        a.step(it)
+    ref = a.download()
+    a.close()
+    b = Solver(cfg, precision)
+    b.init_taylor_green()
+    for it in range(1, 5):
+        b.step(it)
+    path = str(tmp_path / "fields.bin")
+    checkpoint.save(b, path, 4, 4 * b.dt)
+    b.close()
+    c = Solver(cfg, precision)
+    itime, t = checkpoint.load(c, path)
+    assert itime == 4 and abs(t - 4 * c.dt) < 1e-15
+    for it in range(itime + 1, 9):
+        c.step(it)
+    got = c.download()
+    c.close()
+    for x, y in zip(got, ref):
+        assert np.array_equal(x, y)
