@@ -122,6 +122,21 @@ int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
                    double dx, double dy, double dz, double re, double t, const int *bc,
                    double *out17);
 
+/* transeq, src/integration.f90:332-468: passive scalar transport.  fphi is
+ * the (nx,ny,nz,3) history (slot 1 = newest, shifted by copy as the
+ * reference does); the p (even) derivative variants; alpha_eff =
+ * 1/(re sc) (+ nu_t/sc with iles = 1, nu_t read only then); the AB update;
+ * then the conservative clipping to [0, 1] with FP64 global sums (their
+ * summation order differs from the Fortran sum's, so clipped values agree to
+ * round-off; without clipping the result is bit-identical).  A zero weight
+ * sum (every value at a bound, where the reference divides 0 by 0) leaves the
+ * clipped field unchanged.  Returns -2 for an unrecognized itscheme. */
+int o3d_transeq(double *phi, const double *ux, const double *uy, const double *uz,
+                const double *src, double *fphi, double re, double sc, const double *adt,
+                const double *bdt, const double *cdt, int itime, int itscheme, double dx,
+                double dy, double dz, int nx, int ny, int nz, int iles, const double *nu_t,
+                const int *bc);
+
 /* rotational(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, nz),
  * src/differential_operators.f90:40-76: the curl from the p (even) D1
  * variants.  One fused pass over (ux, uy, uz); bc as for o3d_divergence. */

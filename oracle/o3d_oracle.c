@@ -246,6 +246,58 @@ int orc_predict(double *ux_pred, double *uy_pred, double *uz_pred,
     return 0;
 }
 
+/* transeq, src/integration.f90:332-468: scalar transport with the p (even)
+ * variants (:415-421), alpha_eff (:405-411), F1 and the AB update
+ * (:422-426), conservative clipping (:428-450, sums in index order) and the
+ * history shift (:453-460).  fphi: 3 planes, slot 1 first.  A zero weight
+ * sum leaves the clipped field unchanged (the reference divides 0 by 0). */
+int orc_transeq(double *phi, const double *ux, const double *uy,
+                const double *uz, const double *src, double *fphi, double re,
+                double sc, const double *adt, const double *bdt,
+                const double *cdt, int itime, int itscheme, double dx,
+                double dy, double dz, int nx, int ny, int nz, int iles,
+                const double *nu_t, const int *bc) {
+    const size_t N = (size_t)nx * ny * nz;
+    double adu, bdu, cdu;
+    if (itscheme == 1 || itime == 1) { adu = adt[0]; bdu = bdt[0]; cdu = cdt[0]; }
+    else if (itscheme == 2 || itime == 2) { adu = adt[1]; bdu = bdt[1]; cdu = cdt[1]; }
+    else if (itscheme == 3) { adu = adt[2]; bdu = bdt[2]; cdu = cdt[2]; }
+    else return 1;
+    const double h[3] = {dx, dy, dz};
+    double *g[3], *l[3];
+    for (int a = 0; a < 3; ++a) {
+        g[a] = malloc(N * sizeof(double));
+        l[a] = malloc(N * sizeof(double));
+        orc_der1(g[a], phi, nx, ny, nz, a, kind_of(bc, a, 0), h[a]);
+        orc_der2(l[a], phi, nx, ny, nz, a, kind_of(bc, a, 0), h[a]);
+    }
+    const double alpha0 = 1.0 / (re * sc);
+    for (size_t q = 0; q < N; ++q) {
+        const double alpha = iles == 1 ? alpha0 + nu_t[q] / sc : alpha0;
+        const double F = alpha * (l[0][q] + l[1][q] + l[2][q]) -
+                         (ux[q] * g[0][q] + uy[q] * g[1][q] + uz[q] * g[2][q]) + src[q];
+        fphi[q] = F;
+        phi[q] = phi[q] + adu * F + bdu * fphi[N + q] + cdu * fphi[2 * N + q];
+    }
+    double s_old = 0.0, s_new = 0.0, s_w = 0.0;
+    for (size_t q = 0; q < N; ++q) s_old += phi[q];
+    for (size_t q = 0; q < N; ++q) {
+        phi[q] = fmax(0.0, fmin(1.0, phi[q]));
+        s_new += phi[q];
+    }
+    const double excess = s_old / (double)N - s_new / (double)N;
+    for (size_t q = 0; q < N; ++q) s_w += fmin(phi[q], 1.0 - phi[q]);
+    if (s_w != 0.0)
+        for (size_t q = 0; q < N; ++q) {
+            const double w = fmin(phi[q], 1.0 - phi[q]) / s_w;
+            phi[q] = fmax(0.0, fmin(1.0, phi[q] + excess * w));
+        }
+    if (itscheme == 3) memcpy(fphi + 2 * N, fphi + N, N * sizeof(double));
+    if (itscheme >= 2) memcpy(fphi + N, fphi, N * sizeof(double));
+    for (int a = 0; a < 3; ++a) { free(g[a]); free(l[a]); }
+    return 0;
+}
+
 /* Neighbour rules of the three SOR solvers (src/poisson.f90:57-92,
  * :186-218, :310-344): 0000 all periodic, 0011 free-slip in y only,
  * 111111 free-slip in x, y and z.  Free-slip: m1 = p1 = 2 at node 1 and

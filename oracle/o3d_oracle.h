@@ -65,4 +65,10 @@ void orc_rotational(double *rx, double *ry, double *rz, const double *ux,
 void orc_q_criterion(double *Q, const double *ux, const double *uy,
                      const double *uz, double dx, double dy, double dz, int nx,
                      int ny, int nz, const int *bc);
+int orc_transeq(double *phi, const double *ux, const double *uy,
+                const double *uz, const double *src, double *fphi, double re,
+                double sc, const double *adt, const double *bdt,
+                const double *cdt, int itime, int itscheme, double dx,
+                double dy, double dz, int nx, int ny, int nz, int iles,
+                const double *nu_t, const int *bc);
 #endif

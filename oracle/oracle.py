@@ -61,6 +61,16 @@ class Oracle:
                                 *_dims(fx), _bc(bc), int(odd))
         return out
 
+    def transeq(self, phi, u, src, fphi, re, sc, adt, bdt, cdt, itime, itscheme, d, iles,
+                nu_t, bc):
+        """In place on phi and fphi (3, nz, ny, nx)."""
+        arr = lambda v: (C.c_double * 3)(*[float(x) for x in v])
+        return self.lib.orc_transeq(_p(phi), _p(u[0]), _p(u[1]), _p(u[2]), _p(src), _p(fphi),
+                                    C.c_double(re), C.c_double(sc), arr(adt), arr(bdt),
+                                    arr(cdt), int(itime), int(itscheme),
+                                    *[C.c_double(v) for v in d], *_dims(phi), int(iles),
+                                    _p(nu_t), _bc(bc))
+
     def rotational(self, ux, uy, uz, d, bc):
         out = [np.empty_like(ux) for _ in range(3)]
         self.lib.orc_rotational(*[_p(o) for o in out], _p(ux), _p(uy), _p(uz),
@@ -159,6 +169,15 @@ class Reference:
         self.lib.ref_divergence(_p(out), _p(fx), _p(fy), _p(fz), *[C.c_double(v) for v in d],
                                 *_dims(fx), int(odd))
         return out
+
+    def transeq(self, phi, u, src, fphi, re, sc, adt, bdt, cdt, itime, itscheme, d, iles,
+                nu_t):
+        """The reference's transeq, in place on phi and fphi."""
+        arr = lambda v: (C.c_double * 3)(*[float(x) for x in v])
+        self.lib.ref_transeq(_p(phi), _p(u[0]), _p(u[1]), _p(u[2]), _p(src), _p(fphi),
+                             C.c_double(re), C.c_double(sc), arr(adt), arr(bdt), arr(cdt),
+                             int(itime), int(itscheme), *[C.c_double(v) for v in d],
+                             *_dims(phi), int(iles), _p(nu_t))
 
     def rotational(self, ux, uy, uz, d):
         out = [np.empty_like(ux) for _ in range(3)]

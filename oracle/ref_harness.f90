@@ -76,6 +76,18 @@ contains
     call divergence(divf, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3, odd)
   end subroutine ref_divergence
 
+  subroutine ref_transeq(ph, u, v, w, sr, fph, rey, schm, a3, b3, c3, it, its, ddx, ddy, ddz, &
+       n1, n2, n3, les, nut) bind(C, name="ref_transeq")
+    integer(c_int), value :: it, its, n1, n2, n3, les
+    real(c_double), value :: rey, schm, ddx, ddy, ddz
+    real(c_double), intent(in) :: a3(3), b3(3), c3(3)
+    real(c_double), intent(in) :: u(n1,n2,n3), v(n1,n2,n3), w(n1,n2,n3), sr(n1,n2,n3)
+    real(c_double), intent(in) :: nut(n1,n2,n3)
+    real(c_double), intent(inout) :: ph(n1,n2,n3), fph(n1,n2,n3,3)
+    call transeq(ph, u, v, w, sr, fph, rey, schm, a3, b3, c3, it, its, ddx, ddy, ddz, &
+         n1, n2, n3, les, nut)
+  end subroutine ref_transeq
+
   subroutine ref_rotational(rx, ry, rz, fx, fy, fz, ddx, ddy, ddz, n1, n2, n3) &
        bind(C, name="ref_rotational")
     integer(c_int), value :: n1, n2, n3

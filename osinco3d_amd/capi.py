@@ -75,7 +75,7 @@ EXPORTS = [
     "o3d_der", "o3d_der_f32", "o3d_divergence", "o3d_nu_t", "o3d_predict_velocity",
     "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity",
     "o3d_function_stats",
-    "o3d_statistics", "o3d_rotational", "o3d_q_criterion", "o3d_solver_create",
+    "o3d_statistics", "o3d_transeq", "o3d_rotational", "o3d_q_criterion", "o3d_solver_create",
     "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
@@ -245,6 +245,17 @@ def function_stats(f):
     st = np.zeros(6)
     check(load().o3d_function_stats(ptr(f), nx, ny, nz, st.ctypes.data_as(_dp)))
     return st
+
+
+def transeq(phi, u, src, fphi, re, sc, adt, bdt, cdt, itime, itscheme, d, iles, nu_t, bc):
+    """Scalar transport (src/integration.f90:332-468) in place on phi and
+    fphi (shape (3, nz, ny, nx), slot 0 = newest)."""
+    nx, ny, nz = _dims(phi)
+    arr = lambda v: (C.c_double * 3)(*[float(x) for x in v])
+    return check(load().o3d_transeq(ptr(phi), ptr(u[0]), ptr(u[1]), ptr(u[2]), ptr(src),
+                                    ptr(fphi), _d(re), _d(sc), arr(adt), arr(bdt), arr(cdt),
+                                    int(itime), int(itscheme), *map(_d, d), nx, ny, nz,
+                                    int(iles), ptr(nu_t), _bc(bc)))
 
 
 def rotational(ux, uy, uz, d, bc):

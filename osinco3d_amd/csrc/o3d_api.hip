@@ -734,6 +734,63 @@ int diag_dev(int mode, double *out0, double *out1, double *out2, const double *u
 }
 }  // namespace
 
+extern "C" int o3d_transeq(double *phi, const double *ux, const double *uy, const double *uz,
+                           const double *src, double *fphi, double re, double sc,
+                           const double *adt, const double *bdt, const double *cdt, int itime,
+                           int itscheme, double dx, double dy, double dz, int nx, int ny, int nz,
+                           int iles, const double *nu_t, const int *bc) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc));
+  if (!adt || !bdt || !cdt) return fail("NULL coefficient array");
+  int ci;
+  if (itscheme == 1 || itime == 1) ci = 0;
+  else if (itscheme == 2 || itime == 2) ci = 1;
+  else if (itscheme == 3) ci = 2;
+  else { fail("itscheme: %d unrecognized", itscheme); return -2; }
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  TranseqArgs<double> a;
+  double *ph, *F;
+  CHK(s.out(phi, n, &ph, true));
+  CHK(s.out(fphi, 3 * n, &F, true));
+  CHK(s.in(ux, n, &a.u[0]));
+  CHK(s.in(uy, n, &a.u[1]));
+  CHK(s.in(uz, n, &a.u[2]));
+  CHK(s.in(src, n, &a.src));
+  a.les = iles == 1;
+  if (a.les) {
+    if (!nu_t) return fail("nu_t is NULL with iles = 1");
+    CHK(s.in(nu_t, n, &a.nu_t));
+  } else {
+    a.nu_t = nullptr;
+  }
+  a.phi = ph;
+  a.phi_out = ph;
+  CHK(s.scratch(n, &a.phi_new));
+  a.f1 = F;
+  a.f2 = F + n;
+  a.f3 = F + 2 * n;
+  a.nh = ci + 1;
+  a.adu = adt[ci]; a.bdu = bdt[ci]; a.cdu = cdt[ci];
+  a.alpha0 = 1.0 / (re * sc);  // src/integration.f90:407,410
+  a.sc = sc;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
+  a.tl = make_tiling(nx, ny, nz);
+  const double h[3] = {dx, dy, dz};
+  fill_coef<double>(a.d1, h, false);
+  fill_coef<double>(a.d2, h, true);
+  CHK(s.scratch(3 * a.tl.blocks(), &a.part));
+  double *sums;
+  CHK(s.scratch(3, &sums));
+  HIPCHK(launch_transeq<double>(a, sums, g_stream));
+  // History shift by copy, as the reference (src/integration.f90:453-460).
+  if (itscheme == 3) HIPCHK(launch_copy<double>(F + 2 * n, F + n, (int)n, g_stream));
+  if (itscheme >= 2) HIPCHK(launch_copy<double>(F + n, F, (int)n, g_stream));
+  return s.finish();
+}
+
 extern "C" int o3d_rotational(double *rotx, double *roty, double *rotz, const double *ux,
                               const double *uy, const double *uz, double dx, double dy,
                               double dz, int nx, int ny, int nz, const int *bc) {

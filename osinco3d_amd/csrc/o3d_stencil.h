@@ -86,6 +86,27 @@ struct DiagArgs {
 };
 
 template <typename T>
+struct TranseqArgs {
+  const T *phi;      // phi before the step (not modified)
+  T *phi_new;        // updated phi before clipping (scratch)
+  T *phi_out;        // clipped phi (may alias phi)
+  const T *u[3];
+  const T *src;
+  const T *nu_t;     // read when les
+  T *f1;             // F of this step
+  const T *f2, *f3;  // F of the two previous steps (read when nh >= 2 / 3)
+  double *part;      // 3 sums per block
+  int nh, les;
+  T adu, bdu, cdu, alpha0, sc;
+  int nx, ny, nz, bc[3];
+  Tiling tl;
+  T d1[3][3], d2[3][3];
+};
+
+template <typename T>
+hipError_t launch_transeq(const TranseqArgs<T> &a, double *sums, hipStream_t s);
+
+template <typename T>
 hipError_t launch_der(int which, T *df, const T *f, int nx, int ny, int nz, double h,
                       hipStream_t s);
 template <typename T>

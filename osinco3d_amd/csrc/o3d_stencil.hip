@@ -648,6 +648,132 @@ __global__ void __launch_bounds__(kTX *kTY) stats_kernel(const StatsArgs<T> A) {
   }
 }
 
+__global__ void __launch_bounds__(1024) finalize_sums_kernel(const double *part, int nparts,
+                                                             int width, double *out);
+
+// ------------------------------------------------------- scalar transport
+// transeq, src/integration.f90:332-468: one z-march computes the p-variant
+// first and second derivatives of phi, the right-hand side F1 = alpha_eff
+// lap(phi) - u.grad(phi) + src, the AB update (into phi_new, before
+// clipping) and per-block FP64 sums of phi_new, of clip(phi_new) and of
+// min(c, 1 - c) with c = clip(phi_new) (the three global sums of the
+// conservative clipping, :428-450); transeq_clip_kernel then redistributes
+// the excess and clips again.
+template <typename T>
+__global__ void __launch_bounds__(kTX *kTY) transeq_kernel(const TranseqArgs<T> A) {
+  __shared__ T pl[kTY + 6][kTX + 6];
+  __shared__ double sv[kNW];
+  const int nx = A.nx, ny = A.ny, nz = A.nz;
+  const TilePos P = tile_pos(A.tl, nx, ny, nz, A.bc);
+  const int plane = nx * ny;
+  const int col = P.gi + nx * P.gj;
+  const int tx = P.tx, ty = P.ty;
+  int hx_col = -1, hx_g = 0;
+  bool hx_f = false;
+  if (tx < 3) { hx_col = tx; hx_g = ghost_safe(P.x0 + tx - 3, nx, A.bc[0], hx_f); }
+  else if (tx >= kTX - 3) { hx_col = tx + 6; hx_g = ghost_safe(P.x0 + tx + 3, nx, A.bc[0], hx_f); }
+  int hy_row = -1, hy_g = 0;
+  bool hy_f = false;
+  if (ty < 3) { hy_row = ty; hy_g = ghost_safe(P.y0 + ty - 3, ny, A.bc[1], hy_f); }
+  else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
+  const int hx_idx = hx_g + nx * P.gj;
+  const int hy_idx = P.gi + nx * hy_g;
+  // phi is even about free-slip walls (p variants): no sign flips; the even
+  // first derivative is forced to zero on the wall nodes.
+  const bool edge_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  const bool edge_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  double s_old = 0.0, s_clip = 0.0, s_w = 0.0;
+  T q[8];
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    bool fz;
+    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
+    q[s] = A.phi[col + plane * kk];
+  }
+  for (int k = P.z0; k < P.z1; ++k) {
+    {
+      bool fz;  // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6
+      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
+      q[7] = A.phi[col + plane * kk];
+    }
+    const int pk = plane * k;
+    pl[ty + 3][tx + 3] = q[3];
+    if (hx_col >= 0) pl[ty + 3][hx_col] = A.phi[hx_idx + pk];
+    if (hy_row >= 0) pl[hy_row][tx + 3] = A.phi[hy_idx + pk];
+    __syncthreads();
+    if (P.valid) {
+      const int idx = P.i + nx * P.j + pk;
+      const int r = ty + 3, cc = tx + 3;
+      const T ux = A.u[0][idx], uy = A.u[1][idx], uz = A.u[2][idx], sr = A.src[idx];
+      T gx = d1<T>(pl[r][cc - 3], pl[r][cc - 2], pl[r][cc - 1], pl[r][cc + 1], pl[r][cc + 2],
+                   pl[r][cc + 3], A.d1[0]);
+      T gy = d1<T>(pl[r - 3][cc], pl[r - 2][cc], pl[r - 1][cc], pl[r + 1][cc], pl[r + 2][cc],
+                   pl[r + 3][cc], A.d1[1]);
+      T gz = d1<T>(q[0], q[1], q[2], q[4], q[5], q[6], A.d1[2]);
+      if (edge_x) gx = T(0);
+      if (edge_y) gy = T(0);
+      if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) gz = T(0);
+      const T lx = d2<T>(pl[r][cc - 2], pl[r][cc - 1], pl[r][cc], pl[r][cc + 1], pl[r][cc + 2],
+                         A.d2[0]);
+      const T ly = d2<T>(pl[r - 2][cc], pl[r - 1][cc], pl[r][cc], pl[r + 1][cc], pl[r + 2][cc],
+                         A.d2[1]);
+      const T lz = d2<T>(q[1], q[2], q[3], q[4], q[5], A.d2[2]);
+      // src/integration.f90:405-411 (alpha_eff), :422-426 (F1, update)
+      const T alpha = A.les ? A.alpha0 + A.nu_t[idx] / A.sc : A.alpha0;
+      const T F = alpha * (lx + ly + lz) - (ux * gx + uy * gy + uz * gz) + sr;
+      T pn = q[3] + A.adu * F;
+      if (A.nh >= 2) pn = pn + A.bdu * A.f2[idx];
+      if (A.nh >= 3) pn = pn + A.cdu * A.f3[idx];
+      A.f1[idx] = F;
+      A.phi_new[idx] = pn;
+      const T c = fmax(T(0), fmin(T(1), pn));
+      s_old += (double)pn;
+      s_clip += (double)c;
+      s_w += (double)fmin(c, T(1) - c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 7; ++s) q[s] = q[s + 1];
+  }
+  const double a = block_sum<kNW>(s_old, sv);
+  const double b = block_sum<kNW>(s_clip, sv);
+  const double c = block_sum<kNW>(s_w, sv);
+  if (threadIdx.x == 0) {
+    A.part[3 * blockIdx.x + 0] = a;
+    A.part[3 * blockIdx.x + 1] = b;
+    A.part[3 * blockIdx.x + 2] = c;
+  }
+}
+
+// phi = clip(c + excess * (w / sum w)), c = clip(phi_new), w = min(c, 1 - c),
+// excess = mean(phi_new) - mean(c) (src/integration.f90:436-450).  A zero
+// weight sum (every value at a bound) leaves c unchanged.
+template <typename T>
+__global__ void __launch_bounds__(256) transeq_clip_kernel(T *phi, const T *phi_new,
+                                                           const double *sums, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T c = fmax(T(0), fmin(T(1), phi_new[i]));
+  const double N = (double)n;
+  const T excess = T(sums[0] / N - sums[1] / N);
+  const T sw = T(sums[2]);
+  T v = c;
+  if (sw != T(0)) {
+    const T w = fmin(c, T(1) - c) / sw;
+    v = c + excess * w;
+  }
+  phi[i] = fmax(T(0), fmin(T(1), v));
+}
+
+template <typename T>
+hipError_t launch_transeq(const TranseqArgs<T> &a, double *sums, hipStream_t s) {
+  transeq_kernel<T><<<dim3(a.tl.blocks()), dim3(kTX * kTY), 0, s>>>(a);
+  finalize_sums_kernel<<<1, 1024, 0, s>>>(a.part, a.tl.blocks(), 3, sums);
+  const int n = a.nx * a.ny * a.nz;
+  transeq_clip_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(a.phi_out, a.phi_new, sums, n);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------- vortical diagnostics
 // rotational (mode 0: out[0..2] = curl u, src/differential_operators.f90:40-76)
 // and calculate_Q_criterion (mode 1: out[0] = Q, :78-108) in one pass over
@@ -846,6 +972,7 @@ template hipError_t launch_correct<float, double>(const CorrectArgs<float, doubl
                                                   hipStream_t);
 template hipError_t launch_statistics<double>(const StatsArgs<double> &, hipStream_t);
 template hipError_t launch_diag<double>(const DiagArgs<double> &, hipStream_t);
+template hipError_t launch_transeq<double>(const TranseqArgs<double> &, double *, hipStream_t);
 template hipError_t launch_statistics<float>(const StatsArgs<float> &, hipStream_t);
 template hipError_t launch_field_stats<double>(const double *, int, FieldPartial *, int,
                                                hipStream_t);

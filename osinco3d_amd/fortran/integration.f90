@@ -8,8 +8,8 @@
 !   correct_velocity  -> o3d_correct_velocity (grad p, update, NaN/max guard)
 ! Host arrays are staged through device memory on every call (the reference
 ! owns all state), so results come back in the caller's arrays exactly as
-! the reference leaves them.  The passive scalar (transeq) is not part of the
-! GPU path: a run with nscr = 1 stops with a message.
+! the reference leaves them.  The passive scalar transeq runs o3d_transeq
+! (derivatives, update, conservative clipping in one kernel chain).
 module integration
   use functions
   use derivation
@@ -150,8 +150,7 @@ contains
     rc = r
   end subroutine correct_x
 
-  !> Passive scalar transport (src/integration.f90:332-468) is outside the
-  !> GPU path.
+  !> Passive scalar transport (src/integration.f90:332-468) -> o3d_transeq.
   subroutine transeq(phi, ux, uy, uz, src, fphi, re, sc, adt, bdt, cdt, &
        itime, itscheme, dx, dy, dz, nx, ny, nz, iles, nu_t)
     integer, intent(in) :: itime, itscheme, nx, ny, nz, iles
@@ -160,8 +159,25 @@ contains
     real(kind=8), dimension(nx,ny,nz), intent(in) :: ux, uy, uz, src, nu_t
     real(kind=8), dimension(nx,ny,nz), intent(inout) :: phi
     real(kind=8), dimension(nx,ny,nz,3), intent(inout) :: fphi
-    print *, "nscr = 1 (passive scalar) is not supported by the GPU path"
-    stop
+    integer(c_int) :: bc(3), rc
+    print *, "* Transport equation"
+    if (itscheme == 1 .or. itime == 1) then
+       print *, " Euler"
+    elseif (itscheme == 2 .or. itime == 2) then
+       print *, " Adams-Bashforth 2"
+    elseif (itscheme == 3) then
+       print *, " Adams-Bashforth 3"
+    end if
+    bc = [int(nbcx1, c_int), int(nbcy1, c_int), int(nbcz1, c_int)]
+    rc = o3d_transeq(phi, ux, uy, uz, src, fphi, re, sc, adt, bdt, cdt, int(itime, c_int), &
+         int(itscheme, c_int), dx, dy, dz, int(nx, c_int), int(ny, c_int), int(nz, c_int), &
+         int(iles, c_int), nu_t, bc)
+    if (rc == -2) then
+       print *, 'itscheme:', itscheme, ' unrecognized'
+       stop
+    end if
+    call o3d_check(rc, 'transeq')
+    print *, ""
   end subroutine transeq
 
 end module integration

@@ -26,6 +26,17 @@ module o3d_capi
        real(c_double) :: stats6(6)
      end function o3d_divergence
 
+     integer(c_int) function o3d_transeq(phi, ux, uy, uz, src, fphi, re, sc, adt, bdt, cdt, &
+          itime, itscheme, dx, dy, dz, nx, ny, nz, iles, nu_t, bc) bind(C, name="o3d_transeq")
+       import :: c_int, c_double
+       real(c_double) :: phi(*), fphi(*)
+       real(c_double), intent(in) :: ux(*), uy(*), uz(*), src(*), nu_t(*)
+       real(c_double), value :: re, sc, dx, dy, dz
+       real(c_double), intent(in) :: adt(3), bdt(3), cdt(3)
+       integer(c_int), value :: itime, itscheme, nx, ny, nz, iles
+       integer(c_int), intent(in) :: bc(3)
+     end function o3d_transeq
+
      integer(c_int) function o3d_rotational(rotx, roty, rotz, ux, uy, uz, dx, dy, dz, nx, ny, &
           nz, bc) bind(C, name="o3d_rotational")
        import :: c_int, c_double

@@ -298,3 +298,31 @@ def test_vortical_diagnostics(orc, bc):
     for a, b in zip(capi.rotational(*u, d, bc), orc.rotational(*u, d, bc)):
         assert np.array_equal(a, b)
     assert np.array_equal(capi.q_criterion(*u, d, bc), orc.q_criterion(*u, d, bc))
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
+@pytest.mark.parametrize("itscheme,iles", [(1, 0), (2, 1), (3, 0), (3, 1)])
+def test_transeq(orc, bc, itscheme, iles):
+    """Scalar transport: bit for bit without clipping, round-off with it."""
+    from tests.test_oracle import _transeq_case
+    sh = (10, 12, 70)
+    d = (0.12, 0.14, 0.16)
+    dt = 0.01
+    adt = [dt, 3 * dt / 2, 23 * dt / 12]
+    bdt = [0.0, -dt / 2, -16 * dt / 12]
+    cdt = [0.0, 0.0, 5 * dt / 12]
+    for clip in (False, True):
+        phi, u, src, nut, fphi = _transeq_case(sh, 17 + itscheme, clip)
+        p_g, f_g = phi.copy(), fphi.copy()
+        p_o, f_o = phi.copy(), fphi.copy()
+        for itime in (1, 2, 3):
+            capi.transeq(p_g, u, src, f_g, 1000.0, 0.7, adt, bdt, cdt, itime, itscheme, d, iles,
+                         nut, bc)
+            orc.transeq(p_o, u, src, f_o, 1000.0, 0.7, adt, bdt, cdt, itime, itscheme, d, iles,
+                        nut, bc)
+            assert np.array_equal(f_g, f_o)
+            if clip:
+                assert np.abs(p_g - p_o).max() <= 1e-14
+                p_g = p_o.copy()
+            else:
+                assert np.array_equal(p_g, p_o), np.abs(p_g - p_o).max()

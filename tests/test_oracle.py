@@ -334,3 +334,65 @@ def test_vortical_diagnostics_closed_form(orc, bc):
     qe = k1 ** 2 * np.cos(Z) ** 2 * (np.sin(X) ** 2 * np.sin(Y) ** 2 -
                                      np.cos(X) ** 2 * np.cos(Y) ** 2)
     assert np.abs(q - qe).max() < 1e-12
+
+
+def _transeq_case(sh, seed, clip):
+    rng = np.random.default_rng(seed)
+    if clip:
+        phi = rng.uniform(-0.2, 1.2, sh)
+    else:
+        phi = 0.5 + 0.2 * rng.standard_normal(sh).clip(-1.4, 1.4)
+    u = [np.ascontiguousarray(0.3 * rng.standard_normal(sh)) for _ in range(3)]
+    src = np.ascontiguousarray(0.01 * rng.standard_normal(sh))
+    nut = np.ascontiguousarray(np.abs(0.01 * rng.standard_normal(sh)))
+    fphi = np.ascontiguousarray(0.05 * rng.standard_normal((3,) + sh))
+    return np.ascontiguousarray(phi), u, src, nut, fphi
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
+@pytest.mark.parametrize("itscheme,iles", [(1, 0), (2, 1), (3, 0), (3, 1)])
+def test_transeq_matches_reference_live(orc, ref, bc, itscheme, iles):
+    """Scalar transport against the reference's transeq: bit for bit while no
+    value leaves [0, 1] (the clipping then moves nothing); with clipping the
+    global sums' order differs, so values agree to round-off."""
+    ref.set_bc(bc)
+    sh = (8, 9, 11)
+    d = (0.12, 0.14, 0.16)
+    dt = 0.01
+    adt = [dt, 3 * dt / 2, 23 * dt / 12]
+    bdt = [0.0, -dt / 2, -16 * dt / 12]
+    cdt = [0.0, 0.0, 5 * dt / 12]
+    for clip in (False, True):
+        phi, u, src, nut, fphi = _transeq_case(sh, 7 + itscheme, clip)
+        p_o, f_o = phi.copy(), fphi.copy()
+        p_r, f_r = phi.copy(), fphi.copy()
+        for itime in (1, 2, 3):
+            assert orc.transeq(p_o, u, src, f_o, 1000.0, 0.7, adt, bdt, cdt, itime, itscheme, d,
+                               iles, nut, bc) == 0
+            ref.transeq(p_r, u, src, f_r, 1000.0, 0.7, adt, bdt, cdt, itime, itscheme, d, iles,
+                        nut)
+            assert np.array_equal(f_o, f_r)
+            if clip:
+                assert np.abs(p_o - p_r).max() <= 1e-14
+                p_r = p_o.copy()   # keep the two runs on the same trajectory
+            else:
+                assert np.array_equal(p_o, p_r), np.abs(p_o - p_r).max()
+
+
+def test_transeq_conserves_and_bounds(orc):
+    """Invariants of the conservative clipping: the result lies in [0, 1]
+    and keeps the mean of the unclipped update (when the redistribution
+    itself needs no second clip)."""
+    sh = (8, 9, 11)
+    phi, u, src, nut, fphi = _transeq_case(sh, 3, True)
+    phi = np.ascontiguousarray(0.5 + 0.55 * (phi - 0.5))   # slight overshoot only
+    d = (0.12, 0.14, 0.16)
+    adt, bdt, cdt = [0.01] * 3, [0.0] * 3, [0.0] * 3
+    # the unclipped update, from the same F
+    p1 = phi.copy()
+    f1 = fphi.copy()
+    orc.transeq(p1, u, src, f1, 1000.0, 0.7, adt, bdt, cdt, 1, 1, d, 0, nut, (0, 0, 0))
+    F = f1[0]
+    mean_unclipped = np.mean(phi + 0.01 * F)
+    assert p1.min() >= 0.0 and p1.max() <= 1.0
+    assert abs(p1.mean() - mean_unclipped) < 1e-12
