@@ -583,7 +583,7 @@ extern "C" int o3d_correct_pression(double *pp, const double *ux_pred, const dou
   a.nx = nx; a.ny = ny; a.nz = nz;
   for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
   a.odd = 1;  // src/integration.f90:235-236
-  a.tl = make_tiling(nx, ny, nz);
+  a.tl = div_tiling<double>(nx, ny, nz);
   const double h[3] = {dx, dy, dz};
   fill_coef<double>(a.d1, h, false);
   a.divisor = dt;
@@ -613,7 +613,7 @@ extern "C" int o3d_correct_velocity(double *ux, double *uy, double *uz, const do
   CHK(s.in(pp, n, &a.p));
   a.nx = nx; a.ny = ny; a.nz = nz;
   for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
-  a.tl = make_tiling(nx, ny, nz);
+  a.tl = correct_tiling<double, double>(nx, ny, nz);
   const double h[3] = {dx, dy, dz};
   fill_coef<double>(a.d1, h, false);
   a.dt = dt;
@@ -810,7 +810,9 @@ struct Solver {
   o3d_params p;
   int n = 0;
   bool mixed = false;
-  Tiling tl;
+  Tiling tl;      // predict
+  Tiling tl_div;  // divergence (its own occupancy)
+  Tiling tl_cor;  // correction
   int kind = 0;
   SorConst sc;
   double omega = 0.0;
@@ -840,7 +842,7 @@ struct Solver {
   float *Feat = nullptr;       // network input, 2 coarse planes
   float *Out = nullptr;        // fine-grid scratch
   float *Outc = nullptr;       // network output (coarse)
-  double *npart = nullptr;     // reduction partials (two residual norms)
+  double *npart = nullptr;     // reduction partials (residual norms, sums of squares)
   double *hnpart = nullptr;    // pinned copy
   double *epart = nullptr;     // sum-of-squares partials of e_prev
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
@@ -923,7 +925,7 @@ int solver_divergence(Solver &S, const void *const f[3], T *out, double divisor,
   a.nx = S.p.nx; a.ny = S.p.ny; a.nz = S.p.nz;
   for (int q = 0; q < 3; ++q) a.bc[q] = S.p.bc[q];
   a.odd = 1;
-  a.tl = S.tl;
+  a.tl = S.tl_div;
   const double h[3] = {S.p.dx, S.p.dy, S.p.dz};
   fill_coef<T>(a.d1, h, false);
   a.divisor = T(divisor);
@@ -939,7 +941,7 @@ int solver_correct(Solver &S) {
   a.part = S.part_cor;
   a.nx = S.p.nx; a.ny = S.p.ny; a.nz = S.p.nz;
   for (int q = 0; q < 3; ++q) a.bc[q] = S.p.bc[q];
-  a.tl = S.tl;
+  a.tl = S.tl_cor;
   const double h[3] = {S.p.dx, S.p.dy, S.p.dz};
   fill_coef<double>(a.d1, h, false);
   a.dt = S.p.dt;
@@ -1224,8 +1226,11 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.Out, n));
   CHK(S.alloc(&S.Feat, 2 * nc));
   CHK(S.alloc(&S.Outc, nc));
-  CHK(S.alloc(&S.npart, 4 * kLwBlocks));
-  HIPCHK(hipHostMalloc((void **)&S.hnpart, 4 * kLwBlocks * sizeof(double)));
+  // Shared by the residual-norm pair (4 per block + 4) and lw_sumsq
+  // (kLwBlocks partials, copied whole to the host).
+  const int np = std::max(4 * residual_blocks(S.p.nx, S.p.ny, S.p.nz) + 4, 2 * kLwBlocks);
+  CHK(S.alloc(&S.npart, np));
+  HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
   CHK(S.alloc(&S.epart, kLwBlocks));
   HIPCHK(hipHostMalloc((void **)&S.hepart, kLwBlocks * sizeof(double)));
   return 0;
@@ -1242,26 +1247,20 @@ LwSums lw_of(Solver &S) {
 int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *out) {
   int per[3];
   solver_periodic(S.kind, per);
-  const double *pp[2] = {p0, p1};
-  for (int q = 0; q < 2; ++q) {
-    double *part = S.npart + 2 * kLwBlocks * q;
-    if (S.mixed)
-      HIPCHK(launch_residual_norm<float>(pp[q], (const float *)S.rhs, part, kLwBlocks, S.p.nx,
-                                         S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz,
-                                         S.sc.A, g_stream));
-    else
-      HIPCHK(launch_residual_norm<double>(pp[q], (const double *)S.rhs, part, kLwBlocks,
-                                          S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy,
-                                          S.sc.oz, S.sc.A, g_stream));
-  }
-  HIPCHK(hipMemcpyAsync(S.hnpart, S.npart, 4 * kLwBlocks * sizeof(double),
-                        hipMemcpyDeviceToHost, g_stream));
+  const int nb = residual_blocks(S.p.nx, S.p.ny, S.p.nz);
+  double *fin = S.npart + 4 * nb;
+  if (S.mixed)
+    HIPCHK(launch_residual_norm_pair<float>(p0, p1, (const float *)S.rhs, S.npart, fin, S.p.nx,
+                                            S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz,
+                                            S.sc.A, g_stream));
+  else
+    HIPCHK(launch_residual_norm_pair<double>(p0, p1, (const double *)S.rhs, S.npart, fin,
+                                             S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy,
+                                             S.sc.oz, S.sc.A, g_stream));
+  HIPCHK(hipMemcpyAsync(S.hnpart, fin, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
-  for (int q = 0; q < 2; ++q) {
-    double s = 0.0;
-    for (int b = 0; b < kLwBlocks; ++b) s += S.hnpart[2 * kLwBlocks * q + 2 * b + 1];
-    out[q] = sqrt(s);
-  }
+  out[0] = sqrt(S.hnpart[1]);
+  out[1] = sqrt(S.hnpart[3]);
   return 0;
 }
 
@@ -1367,7 +1366,7 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   HIPCHK(hipEventRecord(S.ev[2], g_stream));
   CHK(solver_predict<T>(S, itime));
   CHK(solver_divergence<T>(S, S.up, (T *)S.rhs, p.dt, S.part_div));
-  HIPCHK(launch_finalize_partials(S.part_div, S.tl.blocks(), 1, 0, S.fin + 0, g_stream));
+  HIPCHK(launch_finalize_partials(S.part_div, S.tl_div.blocks(), 1, 0, S.fin + 0, g_stream));
   if (p.iles == 1)
     HIPCHK(launch_finalize_partials(S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5, g_stream));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
@@ -1384,9 +1383,10 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   if (learning) CHK(solver_learning_finish(S));
   CHK(solver_correct<T>(S));
   for (int c = 0; c < 3; ++c)
-    HIPCHK(launch_finalize_partials(S.part_cor, S.tl.blocks(), 3, c, S.fin + 2 + c, g_stream));
+    HIPCHK(launch_finalize_partials(S.part_cor, S.tl_cor.blocks(), 3, c, S.fin + 2 + c,
+                                    g_stream));
   CHK(solver_divergence<T>(S, S.u, nullptr, 1.0, S.part_div2));
-  HIPCHK(launch_finalize_partials(S.part_div2, S.tl.blocks(), 1, 0, S.fin + 1, g_stream));
+  HIPCHK(launch_finalize_partials(S.part_div2, S.tl_div.blocks(), 1, 0, S.fin + 1, g_stream));
   HIPCHK(hipEventRecord(S.ev[3], g_stream));
   HIPCHK(hipMemcpyAsync(S.hfin, S.fin, 6 * sizeof(FieldPartial), hipMemcpyDeviceToHost,
                         g_stream));
@@ -1459,6 +1459,10 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
   S->n = p->nx * p->ny * p->nz;
   S->mixed = p->precision == O3D_MIXED;
   S->tl = make_tiling(p->nx, p->ny, p->nz);
+  S->tl_div = S->mixed ? div_tiling<float>(p->nx, p->ny, p->nz)
+                       : div_tiling<double>(p->nx, p->ny, p->nz);
+  S->tl_cor = S->mixed ? correct_tiling<float, double>(p->nx, p->ny, p->nz)
+                       : correct_tiling<double, double>(p->nx, p->ny, p->nz);
   S->kind = kind;
   S->sc = sor_const(p->dx, p->dy, p->dz);
   S->omega = p->omega;
@@ -1478,7 +1482,7 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     rc |= S->alloc(&S->E[1], n);
     rc |= S->alloc(&S->R, n);
   }
-  const int nb = S->tl.blocks();
+  const int nb = std::max(S->tl.blocks(), std::max(S->tl_div.blocks(), S->tl_cor.blocks()));
   if (!rc) rc |= S->alloc(&S->part_div, nb);
   if (!rc) rc |= S->alloc(&S->part_div2, nb);
   if (!rc) rc |= S->alloc(&S->part_nut, nb);

@@ -78,11 +78,15 @@ hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *
                                int ny, int nz, const int per[3], double ox, double oy,
                                double oz, double A, hipStream_t s);
 hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s);
-// FP64 residual norms: out[0] = max |r|, out[1] = sum r^2 (r = rhs - L p).
+// FP64 residual norms of two iterates p0, p1 (r = rhs - L p) in one launch:
+// out (device, 4 doubles) = (max |r0|, sum r0^2, max |r1|, sum r1^2).  part
+// holds 4 * residual_blocks(nx, ny, nz) doubles.
 template <typename TR>
-hipError_t launch_residual_norm(const double *p, const TR *rhs, double *part, int nblocks,
-                                int nx, int ny, int nz, const int per[3], double ox,
-                                double oy, double oz, double A, hipStream_t s);
+hipError_t launch_residual_norm_pair(const double *p0, const double *p1, const TR *rhs,
+                                     double *part, double *out, int nx, int ny, int nz,
+                                     const int per[3], double ox, double oy, double oz, double A,
+                                     hipStream_t s);
+int residual_blocks(int nx, int ny, int nz);
 
 template <typename T>
 hipError_t launch_copy(T *dst, const T *src, int n, hipStream_t s);

@@ -719,39 +719,72 @@ hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s) 
 }
 
 // ------------------------------------------------------ mixed precision
+// The residual kernels march z along one (i, j) column per thread (RZC
+// planes; the p values of planes k-1, k, k+1 stay in registers) over 64 x 4
+// (i, j) blocks, so a launch has many blocks and no index divisions.  The
+// residual keeps the operation order of the red-black sweep's Laplacian.
+constexpr int RBX = 64, RBY = 4, RZC = 8, RNT = RBX * RBY;
+
+dim3 residual_grid(int nx, int ny, int nz) {
+  return dim3((nx + RBX - 1) / RBX, (ny + RBY - 1) / RBY, (nz + RZC - 1) / RZC);
+}
+int residual_blocks(int nx, int ny, int nz) {
+  const dim3 g = residual_grid(nx, ny, nz);
+  return (int)(g.x * g.y * g.z);
+}
+
+// f(offset of (i, j, k), residual) for the planes of this thread's column.
+template <typename TR, typename F>
+__device__ __forceinline__ void residual_column(const double *__restrict__ p,
+                                                const TR *__restrict__ rhs, int nx, int ny,
+                                                int nz, int px, int py, int pz, double ox,
+                                                double oy, double oz, double A, F &&f) {
+  const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
+  const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
+  if (i >= nx || j >= ny) return;
+  const int k0 = blockIdx.z * RZC;
+  const size_t plane = (size_t)nx * ny;
+  const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+  const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+  const int c = i + nx * j;
+  const int cim = im + nx * j, cip = ip + nx * j, cjm = i + nx * jm, cjp = i + nx * jp;
+  double pm = p[c + plane * sor_ghost(k0 - 1, nz, pz)];
+  double pc = p[c + plane * k0];
+#pragma unroll
+  for (int kk = 0; kk < RZC; ++kk) {
+    const int k = k0 + kk;
+    if (k < nz) {
+      const double pp = p[c + plane * sor_ghost(k + 1, nz, pz)];
+      const size_t o = plane * k;
+      const double lap = ox * (p[cim + o] + p[cip + o]) + oy * (p[cjm + o] + p[cjp + o]) +
+                         oz * (pm + pp) + A * pc;
+      f(c + o, (double)rhs[c + o] - lap);
+      pm = pc;
+      pc = pp;
+    }
+  }
+}
+
 template <typename TR>
-__global__ void __launch_bounds__(256) residual_f32_kernel(const double *__restrict__ p,
+__global__ void __launch_bounds__(RNT) residual_f32_kernel(const double *__restrict__ p,
                                                            const TR *__restrict__ rhs,
                                                            float *__restrict__ r,
                                                            float *__restrict__ e, int nx, int ny,
                                                            int nz, int px, int py, int pz,
                                                            double ox, double oy, double oz,
                                                            double A) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = nx * ny * nz;
-  if (idx >= N) return;
-  const int i = idx % nx, j = (idx / nx) % ny, k = idx / (nx * ny);
-  const int plane = nx * ny;
-  const int base = idx - i - nx * j - plane * k;
-  (void)base;
-  const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
-  const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
-  const int km = sor_ghost(k - 1, nz, pz), kp = sor_ghost(k + 1, nz, pz);
-  const int rowk = plane * k, rowj = nx * j;
-  const double lap = ox * (p[im + rowj + rowk] + p[ip + rowj + rowk]) +
-                     oy * (p[i + nx * jm + rowk] + p[i + nx * jp + rowk]) +
-                     oz * (p[i + rowj + plane * km] + p[i + rowj + plane * kp]) + A * p[idx];
-  r[idx] = (float)((double)rhs[idx] - lap);
-  e[idx] = 0.0f;
+  residual_column(p, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A, [&](size_t o, double res) {
+    r[o] = (float)res;
+    e[o] = 0.0f;
+  });
 }
 
 template <typename TR>
 hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *e, int nx, int ny,
                                int nz, const int per[3], double ox, double oy, double oz,
                                double A, hipStream_t s) {
-  const int N = nx * ny * nz;
-  residual_f32_kernel<TR><<<(N + 255) / 256, 256, 0, s>>>(p, rhs, r, e, nx, ny, nz, per[0],
-                                                          per[1], per[2], ox, oy, oz, A);
+  residual_f32_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
+      p, rhs, r, e, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
   return hipGetLastError();
 }
 
@@ -766,40 +799,64 @@ hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Residual norms of two iterates against one rhs in one launch: per block
+// (max |r0|, sum r0^2, max |r1|, sum r1^2).
 template <typename TR>
-__global__ void __launch_bounds__(256) residual_norm_kernel(const double *__restrict__ p,
-                                                            const TR *__restrict__ rhs,
-                                                            double *part, int nx, int ny, int nz,
-                                                            int px, int py, int pz, double ox,
-                                                            double oy, double oz, double A) {
-  __shared__ double sc[4];
-  const int N = nx * ny * nz;
-  const int plane = nx * ny;
-  double mx = 0.0, ss = 0.0;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += gridDim.x * blockDim.x) {
-    const int i = idx % nx, j = (idx / nx) % ny, k = idx / plane;
-    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
-    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
-    const int km = sor_ghost(k - 1, nz, pz), kp = sor_ghost(k + 1, nz, pz);
-    const int rowk = plane * k, rowj = nx * j;
-    const double lap = ox * (p[im + rowj + rowk] + p[ip + rowj + rowk]) +
-                       oy * (p[i + nx * jm + rowk] + p[i + nx * jp + rowk]) +
-                       oz * (p[i + rowj + plane * km] + p[i + rowj + plane * kp]) + A * p[idx];
-    const double r = (double)rhs[idx] - lap;
-    mx = fmax(mx, fabs(r));
-    ss += r * r;
+__global__ void __launch_bounds__(RNT) residual_norm_pair_kernel(
+    const double *__restrict__ p0, const double *__restrict__ p1, const TR *__restrict__ rhs,
+    double *__restrict__ part, int nx, int ny, int nz, int px, int py, int pz, double ox,
+    double oy, double oz, double A) {
+  __shared__ double sc[RNT / O3D_WAVE];
+  double m0 = 0.0, s0 = 0.0, m1 = 0.0, s1 = 0.0;
+  residual_column(p0, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A, [&](size_t, double r) {
+    m0 = fmax(m0, fabs(r));
+    s0 += r * r;
+  });
+  residual_column(p1, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A, [&](size_t, double r) {
+    m1 = fmax(m1, fabs(r));
+    s1 += r * r;
+  });
+  m0 = block_max<RNT / O3D_WAVE>(m0, sc);
+  s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
+  m1 = block_max<RNT / O3D_WAVE>(m1, sc);
+  s1 = block_sum<RNT / O3D_WAVE>(s1, sc);
+  if (threadIdx.x == 0) {
+    const size_t b =
+        blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+    part[4 * b] = m0;
+    part[4 * b + 1] = s0;
+    part[4 * b + 2] = m1;
+    part[4 * b + 3] = s1;
   }
-  mx = block_max<4>(mx, sc);
-  ss = block_sum<4>(ss, sc);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = mx; part[2 * blockIdx.x + 1] = ss; }
+}
+
+// out[0..3] = (max, sum, max, sum) over the block records, in a fixed order.
+__global__ void __launch_bounds__(1024) residual_norm_final_kernel(const double *__restrict__ part,
+                                                                  int nb, double *out) {
+  __shared__ double sc[1024 / O3D_WAVE];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    v[0] = fmax(v[0], part[4 * b]);
+    v[1] += part[4 * b + 1];
+    v[2] = fmax(v[2], part[4 * b + 2]);
+    v[3] += part[4 * b + 3];
+  }
+  v[0] = block_max<1024 / O3D_WAVE>(v[0], sc);
+  v[1] = block_sum<1024 / O3D_WAVE>(v[1], sc);
+  v[2] = block_max<1024 / O3D_WAVE>(v[2], sc);
+  v[3] = block_sum<1024 / O3D_WAVE>(v[3], sc);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; ++q) out[q] = v[q];
 }
 
 template <typename TR>
-hipError_t launch_residual_norm(const double *p, const TR *rhs, double *part, int nblocks, int nx,
-                                int ny, int nz, const int per[3], double ox, double oy,
-                                double oz, double A, hipStream_t s) {
-  residual_norm_kernel<TR><<<nblocks, 256, 0, s>>>(p, rhs, part, nx, ny, nz, per[0], per[1],
-                                                   per[2], ox, oy, oz, A);
+hipError_t launch_residual_norm_pair(const double *p0, const double *p1, const TR *rhs,
+                                     double *part, double *out, int nx, int ny, int nz,
+                                     const int per[3], double ox, double oy, double oz, double A,
+                                     hipStream_t s) {
+  residual_norm_pair_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
+      p0, p1, rhs, part, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
+  residual_norm_final_kernel<<<1, 1024, 0, s>>>(part, residual_blocks(nx, ny, nz), out);
   return hipGetLastError();
 }
 
@@ -859,12 +916,14 @@ template hipError_t launch_residual_f32<double>(const double *, const double *, 
 template hipError_t launch_residual_f32<float>(const double *, const float *, float *, float *,
                                                int, int, int, const int *, double, double, double,
                                                double, hipStream_t);
-template hipError_t launch_residual_norm<double>(const double *, const double *, double *, int,
-                                                 int, int, int, const int *, double, double,
-                                                 double, double, hipStream_t);
-template hipError_t launch_residual_norm<float>(const double *, const float *, double *, int, int,
-                                                int, int, const int *, double, double, double,
-                                                double, hipStream_t);
+template hipError_t launch_residual_norm_pair<double>(const double *, const double *,
+                                                      const double *, double *, double *, int,
+                                                      int, int, const int *, double, double,
+                                                      double, double, hipStream_t);
+template hipError_t launch_residual_norm_pair<float>(const double *, const double *,
+                                                     const float *, double *, double *, int, int,
+                                                     int, const int *, double, double, double,
+                                                     double, hipStream_t);
 template hipError_t launch_copy<double>(double *, const double *, int, hipStream_t);
 template hipError_t launch_copy<float>(float *, const float *, int, hipStream_t);
 template hipError_t launch_convert<float, double>(float *, const double *, int, hipStream_t);

@@ -26,6 +26,14 @@ struct Tiling {
 constexpr int kTX = 64;  // one wave per x-row
 constexpr int kTY = 8;
 Tiling make_tiling(int nx, int ny, int nz);
+// Tiling whose grid fills whole rounds of `slots` resident blocks.
+Tiling make_tiling_slots(int nx, int ny, int nz, int slots);
+// The tilings the divergence and correction kernels launch with (from
+// their occupancy on the current device).
+template <typename T>
+Tiling div_tiling(int nx, int ny, int nz);
+template <typename T, typename TP>
+Tiling correct_tiling(int nx, int ny, int nz);
 
 template <typename T>
 struct PredictArgs {

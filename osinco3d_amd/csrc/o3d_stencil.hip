@@ -19,6 +19,7 @@
 #include <limits.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "o3d_stencil.h"
 
@@ -34,6 +35,43 @@ Tiling make_tiling(int nx, int ny, int nz) {
   t.zc = (nz + ntz - 1) / ntz;
   t.ntz = (nz + t.zc - 1) / t.zc;
   return t;
+}
+
+// Tiling for a kernel with `slots` concurrently resident blocks on the
+// device: the z chunk count that minimises rounds * (zc + 8) (a chunk
+// re-reads 6 planes for its z stencil and prefetches 2 past its end), so the
+// grid fills whole rounds of resident blocks.
+Tiling make_tiling_slots(int nx, int ny, int nz, int slots) {
+  Tiling t;
+  t.ntx = (nx + kTX - 1) / kTX;
+  t.nty = (ny + kTY - 1) / kTY;
+  const int layer = t.ntx * t.nty;
+  if (slots <= 0) return make_tiling(nx, ny, nz);
+  double best = 1e30;
+  int best_ntz = 1;
+  for (int ntz = 1; ntz <= std::max(1, nz / 8); ++ntz) {
+    const int zc = (nz + ntz - 1) / ntz;
+    const int n_t = (nz + zc - 1) / zc;
+    const int rounds = (layer * n_t + slots - 1) / slots;
+    const double cost = (double)rounds * (zc + 8.0);
+    if (cost < best - 1e-9) { best = cost; best_ntz = ntz; }
+  }
+  t.zc = (nz + best_ntz - 1) / best_ntz;
+  t.ntz = (nz + t.zc - 1) / t.zc;
+  return t;
+}
+
+// Resident blocks of `kernel` (kTX*kTY threads) over the device, or 0.
+static int stencil_slots(const void *kernel) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTX * kTY, 0) !=
+          hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return per_cu * cus;
 }
 
 // Ghost mapping that stays in bounds for threads of a partial tile that lie
@@ -377,37 +415,54 @@ __global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
 
   FieldPartial st;
   partial_init(st);
-  // w column queue (planes k-3..k+3) plus the plane k+4 in flight, and the
-  // u / v values of plane k+1, loaded one step before they are staged.
-  T qw[8];
+  // w column queue (planes k-3..k+3, signs applied) and raw loads in flight
+  // in a two-slot ring: the load of plane k+5 is issued at step k and moved
+  // into the queue at the end of step k+1.  u / v values of plane k+2 are
+  // loaded at step k and staged (with their sign) at step k+2.  Slots are
+  // compile-time constants of the unrolled pair of steps, and no register
+  // that a load is still writing is read before its use, so the loads of
+  // two steps stay in flight across the barriers.
+  const bool neg_u = odd && P.fi, neg_v = odd && P.fj;
+  const bool neg_hx = odd && hx_f, neg_hy = odd && hy_f;
+  auto sgn = [](T v, bool neg) { return neg ? -v : v; };
+  auto wz = [&](int q, bool &neg) {
+    bool fz;
+    const int kk = ghost_index(q, nz, A.bc[2], fz);
+    neg = odd && fz;
+    return col + plane * kk;
+  };
+  T qw[7];
 #pragma unroll
   for (int s = 0; s < 7; ++s) {
-    bool fz;
-    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
-    qw[s] = load_ghost(A.f[2], col + plane * kk, odd && fz);
+    bool ng;
+    const int ix = wz(P.z0 - 3 + s, ng);
+    qw[s] = load_ghost(A.f[2], ix, ng);
   }
-  T nu, nv, nhx = T(0), nhy = T(0);
-  auto load_uv = [&](int kp) {
-    const int pk = plane * min(kp, nz - 1);  // the plane after the last is not used
-    nu = load_ghost(A.f[0], col + pk, odd && P.fi);
-    nv = load_ghost(A.f[1], col + pk, odd && P.fj);
-    if (hx_col >= 0) nhx = load_ghost(A.f[0], hx_idx + pk, odd && hx_f);
-    if (hy_row >= 0) nhy = load_ghost(A.f[1], hy_idx + pk, odd && hy_f);
+  T wr[2];
+  {
+    bool ng;
+    wr[1] = A.f[2][wz(P.z0 + 4, ng)];
+  }
+  T nu[2], nv[2], nhx[2] = {T(0), T(0)}, nhy[2] = {T(0), T(0)};
+  // Every lane issues the halo loads (lanes without a halo cell load their
+  // own column), so the number of loads in flight is the same on every path.
+  const int hxi = hx_col >= 0 ? hx_idx : col, hyi = hy_row >= 0 ? hy_idx : col;
+  auto load_uv = [&](int kp, int sl) {
+    const int pk = plane * min(kp, nz - 1);  // planes after the last are not used
+    nu[sl] = A.f[0][col + pk];
+    nv[sl] = A.f[1][col + pk];
+    nhx[sl] = A.f[0][hxi + pk];
+    nhy[sl] = A.f[1][hyi + pk];
   };
-  load_uv(P.z0);
-  for (int k = P.z0; k < P.z1; ++k) {
+  load_uv(P.z0, 0);
+  load_uv(P.z0 + 1, 1);
+  auto step = [&](int k, auto SL) {
+    constexpr int sl = decltype(SL)::value;
     const int pk = plane * k;
-    su[ty][tx + 3] = nu;
-    sv[ty + 3][tx] = nv;
-    if (hx_col >= 0) su[ty][hx_col] = nhx;
-    if (hy_row >= 0) sv[hy_row][tx] = nhy;
-    {
-      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
-      bool fz;
-      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
-      qw[7] = load_ghost(A.f[2], col + plane * kk, odd && fz);
-    }
-    load_uv(k + 1);
+    su[ty][tx + 3] = sgn(nu[sl], neg_u);
+    sv[ty + 3][tx] = sgn(nv[sl], neg_v);
+    if (hx_col >= 0) su[ty][hx_col] = sgn(nhx[sl], neg_hx);
+    if (hy_row >= 0) sv[hy_row][tx] = sgn(nhy[sl], neg_hy);
     __syncthreads();
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
@@ -424,11 +479,34 @@ __global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
       if (A.out) A.out[idx] = div / A.divisor;
       partial_add(st, (double)div, idx);
     }
+    {
+      // k + 5 <= nz + 4 stays a valid mirror / wrap index for nz >= 6.
+      bool ng;
+      wr[sl] = A.f[2][wz(k + 5, ng)];
+    }
+    load_uv(k + 2, sl);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 7; ++s) qw[s] = qw[s + 1];
+    for (int s = 0; s < 6; ++s) qw[s] = qw[s + 1];
+    bool ng;
+    (void)wz(k + 4, ng);
+    qw[6] = sgn(wr[1 - sl], ng);
+  };
+  // Whole pairs in the loop (every back edge leaves slot 1's loads last in
+  // flight), the odd last plane after it.
+  int k = P.z0;
+  for (; k + 1 < P.z1; k += 2) {
+    step(k, std::integral_constant<int, 0>{});
+    step(k + 1, std::integral_constant<int, 1>{});
   }
+  if (k < P.z1) step(k, std::integral_constant<int, 0>{});
   partial_store(st, A.part + blockIdx.x, sc);
+}
+
+template <typename T>
+Tiling div_tiling(int nx, int ny, int nz) {
+  static const int slots = stencil_slots((const void *)div_kernel<T>);
+  return make_tiling_slots(nx, ny, nz, slots);
 }
 
 template <typename T>
@@ -466,42 +544,51 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
 
   FieldPartial st[3];
   partial_init(st[0]); partial_init(st[1]); partial_init(st[2]);
-  // p column queue (planes k-3..k+3) plus plane k+4 in flight; the halo
-  // values of plane k+1 are loaded one step before they are staged.
-  TP q[8];
-#pragma unroll
-  for (int s = 0; s < 7; ++s) {
+  // p column queue (planes k-3..k+3) and raw loads in flight in two-slot
+  // rings (slots fixed at compile time by the unrolled pair of steps): the
+  // load of plane k+5 is issued at step k and joins the queue at the end of
+  // step k+1; the halo and u* values of plane k+2 are loaded at step k and
+  // used at step k+2.  No register that a load is still writing is read
+  // before its use.
+  auto pz = [&](int qz) {
     bool fz;
-    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
-    q[s] = A.p[col + plane * kk];
-  }
-  TP nhx = TP(0), nhy = TP(0);
-  auto load_halo = [&](int kp) {
-    const int pk = plane * min(kp, nz - 1);  // the plane after the last is not used
-    if (hx_col >= 0) nhx = A.p[hx_idx + pk];
-    if (hy_row >= 0) nhy = A.p[hy_idx + pk];
+    return col + plane * ghost_index(qz, nz, A.bc[2], fz);
   };
-  load_halo(P.z0);
-  for (int k = P.z0; k < P.z1; ++k) {
+  TP q[7];
+#pragma unroll
+  for (int s = 0; s < 7; ++s) q[s] = A.p[pz(P.z0 - 3 + s)];
+  TP pr[2];
+  pr[1] = A.p[pz(P.z0 + 4)];
+  TP nhx[2] = {TP(0), TP(0)}, nhy[2] = {TP(0), TP(0)};
+  // Every lane issues the halo loads (lanes without a halo cell load their
+  // own column), so the number of loads in flight is the same on every path.
+  const int hxi = hx_col >= 0 ? hx_idx : col, hyi = hy_row >= 0 ? hy_idx : col;
+  auto load_halo = [&](int kp, int sl) {
+    const int pk = plane * min(kp, nz - 1);  // planes after the last are not used
+    nhx[sl] = A.p[hxi + pk];
+    nhy[sl] = A.p[hyi + pk];
+  };
+  T upv[2][3];
+  const int own = P.valid ? P.i + nx * P.j : 0;
+  auto load_up = [&](int kp, int sl) {
+    const int idx = own + plane * min(kp, nz - 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) upv[sl][c] = A.up[c][idx];
+  };
+  load_halo(P.z0, 0);
+  load_halo(P.z0 + 1, 1);
+  load_up(P.z0, 0);
+  load_up(P.z0 + 1, 1);
+  auto step = [&](int k, auto SL) {
+    constexpr int sl = decltype(SL)::value;
     const int pk = plane * k;
     pl[ty + 3][tx + 3] = q[3];
-    if (hx_col >= 0) pl[ty + 3][hx_col] = nhx;
-    if (hy_row >= 0) pl[hy_row][tx + 3] = nhy;
-    {
-      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
-      bool fz;
-      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
-      q[7] = A.p[col + plane * kk];
-    }
-    load_halo(k + 1);
+    if (hx_col >= 0) pl[ty + 3][hx_col] = nhx[sl];
+    if (hy_row >= 0) pl[hy_row][tx + 3] = nhy[sl];
     __syncthreads();
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
       const int cc = tx + 3, r = ty + 3;
-      // u* loads issued before the derivative arithmetic that hides them.
-      T upv[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) upv[c] = A.up[c][idx];
       TP g[3];
       g[0] = d1<TP>(pl[r][cc - 3], pl[r][cc - 2], pl[r][cc - 1], pl[r][cc + 1], pl[r][cc + 2],
                     pl[r][cc + 3], A.d1[0]);
@@ -513,17 +600,36 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
       if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) g[2] = TP(0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T un = T(TP(upv[c]) - A.dt * g[c]);
+        const T un = T(TP(upv[sl][c]) - A.dt * g[c]);
         A.u[c][idx] = un;
         partial_add(st[c], (double)un, idx);
       }
     }
+    // k + 5 <= nz + 4 stays a valid mirror / wrap index for nz >= 6.
+    pr[sl] = A.p[pz(k + 5)];
+    load_halo(k + 2, sl);
+    load_up(k + 2, sl);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < 7; ++s) q[s] = q[s + 1];
+    for (int s = 0; s < 6; ++s) q[s] = q[s + 1];
+    q[6] = pr[1 - sl];
+  };
+  // Whole pairs in the loop (every back edge leaves slot 1's loads last in
+  // flight), the odd last plane after it.
+  int k = P.z0;
+  for (; k + 1 < P.z1; k += 2) {
+    step(k, std::integral_constant<int, 0>{});
+    step(k + 1, std::integral_constant<int, 1>{});
   }
+  if (k < P.z1) step(k, std::integral_constant<int, 0>{});
 #pragma unroll
   for (int c = 0; c < 3; ++c) partial_store(st[c], A.part + 3 * blockIdx.x + c, sc);
+}
+
+template <typename T, typename TP>
+Tiling correct_tiling(int nx, int ny, int nz) {
+  static const int slots = stencil_slots((const void *)correct_kernel<T, TP>);
+  return make_tiling_slots(nx, ny, nz, slots);
 }
 
 template <typename T, typename TP>
@@ -964,6 +1070,10 @@ template hipError_t launch_der<float>(int, float *, const float *, int, int, int
                                       hipStream_t);
 template hipError_t launch_predict<double>(const PredictArgs<double> &, int, bool, hipStream_t);
 template hipError_t launch_predict<float>(const PredictArgs<float> &, int, bool, hipStream_t);
+template Tiling div_tiling<double>(int, int, int);
+template Tiling div_tiling<float>(int, int, int);
+template Tiling correct_tiling<double, double>(int, int, int);
+template Tiling correct_tiling<float, double>(int, int, int);
 template hipError_t launch_divergence<double>(const DivArgs<double> &, hipStream_t);
 template hipError_t launch_divergence<float>(const DivArgs<float> &, hipStream_t);
 template hipError_t launch_correct<double, double>(const CorrectArgs<double, double> &,
