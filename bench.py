@@ -17,12 +17,14 @@ Each requested form is timed over the same steps:
 The JSON line's value is the steps/s of the headline form (the last of
 --forms); every form's figures are included under "forms".
 
-roofline: the dominant kernel of the headline form is the red-black SOR
-kernel.  Its algorithmic bytes per launch are (read p, read rhs, write p) =
-3 values per cell: 24 B/cell in FP64, 12 B/cell in FP32, times the cell
-count; the FP32 kernel runs two iterations per launch for that traffic.  Its
-average launch duration is measured with HIP events on the launch stream
-over complete batches of launches inside the timed steps.
+roofline: the dominant work of the headline form is the red-black SOR
+iteration (in the mixed forms the FP32 colour-split red and black launches,
+sor_split_kernel<0> and <1>).  Its algorithmic bytes per iteration are
+(read p, read rhs, write p) = 3 values per cell: 24 B/cell in FP64, 12 B/cell
+in FP32, times the cell count.  Its average duration is measured with HIP
+events on the launch stream over complete batches of iterations inside the
+timed steps; `traffic` is the measured HBM bytes of one iteration (both
+launches) from the committed rocprofv3 --pmc summary.
 
 cpu_baseline: the reference Fortran (oracle/_ref/o3d_ref_run, built from the
 reference sources at -O3, single-threaded) on one host core: a bounded sample
@@ -163,10 +165,12 @@ def run_form(form, cfg, steps, warmup, model_path, init=None):
         "enstrophy": st[4],
         "t_end": t_end,
     }
-    # SOR kernel: average launch duration from events over complete batches.
-    # One launch reads the iterate and the rhs and writes the new iterate
-    # once (24 B/cell FP64, 12 B/cell FP32), whether it runs one iteration
-    # (sor_kernel) or two (the FP32 pair kernel).
+    # SOR iterations: average duration from events over complete batches of
+    # launches.  The algorithmic bytes of one iteration are one read of the
+    # iterate and the rhs and one write of the iterate (24 B/cell FP64,
+    # 12 B/cell FP32).  An FP64 iteration is one sor_kernel launch; an FP32
+    # iteration of the mixed forms is the red and the black colour-split
+    # launch (sor_split_kernel<0>, <1>), which together move 16 B/cell.
     for q, name, bpc in ((0, "sor_f64", 24), (1, "sor_f32", 12)):
         if sor_ln[q] > 0:
             avg_ms = sor_ms[q] / sor_ln[q]
@@ -339,7 +343,9 @@ def main():
     dom = head.get("sor_" + which)
     roofline = None
     if dom:
-        roofline = {"bound": "hbm", "kernel": f"sor_kernel (red-black iteration, {which})",
+        kname = ("sor_split_kernel<0> + <1> (one colour-split red-black iteration, f32)"
+                 if which == "f32" else "sor_kernel (one red-black iteration, f64)")
+        roofline = {"bound": "hbm", "kernel": kname,
                     "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": dom["achieved_GBps"] / HBM_PEAK_GBS,
                     "traffic": measured_traffic(which),

@@ -86,13 +86,15 @@ int o3d_poisson(double *pp, const double *rhs, double dx, double dy, double dz, 
                 int *exit_iter, double *dmax, int *reason);
 
 /* The red-black SOR of o3d_poisson in FP32 (the sweeps of the mixed form),
- * same rules and outputs.  pair = 1 runs two iterations per kernel launch
- * (every axis >= 8): the iterates are bit-identical to pair = 0, the stop
- * rules are checked after every second iteration.  No counterpart in the
- * reference (its solver is FP64). */
+ * same rules and outputs.  variant selects the kernels, all with
+ * bit-identical iterates: 0 = one fused red+black iteration per launch;
+ * 1 = two iterations per launch (every axis >= 8; the stop rules are checked
+ * after every second iteration); 2 = colour-split storage, one in-place
+ * launch per colour (nx a multiple of 8; the form the mixed solver uses).
+ * No counterpart in the reference (its solver is FP64). */
 int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz, int nx, int ny,
-                    int nz, double *omega, double eps, int kmax, int idyn, int solver, int pair,
-                    int *exit_iter, double *dmax, int *reason);
+                    int nz, double *omega, double eps, int kmax, int idyn, int solver,
+                    int variant, int *exit_iter, double *dmax, int *reason);
 
 /* correct_pression, src/integration.f90:199-255 (SOR branch): divergence of
  * the predicted velocity (odd variants), rhs = div/dt, then the Poisson solver

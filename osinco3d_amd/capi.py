@@ -203,8 +203,10 @@ def poisson(pp, rhs, d, omega, eps, kmax, idyn, solver):
     return it.value, dm.value, rs.value, om.value
 
 
-def poisson_f32(pp, rhs, d, omega, eps, kmax, idyn, solver, pair=False):
-    """FP32 red-black SOR; returns (exit_iter, dmax, reason, omega)."""
+def poisson_f32(pp, rhs, d, omega, eps, kmax, idyn, solver, variant=0):
+    """FP32 red-black SOR; returns (exit_iter, dmax, reason, omega).
+    variant: 0 fused iteration per launch, 1 two iterations per launch,
+    2 colour-split half sweeps (identical iterates)."""
     nx, ny, nz = _dims(pp)
     om = C.c_double(omega)
     it = C.c_int(0)
@@ -212,7 +214,7 @@ def poisson_f32(pp, rhs, d, omega, eps, kmax, idyn, solver, pair=False):
     rs = C.c_int(0)
     code = {"0000": 0, "0011": 1, "111111": 2}.get(solver, solver)
     check(load().o3d_poisson_f32(ptr(pp), ptr(rhs), *map(_d, d), nx, ny, nz, C.byref(om),
-                                 _d(eps), int(kmax), int(idyn), int(code), int(bool(pair)),
+                                 _d(eps), int(kmax), int(idyn), int(code), int(variant),
                                  C.byref(it), C.byref(dm), C.byref(rs)))
     return it.value, dm.value, rs.value, om.value
 

@@ -196,7 +196,8 @@ SorConst sor_const(double dx, double dy, double dz) {
 // Batched launcher of red-black iterations with device-side stop control.
 // Buffers: buf[0] holds the start iterate; launch L reads buf[L & 1] and
 // writes buf[(L + 1) & 1].  With `pair` (FP32 only) each launch runs two
-// iterations (launch_sor2_iteration).
+// iterations (launch_sor2_iteration).  With `split` (FP32 only) each launch
+// is one iteration of the colour-split in-place sweeps on `sa` (buf unused).
 template <typename T, typename TR>
 struct SorRun {
   SorArgs<T, TR> a;
@@ -204,6 +205,8 @@ struct SorRun {
   SorState *hst = nullptr;   // pinned host copy
   T *buf[2] = {nullptr, nullptr};
   bool pair = false;
+  bool split = false;
+  SorSplitArgs sa{};
   // Timing of complete batches (all launches did their iterations).
   double ms_full = 0.0;
   int sweeps_full = 0;
@@ -212,6 +215,10 @@ struct SorRun {
 
   hipError_t launch() {
     if constexpr (std::is_same<T, float>::value && std::is_same<TR, float>::value) {
+      if (split) {
+        sa.it = a.it;
+        return launch_sor_split_iteration(sa, g_stream);
+      }
       if (pair) return launch_sor2_iteration(a, g_stream);
     }
     return launch_sor_iteration<T, TR>(a, g_stream);
@@ -222,6 +229,7 @@ struct SorRun {
 
   int run(double omega, int first_batch, int *sweeps_out) {
     HIPCHK(launch_sor_init(st, omega, 0, g_stream));
+    if (split) HIPCHK(launch_sor_split_init(sa.dslot, g_stream));
     const int step = pair ? 2 : 1;
     int it = 1, L = 0;
     int batch = std::max(1, first_batch / step);
@@ -474,15 +482,43 @@ extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz
   return s.finish();
 }
 
+// The colour-split sweep arguments matching `a` (red, black and the rhs
+// halves already in split storage).
+SorSplitArgs split_args(const SorArgs<float, float> &a, float *red, float *black,
+                        const float *rhs_red, const float *rhs_black,
+                        unsigned long long *dslot) {
+  SorSplitArgs sa{};
+  sa.red = red;
+  sa.black = black;
+  sa.rhs_red = rhs_red;
+  sa.rhs_black = rhs_black;
+  sa.st = a.st;
+  sa.dslot = dslot;
+  sa.log = a.log;
+  sa.log_len = a.log_len;
+  sa.nx = a.nx; sa.ny = a.ny; sa.nz = a.nz;
+  for (int q = 0; q < 3; ++q) sa.per[q] = a.per[q];
+  sa.ox = a.ox; sa.oy = a.oy; sa.oz = a.oz; sa.A = a.A;
+  sa.eps = a.eps;
+  sa.factor = a.factor;
+  sa.kmax = a.kmax;
+  sa.idyn = a.idyn;
+  sa.it = a.it;
+  return sa;
+}
+
 namespace {
+// variant (FP32 only): 0 one iteration per launch, 1 two per launch,
+// 2 colour-split half sweeps.
 template <typename T>
 int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, int ny, int nz,
                 double *omega, double eps, int kmax, int idyn, int solver, int *exit_iter,
-                double *dmax, int *reason, Staged &s, bool pair = false) {
+                double *dmax, int *reason, Staged &s, int variant = 0) {
   if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
   CHK(check_red_black(solver, nx, ny, nz));
   if (kmax < 1) return fail("kmax must be >= 1");
   const int n = nx * ny * nz;
+  const bool pair = variant == 1;
   SorRun<T, T> r;
   r.pair = pair;
   CHK(s.scratch(1, &r.st));
@@ -509,9 +545,28 @@ int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, in
   a.kmax = kmax;
   a.idyn = idyn;
   int sweeps = 0;
-  int rc = r.run(*omega, 16, &sweeps);
-  if (rc == 0 && r.result(sweeps) != pp)
-    rc = launch_copy<T>(pp, other, n, g_stream) == hipSuccess ? 0 : fail("copy failed");
+  int rc = 0;
+  if constexpr (std::is_same<T, float>::value) {
+    if (variant == 2) {
+      // Colour-split copies of the iterate (in `other`) and of the rhs.
+      float *rs;
+      unsigned long long *dslot;
+      CHK(s.scratch(n, &rs));
+      CHK(s.scratch(sor_split_slot_words(), &dslot));
+      const int h = n / 2;
+      HIPCHK(launch_split(pp, other, other + h, nx, ny, nz, g_stream));
+      HIPCHK(launch_split(rhs, rs, rs + h, nx, ny, nz, g_stream));
+      r.split = true;
+      r.sa = split_args(a, other, other + h, rs, rs + h, dslot);
+      rc = r.run(*omega, 16, &sweeps);
+      if (rc == 0) HIPCHK(launch_unsplit(pp, other, other + h, nx, ny, nz, g_stream));
+    }
+  }
+  if (!r.split) {
+    rc = r.run(*omega, 16, &sweeps);
+    if (rc == 0 && r.result(sweeps) != pp)
+      rc = launch_copy<T>(pp, other, n, g_stream) == hipSuccess ? 0 : fail("copy failed");
+  }
   if (rc == 0) {
     *omega = r.hst->omega_final;
     if (exit_iter) *exit_iter = r.hst->exit_iter;
@@ -543,13 +598,16 @@ extern "C" int o3d_poisson(double *pp, const double *rhs, double dx, double dy, 
 
 extern "C" int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz,
                                int nx, int ny, int nz, double *omega, double eps, int kmax,
-                               int idyn, int solver, int pair, int *exit_iter, double *dmax,
+                               int idyn, int solver, int variant, int *exit_iter, double *dmax,
                                int *reason) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
   if (!omega) return fail("omega is NULL");
-  if (pair && !sor2_supported(nx, ny, nz))
+  if (variant < 0 || variant > 2) return fail("SOR kernel variant %d unknown", variant);
+  if (variant == 1 && !sor2_supported(nx, ny, nz))
     return fail("two iterations per launch need every axis >= 8 (grid %dx%dx%d)", nx, ny, nz);
+  if (variant == 2 && !sor_split_supported(nx, ny, nz))
+    return fail("colour-split sweeps need nx a multiple of 8 (nx = %d)", nx);
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
   float *p;
@@ -557,7 +615,7 @@ extern "C" int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy
   CHK(s.out(pp, n, &p, true));
   CHK(s.in(rhs, n, &r));
   CHK(poisson_dev<float>(p, r, dx, dy, dz, nx, ny, nz, omega, eps, kmax, idyn, solver, exit_iter,
-                         dmax, reason, s, pair != 0));
+                         dmax, reason, s, variant));
   return s.finish();
 }
 
@@ -821,6 +879,7 @@ struct Solver {
   double *P[2] = {nullptr, nullptr};
   int cur = 0;                 // P[cur] holds the pressure
   float *E[2] = {nullptr, nullptr}, *R = nullptr;  // mixed: correction ping-pong, residual
+  unsigned long long *dslot = nullptr;  // mixed, colour-split sweeps: dmax slots
   int ring = 0;                // predict calls so far (history ring position)
   // Reductions.
   FieldPartial *part_div = nullptr, *part_div2 = nullptr, *part_nut = nullptr,
@@ -1006,8 +1065,19 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   while (total < p.kmax) {
     int per[3];
     solver_periodic(S.kind, per);
-    HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx, p.ny,
-                                      p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
+    // Colour-split correction sweeps where the grid allows (nx a multiple
+    // of 8), else one fused iteration per launch; both give the same
+    // iterates.  (Two iterations per launch, sor2_kernel, measured slower
+    // than either at 256^3.)
+    const bool split = S.dslot != nullptr;
+    if (split)
+      HIPCHK(launch_residual_split<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx,
+                                          p.ny, p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                          g_stream));
+    else
+      HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx,
+                                        p.ny, p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                        g_stream));
     SorRun<float, float> r;
     r.st = S.st;
     r.hst = S.hst;
@@ -1016,9 +1086,12 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     r.buf[1] = S.E[1];
     const int kin = std::min(inner_max, p.kmax - total);
     sor_args_common<float, float>(S, r.a, S.R, S.st, kin);
-    // Two iterations per launch (sor2_kernel) measured slower than single
-    // launches at 256^3 (56 vs 47.5 us per iteration): single launches.
     r.pair = false;
+    if (split) {
+      const int h = n / 2;
+      r.split = true;
+      r.sa = split_args(r.a, S.E[0], S.E[0] + h, S.R, S.R + h, S.dslot);
+    }
     int sweeps = 0;
     const int first = cycles == 0 && S.last_sweeps[1] > 0
                           ? std::min(kin, std::max(4, S.last_sweeps[1] - 2))
@@ -1032,7 +1105,8 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     total += sweeps;
     sw32 += sweeps;
     ++cycles;
-    HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
+    if (split) HIPCHK(launch_fold_split(S.P[S.cur], S.E[0], p.nx, p.ny, p.nz, g_stream));
+    else HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
     if (rin == 2 && total < p.kmax) continue;  // cycle cap: refresh the residual
     // FP64 judge sweep (applied, like the reference's last sweep).
     SorRun<double, float> j;
@@ -1481,6 +1555,8 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     rc |= S->alloc(&S->E[0], n);
     rc |= S->alloc(&S->E[1], n);
     rc |= S->alloc(&S->R, n);
+    if (!rc && sor_split_supported(p->nx, p->ny, p->nz))
+      rc |= S->alloc(&S->dslot, sor_split_slot_words());
   }
   const int nb = std::max(S->tl.blocks(), std::max(S->tl_div.blocks(), S->tl_cor.blocks()));
   if (!rc) rc |= S->alloc(&S->part_div, nb);

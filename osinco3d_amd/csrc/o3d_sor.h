@@ -71,6 +71,33 @@ SorTiling sor2_tiling(int nx, int ny, int nz);
 bool sor2_supported(int nx, int ny, int nz);
 hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s);
 
+// Colour-split FP32 sweeps: the iterate and the rhs stored per row as the
+// red nodes (i + j + k even) then the black ones, each compressed to nx / 2
+// entries; an iteration is a red and a black in-place launch with the point
+// update of sor_kernel (identical iterates).  dmax partials go to dslot
+// (sor_split_slot_words() words, zeroed by launch_sor_split_init).
+struct SorSplitArgs {
+  float *red, *black;               // iterate, updated in place
+  const float *rhs_red, *rhs_black;
+  SorState *st;
+  unsigned long long *dslot;
+  double *log;  // dmax per iteration (may be null), log[it-1]
+  int log_len;
+  int nx, ny, nz, per[3];
+  double ox, oy, oz, A;
+  double eps, factor;
+  int kmax, idyn;
+  int it;       // iteration number of this launch pair (added to st->base)
+};
+bool sor_split_supported(int nx, int ny, int nz);  // nx a multiple of 8
+size_t sor_split_slot_words();
+hipError_t launch_sor_split_init(unsigned long long *dslot, hipStream_t s);
+hipError_t launch_sor_split_iteration(const SorSplitArgs &a, hipStream_t s);
+hipError_t launch_split(const float *f, float *red, float *black, int nx, int ny, int nz,
+                        hipStream_t s);
+hipError_t launch_unsplit(float *f, const float *red, const float *black, int nx, int ny, int nz,
+                          hipStream_t s);
+
 // Mixed precision helpers.  r = rhs - L p (FP64 arithmetic) stored in FP32,
 // and e = 0; and p += e.
 template <typename TR>
@@ -78,6 +105,12 @@ hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *
                                int ny, int nz, const int per[3], double ox, double oy,
                                double oz, double A, hipStream_t s);
 hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s);
+// The same with r and e colour-split (red half-array, then black).
+template <typename TR>
+hipError_t launch_residual_split(const double *p, const TR *rhs, float *r, float *e, int nx,
+                                 int ny, int nz, const int per[3], double ox, double oy,
+                                 double oz, double A, hipStream_t s);
+hipError_t launch_fold_split(double *p, const float *e, int nx, int ny, int nz, hipStream_t s);
 // FP64 residual norms of two iterates p0, p1 (r = rhs - L p) in one launch:
 // out (device, 4 doubles) = (max |r0|, sum r0^2, max |r1|, sum r1^2).  part
 // holds 4 * residual_blocks(nx, ny, nz) doubles.

@@ -113,14 +113,16 @@ __device__ __forceinline__ double bits_to_double(unsigned long long b) {
 
 // The reference's loop tail for iteration it-1 (src/poisson.f90:110-122).
 // Returns true when the solve is over; sets om to omega for iteration it.
-__device__ __forceinline__ bool sor_decide(const SorState *st, int it, double eps, double factor,
-                                           int kmax, int idyn, double &om, int &reason,
-                                           double &dlast) {
+// dm(q) is dmax of iteration q (q >= 1).
+template <typename DM>
+__device__ __forceinline__ bool sor_decide_with(const SorState *st, int it, double eps,
+                                                double factor, int kmax, int idyn, double &om,
+                                                int &reason, double &dlast, DM &&dm) {
   reason = -1;
   dlast = 0.0;
   if (it <= 1) { om = st->omega[0]; return false; }
-  const double d1 = bits_to_double(st->dmax_bits[(it - 1) & 7]);
-  const double d0 = (it - 1 == 1) ? 1609.0 : bits_to_double(st->dmax_bits[(it - 2) & 7]);
+  const double d1 = dm(it - 1);
+  const double d0 = (it - 1 == 1) ? 1609.0 : dm(it - 2);
   dlast = d1;
   om = st->omega[(it - 1) & 7];
   if (d1 < eps) { reason = 0; return true; }
@@ -131,6 +133,13 @@ __device__ __forceinline__ bool sor_decide(const SorState *st, int it, double ep
   }
   if (it > kmax) { reason = 2; return true; }
   return false;
+}
+
+__device__ __forceinline__ bool sor_decide(const SorState *st, int it, double eps, double factor,
+                                           int kmax, int idyn, double &om, int &reason,
+                                           double &dlast) {
+  return sor_decide_with(st, it, eps, factor, kmax, idyn, om, reason, dlast,
+                         [&](int q) { return bits_to_double(st->dmax_bits[q & 7]); });
 }
 
 // Point update of src/poisson.f90:95-98.  The FP64 form divides by A as the
@@ -196,8 +205,10 @@ __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsi
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// At most 64 VGPRs where the compiler can fit them (8 waves per SIMD: four
+// 512-thread blocks per CU).
 template <typename T, typename TR>
-__global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
+__global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   constexpr int PPS = SorShape<T>::PPS;
   constexpr int PD = SorShape<T>::PD;
   constexpr int NO = SorShape<T>::NO, NRD = SorShape<T>::NRD, NRH = SorShape<T>::NRH;
@@ -392,15 +403,246 @@ __global__ void __launch_bounds__(SNT) sor_kernel(const SorArgs<T, TR> a) {
       }
     }
   };
-  for (int s = z0 - 1; s <= z1; s += PD * PPS)
-    unroll_steps(std::make_integer_sequence<int, PD>{}, [&](auto K) {
-      const int sk = s + decltype(K)::value * PPS;
-      if (decltype(K)::value == 0 || sk <= z1) step(sk, K);
-    });
+  // Whole groups of PD steps in the loop (every back edge comes from the
+  // last step of a group, so the loads of a group stay in flight across the
+  // next one), then the remaining steps.
+  int s = z0 - 1;
+  for (; s + (PD - 1) * PPS <= z1; s += PD * PPS)
+    unroll_steps(std::make_integer_sequence<int, PD>{},
+                 [&](auto K) { step(s + decltype(K)::value * PPS, K); });
+  unroll_steps(std::make_integer_sequence<int, PD>{}, [&](auto K) {
+    const int sk = s + decltype(K)::value * PPS;
+    if (sk <= z1) step(sk, K);
+  });
   // Block max, one atomic per block.
   const double dm = block_max<SNW>((double)dmax, scr);
   if (threadIdx.x == 0 && dm != 0.0)
     atomicMax(&st->dmax_bits[it & 7], (unsigned long long)__double_as_longlong(dm));
+}
+
+// ------------------------------------------------ colour-split half sweeps
+// The FP32 correction sweeps of the mixed form keep the iterate and the rhs
+// in colour-split storage: every row (j, k) stores its red nodes
+// (i + j + k even) in the red half-array and its black nodes in the black
+// one, node i at entry i >> 1 of its colour's row (nx even).  An iteration
+// is two in-place launches, red then black; each updates one colour from the
+// other with 16-byte loads and stores, no LDS and no halo recomputation
+// (16 B/cell per iteration instead of 12, at close to streaming rates).
+// Red node (c, j, k) of row parity q = (j + k) & 1 sits at i = 2c + q; its
+// x neighbours are black entries c - 1 + q and c + q; black node c sits at
+// i = 2c + 1 - q with red x neighbours c - q and c + 1 - q; the y and z
+// neighbours of either colour are entry c of the other colour's row.
+// Each launch applies the same point update (sor_gs, then the relaxation) in
+// the same operation order as sor_kernel, so the iterates are identical.
+// dmax is reduced per block and max-accumulated into one of kSplitSlots
+// words (on separate cache lines) so that thousands of blocks do not contend
+// for one address; the next red launch reduces the slots with one wave.
+constexpr int kSplitV = 4, kSplitZC = 2, kSplitBX = 32, kSplitBY = 8;
+constexpr int kSplitNT = kSplitBX * kSplitBY;
+constexpr int kSplitSlots = 64, kSplitStride = 16;
+
+bool sor_split_supported(int nx, int ny, int nz) {
+  (void)ny; (void)nz;
+  return nx % (2 * kSplitV) == 0;
+}
+size_t sor_split_slot_words() { return (size_t)8 * kSplitSlots * kSplitStride; }
+
+__device__ __forceinline__ int sor_ghost1(int q, int n, int periodic) {
+  if (q < 0) return periodic ? q + n : -q;
+  if (q >= n) return periodic ? q - n : 2 * (n - 1) - q;
+  return q;
+}
+
+template <int COLOUR>
+__global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs a) {
+  typedef float vec __attribute__((ext_vector_type(kSplitV)));
+  __shared__ double scr[kSplitNT / O3D_WAVE];
+  SorState *st = a.st;
+  if (st->done) return;
+  const int it = a.it + st->base;
+  const int tl = threadIdx.x + kSplitBX * threadIdx.y;
+  const bool first = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+  float *__restrict__ X = COLOUR == 0 ? a.red : a.black;
+  const float *__restrict__ Y = COLOUR == 0 ? a.black : a.red;
+  const float *__restrict__ RX = COLOUR == 0 ? a.rhs_red : a.rhs_black;
+  const int hx = a.nx / 2, ny = a.ny, nz = a.nz;
+  const int px = a.per[0], py = a.per[1], pz = a.per[2];
+  const int nv = hx / kSplitV;
+  const int cv = blockIdx.x * kSplitBX + threadIdx.x;
+  const int j = blockIdx.y * kSplitBY + threadIdx.y;
+  const int k0 = blockIdx.z * kSplitZC;
+  const bool active = cv < nv && j < ny;
+  // The dmax slots of the two previous iterations are loaded first and
+  // every load of the sweep right after them, so the slot reduction and the
+  // stop decision overlap the sweep's loads (the load counter completes in
+  // order); the decision only gates the stores.
+  double sl1 = 0.0, sl2 = 0.0;
+  if (COLOUR == 0) {
+    const size_t lane = __lane_id();
+    sl1 = bits_to_double(a.dslot[((size_t)((it - 1) & 7) * kSplitSlots + lane) * kSplitStride]);
+    sl2 = bits_to_double(a.dslot[((size_t)((it - 2) & 7) * kSplitSlots + lane) * kSplitStride]);
+  }
+  const int c0 = active ? cv * kSplitV : 0;
+  const int jj = active ? j : 0;
+  const size_t pl = (size_t)hx * ny;
+  const size_t rj = (size_t)hx * jj + c0;
+  const size_t rjm = (size_t)hx * sor_ghost1(jj - 1, ny, py) + c0;
+  const size_t rjp = (size_t)hx * sor_ghost1(jj + 1, ny, py) + c0;
+  // Entry beside the vector: left for s = 0, right for s = 1 (periodic
+  // wrap, or the free-slip mirror, which keeps the colour).
+  const int cl = c0 - 1 < 0 ? (px ? hx - 1 : 0) : c0 - 1;
+  const int cr = c0 + kSplitV >= hx ? (px ? 0 : hx - 1) : c0 + kSplitV;
+  vec yz[kSplitZC + 2], ym[kSplitZC], yp[kSplitZC], pc[kSplitZC], rr[kSplitZC];
+  float side[kSplitZC];
+  yz[0] = *(const vec *)(Y + rj + pl * sor_ghost1(k0 - 1, nz, pz));
+#pragma unroll
+  for (int kk = 0; kk <= kSplitZC; ++kk)
+    yz[kk + 1] = *(const vec *)(Y + rj + pl * sor_ghost1(min(k0 + kk, nz), nz, pz));
+#pragma unroll
+  for (int kk = 0; kk < kSplitZC; ++kk) {
+    const int k = min(k0 + kk, nz - 1);  // planes past the end are not stored
+    const int sd = COLOUR == 0 ? ((jj + k) & 1) : 1 - ((jj + k) & 1);
+    const size_t pk = pl * k;
+    side[kk] = Y[pk + (rj - c0) + (sd == 0 ? cl : cr)];
+    ym[kk] = *(const vec *)(Y + rjm + pk);
+    yp[kk] = *(const vec *)(Y + rjp + pk);
+    pc[kk] = *(const vec *)(X + rj + pk);
+    rr[kk] = *(const vec *)(RX + rj + pk);
+  }
+  double om;
+  bool done = false;
+  if (COLOUR == 0) {
+    double dlast;
+    int reason;
+    done = sor_decide_with(st, it, a.eps, a.factor, a.kmax, a.idyn, om, reason, dlast,
+                           [&](int q) { return wave_max(q == it - 1 ? sl1 : sl2); });
+    if (first && done && tl == 0) {
+      st->done = 1;
+      st->reason = reason;
+      st->exit_iter = reason == 2 ? a.kmax + 1 : it - 1;
+      st->sweeps = it - 1;
+      st->dmax_final = dlast;
+      st->omega_final = om;
+      if (a.log && it - 2 < a.log_len) a.log[it - 2] = dlast;
+    }
+    if (first && !done) {
+      if (tl < kSplitSlots)
+        a.dslot[((size_t)((it + 1) & 7) * kSplitSlots + tl) * kSplitStride] = 0ull;
+      if (tl == 0) {
+        st->omega[it & 7] = om;
+        if (a.log && it >= 2 && it - 2 < a.log_len) a.log[it - 2] = dlast;
+      }
+    }
+  } else {
+    om = st->omega[it & 7];
+  }
+  if (done) return;
+  const float ox = float(a.ox), oy = float(a.oy), oz = float(a.oz);
+  const float invA = float(1.0 / a.A);
+  const float w = float(om), w1 = float(1.0 - om);
+  float dmax = 0.0f;
+  if (active) {
+#pragma unroll
+    for (int kk = 0; kk < kSplitZC; ++kk) {
+      const int k = k0 + kk;
+      if (k < nz) {
+        const int sd = COLOUR == 0 ? ((j + k) & 1) : 1 - ((j + k) & 1);
+        const vec zm = yz[kk], zc = yz[kk + 1], zp = yz[kk + 2];
+        vec out;
+#pragma unroll
+        for (int v = 0; v < kSplitV; ++v) {
+          const float xm = sd == 0 ? (v == 0 ? side[kk] : zc[v - 1]) : zc[v];
+          const float xp = sd == 0 ? zc[v] : (v == kSplitV - 1 ? side[kk] : zc[v + 1]);
+          const float pn = sor_gs<float, float>(xm, xp, ym[kk][v], yp[kk][v], zm[v], zp[v],
+                                                rr[kk][v], ox, oy, oz, float(a.A), invA);
+          out[v] = w1 * pc[kk][v] + w * pn;
+          const float d = fabsf(pn - pc[kk][v]);
+          dmax = d > dmax ? d : dmax;
+        }
+        *(vec *)(X + rj + pl * k) = out;
+      }
+    }
+  }
+  double dm = wave_max((double)dmax);
+  if (tl % O3D_WAVE == 0) scr[tl / O3D_WAVE] = dm;
+  __syncthreads();
+  dm = scr[0];
+#pragma unroll
+  for (int q = 1; q < kSplitNT / O3D_WAVE; ++q) dm = fmax(dm, scr[q]);
+  if (tl == 0 && dm != 0.0) {
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    atomicMax(&a.dslot[((size_t)(it & 7) * kSplitSlots + (b % kSplitSlots)) * kSplitStride],
+              (unsigned long long)__double_as_longlong(dm));
+  }
+}
+
+__global__ void sor_split_init_kernel(unsigned long long *dslot, int words) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) dslot[i] = 0ull;
+}
+
+hipError_t launch_sor_split_init(unsigned long long *dslot, hipStream_t s) {
+  const int words = (int)sor_split_slot_words();
+  sor_split_init_kernel<<<(words + 255) / 256, 256, 0, s>>>(dslot, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_sor_split_iteration(const SorSplitArgs &a, hipStream_t s) {
+  if (!sor_split_supported(a.nx, a.ny, a.nz) ||
+      (long long)a.nx * a.ny * a.nz * 4LL > 0x7fffffffLL)
+    return hipErrorInvalidValue;
+  const int nv = a.nx / 2 / kSplitV;
+  const dim3 grd((nv + kSplitBX - 1) / kSplitBX, (a.ny + kSplitBY - 1) / kSplitBY,
+                 (a.nz + kSplitZC - 1) / kSplitZC);
+  const dim3 blk(kSplitBX, kSplitBY);
+  sor_split_kernel<0><<<grd, blk, 0, s>>>(a);
+  sor_split_kernel<1><<<grd, blk, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// Natural <-> colour-split storage (nx even).
+__device__ __forceinline__ size_t split_entry(int idx, int nx, int ny, bool &black) {
+  const int i = idx % nx, r = idx / nx;  // r = j + ny * k
+  const int j = r % ny, k = r / ny;
+  black = ((i + j + k) & 1) != 0;
+  return (size_t)(nx / 2) * r + (i >> 1);
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) split_kernel(const TS *__restrict__ f,
+                                                    float *__restrict__ red,
+                                                    float *__restrict__ black, int nx, int ny,
+                                                    int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  bool b;
+  const size_t o = split_entry(idx, nx, ny, b);
+  (b ? black : red)[o] = (float)f[idx];
+}
+
+__global__ void __launch_bounds__(256) unsplit_kernel(float *__restrict__ f,
+                                                      const float *__restrict__ red,
+                                                      const float *__restrict__ black, int nx,
+                                                      int ny, int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  bool b;
+  const size_t o = split_entry(idx, nx, ny, b);
+  f[idx] = (b ? black : red)[o];
+}
+
+hipError_t launch_split(const float *f, float *red, float *black, int nx, int ny, int nz,
+                        hipStream_t s) {
+  const int n = nx * ny * nz;
+  split_kernel<float><<<(n + 255) / 256, 256, 0, s>>>(f, red, black, nx, ny, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_unsplit(float *f, const float *red, const float *black, int nx, int ny, int nz,
+                          hipStream_t s) {
+  const int n = nx * ny * nz;
+  unsplit_kernel<<<(n + 255) / 256, 256, 0, s>>>(f, red, black, nx, ny, n);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------- two iterations per launch
@@ -733,7 +975,8 @@ int residual_blocks(int nx, int ny, int nz) {
   return (int)(g.x * g.y * g.z);
 }
 
-// f(offset of (i, j, k), residual) for the planes of this thread's column.
+// f(offset of (i, j, k), i, j, k, residual) for the planes of this
+// thread's column.
 template <typename TR, typename F>
 __device__ __forceinline__ void residual_column(const double *__restrict__ p,
                                                 const TR *__restrict__ rhs, int nx, int ny,
@@ -758,7 +1001,7 @@ __device__ __forceinline__ void residual_column(const double *__restrict__ p,
       const size_t o = plane * k;
       const double lap = ox * (p[cim + o] + p[cip + o]) + oy * (p[cjm + o] + p[cjp + o]) +
                          oz * (pm + pp) + A * pc;
-      f(c + o, (double)rhs[c + o] - lap);
+      f(c + o, i, j, k, (double)rhs[c + o] - lap);
       pm = pc;
       pc = pp;
     }
@@ -773,10 +1016,59 @@ __global__ void __launch_bounds__(RNT) residual_f32_kernel(const double *__restr
                                                            int nz, int px, int py, int pz,
                                                            double ox, double oy, double oz,
                                                            double A) {
-  residual_column(p, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A, [&](size_t o, double res) {
-    r[o] = (float)res;
-    e[o] = 0.0f;
-  });
+  residual_column(p, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
+                  [&](size_t o, int, int, int, double res) {
+                    r[o] = (float)res;
+                    e[o] = 0.0f;
+                  });
+}
+
+// The same residual stored colour-split (r and e hold the red half-array
+// followed by the black one).
+template <typename TR>
+__global__ void __launch_bounds__(RNT) residual_split_kernel(const double *__restrict__ p,
+                                                             const TR *__restrict__ rhs,
+                                                             float *__restrict__ r,
+                                                             float *__restrict__ e, int nx,
+                                                             int ny, int nz, int px, int py,
+                                                             int pz, double ox, double oy,
+                                                             double oz, double A) {
+  const size_t half = (size_t)nx * ny * nz / 2;
+  residual_column(p, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
+                  [&](size_t, int i, int j, int k, double res) {
+                    const size_t o = (size_t)(nx / 2) * (j + (size_t)ny * k) + (i >> 1) +
+                                     (((i + j + k) & 1) ? half : 0);
+                    r[o] = (float)res;
+                    e[o] = 0.0f;
+                  });
+}
+
+template <typename TR>
+hipError_t launch_residual_split(const double *p, const TR *rhs, float *r, float *e, int nx,
+                                 int ny, int nz, const int per[3], double ox, double oy,
+                                 double oz, double A, hipStream_t s) {
+  residual_split_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
+      p, rhs, r, e, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
+  return hipGetLastError();
+}
+
+// p += e with e colour-split (red half-array, then black).
+__global__ void __launch_bounds__(256) fold_split_kernel(double *__restrict__ p,
+                                                         const float *__restrict__ e, int nx,
+                                                         int ny, int nz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= nx) return;
+  const size_t half = (size_t)nx * ny * nz / 2;
+  const size_t row = j + (size_t)ny * k;
+  const size_t o = (size_t)(nx / 2) * row + (i >> 1) + (((i + j + k) & 1) ? half : 0);
+  const size_t idx = (size_t)nx * row + i;
+  p[idx] = p[idx] + (double)e[o];
+}
+
+hipError_t launch_fold_split(double *p, const float *e, int nx, int ny, int nz, hipStream_t s) {
+  fold_split_kernel<<<dim3((nx + 255) / 256, ny, nz), 256, 0, s>>>(p, e, nx, ny, nz);
+  return hipGetLastError();
 }
 
 template <typename TR>
@@ -808,14 +1100,16 @@ __global__ void __launch_bounds__(RNT) residual_norm_pair_kernel(
     double oy, double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0, m1 = 0.0, s1 = 0.0;
-  residual_column(p0, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A, [&](size_t, double r) {
-    m0 = fmax(m0, fabs(r));
-    s0 += r * r;
-  });
-  residual_column(p1, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A, [&](size_t, double r) {
-    m1 = fmax(m1, fabs(r));
-    s1 += r * r;
-  });
+  residual_column(p0, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
+                  [&](size_t, int, int, int, double r) {
+                    m0 = fmax(m0, fabs(r));
+                    s0 += r * r;
+                  });
+  residual_column(p1, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
+                  [&](size_t, int, int, int, double r) {
+                    m1 = fmax(m1, fabs(r));
+                    s1 += r * r;
+                  });
   m0 = block_max<RNT / O3D_WAVE>(m0, sc);
   s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
   m1 = block_max<RNT / O3D_WAVE>(m1, sc);
@@ -916,6 +1210,9 @@ template hipError_t launch_residual_f32<double>(const double *, const double *, 
 template hipError_t launch_residual_f32<float>(const double *, const float *, float *, float *,
                                                int, int, int, const int *, double, double, double,
                                                double, hipStream_t);
+template hipError_t launch_residual_split<float>(const double *, const float *, float *, float *,
+                                                 int, int, int, const int *, double, double,
+                                                 double, double, hipStream_t);
 template hipError_t launch_residual_norm_pair<double>(const double *, const double *,
                                                       const double *, double *, double *, int,
                                                       int, int, const int *, double, double,
