@@ -196,7 +196,7 @@ SorConst sor_const(double dx, double dy, double dz) {
 // Batched launcher of red-black iterations with device-side stop control.
 // Buffers: buf[0] holds the start iterate; launch L reads buf[L & 1] and
 // writes buf[(L + 1) & 1].  With `pair` (FP32 only) each launch runs two
-// iterations (launch_sor2_iteration).  With `split` (FP32 only) each launch
+// iterations (launch_sor2_iteration).  With `split` (T == TR) each launch
 // is one iteration of the colour-split in-place sweeps on `sa` (buf unused).
 template <typename T, typename TR>
 struct SorRun {
@@ -206,7 +206,7 @@ struct SorRun {
   T *buf[2] = {nullptr, nullptr};
   bool pair = false;
   bool split = false;
-  SorSplitArgs sa{};
+  SorSplitArgs<T> sa{};
   // Timing of complete batches (all launches did their iterations).
   double ms_full = 0.0;
   int sweeps_full = 0;
@@ -214,11 +214,13 @@ struct SorRun {
   std::vector<hipEvent_t> *events = nullptr;
 
   hipError_t launch() {
-    if constexpr (std::is_same<T, float>::value && std::is_same<TR, float>::value) {
+    if constexpr (std::is_same<T, TR>::value) {
       if (split) {
         sa.it = a.it;
-        return launch_sor_split_iteration(sa, g_stream);
+        return launch_sor_split_iteration<T>(sa, g_stream);
       }
+    }
+    if constexpr (std::is_same<T, float>::value && std::is_same<TR, float>::value) {
       if (pair) return launch_sor2_iteration(a, g_stream);
     }
     return launch_sor_iteration<T, TR>(a, g_stream);
@@ -484,10 +486,10 @@ extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz
 
 // The colour-split sweep arguments matching `a` (red, black and the rhs
 // halves already in split storage).
-SorSplitArgs split_args(const SorArgs<float, float> &a, float *red, float *black,
-                        const float *rhs_red, const float *rhs_black,
-                        unsigned long long *dslot) {
-  SorSplitArgs sa{};
+template <typename T>
+SorSplitArgs<T> split_args(const SorArgs<T, T> &a, T *red, T *black, const T *rhs_red,
+                           const T *rhs_black, unsigned long long *dslot) {
+  SorSplitArgs<T> sa{};
   sa.red = red;
   sa.black = black;
   sa.rhs_red = rhs_red;
@@ -508,12 +510,13 @@ SorSplitArgs split_args(const SorArgs<float, float> &a, float *red, float *black
 }
 
 namespace {
-// variant (FP32 only): 0 one iteration per launch, 1 two per launch,
-// 2 colour-split half sweeps.
+// variant: 0 one fused iteration per launch, 1 two per launch (FP32),
+// 2 colour-split half sweeps, -1 the colour-split sweeps where the grid
+// allows them, else the fused kernel (identical iterates in every case).
 template <typename T>
 int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, int ny, int nz,
                 double *omega, double eps, int kmax, int idyn, int solver, int *exit_iter,
-                double *dmax, int *reason, Staged &s, int variant = 0) {
+                double *dmax, int *reason, Staged &s, int variant = -1) {
   if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
   CHK(check_red_black(solver, nx, ny, nz));
   if (kmax < 1) return fail("kmax must be >= 1");
@@ -546,21 +549,19 @@ int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, in
   a.idyn = idyn;
   int sweeps = 0;
   int rc = 0;
-  if constexpr (std::is_same<T, float>::value) {
-    if (variant == 2) {
-      // Colour-split copies of the iterate (in `other`) and of the rhs.
-      float *rs;
-      unsigned long long *dslot;
-      CHK(s.scratch(n, &rs));
-      CHK(s.scratch(sor_split_slot_words(), &dslot));
-      const int h = n / 2;
-      HIPCHK(launch_split(pp, other, other + h, nx, ny, nz, g_stream));
-      HIPCHK(launch_split(rhs, rs, rs + h, nx, ny, nz, g_stream));
-      r.split = true;
-      r.sa = split_args(a, other, other + h, rs, rs + h, dslot);
-      rc = r.run(*omega, 16, &sweeps);
-      if (rc == 0) HIPCHK(launch_unsplit(pp, other, other + h, nx, ny, nz, g_stream));
-    }
+  if (variant == 2 || (variant < 0 && sor_split_supported<T>(nx, ny, nz))) {
+    // Colour-split copies of the iterate (in `other`) and of the rhs.
+    T *rs;
+    unsigned long long *dslot;
+    CHK(s.scratch(n, &rs));
+    CHK(s.scratch(sor_split_slot_words(), &dslot));
+    const int h = n / 2;
+    HIPCHK(launch_split<T>(pp, other, other + h, nx, ny, nz, g_stream));
+    HIPCHK(launch_split<T>(rhs, rs, rs + h, nx, ny, nz, g_stream));
+    r.split = true;
+    r.sa = split_args<T>(a, other, other + h, rs, rs + h, dslot);
+    rc = r.run(*omega, 16, &sweeps);
+    if (rc == 0) HIPCHK(launch_unsplit<T>(pp, other, other + h, nx, ny, nz, g_stream));
   }
   if (!r.split) {
     rc = r.run(*omega, 16, &sweeps);
@@ -606,7 +607,7 @@ extern "C" int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy
   if (variant < 0 || variant > 2) return fail("SOR kernel variant %d unknown", variant);
   if (variant == 1 && !sor2_supported(nx, ny, nz))
     return fail("two iterations per launch need every axis >= 8 (grid %dx%dx%d)", nx, ny, nz);
-  if (variant == 2 && !sor_split_supported(nx, ny, nz))
+  if (variant == 2 && !sor_split_supported<float>(nx, ny, nz))
     return fail("colour-split sweeps need nx a multiple of 8 (nx = %d)", nx);
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
@@ -879,7 +880,8 @@ struct Solver {
   double *P[2] = {nullptr, nullptr};
   int cur = 0;                 // P[cur] holds the pressure
   float *E[2] = {nullptr, nullptr}, *R = nullptr;  // mixed: correction ping-pong, residual
-  unsigned long long *dslot = nullptr;  // mixed, colour-split sweeps: dmax slots
+  unsigned long long *dslot = nullptr;  // colour-split sweeps: dmax slots
+  double *rhs_split = nullptr;          // FP64 form: the rhs in colour-split storage
   int ring = 0;                // predict calls so far (history ring position)
   // Reductions.
   FieldPartial *part_div = nullptr, *part_div2 = nullptr, *part_nut = nullptr,
@@ -1035,8 +1037,23 @@ int solver_poisson_f64(Solver &S, o3d_step_info *info) {
   sor_args_common<double, double>(S, r.a, (const double *)S.rhs, S.st, S.p.kmax);
   int sweeps = 0;
   const int first = S.last_sweeps[0] > 0 ? std::max(4, S.last_sweeps[0] - 2) : 16;
-  CHK(r.run(S.omega, first, &sweeps));
-  if (sweeps & 1) S.cur = 1 - S.cur;
+  if (S.rhs_split) {
+    // Colour-split sweeps (identical iterates): the iterate in split storage
+    // in the spare pressure buffer, back into P[cur] after the solve.
+    const int n = S.n, h = n / 2;
+    const int nx = S.p.nx, ny = S.p.ny, nz = S.p.nz;
+    double *q = S.P[1 - S.cur];
+    HIPCHK(launch_split<double>(S.P[S.cur], q, q + h, nx, ny, nz, g_stream));
+    HIPCHK(launch_split<double>((const double *)S.rhs, S.rhs_split, S.rhs_split + h, nx, ny, nz,
+                                g_stream));
+    r.split = true;
+    r.sa = split_args<double>(r.a, q, q + h, S.rhs_split, S.rhs_split + h, S.dslot);
+    CHK(r.run(S.omega, first, &sweeps));
+    HIPCHK(launch_unsplit<double>(S.P[S.cur], q, q + h, nx, ny, nz, g_stream));
+  } else {
+    CHK(r.run(S.omega, first, &sweeps));
+    if (sweeps & 1) S.cur = 1 - S.cur;
+  }
   S.last_sweeps[0] = sweeps;
   S.omega = S.hst->omega_final;
   S.sor_ms[0] += r.ms_full;
@@ -1090,7 +1107,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     if (split) {
       const int h = n / 2;
       r.split = true;
-      r.sa = split_args(r.a, S.E[0], S.E[0] + h, S.R, S.R + h, S.dslot);
+      r.sa = split_args<float>(r.a, S.E[0], S.E[0] + h, S.R, S.R + h, S.dslot);
     }
     int sweeps = 0;
     const int first = cycles == 0 && S.last_sweeps[1] > 0
@@ -1555,8 +1572,12 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     rc |= S->alloc(&S->E[0], n);
     rc |= S->alloc(&S->E[1], n);
     rc |= S->alloc(&S->R, n);
-    if (!rc && sor_split_supported(p->nx, p->ny, p->nz))
+    if (!rc && sor_split_supported<float>(p->nx, p->ny, p->nz))
       rc |= S->alloc(&S->dslot, sor_split_slot_words());
+  }
+  if (!rc && !S->mixed && sor_split_supported<double>(p->nx, p->ny, p->nz)) {
+    rc |= S->alloc(&S->dslot, sor_split_slot_words());
+    rc |= S->alloc(&S->rhs_split, n);
   }
   const int nb = std::max(S->tl.blocks(), std::max(S->tl_div.blocks(), S->tl_cor.blocks()));
   if (!rc) rc |= S->alloc(&S->part_div, nb);

@@ -71,14 +71,15 @@ SorTiling sor2_tiling(int nx, int ny, int nz);
 bool sor2_supported(int nx, int ny, int nz);
 hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s);
 
-// Colour-split FP32 sweeps: the iterate and the rhs stored per row as the
-// red nodes (i + j + k even) then the black ones, each compressed to nx / 2
+// Colour-split sweeps: the iterate and the rhs stored per row as the red
+// nodes (i + j + k even) then the black ones, each compressed to nx / 2
 // entries; an iteration is a red and a black in-place launch with the point
 // update of sor_kernel (identical iterates).  dmax partials go to dslot
 // (sor_split_slot_words() words, zeroed by launch_sor_split_init).
+template <typename T>
 struct SorSplitArgs {
-  float *red, *black;               // iterate, updated in place
-  const float *rhs_red, *rhs_black;
+  T *red, *black;                   // iterate, updated in place
+  const T *rhs_red, *rhs_black;
   SorState *st;
   unsigned long long *dslot;
   double *log;  // dmax per iteration (may be null), log[it-1]
@@ -89,13 +90,17 @@ struct SorSplitArgs {
   int kmax, idyn;
   int it;       // iteration number of this launch pair (added to st->base)
 };
-bool sor_split_supported(int nx, int ny, int nz);  // nx a multiple of 8
+// nx a multiple of 8 (FP32) or 4 (FP64): 16-byte vectors of one colour.
+template <typename T>
+bool sor_split_supported(int nx, int ny, int nz);
 size_t sor_split_slot_words();
 hipError_t launch_sor_split_init(unsigned long long *dslot, hipStream_t s);
-hipError_t launch_sor_split_iteration(const SorSplitArgs &a, hipStream_t s);
-hipError_t launch_split(const float *f, float *red, float *black, int nx, int ny, int nz,
-                        hipStream_t s);
-hipError_t launch_unsplit(float *f, const float *red, const float *black, int nx, int ny, int nz,
+template <typename T>
+hipError_t launch_sor_split_iteration(const SorSplitArgs<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_split(const T *f, T *red, T *black, int nx, int ny, int nz, hipStream_t s);
+template <typename T>
+hipError_t launch_unsplit(T *f, const T *red, const T *black, int nx, int ny, int nz,
                           hipStream_t s);
 
 // Mixed precision helpers.  r = rhs - L p (FP64 arithmetic) stored in FP32,

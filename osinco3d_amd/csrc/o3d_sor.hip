@@ -437,13 +437,17 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
 // dmax is reduced per block and max-accumulated into one of kSplitSlots
 // words (on separate cache lines) so that thousands of blocks do not contend
 // for one address; the next red launch reduces the slots with one wave.
-constexpr int kSplitV = 4, kSplitZC = 2, kSplitBX = 32, kSplitBY = 8;
+// Entries per 16-byte vector: 4 in FP32, 2 in FP64.
+template <typename T>
+constexpr int split_v() { return 16 / (int)sizeof(T); }
+constexpr int kSplitZC = 2, kSplitBX = 32, kSplitBY = 8;
 constexpr int kSplitNT = kSplitBX * kSplitBY;
 constexpr int kSplitSlots = 64, kSplitStride = 16;
 
+template <typename T>
 bool sor_split_supported(int nx, int ny, int nz) {
   (void)ny; (void)nz;
-  return nx % (2 * kSplitV) == 0;
+  return nx % (2 * split_v<T>()) == 0;
 }
 size_t sor_split_slot_words() { return (size_t)8 * kSplitSlots * kSplitStride; }
 
@@ -453,18 +457,19 @@ __device__ __forceinline__ int sor_ghost1(int q, int n, int periodic) {
   return q;
 }
 
-template <int COLOUR>
-__global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs a) {
-  typedef float vec __attribute__((ext_vector_type(kSplitV)));
+template <typename T, int COLOUR>
+__global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs<T> a) {
+  constexpr int kSplitV = split_v<T>();
+  typedef T vec __attribute__((ext_vector_type(kSplitV)));
   __shared__ double scr[kSplitNT / O3D_WAVE];
   SorState *st = a.st;
   if (st->done) return;
   const int it = a.it + st->base;
   const int tl = threadIdx.x + kSplitBX * threadIdx.y;
   const bool first = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
-  float *__restrict__ X = COLOUR == 0 ? a.red : a.black;
-  const float *__restrict__ Y = COLOUR == 0 ? a.black : a.red;
-  const float *__restrict__ RX = COLOUR == 0 ? a.rhs_red : a.rhs_black;
+  T *__restrict__ X = COLOUR == 0 ? a.red : a.black;
+  const T *__restrict__ Y = COLOUR == 0 ? a.black : a.red;
+  const T *__restrict__ RX = COLOUR == 0 ? a.rhs_red : a.rhs_black;
   const int hx = a.nx / 2, ny = a.ny, nz = a.nz;
   const int px = a.per[0], py = a.per[1], pz = a.per[2];
   const int nv = hx / kSplitV;
@@ -493,7 +498,7 @@ __global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs 
   const int cl = c0 - 1 < 0 ? (px ? hx - 1 : 0) : c0 - 1;
   const int cr = c0 + kSplitV >= hx ? (px ? 0 : hx - 1) : c0 + kSplitV;
   vec yz[kSplitZC + 2], ym[kSplitZC], yp[kSplitZC], pc[kSplitZC], rr[kSplitZC];
-  float side[kSplitZC];
+  T side[kSplitZC];
   yz[0] = *(const vec *)(Y + rj + pl * sor_ghost1(k0 - 1, nz, pz));
 #pragma unroll
   for (int kk = 0; kk <= kSplitZC; ++kk)
@@ -537,10 +542,10 @@ __global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs 
     om = st->omega[it & 7];
   }
   if (done) return;
-  const float ox = float(a.ox), oy = float(a.oy), oz = float(a.oz);
-  const float invA = float(1.0 / a.A);
-  const float w = float(om), w1 = float(1.0 - om);
-  float dmax = 0.0f;
+  const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A);
+  const T invA = T(1.0 / a.A);
+  const T w = T(om), w1 = T(1.0 - om);
+  T dmax = T(0);
   if (active) {
 #pragma unroll
     for (int kk = 0; kk < kSplitZC; ++kk) {
@@ -551,12 +556,12 @@ __global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs 
         vec out;
 #pragma unroll
         for (int v = 0; v < kSplitV; ++v) {
-          const float xm = sd == 0 ? (v == 0 ? side[kk] : zc[v - 1]) : zc[v];
-          const float xp = sd == 0 ? zc[v] : (v == kSplitV - 1 ? side[kk] : zc[v + 1]);
-          const float pn = sor_gs<float, float>(xm, xp, ym[kk][v], yp[kk][v], zm[v], zp[v],
-                                                rr[kk][v], ox, oy, oz, float(a.A), invA);
+          const T xm = sd == 0 ? (v == 0 ? side[kk] : zc[v - 1]) : zc[v];
+          const T xp = sd == 0 ? zc[v] : (v == kSplitV - 1 ? side[kk] : zc[v + 1]);
+          const T pn = sor_gs<T, T>(xm, xp, ym[kk][v], yp[kk][v], zm[v], zp[v], rr[kk][v], ox,
+                                    oy, oz, A, invA);
           out[v] = w1 * pc[kk][v] + w * pn;
-          const float d = fabsf(pn - pc[kk][v]);
+          const T d = T(fabs(pn - pc[kk][v]));
           dmax = d > dmax ? d : dmax;
         }
         *(vec *)(X + rj + pl * k) = out;
@@ -587,16 +592,16 @@ hipError_t launch_sor_split_init(unsigned long long *dslot, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_sor_split_iteration(const SorSplitArgs &a, hipStream_t s) {
-  if (!sor_split_supported(a.nx, a.ny, a.nz) ||
-      (long long)a.nx * a.ny * a.nz * 4LL > 0x7fffffffLL)
+template <typename T>
+hipError_t launch_sor_split_iteration(const SorSplitArgs<T> &a, hipStream_t s) {
+  if (!sor_split_supported<T>(a.nx, a.ny, a.nz))
     return hipErrorInvalidValue;
-  const int nv = a.nx / 2 / kSplitV;
+  const int nv = a.nx / 2 / split_v<T>();
   const dim3 grd((nv + kSplitBX - 1) / kSplitBX, (a.ny + kSplitBY - 1) / kSplitBY,
                  (a.nz + kSplitZC - 1) / kSplitZC);
   const dim3 blk(kSplitBX, kSplitBY);
-  sor_split_kernel<0><<<grd, blk, 0, s>>>(a);
-  sor_split_kernel<1><<<grd, blk, 0, s>>>(a);
+  sor_split_kernel<T, 0><<<grd, blk, 0, s>>>(a);
+  sor_split_kernel<T, 1><<<grd, blk, 0, s>>>(a);
   return hipGetLastError();
 }
 
@@ -608,21 +613,21 @@ __device__ __forceinline__ size_t split_entry(int idx, int nx, int ny, bool &bla
   return (size_t)(nx / 2) * r + (i >> 1);
 }
 
-template <typename TS>
-__global__ void __launch_bounds__(256) split_kernel(const TS *__restrict__ f,
-                                                    float *__restrict__ red,
-                                                    float *__restrict__ black, int nx, int ny,
+template <typename T>
+__global__ void __launch_bounds__(256) split_kernel(const T *__restrict__ f, T *__restrict__ red,
+                                                    T *__restrict__ black, int nx, int ny,
                                                     int n) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   bool b;
   const size_t o = split_entry(idx, nx, ny, b);
-  (b ? black : red)[o] = (float)f[idx];
+  (b ? black : red)[o] = f[idx];
 }
 
-__global__ void __launch_bounds__(256) unsplit_kernel(float *__restrict__ f,
-                                                      const float *__restrict__ red,
-                                                      const float *__restrict__ black, int nx,
+template <typename T>
+__global__ void __launch_bounds__(256) unsplit_kernel(T *__restrict__ f,
+                                                      const T *__restrict__ red,
+                                                      const T *__restrict__ black, int nx,
                                                       int ny, int n) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
@@ -631,19 +636,34 @@ __global__ void __launch_bounds__(256) unsplit_kernel(float *__restrict__ f,
   f[idx] = (b ? black : red)[o];
 }
 
-hipError_t launch_split(const float *f, float *red, float *black, int nx, int ny, int nz,
-                        hipStream_t s) {
+template <typename T>
+hipError_t launch_split(const T *f, T *red, T *black, int nx, int ny, int nz, hipStream_t s) {
   const int n = nx * ny * nz;
-  split_kernel<float><<<(n + 255) / 256, 256, 0, s>>>(f, red, black, nx, ny, n);
+  split_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(f, red, black, nx, ny, n);
   return hipGetLastError();
 }
 
-hipError_t launch_unsplit(float *f, const float *red, const float *black, int nx, int ny, int nz,
+template <typename T>
+hipError_t launch_unsplit(T *f, const T *red, const T *black, int nx, int ny, int nz,
                           hipStream_t s) {
   const int n = nx * ny * nz;
-  unsplit_kernel<<<(n + 255) / 256, 256, 0, s>>>(f, red, black, nx, ny, n);
+  unsplit_kernel<T><<<(n + 255) / 256, 256, 0, s>>>(f, red, black, nx, ny, n);
   return hipGetLastError();
 }
+
+template bool sor_split_supported<float>(int, int, int);
+template bool sor_split_supported<double>(int, int, int);
+template hipError_t launch_sor_split_iteration<float>(const SorSplitArgs<float> &, hipStream_t);
+template hipError_t launch_sor_split_iteration<double>(const SorSplitArgs<double> &,
+                                                       hipStream_t);
+template hipError_t launch_split<float>(const float *, float *, float *, int, int, int,
+                                        hipStream_t);
+template hipError_t launch_split<double>(const double *, double *, double *, int, int, int,
+                                         hipStream_t);
+template hipError_t launch_unsplit<float>(float *, const float *, const float *, int, int, int,
+                                          hipStream_t);
+template hipError_t launch_unsplit<double>(double *, const double *, const double *, int, int,
+                                           int, hipStream_t);
 
 // ------------------------------------------------- two iterations per launch
 // The FP32 correction sweeps of the mixed form run two red-black iterations

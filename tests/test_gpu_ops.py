@@ -199,8 +199,12 @@ def test_statistics(orc, bc):
     assert np.allclose(g, o, rtol=1e-12, atol=0)
 
 
+# nx = 14, 13, 130: the fused kernel; nx = 16, 20, 12 (multiples of 4): the
+# colour-split sweeps.
 @pytest.mark.parametrize("solver,shape", [("0000", (10, 12, 14)), ("0011", (10, 9, 14)),
-                                          ("111111", (9, 11, 13)), ("0000", (16, 16, 130))])
+                                          ("111111", (9, 11, 13)), ("0000", (16, 16, 130)),
+                                          ("0000", (10, 12, 16)), ("0011", (10, 9, 20)),
+                                          ("111111", (9, 11, 12))])
 @pytest.mark.parametrize("idyn", [0, 1])
 def test_sor_red_black_bit_exact(orc, solver, shape, idyn):
     rhs, p0 = rand_fields(shape, 2, seed=19)
@@ -243,9 +247,9 @@ def test_sor_rejects_odd_periodic():
         capi.poisson(np.zeros(sh), np.zeros(sh), (0.1, 0.1, 0.1), 1.5, 1e-8, 10, 0, "0000")
 
 
+@pytest.mark.parametrize("sh", [(10, 12, 14), (10, 12, 16)])
 @pytest.mark.parametrize("bc", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
-def test_correct_pression(orc, bc):
-    sh = (10, 12, 14)
+def test_correct_pression(orc, bc, sh):
     up = rand_fields(sh, 3, seed=23)
     d = (0.3, 0.32, 0.29)
     dt = 0.05
