@@ -12,8 +12,13 @@ Each requested form is timed over the same steps:
   mixed  FP32 stencils / AB update / SOR sweeps, FP64-refined Poisson
   extrap mixed + warm start by linear extrapolation p_prev + (p_prev - p_prev2)
   extrap2 mixed + quadratic extrapolation p_prev + 2 e_prev - e_prev2
-  nn     mixed + learned warm start (the trained network of
+  mg     extrap + the coarse-grid correction of the guess (multigrid V-cycles
+         on the 4^3 block averages of its residual) and Gauss-Seidel sweeps
+         opening the FP32 correction solve
+  nn0    mixed + learned warm start (the trained network of
          osinco3d_amd/warmstart_weights.npy; skipped when it is missing)
+  nn     nn0 + the coarse-grid correction and Gauss-Seidel sweeps of mg (the
+         headline: every warm-start stage)
 The JSON line's value is the steps/s of the headline form (the last of
 --forms); every form's figures are included under "forms".
 
@@ -60,7 +65,11 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--forms", default="fp64,mixed,extrap,nn")
+    ap.add_argument("--forms", default="fp64,mixed,extrap,mg,nn0,nn")
+    ap.add_argument("--coarse-cycles", type=int, default=2,
+                    help="V-cycles of the coarse-grid correction (forms mg, nn)")
+    ap.add_argument("--smooth-sweeps", type=int, default=8,
+                    help="Gauss-Seidel sweeps opening the FP32 solve (forms mg, nn)")
     ap.add_argument("--workload", choices=("tgv", "mixing"), default="tgv",
                     help="tgv: the headline Taylor-Green case; mixing: the 384x256x128 LES "
                          "temporal mixing layer (free-slip y, AB2), no CPU baseline")
@@ -97,22 +106,23 @@ def pct(v, q):
     return v[k]
 
 
-def run_form(form, cfg, steps, warmup, model_path, init=None):
+def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
     import torch
     from osinco3d_amd import solver as S
 
     prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = {"nn": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE,
-            "extrap2": S.WARM_EXTRAPOLATE2}.get(form, S.WARM_PREVIOUS)
+    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE,
+            "mg": S.WARM_EXTRAPOLATE, "extrap2": S.WARM_EXTRAPOLATE2}.get(form, S.WARM_PREVIOUS)
+    cc, gs = coarse if form in ("mg", "nn") else (0, 0)
     net = None
-    if form == "nn":
+    if form in ("nn", "nn0"):
         from osinco3d_amd import warmstart
         net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
         if net is None:
             print(f"[bench] form nn skipped: no model at {model_path}", file=sys.stderr,
                   flush=True)
             return None
-    s = S.Solver(cfg, prec, warm)
+    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs)
     if net is not None:
         net.attach(s)
     if init is None:
@@ -179,7 +189,10 @@ def run_form(form, cfg, steps, warmup, model_path, init=None):
                          "iterations_per_launch": sor_sw[q] / sor_ln[q],
                          "us_per_iteration": 1e3 * sor_ms[q] / sor_sw[q],
                          "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs}
-    if form in ("nn", "extrap", "extrap2"):
+    if cc or gs:
+        out["coarse_cycles"] = cc
+        out["smooth_sweeps"] = gs
+    if form in ("nn", "nn0", "extrap", "extrap2", "mg"):
         out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)
         out["warm_residual_l2"] = {"previous": statistics.mean(i["warm_dmax"][0] for i in infos),
                                    "guess": statistics.mean(i["warm_dmax"][1] for i in infos)}
@@ -326,7 +339,8 @@ def main():
     for form in forms:
         print(f"[bench] form {form}: {cfg.nx}x{cfg.ny}x{cfg.nz}, warmup {args.warmup}, "
               f"steps {args.steps}", file=sys.stderr, flush=True)
-        r = run_form(form, cfg, args.steps, args.warmup, args.model, init)
+        r = run_form(form, cfg, args.steps, args.warmup, args.model, init,
+                     (args.coarse_cycles, args.smooth_sweeps))
         if r is not None:
             results[form] = r
             print(f"[bench] form {form}: {r['steps_per_s']:.2f} steps/s, "
@@ -371,7 +385,9 @@ def main():
         "dtype": "f64" if head_form == "fp64" else "f32+f64",
         "data": data,
         "config": {"workload": workload, "grid": [cfg.nx, cfg.ny, cfg.nz],
-                   "form": head_form, "parallelism": "none (1 GPU)"},
+                   "form": head_form, "parallelism": "none (1 GPU)",
+                   "warm_start": {"coarse_cycles": head.get("coarse_cycles", 0),
+                                  "smooth_sweeps": head.get("smooth_sweeps", 0)}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "forms": results,

@@ -96,6 +96,17 @@ int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz
                     int nz, double *omega, double eps, int kmax, int idyn, int solver,
                     int variant, int *exit_iter, double *dmax, int *reason);
 
+/* The coarse-grid solver of the warm starts on its own: `cycles` multigrid
+ * V(2,2) cycles on L u = f from u = 0, L the 7-point Laplacian of a
+ * cell-centred mx x my x mz grid with spacings hx, hy, hz; per[a] != 0 makes
+ * axis a periodic, else the ghost cell equals the edge cell (homogeneous
+ * Neumann).  Levels are halved while every axis is even and >= 4; the
+ * coarsest (at most 512 cells, or where halving stops) gets 60 Gauss-Seidel
+ * sweeps.  u, f: host or device float32.  No counterpart in the reference
+ * (used by the mixed forms' coarse-grid correction, o3d_params.coarse_cycles). */
+int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx, double hy,
+                  double hz, const int *per, int cycles);
+
 /* correct_pression, src/integration.f90:199-255 (SOR branch): divergence of
  * the predicted velocity (odd variants), rhs = div/dt, then the Poisson solver
  * that schemes() binds for bc (src/initialization.f90:283-301). */
@@ -179,6 +190,11 @@ typedef struct o3d_params {
   int precision;        /* O3D_FP64 or O3D_MIXED */
   int warm_start;       /* O3D_WARM_PREVIOUS, _LEARNED, _EXTRAPOLATE or _EXTRAPOLATE2 */
   int inner_max;        /* O3D_MIXED: FP32 sweeps between FP64 refinements (0 = default) */
+  int coarse_cycles;    /* O3D_MIXED with a warm start: multigrid V-cycles of the coarse-grid
+                           correction added to the guess (0 = none; every axis a multiple of
+                           O3D_WARM_COARSEN) */
+  int smooth_sweeps;    /* O3D_MIXED: Gauss-Seidel sweeps (omega = 1) that open the first FP32
+                           correction solve of each step (0 = none) */
 } o3d_params;
 
 typedef struct o3d_step_info {

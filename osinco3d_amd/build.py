@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["o3d_stencil.hip", "o3d_sor.hip", "o3d_nn.hip", "o3d_api.hip"]
+SOURCES = ["o3d_stencil.hip", "o3d_sor.hip", "o3d_mg.hip", "o3d_nn.hip", "o3d_api.hip"]
 # Modules that compile on their own.  differential_operators, les_turbulence
 # and integration use the reference's initialization module (as the
 # reference's own versions do); they are built together with it by

@@ -43,6 +43,8 @@ class O3DParams(C.Structure):
         ("precision", C.c_int),
         ("warm_start", C.c_int),
         ("inner_max", C.c_int),
+        ("coarse_cycles", C.c_int),
+        ("smooth_sweeps", C.c_int),
     ]
 
 
@@ -79,7 +81,7 @@ EXPORTS = [
     "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
-    "o3d_warm_features", "o3d_solver_history",
+    "o3d_warm_features", "o3d_solver_history", "o3d_multigrid",
 ]
 NN_EXPORTS = ["o3d_nn_create", "o3d_nn_destroy", "o3d_nn_resize", "o3d_nn_num_params",
               "o3d_nn_get_params", "o3d_nn_set_params", "o3d_nn_get_grads", "o3d_nn_zero_grad",
@@ -201,6 +203,15 @@ def poisson(pp, rhs, d, omega, eps, kmax, idyn, solver):
                              int(kmax), int(idyn), int(code), C.byref(it), C.byref(dm),
                              C.byref(rs)))
     return it.value, dm.value, rs.value, om.value
+
+
+def multigrid(u, f, h, per, cycles):
+    """V(2,2) multigrid cycles on the cell-centred 7-point Poisson problem
+    L u = f from u = 0 (u written in place); h = (hx, hy, hz), per = three
+    flags (periodic axis, else homogeneous Neumann)."""
+    nx, ny, nz = _dims(u)
+    check(load().o3d_multigrid(ptr(u), ptr(f), nx, ny, nz, *map(_d, h),
+                               (C.c_int * 3)(*[int(bool(q)) for q in per]), int(cycles)))
 
 
 def poisson_f32(pp, rhs, d, omega, eps, kmax, idyn, solver, variant=0):

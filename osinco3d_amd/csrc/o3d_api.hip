@@ -19,6 +19,7 @@
 #include "../../include/o3d.h"
 #include "../../include/o3d_nn.h"
 #include "o3d_nn.h"
+#include "o3d_mg.h"
 #include "o3d_sor.h"
 #include "o3d_stencil.h"
 
@@ -506,6 +507,7 @@ SorSplitArgs<T> split_args(const SorArgs<T, T> &a, T *red, T *black, const T *rh
   sa.kmax = a.kmax;
   sa.idyn = a.idyn;
   sa.it = a.it;
+  sa.gs = a.gs;
   return sa;
 }
 
@@ -547,6 +549,8 @@ int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, in
   a.factor = solver == SOR_0011 ? 1.01 : 1.05;  // src/poisson.f90:38,162,285
   a.kmax = kmax;
   a.idyn = idyn;
+  a.it = 0;
+  a.gs = 0;
   int sweeps = 0;
   int rc = 0;
   if (variant == 2 || (variant < 0 && sor_split_supported<T>(nx, ny, nz))) {
@@ -594,6 +598,32 @@ extern "C" int o3d_poisson(double *pp, const double *rhs, double dx, double dy, 
   CHK(s.in(rhs, n, &r));
   CHK(poisson_dev(p, r, dx, dy, dz, nx, ny, nz, omega, eps, kmax, idyn, solver, exit_iter, dmax,
                   reason, s));
+  return s.finish();
+}
+
+extern "C" int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx,
+                             double hy, double hz, const int *per, int cycles) {
+  CHK(ensure_init());
+  if (mx < 1 || my < 1 || mz < 1 || (long long)mx * my * mz >= (1ll << 31))
+    return fail("multigrid grid %dx%dx%d out of range", mx, my, mz);
+  if (!per || !u || !f) return fail("multigrid: NULL argument");
+  if (cycles < 1) return fail("multigrid: cycles must be >= 1");
+  const size_t n = (size_t)mx * my * mz;
+  Staged s;
+  float *du;
+  const float *df;
+  CHK(s.out(u, n, &du, false));
+  CHK(s.in(f, n, &df));
+  MgHierarchy H;
+  const int m0[3] = {mx, my, mz};
+  const double h[3] = {hx, hy, hz};
+  const int p01[3] = {per[0] != 0, per[1] != 0, per[2] != 0};
+  float *buf;
+  CHK(s.scratch(mg_plan(H, m0, h, p01), &buf));
+  mg_bind(H, buf);
+  HIPCHK(hipMemcpyAsync(H.f[0], df, n * sizeof(float), hipMemcpyDeviceToDevice, g_stream));
+  HIPCHK(mg_solve(H, cycles, 2, 2, g_stream));
+  HIPCHK(hipMemcpyAsync(du, H.u[0], n * sizeof(float), hipMemcpyDeviceToDevice, g_stream));
   return s.finish();
 }
 
@@ -903,6 +933,8 @@ struct Solver {
   float *Feat = nullptr;       // network input, 2 coarse planes
   float *Out = nullptr;        // fine-grid scratch
   float *Outc = nullptr;       // network output (coarse)
+  MgHierarchy mg;              // coarse-grid correction (coarse_cycles > 0)
+  float *mgbuf = nullptr;
   double *npart = nullptr;     // reduction partials (residual norms, sums of squares)
   double *hnpart = nullptr;    // pinned copy
   double *epart = nullptr;     // sum-of-squares partials of e_prev
@@ -1024,6 +1056,8 @@ void sor_args_common(Solver &S, SorArgs<T, TR> &a, const TR *rhs, SorState *st, 
   a.factor = S.kind == SOR_0011 ? 1.01 : 1.05;
   a.kmax = kmax;
   a.idyn = S.p.idyn;
+  a.it = 0;
+  a.gs = 0;
 }
 
 // Plain FP64 Poisson solve on P[cur] (src/poisson.f90 semantics, red-black).
@@ -1103,6 +1137,9 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     r.buf[1] = S.E[1];
     const int kin = std::min(inner_max, p.kmax - total);
     sor_args_common<float, float>(S, r.a, S.R, S.st, kin);
+    // The first correction solve of the step opens with Gauss-Seidel sweeps
+    // (smoothing the high-frequency error a warm start leaves).
+    if (cycles == 0) r.a.gs = p.smooth_sweeps;
     r.pair = false;
     if (split) {
       const int h = n / 2;
@@ -1222,6 +1259,29 @@ __global__ void lw_guess_coarse_kernel(const double *__restrict__ p, const float
   q[o] = p[o] + (double)ep[o] + s * (double)v;
 }
 
+// q += trilinear interpolation of the coarse correction c (the multigrid
+// solution on the grid of F^3 block averages).
+__global__ void mg_prolong_fine_kernel(double *__restrict__ q, const float *__restrict__ c,
+                                       int nx, int ny, int nz, int F, int px, int py, int pz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= nx) return;
+  const int mx = nx / F, my = ny / F, mz = nz / F;
+  int x0, x1, y0, y1, z0, z1;
+  float wx, wy, wz;
+  lw_axis(i, F, mx, px, x0, x1, wx);
+  lw_axis(j, F, my, py, y0, y1, wy);
+  lw_axis(k, F, mz, pz, z0, z1, wz);
+  auto v = [&](int a, int b, int d) { return c[(size_t)a + (size_t)mx * (b + (size_t)my * d)]; };
+  const float t =
+      (1.f - wz) * ((1.f - wy) * ((1.f - wx) * v(x0, y0, z0) + wx * v(x1, y0, z0)) +
+                    wy * ((1.f - wx) * v(x0, y1, z0) + wx * v(x1, y1, z0))) +
+      wz * ((1.f - wy) * ((1.f - wx) * v(x0, y0, z1) + wx * v(x1, y0, z1)) +
+            wy * ((1.f - wx) * v(x0, y1, z1) + wx * v(x1, y1, z1)));
+  const size_t o = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+  q[o] = q[o] + (double)t;
+}
+
 __global__ void lw_guess_kernel(const double *p, const float *o, double *q, int n, double s) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) q[i] = p[i] + s * (double)o[i];
@@ -1317,6 +1377,16 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.Out, n));
   CHK(S.alloc(&S.Feat, 2 * nc));
   CHK(S.alloc(&S.Outc, nc));
+  if (S.p.coarse_cycles > 0) {
+    const int F = O3D_WARM_COARSEN;
+    const int m0[3] = {S.p.nx / F, S.p.ny / F, S.p.nz / F};
+    const double h[3] = {S.p.dx * F, S.p.dy * F, S.p.dz * F};
+    int per[3];
+    solver_periodic(S.kind, per);
+    const size_t floats = mg_plan(S.mg, m0, h, per);
+    CHK(S.alloc(&S.mgbuf, floats));
+    mg_bind(S.mg, S.mgbuf);
+  }
   // Shared by the residual-norm pair (4 per block + 4) and lw_sumsq
   // (kLwBlocks partials, copied whole to the host).
   const int np = std::max(4 * residual_blocks(S.p.nx, S.p.ny, S.p.nz) + 4, 2 * kLwBlocks);
@@ -1428,6 +1498,29 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
   }
   // Otherwise q already holds p_prev + e_prev (solver_learning_prepare).
   HIPCHK(hipGetLastError());
+  if (S.p.coarse_cycles > 0) {
+    // Coarse-grid correction: the residual of q averaged over F^3 blocks,
+    // the coarse Poisson problem solved by multigrid V-cycles, its trilinear
+    // interpolation added to q.  S.Rl holds the residual of the plain
+    // extrapolation; other guesses need their own.
+    const int F = O3D_WARM_COARSEN;
+    int per[3];
+    solver_periodic(S.kind, per);
+    const bool own = S.nn || (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2);
+    if (own)
+      HIPCHK(launch_residual_f32<float>(q, (const float *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
+                                        S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                        g_stream));
+    const int nc = n / (F * F * F);
+    lw_pool_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.mg.f[0], S.p.nx, S.p.ny,
+                                                           S.p.nz, F, 1.0f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 2, 2, g_stream));
+    dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
+    mg_prolong_fine_kernel<<<grid, 256, 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny, S.p.nz, F,
+                                                       per[0], per[1], per[2]);
+    HIPCHK(hipGetLastError());
+  }
   CHK(lw_residual_l2_pair(S, S.P[S.cur], q, warm));
   if (warm[1] < warm[0]) {
     S.cur = 1 - S.cur;
@@ -1544,6 +1637,13 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     return fail("unknown warm start %d", p->warm_start);
   const int kind = solver_from_bc(p->bc);
   if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");
+  if (p->coarse_cycles < 0 || p->smooth_sweeps < 0)
+    return fail("coarse_cycles and smooth_sweeps must be >= 0");
+  if ((p->coarse_cycles > 0 || p->smooth_sweeps > 0) && p->precision != O3D_MIXED)
+    return fail("coarse_cycles and smooth_sweeps apply to the mixed form only");
+  if (p->coarse_cycles > 0 &&
+      (p->nx % O3D_WARM_COARSEN || p->ny % O3D_WARM_COARSEN || p->nz % O3D_WARM_COARSEN))
+    return fail("the coarse-grid correction needs every axis a multiple of %d", O3D_WARM_COARSEN);
   CHK(check_red_black(kind, p->nx, p->ny, p->nz));
   Solver *S = new Solver();
   S->p = *p;

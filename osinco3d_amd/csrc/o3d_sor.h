@@ -58,6 +58,7 @@ struct SorArgs {
   double eps, factor;
   int kmax, idyn;
   int it;        // iteration number of this launch (added to st->base)
+  int gs;        // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing)
 };
 
 template <typename T, typename TR>
@@ -89,6 +90,7 @@ struct SorSplitArgs {
   double eps, factor;
   int kmax, idyn;
   int it;       // iteration number of this launch pair (added to st->base)
+  int gs;       // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing)
 };
 // nx a multiple of 8 (FP32) or 4 (FP64): 16-byte vectors of one colour.
 template <typename T>

@@ -255,8 +255,11 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   const int z1 = min(z0 + a.tl.zc, nz);
   const int tid = threadIdx.x;
   const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A), invA = T(1.0 / a.A);
-  const T w = T(om);
-  const T w1 = T(1.0 - om);
+  // The first a.gs iterations are Gauss-Seidel sweeps (omega = 1); the
+  // omega ring keeps the solve's omega.
+  const double omu = it <= a.gs ? 1.0 : om;
+  const T w = T(omu);
+  const T w1 = T(1.0 - omu);
 
   // Global column offsets of this thread's R2 and R1 cells (the second of
   // each exists for the first R2N-SNT / R1N-SNT threads; the others load a
@@ -544,7 +547,10 @@ __global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs<
   if (done) return;
   const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A);
   const T invA = T(1.0 / a.A);
-  const T w = T(om), w1 = T(1.0 - om);
+  // The first a.gs iterations are Gauss-Seidel sweeps (omega = 1); the
+  // omega ring keeps the solve's omega.
+  const double omu = it <= a.gs ? 1.0 : om;
+  const T w = T(omu), w1 = T(1.0 - omu);
   T dmax = T(0);
   if (active) {
 #pragma unroll
@@ -944,7 +950,7 @@ SorTiling sor2_tiling(int nx, int ny, int nz) {
 }
 
 hipError_t launch_sor2_iteration(const SorArgs<float, float> &a, hipStream_t s) {
-  if (a.tl.zc % kPairRing != 0 || !sor2_supported(a.nx, a.ny, a.nz) ||
+  if (a.tl.zc % kPairRing != 0 || !sor2_supported(a.nx, a.ny, a.nz) || a.gs != 0 ||
       (long long)a.nx * a.ny * a.nz * 4LL > 0x7fffffffLL)
     return hipErrorInvalidValue;
   sor2_kernel<<<dim3(a.tl.blocks()), dim3(SNT), 0, s>>>(a);

@@ -83,7 +83,8 @@ def mixing_layer(cfg: Config, seed: int = 1234, amplitude: float = 0.05):
 
 
 def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
-                inner_max: int = 0) -> capi.O3DParams:
+                inner_max: int = 0, coarse_cycles: int = 0,
+                smooth_sweeps: int = 0) -> capi.O3DParams:
     p = capi.O3DParams()
     p.nx, p.ny, p.nz = cfg.nx, cfg.ny, cfg.nz
     p.dx, p.dy, p.dz = cfg.spacing()
@@ -107,6 +108,8 @@ def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
     p.precision = precision
     p.warm_start = warm
     p.inner_max = inner_max
+    p.coarse_cycles = coarse_cycles
+    p.smooth_sweeps = smooth_sweeps
     return p
 
 
@@ -114,11 +117,12 @@ class Solver:
     """Device-resident time stepping of one configuration."""
 
     def __init__(self, cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
-                 inner_max: int = 0):
+                 inner_max: int = 0, coarse_cycles: int = 0, smooth_sweeps: int = 0):
         self.cfg = cfg
         self.precision = precision
         self.lib = capi.load()
-        self.params = make_params(cfg, precision, warm, inner_max)
+        self.params = make_params(cfg, precision, warm, inner_max, coarse_cycles,
+                                  smooth_sweeps)
         self.handle = C.c_void_p(0)
         capi.check(self.lib.o3d_solver_create(C.byref(self.params), C.byref(self.handle)))
         self.dt = self.params.dt

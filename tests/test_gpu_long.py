@@ -5,8 +5,10 @@ enstrophy series (a 1000-step run of the reference on one CPU core would
 take about 16 hours at its measured ~57 s per step); the FP64 form's
 operators are bit-identical to the reference (tested at small sizes) and its
 Poisson solve stops on the reference's rule.  The mixed-precision form and
-the warm-started forms must track the FP64 series within 1e-6 relative at
-every sample, taken every 25 steps as the reference's statistics are.
+the warm-started forms (extrapolation, the learned guess, and both with the
+coarse-grid correction and Gauss-Seidel smoothing) must track the FP64
+series within 1e-6 relative at every sample, taken every 25 steps as the
+reference's statistics are.
 """
 from __future__ import annotations
 
@@ -26,9 +28,13 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
                        "osinco3d_amd", "warmstart_weights.npy")
 
 
-def series(precision, warm=WARM_PREVIOUS, net=None):
+# The coarse-grid correction and smoothing of the bench's mg and nn forms.
+COARSE = (2, 8)
+
+
+def series(precision, warm=WARM_PREVIOUS, net=None, coarse=(0, 0)):
     cfg = tgv_config(N, STEPS, 3)
-    s = Solver(cfg, precision, warm)
+    s = Solver(cfg, precision, warm, coarse_cycles=coarse[0], smooth_sweeps=coarse[1])
     if net is not None:
         net.attach(s)
     s.init_taylor_green()
@@ -44,7 +50,7 @@ def series(precision, warm=WARM_PREVIOUS, net=None):
             ek.append(st[1])
             ens.append(st[4])
         if it % 250 == 0:
-            print(f"[long] precision {precision} warm {warm}: step {it}, "
+            print(f"[long] precision {precision} warm {warm} coarse {coarse}: step {it}, "
                   f"E_k {st[1]:.10e}, iters {np.mean(iters[-250:]):.1f}", flush=True)
     s.close()
     return np.array(ek), np.array(ens), np.array(iters)
@@ -81,5 +87,12 @@ def test_learned_warm_start_tracks_fp64(fp64_series):
     net = warmstart.load_or_none(WEIGHTS, (N, N, N))
     try:
         _check(fp64_series, series(MIXED, WARM_LEARNED, net), "mixed + learned")
+        _check(fp64_series, series(MIXED, WARM_LEARNED, net, COARSE),
+               "mixed + learned + coarse-grid correction")
     finally:
         net.close()
+
+
+def test_coarse_grid_warm_start_tracks_fp64(fp64_series):
+    _check(fp64_series, series(MIXED, WARM_EXTRAPOLATE, coarse=COARSE),
+           "mixed + extrapolation + coarse-grid correction")

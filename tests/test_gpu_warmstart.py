@@ -1,12 +1,15 @@
-"""What the trained warm-start network gains, on the configuration it is meant
-for: Taylor-Green Re 1600 at 256^3, mixed-precision form, SOR to eps = 1e-8.
+"""What the warm starts gain, on the configuration they are meant for:
+Taylor-Green Re 1600 at 256^3, mixed-precision form, SOR to eps = 1e-8.
 
-The method without the network starts each solve from the previous step's
-pressure (the reference's behaviour).  The learned form starts from the
-linear extrapolation corrected by the network (the previous pressure stays
+The method without them starts each solve from the previous step's pressure
+(the reference's behaviour).  The learned form starts from the linear
+extrapolation corrected by the trained network (the previous pressure stays
 the fallback); the plain extrapolation is measured beside it so the
-network's own share of the gain is visible.  The solve's stop rule is the
-same in every form, so the fields agree to a tolerance tied to eps.
+network's own share of the gain is visible.  The coarse-grid correction
+(multigrid on the 4^3 block averages of the guess's residual) with
+Gauss-Seidel sweeps opening the FP32 solve is measured on its own and
+after the network (the headline form).  The solve's stop rule is the same
+in every form, so the fields agree to a tolerance tied to eps.
 """
 from __future__ import annotations
 
@@ -21,13 +24,14 @@ from osinco3d_amd.solver import MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREV
 pytestmark = pytest.mark.gpu
 
 N, STEPS, SKIP = 256, 16, 3
+COARSE = (2, 8)
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "osinco3d_amd", "warmstart_weights.npy")
 
 
-def run(warm, net=None):
+def run(warm, net=None, coarse=(0, 0)):
     cfg = tgv_config(N, STEPS, 3)
-    s = Solver(cfg, MIXED, warm)
+    s = Solver(cfg, MIXED, warm, coarse_cycles=coarse[0], smooth_sweeps=coarse[1])
     if net is not None:
         net.attach(s)
     s.init_taylor_green()
@@ -36,14 +40,14 @@ def run(warm, net=None):
         info = s.step(it)
         assert info["sor_reason"] in (0, 1) and not info["diverged"]
         if it > SKIP:
-            iters.append(info["sor_exit_iter"])
+            iters.append(info["sweeps_f32"] + info["sweeps_f64"])
             used.append(info["warm_used"])
     fields = s.download()
     s.close()
     return np.array(iters), np.mean(used), fields
 
 
-def test_trained_network_reduces_sor_iterations():
+def test_warm_starts_reduce_sor_sweeps():
     if not os.path.exists(WEIGHTS):
         pytest.skip("no trained warm-start weights in the package")
     from osinco3d_amd import warmstart
@@ -52,13 +56,20 @@ def test_trained_network_reduces_sor_iterations():
         it_prev, _, f_prev = run(WARM_PREVIOUS)
         it_ex, _, _ = run(WARM_EXTRAPOLATE)
         it_nn, used, f_nn = run(WARM_LEARNED, net)
+        it_mg, used_mg, f_mg = run(WARM_EXTRAPOLATE, coarse=COARSE)
+        it_nm, used_nm, f_nm = run(WARM_LEARNED, net, COARSE)
     finally:
         net.close()
-    print(f"[warm] SOR iterations/step at {N}^3: previous {it_prev.mean():.1f}, "
-          f"extrapolation {it_ex.mean():.1f}, learned {it_nn.mean():.1f} "
-          f"(guess accepted on {used:.2f} of the steps)", flush=True)
-    assert used == 1.0
+    print(f"[warm] SOR sweeps/step at {N}^3: previous {it_prev.mean():.1f}, "
+          f"extrapolation {it_ex.mean():.1f}, learned {it_nn.mean():.1f} (accepted on "
+          f"{used:.2f} of the steps), extrapolation + coarse-grid correction "
+          f"{it_mg.mean():.1f}, learned + coarse-grid correction {it_nm.mean():.1f}",
+          flush=True)
+    assert used == 1.0 and used_mg == 1.0 and used_nm == 1.0
     assert it_nn.mean() <= 0.8 * it_prev.mean()
     assert it_nn.mean() <= 1.02 * it_ex.mean()
-    for c in range(3):
-        assert np.abs(f_nn[c] - f_prev[c]).max() <= 1e-6
+    assert it_mg.mean() <= 0.5 * it_ex.mean()
+    assert it_nm.mean() <= 0.5 * it_ex.mean()
+    for f in (f_nn, f_mg, f_nm):
+        for c in range(3):
+            assert np.abs(f[c] - f_prev[c]).max() <= 1e-6
