@@ -101,8 +101,8 @@ int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz
  * cell-centred mx x my x mz grid with spacings hx, hy, hz; per[a] != 0 makes
  * axis a periodic, else the ghost cell equals the edge cell (homogeneous
  * Neumann).  Levels are halved while every axis is even and >= 4; the
- * coarsest (at most 512 cells, or where halving stops) gets 60 Gauss-Seidel
- * sweeps.  u, f: host or device float32.  No counterpart in the reference
+ * coarsest (at most 64 cells, or where halving stops) is solved by
+ * Gauss-Seidel sweeps (30, or 60 above 64 cells).  u, f: host or device float32.  No counterpart in the reference
  * (used by the mixed forms' coarse-grid correction, o3d_params.coarse_cycles). */
 int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx, double hy,
                   double hz, const int *per, int cycles);

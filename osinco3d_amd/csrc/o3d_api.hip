@@ -911,6 +911,7 @@ struct Solver {
   int cur = 0;                 // P[cur] holds the pressure
   float *E[2] = {nullptr, nullptr}, *R = nullptr;  // mixed: correction ping-pong, residual
   unsigned long long *dslot = nullptr;  // colour-split sweeps: dmax slots
+  bool r_ready = false;  // R, E[0] hold the split residual of P[cur] and e = 0
   double *rhs_split = nullptr;          // FP64 form: the rhs in colour-split storage
   int ring = 0;                // predict calls so far (history ring position)
   // Reductions.
@@ -1121,7 +1122,9 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     // iterates.  (Two iterations per launch, sor2_kernel, measured slower
     // than either at 256^3.)
     const bool split = S.dslot != nullptr;
-    if (split)
+    if (split && cycles == 0 && S.r_ready) {
+      // r and e = 0 already written by the warm-start acceptance check.
+    } else if (split)
       HIPCHK(launch_residual_split<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx,
                                           p.ny, p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
                                           g_stream));
@@ -1181,6 +1184,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     judge_prev = dlast;
   }
   S.last_sweeps[1] = sw32;
+  S.r_ready = false;
   info->sor_exit_iter = reason == 2 ? p.kmax + 1 : total;
   info->sor_reason = reason;
   info->sweeps_f64 = sw64;
@@ -1257,6 +1261,14 @@ __global__ void lw_guess_coarse_kernel(const double *__restrict__ p, const float
             wy * ((1.f - wx) * c(x0, y1, z1) + wx * c(x1, y1, z1)));
   const size_t o = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
   q[o] = p[o] + (double)ep[o] + s * (double)v;
+}
+
+// u0 = s * (out - in1) on the coarse grid: the network's estimate of the
+// coarse correction of the extrapolation, the multigrid's starting iterate.
+__global__ void lw_coarse_init_kernel(float *__restrict__ u0, const float *__restrict__ out,
+                                      const float *__restrict__ in1, int nc, float s) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I < nc) u0[I] = s * (out[I] - in1[I]);
 }
 
 // q += trilinear interpolation of the coarse correction c (the multigrid
@@ -1410,10 +1422,14 @@ int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *o
   solver_periodic(S.kind, per);
   const int nb = residual_blocks(S.p.nx, S.p.ny, S.p.nz);
   double *fin = S.npart + 4 * nb;
+  // In the mixed form with colour-split sweeps, the residual of p1 (the
+  // guess) is kept for the first correction solve (S.r_ready).
+  S.r_ready = S.mixed && S.dslot != nullptr;
   if (S.mixed)
     HIPCHK(launch_residual_norm_pair<float>(p0, p1, (const float *)S.rhs, S.npart, fin, S.p.nx,
                                             S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz,
-                                            S.sc.A, g_stream));
+                                            S.sc.A, g_stream, S.r_ready ? S.R : nullptr,
+                                            S.r_ready ? S.E[0] : nullptr));
   else
     HIPCHK(launch_residual_norm_pair<double>(p0, p1, (const double *)S.rhs, S.npart, fin,
                                              S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy,
@@ -1432,8 +1448,6 @@ int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *o
 int solver_learning_prepare(Solver &S) {
   CHK(lw_alloc(S));
   const int n = S.n;
-  HIPCHK(hipMemcpyAsync(S.Pstart, S.P[S.cur], (size_t)n * 8, hipMemcpyDeviceToDevice,
-                        g_stream));
   const double *base = S.P[S.cur];
   if (S.hist >= 1) {
     lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.P[1 - S.cur],
@@ -1472,6 +1486,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
   if (S.hist < 1) return 0;
   const int n = S.n;
   double *q = S.P[1 - S.cur];
+  bool nn_coarse = false;  // the network's output is the coarse solve's start
   if (S.nn) {
     LwSums w = lw_of(S);
     double re = 0.0;
@@ -1489,10 +1504,18 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     const int nc = n / (F * F * F);
     int per[3];
     solver_periodic(S.kind, per);
-    dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
-    lw_guess_coarse_kernel<<<grid, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Outc,
-                                                       S.Feat + nc, q, S.p.nx, S.p.ny, S.p.nz,
-                                                       F, per[0], per[1], per[2], re);
+    if (S.p.coarse_cycles > 0) {
+      // The network's coarse correction starts the multigrid solve of the
+      // coarse correction below; q keeps the extrapolation.
+      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.mg.u[0], S.Outc,
+                                                                    S.Feat + nc, nc, (float)re);
+      nn_coarse = true;
+    } else {
+      dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
+      lw_guess_coarse_kernel<<<grid, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Outc,
+                                                         S.Feat + nc, q, S.p.nx, S.p.ny,
+                                                         S.p.nz, F, per[0], per[1], per[2], re);
+    }
   } else if (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2) {
     lw_guess2_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Eprev2, q, n);
   }
@@ -1506,7 +1529,8 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     const int F = O3D_WARM_COARSEN;
     int per[3];
     solver_periodic(S.kind, per);
-    const bool own = S.nn || (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2);
+    const bool own = (S.nn && !nn_coarse) ||
+                     (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2);
     if (own)
       HIPCHK(launch_residual_f32<float>(q, (const float *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
                                         S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
@@ -1515,7 +1539,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     lw_pool_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.mg.f[0], S.p.nx, S.p.ny,
                                                            S.p.nz, F, 1.0f);
     HIPCHK(hipGetLastError());
-    HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 2, 2, g_stream));
+    HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 2, 2, g_stream, !nn_coarse));
     dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
     mg_prolong_fine_kernel<<<grid, 256, 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny, S.p.nz, F,
                                                        per[0], per[1], per[2]);
@@ -1525,6 +1549,8 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
   if (warm[1] < warm[0]) {
     S.cur = 1 - S.cur;
     *used = 1;
+  } else {
+    S.r_ready = false;  // the solve starts from p_prev, whose residual was not kept
   }
   return 0;
 }
@@ -1560,6 +1586,15 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   int warm = 0;
   if (learning) CHK(solver_learning_prepare(S));
   if (guess) CHK(solver_warm_guess(S, &warm, info->warm_dmax));
+  if (learning) {
+    // Keep p_prev for the increment: with an accepted guess it sits in the
+    // other pressure buffer, which trades places with the spare one (the
+    // solve ping-pongs between P[cur] and P[1 - cur]); otherwise the solve
+    // works on it in place, so it is copied.
+    if (warm) std::swap(S.P[1 - S.cur], S.Pstart);
+    else HIPCHK(hipMemcpyAsync(S.Pstart, S.P[S.cur], (size_t)S.n * 8, hipMemcpyDeviceToDevice,
+                               g_stream));
+  }
   if (S.mixed) CHK(solver_poisson_mixed(S, info));
   else CHK(solver_poisson_f64(S, info));
   info->warm_used = warm;

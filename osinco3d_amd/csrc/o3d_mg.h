@@ -3,7 +3,8 @@
 // The Poisson problem L u = f on a cell-centred grid (the block averages of
 // the fine grid) is solved approximately by multigrid V-cycles: red-black
 // Gauss-Seidel smoothing, restriction by 2x2x2 averages, trilinear
-// prolongation, and the coarsest level solved by one block in LDS.  Periodic
+// prolongation, and the coarsest level (at most 64 cells where the axes
+// halve that far) solved by one block in LDS.  Periodic
 // axes wrap; the others use the cell-centred mirror (ghost = edge cell).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -27,8 +28,10 @@ struct MgHierarchy {
 // count the hierarchy needs (bind with mg_bind).
 size_t mg_plan(MgHierarchy &H, const int m0[3], const double h[3], const int per[3]);
 void mg_bind(MgHierarchy &H, float *base);
-// `cycles` V(nu1, nu2) cycles on L u0 = f0 from u0 = 0 (f0 = H.f[0] filled by
-// the caller).
-hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s);
+// `cycles` V(nu1, nu2) cycles on L u0 = f0 (f0 = H.f[0] filled by the
+// caller) from u0 = 0, or from the u0 = H.u[0] the caller wrote
+// (from_zero = false).
+hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s,
+                    bool from_zero = true);
 
 }  // namespace o3d

@@ -31,7 +31,7 @@ size_t mg_plan(MgHierarchy &H, const int m0[3], const double h[3], const int per
     H.levels = l + 1;
     const bool halvable = m[0] % 2 == 0 && m[1] % 2 == 0 && m[2] % 2 == 0 && m[0] >= 4 &&
                           m[1] >= 4 && m[2] >= 4;
-    if (cells <= 512 || !halvable) break;
+    if (cells <= 64 || !halvable) break;
     for (int a = 0; a < 3; ++a) {
       m[a] /= 2;
       hh[a] *= 2.0;
@@ -206,8 +206,10 @@ hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) 
   const MgGrid g = grid_of(H, l);
   const int n = g.mx * g.my * g.mz;
   if (l == H.levels - 1) {
+    // At most 64 cells when the axes halve that far: one wave, 30 sweeps.
     if (n <= kMgCoarsestCells) {
-      mg_coarsest_kernel<<<1, 1024, 0, s>>>(H.u[l], H.f[l], g, 60);
+      const int threads = n <= 64 ? 64 : 1024;
+      mg_coarsest_kernel<<<1, threads, 0, s>>>(H.u[l], H.f[l], g, n <= 64 ? 30 : 60);
       return hipGetLastError();
     }
     return smooth(H, l, 60, s);
@@ -233,10 +235,11 @@ __global__ void mg_zero_kernel(float *u, int n) {
 
 }  // namespace
 
-hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s) {
+hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s,
+                    bool from_zero) {
   if (H.levels < 1) return hipErrorInvalidValue;
   const int n0 = H.m[0][0] * H.m[0][1] * H.m[0][2];
-  mg_zero_kernel<<<(n0 + 255) / 256, 256, 0, s>>>(H.u[0], n0);
+  if (from_zero) mg_zero_kernel<<<(n0 + 255) / 256, 256, 0, s>>>(H.u[0], n0);
   hipError_t e = hipGetLastError();
   for (int c = 0; c < cycles && e == hipSuccess; ++c) e = vcycle(H, 0, nu1, nu2, s);
   return e;

@@ -120,12 +120,15 @@ hipError_t launch_residual_split(const double *p, const TR *rhs, float *r, float
 hipError_t launch_fold_split(double *p, const float *e, int nx, int ny, int nz, hipStream_t s);
 // FP64 residual norms of two iterates p0, p1 (r = rhs - L p) in one launch:
 // out (device, 4 doubles) = (max |r0|, sum r0^2, max |r1|, sum r1^2).  part
-// holds 4 * residual_blocks(nx, ny, nz) doubles.
+// holds 4 * residual_blocks(nx, ny, nz) doubles.  With r1s (nx even), the
+// residual of p1 is also stored colour-split in FP32 and e1s is zeroed, as
+// launch_residual_split would do for p1.
 template <typename TR>
 hipError_t launch_residual_norm_pair(const double *p0, const double *p1, const TR *rhs,
                                      double *part, double *out, int nx, int ny, int nz,
                                      const int per[3], double ox, double oy, double oz, double A,
-                                     hipStream_t s);
+                                     hipStream_t s, float *r1s = nullptr,
+                                     float *e1s = nullptr);
 int residual_blocks(int nx, int ny, int nz);
 
 template <typename T>

@@ -1118,23 +1118,32 @@ hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s) {
 }
 
 // Residual norms of two iterates against one rhs in one launch: per block
-// (max |r0|, sum r0^2, max |r1|, sum r1^2).
+// (max |r0|, sum r0^2, max |r1|, sum r1^2).  With r1s, the residual of p1 is
+// also stored colour-split in FP32 (r1s, the first correction solve's rhs
+// when p1 becomes the iterate) and e1s is zeroed.
 template <typename TR>
 __global__ void __launch_bounds__(RNT) residual_norm_pair_kernel(
     const double *__restrict__ p0, const double *__restrict__ p1, const TR *__restrict__ rhs,
-    double *__restrict__ part, int nx, int ny, int nz, int px, int py, int pz, double ox,
-    double oy, double oz, double A) {
+    double *__restrict__ part, float *__restrict__ r1s, float *__restrict__ e1s, int nx, int ny,
+    int nz, int px, int py, int pz, double ox, double oy, double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0, m1 = 0.0, s1 = 0.0;
+  const size_t half = (size_t)nx * ny * nz / 2;
   residual_column(p0, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
                   [&](size_t, int, int, int, double r) {
                     m0 = fmax(m0, fabs(r));
                     s0 += r * r;
                   });
   residual_column(p1, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
-                  [&](size_t, int, int, int, double r) {
+                  [&](size_t, int i, int j, int k, double r) {
                     m1 = fmax(m1, fabs(r));
                     s1 += r * r;
+                    if (r1s) {
+                      const size_t o = (size_t)(nx / 2) * (j + (size_t)ny * k) + (i >> 1) +
+                                       (((i + j + k) & 1) ? half : 0);
+                      r1s[o] = (float)r;
+                      e1s[o] = 0.0f;
+                    }
                   });
   m0 = block_max<RNT / O3D_WAVE>(m0, sc);
   s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
@@ -1173,9 +1182,10 @@ template <typename TR>
 hipError_t launch_residual_norm_pair(const double *p0, const double *p1, const TR *rhs,
                                      double *part, double *out, int nx, int ny, int nz,
                                      const int per[3], double ox, double oy, double oz, double A,
-                                     hipStream_t s) {
+                                     hipStream_t s, float *r1s, float *e1s) {
+  if (r1s && (nx % 2 != 0 || !e1s)) return hipErrorInvalidValue;
   residual_norm_pair_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
-      p0, p1, rhs, part, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
+      p0, p1, rhs, part, r1s, e1s, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
   residual_norm_final_kernel<<<1, 1024, 0, s>>>(part, residual_blocks(nx, ny, nz), out);
   return hipGetLastError();
 }
@@ -1242,11 +1252,12 @@ template hipError_t launch_residual_split<float>(const double *, const float *, 
 template hipError_t launch_residual_norm_pair<double>(const double *, const double *,
                                                       const double *, double *, double *, int,
                                                       int, int, const int *, double, double,
-                                                      double, double, hipStream_t);
+                                                      double, double, hipStream_t, float *,
+                                                      float *);
 template hipError_t launch_residual_norm_pair<float>(const double *, const double *,
                                                      const float *, double *, double *, int, int,
                                                      int, const int *, double, double, double,
-                                                     double, hipStream_t);
+                                                     double, hipStream_t, float *, float *);
 template hipError_t launch_copy<double>(double *, const double *, int, hipStream_t);
 template hipError_t launch_copy<float>(float *, const float *, int, hipStream_t);
 template hipError_t launch_convert<float, double>(float *, const double *, int, hipStream_t);
