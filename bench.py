@@ -79,7 +79,7 @@ def parse_args():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sor_kernel_tgv256_pmc.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sor_split_kernel_tgv256_pmc.md")
 
 
 def measured_traffic(which):
