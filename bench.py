@@ -66,9 +66,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--forms", default="fp64,mixed,extrap,mg,nn0,nn")
-    ap.add_argument("--coarse-cycles", type=int, default=2,
+    ap.add_argument("--coarse-cycles", type=int, default=3,
                     help="V-cycles of the coarse-grid correction (forms mg, nn)")
-    ap.add_argument("--smooth-sweeps", type=int, default=8,
+    ap.add_argument("--smooth-sweeps", type=int, default=24,
                     help="Gauss-Seidel sweeps opening the FP32 solve (forms mg, nn)")
     ap.add_argument("--workload", choices=("tgv", "mixing"), default="tgv",
                     help="tgv: the headline Taylor-Green case; mixing: the 384x256x128 LES "

@@ -58,7 +58,8 @@ struct SorArgs {
   double eps, factor;
   int kmax, idyn;
   int it;        // iteration number of this launch (added to st->base)
-  int gs;        // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing)
+  int gs;        // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing) and
+                 // end the solve only on kmax
 };
 
 template <typename T, typename TR>
@@ -90,7 +91,8 @@ struct SorSplitArgs {
   double eps, factor;
   int kmax, idyn;
   int it;       // iteration number of this launch pair (added to st->base)
-  int gs;       // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing)
+  int gs;       // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing) and
+                // end the solve only on kmax
 };
 // nx a multiple of 8 (FP32) or 4 (FP64): 16-byte vectors of one colour.
 template <typename T>

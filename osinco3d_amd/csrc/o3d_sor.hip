@@ -226,7 +226,10 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   const int it = a.it + st->base;
   double om, dlast;
   int reason;
-  if (sor_decide(st, it, a.eps, a.factor, a.kmax, a.idyn, om, reason, dlast)) {
+  // No eps or stagnation exit inside the Gauss-Seidel opening (iterations
+  // 1..a.gs): its small updates say little about the smooth error left.
+  if (sor_decide(st, it, a.eps, a.factor, a.kmax, a.idyn, om, reason, dlast) &&
+      !(reason != 2 && it - 1 <= a.gs)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done = 1;
       st->reason = reason;
@@ -524,6 +527,9 @@ __global__ void __launch_bounds__(kSplitNT) sor_split_kernel(const SorSplitArgs<
     int reason;
     done = sor_decide_with(st, it, a.eps, a.factor, a.kmax, a.idyn, om, reason, dlast,
                            [&](int q) { return wave_max(q == it - 1 ? sl1 : sl2); });
+    // No eps or stagnation exit inside the Gauss-Seidel opening: its small
+    // updates say little about the smooth error that remains.
+    if (done && reason != 2 && it - 1 <= a.gs) done = false;
     if (first && done && tl == 0) {
       st->done = 1;
       st->reason = reason;

@@ -24,7 +24,7 @@ from osinco3d_amd.solver import MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREV
 pytestmark = pytest.mark.gpu
 
 N, STEPS, SKIP = 256, 16, 3
-COARSE = (2, 8)
+COARSE = (3, 24)
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "osinco3d_amd", "warmstart_weights.npy")
 
