@@ -184,6 +184,101 @@ __global__ void __launch_bounds__(1024) mg_coarsest_kernel(float *__restrict__ u
   for (int q = threadIdx.x; q < n; q += blockDim.x) u[q] = su[q];
 }
 
+// The levels from `first` down, all of at most kMgCoarsestCells cells, as
+// one V-cycle by one block in LDS: f of the first level in, u of the first
+// level out (from u = 0).  Same smoother, residual, restriction and
+// prolongation as the per-level kernels.
+constexpr int kMgSmallLevels = 4;
+struct MgSmall {
+  int levels;
+  MgGrid g[kMgSmallLevels];
+  int off[kMgSmallLevels];  // offsets of each level in the LDS arrays
+  int nu1, nu2, coarse_sweeps;
+};
+
+__device__ __forceinline__ void lds_smooth(float *u, const float *f, const MgGrid &g,
+                                           int sweeps) {
+  const int n = g.mx * g.my * g.mz;
+  const float diag = -2.f * (g.cx + g.cy + g.cz);
+  for (int s = 0; s < sweeps; ++s)
+    for (int colour = 0; colour < 2; ++colour) {
+      for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
+        if (((i + j + k) & 1) == colour) u[q] = (f[q] - mg_lap_off(u, g, i, j, k)) / diag;
+      }
+      __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(1024) mg_small_vcycle_kernel(float *__restrict__ u_out,
+                                                               const float *__restrict__ f_in,
+                                                               MgSmall S) {
+  __shared__ float su[4096 + 512 + 64 + 8],
+      sf[4096 + 512 + 64 + 8], sr[4096 + 512 + 64 + 8];
+  const int n0 = S.g[0].mx * S.g[0].my * S.g[0].mz;
+  for (int q = threadIdx.x; q < n0; q += blockDim.x) {
+    sf[q] = f_in[q];
+    su[q] = 0.f;
+  }
+  __syncthreads();
+  // Down: smooth, residual, restrict.
+  for (int l = 0; l < S.levels - 1; ++l) {
+    const MgGrid &g = S.g[l];
+    float *u = su + S.off[l], *f = sf + S.off[l], *r = sr + S.off[l];
+    lds_smooth(u, f, g, S.nu1);
+    const int n = g.mx * g.my * g.mz;
+    const float diag = -2.f * (g.cx + g.cy + g.cz);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
+      r[q] = f[q] - (mg_lap_off(u, g, i, j, k) + diag * u[q]);
+    }
+    __syncthreads();
+    const MgGrid &c = S.g[l + 1];
+    float *uc = su + S.off[l + 1], *fc = sf + S.off[l + 1];
+    const int nc = c.mx * c.my * c.mz;
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+      const int I = q % c.mx, J = (q / c.mx) % c.my, K = q / (c.mx * c.my);
+      float t = 0.f;
+      for (int e = 0; e < 8; ++e) {
+        const int i = 2 * I + (e & 1), j = 2 * J + ((e >> 1) & 1), k = 2 * K + (e >> 2);
+        t += r[i + g.mx * (j + g.my * k)];
+      }
+      fc[q] = 0.125f * t;
+      uc[q] = 0.f;
+    }
+    __syncthreads();
+  }
+  // Coarsest.
+  {
+    const int l = S.levels - 1;
+    lds_smooth(su + S.off[l], sf + S.off[l], S.g[l], S.coarse_sweeps);
+  }
+  // Up: prolong, smooth.
+  for (int l = S.levels - 2; l >= 0; --l) {
+    const MgGrid &g = S.g[l];
+    const MgGrid &c = S.g[l + 1];
+    float *u = su + S.off[l];
+    const float *uc = su + S.off[l + 1];
+    const int n = g.mx * g.my * g.mz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
+      int x0, x1, y0, y1, z0, z1;
+      float wx, wy, wz;
+      mg_axis(i, c.mx, g.px, x0, x1, wx);
+      mg_axis(j, c.my, g.py, y0, y1, wy);
+      mg_axis(k, c.mz, g.pz, z0, z1, wz);
+      auto cv = [&](int a, int b, int d) { return uc[a + c.mx * (b + c.my * d)]; };
+      u[q] += (1.f - wz) * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z0) + wx * cv(x1, y0, z0)) +
+                            wy * ((1.f - wx) * cv(x0, y1, z0) + wx * cv(x1, y1, z0))) +
+              wz * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z1) + wx * cv(x1, y0, z1)) +
+                    wy * ((1.f - wx) * cv(x0, y1, z1) + wx * cv(x1, y1, z1)));
+    }
+    __syncthreads();
+    lds_smooth(u, sf + S.off[l], g, S.nu2);
+  }
+  for (int q = threadIdx.x; q < n0; q += blockDim.x) u_out[q] = su[q];
+}
+
 MgGrid grid_of(const MgHierarchy &H, int l) {
   MgGrid g;
   g.mx = H.m[l][0]; g.my = H.m[l][1]; g.mz = H.m[l][2];
@@ -202,9 +297,37 @@ hipError_t smooth(const MgHierarchy &H, int l, int sweeps, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The levels from l down fit one block's LDS (every level at most
+// kMgCoarsestCells cells, the sum at most 4096 + 512 + 64 + 8).
+bool small_from(const MgHierarchy &H, int l) {
+  if (H.levels - l > kMgSmallLevels) return false;
+  size_t tot = 0;
+  for (int q = l; q < H.levels; ++q) {
+    const size_t c = (size_t)H.m[q][0] * H.m[q][1] * H.m[q][2];
+    if (c > (size_t)kMgCoarsestCells) return false;
+    tot += c;
+  }
+  return tot <= 4096 + 512 + 64 + 8;
+}
+
 hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) {
   const MgGrid g = grid_of(H, l);
   const int n = g.mx * g.my * g.mz;
+  if (l > 0 && H.levels - l >= 2 && small_from(H, l)) {
+    MgSmall S;
+    S.levels = H.levels - l;
+    int off = 0;
+    for (int q = 0; q < S.levels; ++q) {
+      S.g[q] = grid_of(H, l + q);
+      S.off[q] = off;
+      off += H.m[l + q][0] * H.m[l + q][1] * H.m[l + q][2];
+    }
+    S.nu1 = nu1;
+    S.nu2 = nu2;
+    S.coarse_sweeps = 10;
+    mg_small_vcycle_kernel<<<1, 1024, 0, s>>>(H.u[l], H.f[l], S);
+    return hipGetLastError();
+  }
   if (l == H.levels - 1) {
     // At most 64 cells when the axes halve that far: one wave, 30 sweeps.
     if (n <= kMgCoarsestCells) {
