@@ -385,7 +385,7 @@ int predict_common(double *up[3], const double *u[3], double *f1[3], const doubl
   a.nut_part = nut_part;
   a.nx = nx; a.ny = ny; a.nz = nz;
   for (int q = 0; q < 3; ++q) a.bc[q] = bc[q];
-  a.tl = make_tiling(nx, ny, nz);
+  a.tl = predict_tiling<double>(nx, ny, nz, les);
   fill_coef<double>(a.d1, h, false);
   fill_coef<double>(a.d2, h, true);
   a.adu = adu; a.bdu = bdu; a.cdu = cdu;
@@ -899,7 +899,7 @@ struct Solver {
   o3d_params p;
   int n = 0;
   bool mixed = false;
-  Tiling tl;      // predict
+  Tiling tl;      // predict (its own occupancy)
   Tiling tl_div;  // divergence (its own occupancy)
   Tiling tl_cor;  // correction
   int kind = 0;
@@ -1684,7 +1684,8 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
   S->p = *p;
   S->n = p->nx * p->ny * p->nz;
   S->mixed = p->precision == O3D_MIXED;
-  S->tl = make_tiling(p->nx, p->ny, p->nz);
+  S->tl = S->mixed ? predict_tiling<float>(p->nx, p->ny, p->nz, p->iles == 1)
+                   : predict_tiling<double>(p->nx, p->ny, p->nz, p->iles == 1);
   S->tl_div = S->mixed ? div_tiling<float>(p->nx, p->ny, p->nz)
                        : div_tiling<double>(p->nx, p->ny, p->nz);
   S->tl_cor = S->mixed ? correct_tiling<float, double>(p->nx, p->ny, p->nz)

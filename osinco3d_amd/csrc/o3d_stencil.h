@@ -32,6 +32,8 @@ Tiling make_tiling_slots(int nx, int ny, int nz, int slots);
 // their occupancy on the current device).
 template <typename T>
 Tiling div_tiling(int nx, int ny, int nz);
+template <typename T>
+Tiling predict_tiling(int nx, int ny, int nz, bool les);
 template <typename T, typename TP>
 Tiling correct_tiling(int nx, int ny, int nz);
 

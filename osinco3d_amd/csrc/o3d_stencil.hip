@@ -267,46 +267,59 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
   FieldPartial nut_p;
   partial_init(nut_p);
 
-  // Column queue of planes k-3..k+3 (slots 0..6) plus slot 7, the plane
-  // k+4 whose load is issued one step before it is needed.
-  T q[3][8];
+  // Column queue of planes k-3..k+3 per field (signs applied), and raw
+  // loads in flight in two-slot rings whose slots are compile-time
+  // constants of the unrolled pair of steps: plane k+5 of the queue is
+  // loaded at step k and joins it at the end of step k+1; the x/y halo
+  // values of plane k+2 are loaded at step k (every lane loads, lanes
+  // without a halo cell their own column) and staged, with their signs, at
+  // step k+2.  No register that a load is still writing is read early.
+  auto zidx = [&](int qz, bool &fz) { return col + plane * ghost_index(qz, nz, A.bc[2], fz); };
+  auto sgn = [](T v, bool neg) { return neg ? -v : v; };
+  T q[3][7];
 #pragma unroll
   for (int s = 0; s < 7; ++s) {
     bool fz;
-    const int kk = ghost_index(P.z0 - 3 + s, nz, A.bc[2], fz);
-    q[0][s] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
-    q[1][s] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
-    q[2][s] = load_ghost(A.u[2], col + plane * kk, fz);
+    const int ix = zidx(P.z0 - 3 + s, fz);
+    q[0][s] = load_ghost(A.u[0], ix, own_neg[0]);
+    q[1][s] = load_ghost(A.u[1], ix, own_neg[1]);
+    q[2][s] = load_ghost(A.u[2], ix, fz);
   }
+  T zr[3][2];
+  {
+    bool fz;
+    const int ix = zidx(P.z0 + 4, fz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) zr[c][1] = A.u[c][ix];
+  }
+  const int hxi = hx_col >= 0 ? hx_idx : col, hyi = hy_row >= 0 ? hy_idx : col;
+  // Signs of the halo values: x halo u0 flips with the x mirror, u1 with
+  // this column's y mirror; y halo u0 with this column's x mirror, u1 with
+  // the y mirror; u2 never (z parity only).
+  const bool hneg_x[3] = {hx_col >= 0 && hx_f, own_neg[1], false};
+  const bool hneg_y[3] = {own_neg[0], hy_row >= 0 && hy_f, false};
+  T hxr[3][2], hyr[3][2];
+  auto load_halo = [&](int kp, int sl) {
+    const int pk = plane * min(kp, nz - 1);  // planes after the last are not used
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      hxr[c][sl] = A.u[c][hxi + pk];
+      hyr[c][sl] = A.u[c][hyi + pk];
+    }
+  };
+  load_halo(P.z0, 0);
+  load_halo(P.z0 + 1, 1);
   const T *c1x = A.d1[0], *c1y = A.d1[1], *c1z = A.d1[2];
   const T *c2x = A.d2[0], *c2y = A.d2[1], *c2z = A.d2[2];
 
-  for (int k = P.z0; k < P.z1; ++k) {
-    {
-      // k + 4 <= nz + 3 stays a valid mirror / wrap index for nz >= 6.
-      bool fz;
-      const int kk = ghost_index(k + 4, nz, A.bc[2], fz);
-      q[0][7] = load_ghost(A.u[0], col + plane * kk, own_neg[0]);
-      q[1][7] = load_ghost(A.u[1], col + plane * kk, own_neg[1]);
-      q[2][7] = load_ghost(A.u[2], col + plane * kk, fz);
-    }
+  auto step = [&](int k, auto SL) {
+    constexpr int sl = decltype(SL)::value;
     const int pk = plane * k;
-    T hx[3], hy[3];
-    if (hx_col >= 0) {
-      hx[0] = load_ghost(A.u[0], hx_idx + pk, hx_f);
-      hx[1] = load_ghost(A.u[1], hx_idx + pk, own_neg[1]);
-      hx[2] = A.u[2][hx_idx + pk];
-    }
-    if (hy_row >= 0) {
-      hy[0] = load_ghost(A.u[0], hy_idx + pk, own_neg[0]);
-      hy[1] = load_ghost(A.u[1], hy_idx + pk, hy_f);
-      hy[2] = A.u[2][hy_idx + pk];
-    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       pl[c][ty + 3][tx + 3] = q[c][3];
-      if (hx_col >= 0) pl[c][ty + 3][hx_col] = hx[c];
-      if (hy_row >= 0) pl[c][hy_row][tx + 3] = hy[c];
+      if (hx_col >= 0) pl[c][ty + 3][hx_col] = sgn(hxr[c][sl], hneg_x[c]);
+      if (hy_row >= 0) pl[c][hy_row][tx + 3] = sgn(hyr[c][sl], hneg_y[c]);
     }
     __syncthreads();
     if (P.valid) {
@@ -362,14 +375,42 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
         A.up[c][idx] = un;
       }
     }
+    {
+      // k + 5 <= nz + 4 stays a valid mirror / wrap index for nz >= 6.
+      bool fz;
+      const int ix = zidx(k + 5, fz);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) zr[c][sl] = A.u[c][ix];
+    }
+    load_halo(k + 2, sl);
     __syncthreads();
+    bool fz4;
+    (void)zidx(k + 4, fz4);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
-      for (int s = 0; s < 7; ++s) q[c][s] = q[c][s + 1];
+      for (int s = 0; s < 6; ++s) q[c][s] = q[c][s + 1];
     }
+    q[0][6] = sgn(zr[0][1 - sl], own_neg[0]);
+    q[1][6] = sgn(zr[1][1 - sl], own_neg[1]);
+    q[2][6] = sgn(zr[2][1 - sl], fz4);
+  };
+  // Whole pairs in the loop (every back edge leaves slot 1's loads last in
+  // flight), the odd last plane after it.
+  int k = P.z0;
+  for (; k + 1 < P.z1; k += 2) {
+    step(k, std::integral_constant<int, 0>{});
+    step(k + 1, std::integral_constant<int, 1>{});
   }
+  if (k < P.z1) step(k, std::integral_constant<int, 0>{});
   if (LES) partial_store(nut_p, A.nut_part + blockIdx.x, sc);
+}
+
+template <typename T>
+Tiling predict_tiling(int nx, int ny, int nz, bool les) {
+  static const int slots[2] = {stencil_slots((const void *)predict_kernel<T, 3, false>),
+                               stencil_slots((const void *)predict_kernel<T, 3, true>)};
+  return make_tiling_slots(nx, ny, nz, slots[les ? 1 : 0]);
 }
 
 template <typename T>
@@ -1070,6 +1111,8 @@ template hipError_t launch_der<float>(int, float *, const float *, int, int, int
                                       hipStream_t);
 template hipError_t launch_predict<double>(const PredictArgs<double> &, int, bool, hipStream_t);
 template hipError_t launch_predict<float>(const PredictArgs<float> &, int, bool, hipStream_t);
+template Tiling predict_tiling<double>(int, int, int, bool);
+template Tiling predict_tiling<float>(int, int, int, bool);
 template Tiling div_tiling<double>(int, int, int);
 template Tiling div_tiling<float>(int, int, int);
 template Tiling correct_tiling<double, double>(int, int, int);
