@@ -1294,10 +1294,6 @@ __global__ void mg_prolong_fine_kernel(double *__restrict__ q, const float *__re
   q[o] = q[o] + (double)t;
 }
 
-__global__ void lw_guess_kernel(const double *p, const float *o, double *q, int n, double s) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) q[i] = p[i] + s * (double)o[i];
-}
 __global__ void lw_guess2_kernel(const double *p, const float *e1, const float *e2, double *q,
                                  int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1448,23 +1444,28 @@ int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *o
 int solver_learning_prepare(Solver &S) {
   CHK(lw_alloc(S));
   const int n = S.n;
-  const double *base = S.P[S.cur];
-  if (S.hist >= 1) {
-    lw_guess_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.P[1 - S.cur],
-                                                           n, 1.0);
-    HIPCHK(hipGetLastError());
-    base = S.P[1 - S.cur];
-  }
   int per[3];
   solver_periodic(S.kind, per);
-  // The residual kernel also zeroes its second output; Out is scratch here.
-  if (S.mixed)
-    HIPCHK(launch_residual_f32<float>(base, (const float *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
-                                      S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
-  else
-    HIPCHK(launch_residual_f32<double>(base, (const double *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
-                                       S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+  if (S.hist >= 1) {
+    // The extrapolation into the spare buffer and its residual, one pass.
+    if (S.mixed)
+      HIPCHK(launch_guess_residual<float>(S.P[S.cur], S.Eprev, (const float *)S.rhs,
+                                          S.P[1 - S.cur], S.Rl, S.p.nx, S.p.ny, S.p.nz, per,
+                                          S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
+    else
+      HIPCHK(launch_guess_residual<double>(S.P[S.cur], S.Eprev, (const double *)S.rhs,
+                                           S.P[1 - S.cur], S.Rl, S.p.nx, S.p.ny, S.p.nz, per,
+                                           S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
+  } else if (S.mixed) {
+    // The residual kernel also zeroes its second output; Out is scratch here.
+    HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.Rl, S.Out, S.p.nx,
+                                      S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
+                                      g_stream));
+  } else {
+    HIPCHK(launch_residual_f32<double>(S.P[S.cur], (const double *)S.rhs, S.Rl, S.Out, S.p.nx,
+                                       S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
                                        g_stream));
+  }
   if (S.cap_r)
     HIPCHK(hipMemcpyAsync(S.cap_r, S.Rl, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));
   if (S.cap_eprev)

@@ -114,6 +114,12 @@ hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *
                                int ny, int nz, const int per[3], double ox, double oy,
                                double oz, double A, hipStream_t s);
 hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s);
+// q = p + e and r = rhs - L q (FP32) in one pass (the extrapolated guess and
+// its residual), equal to forming q and then calling launch_residual_f32.
+template <typename TR>
+hipError_t launch_guess_residual(const double *p, const float *e, const TR *rhs, double *q,
+                                 float *r, int nx, int ny, int nz, const int per[3], double ox,
+                                 double oy, double oz, double A, hipStream_t s);
 // The same with r and e colour-split (red half-array, then black).
 template <typename TR>
 hipError_t launch_residual_split(const double *p, const TR *rhs, float *r, float *e, int nx,

@@ -1008,12 +1008,12 @@ int residual_blocks(int nx, int ny, int nz) {
 }
 
 // f(offset of (i, j, k), i, j, k, residual) for the planes of this
-// thread's column.
-template <typename TR, typename F>
-__device__ __forceinline__ void residual_column(const double *__restrict__ p,
-                                                const TR *__restrict__ rhs, int nx, int ny,
-                                                int nz, int px, int py, int pz, double ox,
-                                                double oy, double oz, double A, F &&f) {
+// thread's column; the iterate's value at offset o is ld(o).
+template <typename TR, typename LD, typename F>
+__device__ __forceinline__ void residual_column_ld(LD &&ld, const TR *__restrict__ rhs, int nx,
+                                                   int ny, int nz, int px, int py, int pz,
+                                                   double ox, double oy, double oz, double A,
+                                                   F &&f) {
   const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
   const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
   if (i >= nx || j >= ny) return;
@@ -1023,21 +1023,61 @@ __device__ __forceinline__ void residual_column(const double *__restrict__ p,
   const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
   const int c = i + nx * j;
   const int cim = im + nx * j, cip = ip + nx * j, cjm = i + nx * jm, cjp = i + nx * jp;
-  double pm = p[c + plane * sor_ghost(k0 - 1, nz, pz)];
-  double pc = p[c + plane * k0];
+  double pm = ld(c + plane * sor_ghost(k0 - 1, nz, pz));
+  double pc = ld(c + plane * k0);
 #pragma unroll
   for (int kk = 0; kk < RZC; ++kk) {
     const int k = k0 + kk;
     if (k < nz) {
-      const double pp = p[c + plane * sor_ghost(k + 1, nz, pz)];
+      const double pp = ld(c + plane * sor_ghost(k + 1, nz, pz));
       const size_t o = plane * k;
-      const double lap = ox * (p[cim + o] + p[cip + o]) + oy * (p[cjm + o] + p[cjp + o]) +
+      const double lap = ox * (ld(cim + o) + ld(cip + o)) + oy * (ld(cjm + o) + ld(cjp + o)) +
                          oz * (pm + pp) + A * pc;
-      f(c + o, i, j, k, (double)rhs[c + o] - lap);
+      f(c + o, i, j, k, pc, (double)rhs[c + o] - lap);
       pm = pc;
       pc = pp;
     }
   }
+}
+
+template <typename TR, typename F>
+__device__ __forceinline__ void residual_column(const double *__restrict__ p,
+                                                const TR *__restrict__ rhs, int nx, int ny,
+                                                int nz, int px, int py, int pz, double ox,
+                                                double oy, double oz, double A, F &&f) {
+  residual_column_ld([&](size_t o) { return p[o]; }, rhs, nx, ny, nz, px, py, pz, ox, oy, oz,
+                     A, [&](size_t o, int i, int j, int k, double, double r) {
+                       f(o, i, j, k, r);
+                     });
+}
+
+// q = p + e (the extrapolated guess) and r = rhs - L q stored in FP32, in one
+// pass: the residual is formed from p + e as loaded, with the operation
+// order of residual_f32_kernel, so both equal the two-kernel result.
+template <typename TR>
+__global__ void __launch_bounds__(RNT) guess_residual_kernel(const double *__restrict__ p,
+                                                             const float *__restrict__ e,
+                                                             const TR *__restrict__ rhs,
+                                                             double *__restrict__ q,
+                                                             float *__restrict__ r, int nx,
+                                                             int ny, int nz, int px, int py,
+                                                             int pz, double ox, double oy,
+                                                             double oz, double A) {
+  residual_column_ld([&](size_t o) { return p[o] + 1.0 * (double)e[o]; }, rhs, nx, ny, nz, px,
+                     py, pz, ox, oy, oz, A,
+                     [&](size_t o, int, int, int, double qc, double res) {
+                       q[o] = qc;
+                       r[o] = (float)res;
+                     });
+}
+
+template <typename TR>
+hipError_t launch_guess_residual(const double *p, const float *e, const TR *rhs, double *q,
+                                 float *r, int nx, int ny, int nz, const int per[3], double ox,
+                                 double oy, double oz, double A, hipStream_t s) {
+  guess_residual_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
+      p, e, rhs, q, r, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
+  return hipGetLastError();
 }
 
 template <typename TR>
@@ -1252,6 +1292,12 @@ template hipError_t launch_residual_f32<double>(const double *, const double *, 
 template hipError_t launch_residual_f32<float>(const double *, const float *, float *, float *,
                                                int, int, int, const int *, double, double, double,
                                                double, hipStream_t);
+template hipError_t launch_guess_residual<float>(const double *, const float *, const float *,
+                                                 double *, float *, int, int, int, const int *,
+                                                 double, double, double, double, hipStream_t);
+template hipError_t launch_guess_residual<double>(const double *, const float *, const double *,
+                                                  double *, float *, int, int, int, const int *,
+                                                  double, double, double, double, hipStream_t);
 template hipError_t launch_residual_split<float>(const double *, const float *, float *, float *,
                                                  int, int, int, const int *, double, double,
                                                  double, double, hipStream_t);
