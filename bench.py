@@ -13,7 +13,7 @@ Each requested form is timed over the same steps:
   extrap mixed + warm start by linear extrapolation p_prev + (p_prev - p_prev2)
   extrap2 mixed + quadratic extrapolation p_prev + 2 e_prev - e_prev2
   mg     extrap + the coarse-grid correction of the guess (multigrid V-cycles
-         on the 4^3 block averages of its residual) and Gauss-Seidel sweeps
+         on the 2^3 block averages of its residual) and Gauss-Seidel sweeps
          opening the FP32 correction solve
   nn0    mixed + learned warm start (the trained network of
          osinco3d_amd/warmstart_weights.npy; skipped when it is missing)
@@ -66,9 +66,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--forms", default="fp64,mixed,extrap,mg,nn0,nn")
-    ap.add_argument("--coarse-cycles", type=int, default=3,
+    ap.add_argument("--coarse-cycles", type=int, default=2,
                     help="V-cycles of the coarse-grid correction (forms mg, nn)")
-    ap.add_argument("--smooth-sweeps", type=int, default=24,
+    ap.add_argument("--smooth-sweeps", type=int, default=8,
                     help="Gauss-Seidel sweeps opening the FP32 solve (forms mg, nn)")
     ap.add_argument("--workload", choices=("tgv", "mixing"), default="tgv",
                     help="tgv: the headline Taylor-Green case; mixing: the 384x256x128 LES "

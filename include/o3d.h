@@ -191,8 +191,8 @@ typedef struct o3d_params {
   int warm_start;       /* O3D_WARM_PREVIOUS, _LEARNED, _EXTRAPOLATE or _EXTRAPOLATE2 */
   int inner_max;        /* O3D_MIXED: FP32 sweeps between FP64 refinements (0 = default) */
   int coarse_cycles;    /* O3D_MIXED with a warm start: multigrid V-cycles of the coarse-grid
-                           correction added to the guess (0 = none; every axis a multiple of
-                           O3D_WARM_COARSEN) */
+                           correction added to the guess, on 2^3 block averages (0 = none;
+                           every axis even) */
   int smooth_sweeps;    /* O3D_MIXED: Gauss-Seidel sweeps (omega = 1) that open the first FP32
                            correction solve of each step (0 = none) */
 } o3d_params;

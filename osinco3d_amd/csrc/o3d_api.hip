@@ -1323,6 +1323,14 @@ __global__ void __launch_bounds__(256) lw_sumsq_kernel(const float *a, int n, do
 
 constexpr int kLwBlocks = 512;
 
+// The coarse-grid correction works on 2^3 block averages (its level 1 is
+// the network's grid of O3D_WARM_COARSEN^3 blocks).
+constexpr int kMgCoarsen = 2;
+
+bool mg_level1_is(const MgHierarchy &H, int mx, int my, int mz) {
+  return H.levels >= 2 && H.m[1][0] == mx && H.m[1][1] == my && H.m[1][2] == mz;
+}
+
 // Partial-sum buffers of the warm-start reductions.
 struct LwSums {
   double *part = nullptr;   // device, 2 * kLwBlocks
@@ -1386,7 +1394,7 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.Feat, 2 * nc));
   CHK(S.alloc(&S.Outc, nc));
   if (S.p.coarse_cycles > 0) {
-    const int F = O3D_WARM_COARSEN;
+    const int F = kMgCoarsen;
     const int m0[3] = {S.p.nx / F, S.p.ny / F, S.p.nz / F};
     const double h[3] = {S.p.dx * F, S.p.dy * F, S.p.dz * F};
     int per[3];
@@ -1505,13 +1513,14 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     const int nc = n / (F * F * F);
     int per[3];
     solver_periodic(S.kind, per);
-    if (S.p.coarse_cycles > 0) {
-      // The network's coarse correction starts the multigrid solve of the
-      // coarse correction below; q keeps the extrapolation.
-      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.mg.u[0], S.Outc,
+    if (S.p.coarse_cycles > 0 && mg_level1_is(S.mg, S.p.nx / F, S.p.ny / F, S.p.nz / F)) {
+      // The network's coarse correction (on the multigrid's level 1, the
+      // grid of F^3 blocks) starts the multigrid solve of the coarse-grid
+      // correction below; q keeps the extrapolation.
+      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.mg.u[1], S.Outc,
                                                                     S.Feat + nc, nc, (float)re);
       nn_coarse = true;
-    } else {
+    } else if (S.p.coarse_cycles == 0) {
       dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
       lw_guess_coarse_kernel<<<grid, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Outc,
                                                          S.Feat + nc, q, S.p.nx, S.p.ny,
@@ -1527,7 +1536,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     // the coarse Poisson problem solved by multigrid V-cycles, its trilinear
     // interpolation added to q.  S.Rl holds the residual of the plain
     // extrapolation; other guesses need their own.
-    const int F = O3D_WARM_COARSEN;
+    const int F = kMgCoarsen;
     int per[3];
     solver_periodic(S.kind, per);
     const bool own = (S.nn && !nn_coarse) ||
@@ -1540,6 +1549,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     lw_pool_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.mg.f[0], S.p.nx, S.p.ny,
                                                            S.p.nz, F, 1.0f);
     HIPCHK(hipGetLastError());
+    if (nn_coarse) HIPCHK(mg_seed_from_level1(S.mg, g_stream));
     HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 2, 2, g_stream, !nn_coarse));
     dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
     mg_prolong_fine_kernel<<<grid, 256, 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny, S.p.nz, F,
@@ -1677,9 +1687,8 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     return fail("coarse_cycles and smooth_sweeps must be >= 0");
   if ((p->coarse_cycles > 0 || p->smooth_sweeps > 0) && p->precision != O3D_MIXED)
     return fail("coarse_cycles and smooth_sweeps apply to the mixed form only");
-  if (p->coarse_cycles > 0 &&
-      (p->nx % O3D_WARM_COARSEN || p->ny % O3D_WARM_COARSEN || p->nz % O3D_WARM_COARSEN))
-    return fail("the coarse-grid correction needs every axis a multiple of %d", O3D_WARM_COARSEN);
+  if (p->coarse_cycles > 0 && (p->nx % 2 || p->ny % 2 || p->nz % 2))
+    return fail("the coarse-grid correction needs every axis even");
   CHK(check_red_black(kind, p->nx, p->ny, p->nz));
   Solver *S = new Solver();
   S->p = *p;

@@ -33,5 +33,8 @@ void mg_bind(MgHierarchy &H, float *base);
 // (from_zero = false).
 hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s,
                     bool from_zero = true);
+// u0 = the trilinear interpolation of the level-1 field H.u[1] (a starting
+// iterate given on the next coarser level).  Needs two levels at least.
+hipError_t mg_seed_from_level1(const MgHierarchy &H, hipStream_t s);
 
 }  // namespace o3d

@@ -2,13 +2,13 @@
 //
 // No counterpart in the reference (its pressure solve is SOR from the
 // previous pressure).  In the mixed forms the residual of the warm-start
-// guess is averaged over 4^3 blocks, this module solves the 7-point Poisson
-// problem of those averages (spacing 4h) approximately, and the solver adds
+// guess is averaged over 2^3 blocks, this module solves the 7-point Poisson
+// problem of those averages (spacing 2h) approximately, and the solver adds
 // the trilinear interpolation of the result to the guess.  The SOR solve
 // that follows is still the judge of convergence (same stop rule), so the
-// correction only changes where SOR starts.  The grids are small (64^3 and
+// correction only changes where SOR starts.  The grids are small (128^3 and
 // below at 256^3), so the kernels are plain one-cell-per-thread kernels;
-// their cost is the launch count.
+// their cost is mostly the launch count.
 #include <algorithm>
 
 #include "o3d_mg.h"
@@ -357,6 +357,16 @@ __global__ void mg_zero_kernel(float *u, int n) {
 }
 
 }  // namespace
+
+hipError_t mg_seed_from_level1(const MgHierarchy &H, hipStream_t s) {
+  if (H.levels < 2) return hipErrorInvalidValue;
+  const MgGrid g = grid_of(H, 0);
+  const int n0 = g.mx * g.my * g.mz;
+  mg_zero_kernel<<<(n0 + 255) / 256, 256, 0, s>>>(H.u[0], n0);
+  mg_prolong_kernel<<<blocks_of(g), 256, 0, s>>>(H.u[0], H.u[1], g.mx, g.my, g.mz, g.px, g.py,
+                                                 g.pz);
+  return hipGetLastError();
+}
 
 hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s,
                     bool from_zero) {

@@ -29,7 +29,7 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 
 
 # The coarse-grid correction and smoothing of the bench's mg and nn forms.
-COARSE = (3, 24)
+COARSE = (2, 8)
 
 
 def series(precision, warm=WARM_PREVIOUS, net=None, coarse=(0, 0)):

@@ -6,7 +6,7 @@ The method without them starts each solve from the previous step's pressure
 extrapolation corrected by the trained network (the previous pressure stays
 the fallback); the plain extrapolation is measured beside it so the
 network's own share of the gain is visible.  The coarse-grid correction
-(multigrid on the 4^3 block averages of the guess's residual) with
+(multigrid on the 2^3 block averages of the guess's residual) with
 Gauss-Seidel sweeps opening the FP32 solve is measured on its own and
 after the network (the headline form).  The solve's stop rule is the same
 in every form, so the fields agree to a tolerance tied to eps.
@@ -24,7 +24,7 @@ from osinco3d_amd.solver import MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREV
 pytestmark = pytest.mark.gpu
 
 N, STEPS, SKIP = 256, 16, 3
-COARSE = (3, 24)
+COARSE = (2, 8)
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "osinco3d_amd", "warmstart_weights.npy")
 
