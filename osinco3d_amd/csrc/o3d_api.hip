@@ -28,6 +28,13 @@ using namespace o3d;
 namespace {
 
 hipStream_t g_stream = nullptr;
+
+// The coarse-grid correction works on 2^3 block averages (its level 1 is
+// the network's grid of O3D_WARM_COARSEN^3 blocks).
+constexpr int kMgCoarsen = 2;
+// SOR sweeps after the Gauss-Seidel opening before a correction solve that
+// starts from a coarse-grid correction ends (O3D_MIXED, inner_max = 0).
+constexpr int kMgInnerSor = 8;
 std::string g_err;
 bool g_init = false;
 
@@ -1110,7 +1117,13 @@ int solver_poisson_f64(Solver &S, o3d_step_info *info) {
 int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   const o3d_params &p = S.p;
   const int n = S.n;
-  const int inner_max = p.inner_max > 0 ? p.inner_max : 400;
+  // With the coarse-grid correction, every correction solve is one
+  // two-grid cycle: the coarse correction of its residual, the Gauss-Seidel
+  // opening and a few SOR sweeps; a solve that has not converged by then
+  // ends, and the next refinement cycle starts from the new residual.
+  const bool mgc = S.mgbuf != nullptr && S.dslot != nullptr;
+  const int inner_max = p.inner_max > 0 ? p.inner_max
+                                        : (mgc ? p.smooth_sweeps + kMgInnerSor : 400);
   int total = 0, sw32 = 0, sw64 = 0, cycles = 0;
   int reason = 2;
   double judge_prev = 1609.0, dlast = 0.0;
@@ -1124,10 +1137,18 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     const bool split = S.dslot != nullptr;
     if (split && cycles == 0 && S.r_ready) {
       // r and e = 0 already written by the warm-start acceptance check.
-    } else if (split)
+    } else if (split) {
       HIPCHK(launch_residual_split<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx,
                                           p.ny, p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
                                           g_stream));
+      if (mgc) {
+        // The start of this correction solve: the coarse-grid correction of
+        // its residual (cycle 0 starts from a guess that already had one).
+        HIPCHK(mg_pool_split(S.R, S.mg.f[0], p.nx, p.ny, p.nz, g_stream));
+        HIPCHK(mg_solve(S.mg, p.coarse_cycles, 2, 2, g_stream));
+        HIPCHK(mg_prolong_split(S.E[0], S.mg.u[0], p.nx, p.ny, p.nz, per, g_stream));
+      }
+    }
     else
       HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.R, S.E[0], p.nx,
                                         p.ny, p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
@@ -1141,8 +1162,9 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     const int kin = std::min(inner_max, p.kmax - total);
     sor_args_common<float, float>(S, r.a, S.R, S.st, kin);
     // The first correction solve of the step opens with Gauss-Seidel sweeps
-    // (smoothing the high-frequency error a warm start leaves).
-    if (cycles == 0) r.a.gs = p.smooth_sweeps;
+    // (smoothing the high-frequency error a warm start leaves), and so does
+    // every one that starts from a coarse-grid correction.
+    if (cycles == 0 || mgc) r.a.gs = p.smooth_sweeps;
     r.pair = false;
     if (split) {
       const int h = n / 2;
@@ -1323,9 +1345,6 @@ __global__ void __launch_bounds__(256) lw_sumsq_kernel(const float *a, int n, do
 
 constexpr int kLwBlocks = 512;
 
-// The coarse-grid correction works on 2^3 block averages (its level 1 is
-// the network's grid of O3D_WARM_COARSEN^3 blocks).
-constexpr int kMgCoarsen = 2;
 
 bool mg_level1_is(const MgHierarchy &H, int mx, int my, int mz) {
   return H.levels >= 2 && H.m[1][0] == mx && H.m[1][1] == my && H.m[1][2] == mz;

@@ -33,6 +33,14 @@ void mg_bind(MgHierarchy &H, float *base);
 // (from_zero = false).
 hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s,
                     bool from_zero = true);
+// The coarse-grid correction of a correction solve held in colour-split
+// storage (per row, the red nodes then the black ones, nx/2 entries each;
+// the red half-array first): f0 = 2^3 block averages of the split residual
+// r, and e = the trilinear interpolation of u0, written split.  The level-0
+// grid is (nx/2, ny/2, nz/2).
+hipError_t mg_pool_split(const float *r, float *f0, int nx, int ny, int nz, hipStream_t s);
+hipError_t mg_prolong_split(float *e, const float *u0, int nx, int ny, int nz, const int per[3],
+                            hipStream_t s);
 // u0 = the trilinear interpolation of the level-1 field H.u[1] (a starting
 // iterate given on the next coarser level).  Needs two levels at least.
 hipError_t mg_seed_from_level1(const MgHierarchy &H, hipStream_t s);

@@ -356,7 +356,71 @@ __global__ void mg_zero_kernel(float *u, int n) {
   if (idx < n) u[idx] = 0.f;
 }
 
+__global__ void __launch_bounds__(256) mg_pool_split_kernel(const float *__restrict__ r,
+                                                            float *__restrict__ f0, int nx,
+                                                            int ny, int nz) {
+  const int mx = nx / 2, my = ny / 2, mz = nz / 2;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= mx * my * mz) return;
+  const int I = idx % mx, J = (idx / mx) % my, K = idx / (mx * my);
+  const size_t hx = (size_t)(nx / 2), half = (size_t)nx * ny * nz / 2;
+  float t = 0.f;
+  // Cells 2I and 2I+1 of a row are one red and one black node, both entry I
+  // of their colour's row.
+  for (int q = 0; q < 4; ++q) {
+    const size_t row = (size_t)(2 * J + (q & 1)) + (size_t)ny * (2 * K + (q >> 1));
+    t += r[hx * row + I] + r[half + hx * row + I];
+  }
+  f0[idx] = 0.125f * t;
+}
+
+__global__ void __launch_bounds__(256) mg_prolong_split_kernel(float *__restrict__ e,
+                                                               const float *__restrict__ u0,
+                                                               int nx, int ny, int nz, int px,
+                                                               int py, int pz) {
+  const size_t half = (size_t)nx * ny * nz / 2;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 2 * half) return;
+  const int black = idx >= half ? 1 : 0;
+  const size_t o = idx - (black ? half : 0);
+  const int hx = nx / 2;
+  const int c = (int)(o % hx);
+  const size_t row = o / hx;
+  const int j = (int)(row % ny), k = (int)(row / ny);
+  const int q = (j + k) & 1;
+  const int i = 2 * c + (black ? 1 - q : q);
+  const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+  int x0, x1, y0, y1, z0, z1;
+  float wx, wy, wz;
+  mg_axis(i, cx, px, x0, x1, wx);
+  mg_axis(j, cy, py, y0, y1, wy);
+  mg_axis(k, cz, pz, z0, z1, wz);
+  auto cv = [&](int a, int b, int d) {
+    return u0[(size_t)a + (size_t)cx * (b + (size_t)cy * d)];
+  };
+  e[idx] = (1.f - wz) * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z0) + wx * cv(x1, y0, z0)) +
+                         wy * ((1.f - wx) * cv(x0, y1, z0) + wx * cv(x1, y1, z0))) +
+           wz * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z1) + wx * cv(x1, y0, z1)) +
+                 wy * ((1.f - wx) * cv(x0, y1, z1) + wx * cv(x1, y1, z1)));
+}
+
 }  // namespace
+
+hipError_t mg_pool_split(const float *r, float *f0, int nx, int ny, int nz, hipStream_t s) {
+  if (nx % 2 || ny % 2 || nz % 2) return hipErrorInvalidValue;
+  const int nc = (nx / 2) * (ny / 2) * (nz / 2);
+  mg_pool_split_kernel<<<(nc + 255) / 256, 256, 0, s>>>(r, f0, nx, ny, nz);
+  return hipGetLastError();
+}
+
+hipError_t mg_prolong_split(float *e, const float *u0, int nx, int ny, int nz, const int per[3],
+                            hipStream_t s) {
+  if (nx % 2 || ny % 2 || nz % 2) return hipErrorInvalidValue;
+  const size_t n = (size_t)nx * ny * nz;
+  mg_prolong_split_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(e, u0, nx, ny, nz, per[0],
+                                                                       per[1], per[2]);
+  return hipGetLastError();
+}
 
 hipError_t mg_seed_from_level1(const MgHierarchy &H, hipStream_t s) {
   if (H.levels < 2) return hipErrorInvalidValue;
