@@ -7,8 +7,9 @@
 // the trilinear interpolation of the result to the guess.  The SOR solve
 // that follows is still the judge of convergence (same stop rule), so the
 // correction only changes where SOR starts.  The grids are small (128^3 and
-// below at 256^3), so the kernels are plain one-cell-per-thread kernels;
-// their cost is mostly the launch count.
+// below at 256^3): the per-level kernels give each thread one cell of the
+// colour being relaxed (or a pair of cells), and the levels of at most 16^3
+// cells run as one V-cycle in one block's LDS; the launch count dominates.
 #include <algorithm>
 
 #include "o3d_mg.h"
@@ -63,59 +64,43 @@ __device__ __forceinline__ int mg_nb(int q, int m, int per) {
 struct MgGrid {
   int mx, my, mz, px, py, pz;
   float cx, cy, cz;
+  float diag, invd;  // -2 (cx + cy + cz) and its reciprocal
 };
 
+// Weighted sum of the 6 neighbours (32-bit offsets: mg_plan keeps every
+// level below 2^31 cells).
 __device__ __forceinline__ float mg_lap_off(const float *u, const MgGrid &g, int i, int j, int k) {
-  const size_t pl = (size_t)g.mx * g.my;
-  const size_t row = (size_t)g.mx * j + pl * k;
-  const float sx = u[row + mg_nb(i - 1, g.mx, g.px)] + u[row + mg_nb(i + 1, g.mx, g.px)];
-  const float sy = u[(size_t)g.mx * mg_nb(j - 1, g.my, g.py) + pl * k + i] +
-                   u[(size_t)g.mx * mg_nb(j + 1, g.my, g.py) + pl * k + i];
-  const float sz = u[row - pl * k + pl * mg_nb(k - 1, g.mz, g.pz) + i] +
-                   u[row - pl * k + pl * mg_nb(k + 1, g.mz, g.pz) + i];
+  const int pl = g.mx * g.my;
+  const int jr = g.mx * j, kp = pl * k;
+  const float sx = u[kp + jr + mg_nb(i - 1, g.mx, g.px)] + u[kp + jr + mg_nb(i + 1, g.mx, g.px)];
+  const float sy = u[kp + g.mx * mg_nb(j - 1, g.my, g.py) + i] +
+                   u[kp + g.mx * mg_nb(j + 1, g.my, g.py) + i];
+  const float sz = u[pl * mg_nb(k - 1, g.mz, g.pz) + jr + i] +
+                   u[pl * mg_nb(k + 1, g.mz, g.pz) + jr + i];
   return g.cx * sx + g.cy * sy + g.cz * sz;
 }
 
-// One colour of a red-black Gauss-Seidel sweep.
-__global__ void __launch_bounds__(256) mg_smooth_kernel(float *__restrict__ u,
-                                                        const float *__restrict__ f, MgGrid g,
-                                                        int colour) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = g.mx * g.my * g.mz;
-  if (idx >= n) return;
-  const int i = idx % g.mx, j = (idx / g.mx) % g.my, k = idx / (g.mx * g.my);
-  if (((i + j + k) & 1) != colour) return;
-  const float diag = -2.f * (g.cx + g.cy + g.cz);
-  u[idx] = (f[idx] - mg_lap_off(u, g, i, j, k)) / diag;
+// Gauss-Seidel update of cell (i, j, k) at linear index q.
+__device__ __forceinline__ void mg_relax(float *u, const float *f, const MgGrid &g, int i, int j,
+                                         int k, int q) {
+  u[q] = (f[q] - mg_lap_off(u, g, i, j, k)) * g.invd;
 }
 
-__global__ void __launch_bounds__(256) mg_residual_kernel(float *__restrict__ r,
-                                                          const float *__restrict__ u,
-                                                          const float *__restrict__ f, MgGrid g) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = g.mx * g.my * g.mz;
-  if (idx >= n) return;
-  const int i = idx % g.mx, j = (idx / g.mx) % g.my, k = idx / (g.mx * g.my);
-  const float diag = -2.f * (g.cx + g.cy + g.cz);
-  r[idx] = f[idx] - (mg_lap_off(u, g, i, j, k) + diag * u[idx]);
+// Residual f - L u at cell (i, j, k).
+__device__ __forceinline__ float mg_res(const float *u, const float *f, const MgGrid &g, int i,
+                                        int j, int k) {
+  const int q = i + g.mx * (j + g.my * k);
+  return f[q] - (mg_lap_off(u, g, i, j, k) + g.diag * u[q]);
 }
 
-// fc = average of the 8 children of each coarse cell; uc = 0.
-__global__ void __launch_bounds__(256) mg_restrict_kernel(float *__restrict__ fc,
-                                                          float *__restrict__ uc,
-                                                          const float *__restrict__ r, int mx,
-                                                          int my, int mz) {
-  const int cx = mx / 2, cy = my / 2, cz = mz / 2;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= cx * cy * cz) return;
-  const int I = idx % cx, J = (idx / cx) % cy, K = idx / (cx * cy);
+// The residual averaged over the 8 children of coarse cell (I, J, K).
+__device__ __forceinline__ float mg_child_mean(const float *u, const float *f, const MgGrid &g,
+                                               int I, int J, int K) {
   float s = 0.f;
-  for (int c = 0; c < 8; ++c) {
-    const int i = 2 * I + (c & 1), j = 2 * J + ((c >> 1) & 1), k = 2 * K + (c >> 2);
-    s += r[(size_t)i + (size_t)mx * (j + (size_t)my * k)];
-  }
-  fc[idx] = 0.125f * s;
-  uc[idx] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    s += mg_res(u, f, g, 2 * I + (c & 1), 2 * J + ((c >> 1) & 1), 2 * K + (c >> 2));
+  return 0.125f * s;
 }
 
 // Position of fine cell i on the grid coarser by 2: lower coarse cell and
@@ -136,28 +121,87 @@ __device__ __forceinline__ void mg_axis(int i, int m, int per, int &c0, int &c1,
   c1 = I1;
 }
 
-// u += trilinear interpolation of the coarse correction uc.
+// Trilinear interpolation of the coarse field uc (cx, cy, cz cells) at the
+// fine cells (2I, j, k) and (2I + 1, j, k).
+__device__ __forceinline__ float2 mg_interp_pair(const float *uc, int I, int j, int k, int cx,
+                                                 int cy, int cz, int px, int py, int pz) {
+  int y0, y1, z0, z1;
+  float wy, wz;
+  mg_axis(j, cy, py, y0, y1, wy);
+  mg_axis(k, cz, pz, z0, z1, wz);
+  const int r00 = cx * (y0 + cy * z0), r10 = cx * (y1 + cy * z0);
+  const int r01 = cx * (y0 + cy * z1), r11 = cx * (y1 + cy * z1);
+  float v[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    int x0, x1;
+    float wx;
+    mg_axis(2 * I + h, cx, px, x0, x1, wx);
+    v[h] = (1.f - wz) * ((1.f - wy) * ((1.f - wx) * uc[r00 + x0] + wx * uc[r00 + x1]) +
+                         wy * ((1.f - wx) * uc[r10 + x0] + wx * uc[r10 + x1])) +
+           wz * ((1.f - wy) * ((1.f - wx) * uc[r01 + x0] + wx * uc[r01 + x1]) +
+                 wy * ((1.f - wx) * uc[r11 + x0] + wx * uc[r11 + x1]));
+  }
+  return make_float2(v[0], v[1]);
+}
+
+// One colour of a red-black Gauss-Seidel sweep, one thread per cell of that
+// colour (mx even: every row holds mx / 2 cells of each colour).
+__global__ void __launch_bounds__(256) mg_smooth_kernel(float *__restrict__ u,
+                                                        const float *__restrict__ f, MgGrid g,
+                                                        int colour) {
+  const int hx = g.mx >> 1;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= hx * g.my * g.mz) return;
+  const int row = p / hx, t = p - row * hx;
+  const int k = row / g.my, j = row - k * g.my;
+  const int i = 2 * t + ((j + k + colour) & 1);
+  mg_relax(u, f, g, i, j, k, i + g.mx * row);
+}
+
+// The same for mx odd: one thread per cell, the other colour idle.
+__global__ void __launch_bounds__(256) mg_smooth_any_kernel(float *__restrict__ u,
+                                                            const float *__restrict__ f,
+                                                            MgGrid g, int colour) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.mx * g.my * g.mz) return;
+  const int row = idx / g.mx, i = idx - row * g.mx;
+  const int k = row / g.my, j = row - k * g.my;
+  if (((i + j + k) & 1) != colour) return;
+  mg_relax(u, f, g, i, j, k, idx);
+}
+
+// fc = the residual of the fine level averaged over the 8 children of each
+// coarse cell; uc = 0.
+__global__ void __launch_bounds__(256) mg_restrict_kernel(float *__restrict__ fc,
+                                                          float *__restrict__ uc,
+                                                          const float *__restrict__ u,
+                                                          const float *__restrict__ f, MgGrid g) {
+  const int cx = g.mx >> 1, cy = g.my >> 1, cz = g.mz >> 1;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cx * cy * cz) return;
+  const int crow = idx / cx, I = idx - crow * cx;
+  const int K = crow / cy, J = crow - K * cy;
+  fc[idx] = mg_child_mean(u, f, g, I, J, K);
+  uc[idx] = 0.f;
+}
+
+// u += trilinear interpolation of the coarse correction uc, one thread per
+// pair of fine cells (2I, 2I + 1) of a row.
 __global__ void __launch_bounds__(256) mg_prolong_kernel(float *__restrict__ u,
                                                          const float *__restrict__ uc, int mx,
                                                          int my, int mz, int px, int py, int pz) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= mx * my * mz) return;
-  const int i = idx % mx, j = (idx / mx) % my, k = idx / (mx * my);
-  const int cx = mx / 2, cy = my / 2, cz = mz / 2;
-  int x0, x1, y0, y1, z0, z1;
-  float wx, wy, wz;
-  mg_axis(i, cx, px, x0, x1, wx);
-  mg_axis(j, cy, py, y0, y1, wy);
-  mg_axis(k, cz, pz, z0, z1, wz);
-  auto c = [&](int a, int b, int d) {
-    return uc[(size_t)a + (size_t)cx * (b + (size_t)cy * d)];
-  };
-  const float v =
-      (1.f - wz) * ((1.f - wy) * ((1.f - wx) * c(x0, y0, z0) + wx * c(x1, y0, z0)) +
-                    wy * ((1.f - wx) * c(x0, y1, z0) + wx * c(x1, y1, z0))) +
-      wz * ((1.f - wy) * ((1.f - wx) * c(x0, y0, z1) + wx * c(x1, y0, z1)) +
-            wy * ((1.f - wx) * c(x0, y1, z1) + wx * c(x1, y1, z1)));
-  u[idx] += v;
+  const int cx = mx >> 1;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cx * my * mz) return;
+  const int row = p / cx, I = p - row * cx;
+  const int k = row / my, j = row - k * my;
+  const float2 v = mg_interp_pair(uc, I, j, k, cx, my >> 1, mz >> 1, px, py, pz);
+  float2 *o = reinterpret_cast<float2 *>(u + (size_t)mx * row) + I;
+  float2 w = *o;
+  w.x += v.x;
+  w.y += v.y;
+  *o = w;
 }
 
 // The coarsest level: `sweeps` red-black Gauss-Seidel sweeps from u = 0 by
@@ -172,12 +216,12 @@ __global__ void __launch_bounds__(1024) mg_coarsest_kernel(float *__restrict__ u
     sf[q] = f[q];
   }
   __syncthreads();
-  const float diag = -2.f * (g.cx + g.cy + g.cz);
   for (int s = 0; s < sweeps; ++s)
     for (int colour = 0; colour < 2; ++colour) {
       for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
-        if (((i + j + k) & 1) == colour) su[q] = (sf[q] - mg_lap_off(su, g, i, j, k)) / diag;
+        const int row = q / g.mx, i = q - row * g.mx;
+        const int k = row / g.my, j = row - k * g.my;
+        if (((i + j + k) & 1) == colour) mg_relax(su, sf, g, i, j, k, q);
       }
       __syncthreads();
     }
@@ -187,8 +231,12 @@ __global__ void __launch_bounds__(1024) mg_coarsest_kernel(float *__restrict__ u
 // The levels from `first` down, all of at most kMgCoarsestCells cells, as
 // one V-cycle by one block in LDS: f of the first level in, u of the first
 // level out (from u = 0).  Same smoother, residual, restriction and
-// prolongation as the per-level kernels.
+// prolongation as the per-level kernels.  Each thread keeps the coordinates
+// of its cells of one colour in registers for the sweeps of a level.
 constexpr int kMgSmallLevels = 4;
+constexpr int kMgSmallCells = 4096 + 512 + 64 + 8;
+constexpr int kMgSmallThreads = 512;
+constexpr int kMgSmallPer = kMgCoarsestCells / 2 / kMgSmallThreads;
 struct MgSmall {
   int levels;
   MgGrid g[kMgSmallLevels];
@@ -198,85 +246,90 @@ struct MgSmall {
 
 __device__ __forceinline__ void lds_smooth(float *u, const float *f, const MgGrid &g,
                                            int sweeps) {
-  const int n = g.mx * g.my * g.mz;
-  const float diag = -2.f * (g.cx + g.cy + g.cz);
-  for (int s = 0; s < sweeps; ++s)
-    for (int colour = 0; colour < 2; ++colour) {
-      for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
-        if (((i + j + k) & 1) == colour) u[q] = (f[q] - mg_lap_off(u, g, i, j, k)) / diag;
+  if (g.mx & 1) {
+    const int n = g.mx * g.my * g.mz;
+    for (int s = 0; s < sweeps; ++s)
+      for (int colour = 0; colour < 2; ++colour) {
+        for (int q = threadIdx.x; q < n; q += kMgSmallThreads) {
+          const int row = q / g.mx, i = q - row * g.mx;
+          const int k = row / g.my, j = row - k * g.my;
+          if (((i + j + k) & 1) == colour) mg_relax(u, f, g, i, j, k, q);
+        }
+        __syncthreads();
       }
+    return;
+  }
+  const int hx = g.mx >> 1, nh = hx * g.my * g.mz;
+  int t[kMgSmallPer], jj[kMgSmallPer], kk[kMgSmallPer];
+#pragma unroll
+  for (int s = 0; s < kMgSmallPer; ++s) {
+    const int p = threadIdx.x + s * kMgSmallThreads;
+    const int row = p / hx;
+    t[s] = p - row * hx;
+    kk[s] = row / g.my;
+    jj[s] = row - kk[s] * g.my;
+  }
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int colour = 0; colour < 2; ++colour) {
+#pragma unroll
+      for (int s = 0; s < kMgSmallPer; ++s)
+        if (threadIdx.x + s * kMgSmallThreads < nh) {
+          const int i = 2 * t[s] + ((jj[s] + kk[s] + colour) & 1);
+          mg_relax(u, f, g, i, jj[s], kk[s], i + g.mx * (jj[s] + g.my * kk[s]));
+        }
       __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(1024) mg_small_vcycle_kernel(float *__restrict__ u_out,
-                                                               const float *__restrict__ f_in,
-                                                               MgSmall S) {
-  __shared__ float su[4096 + 512 + 64 + 8],
-      sf[4096 + 512 + 64 + 8], sr[4096 + 512 + 64 + 8];
+__global__ void __launch_bounds__(kMgSmallThreads) mg_small_vcycle_kernel(
+    float *__restrict__ u_out, const float *__restrict__ f_in, MgSmall S) {
+  __shared__ float su[kMgSmallCells], sf[kMgSmallCells];
   const int n0 = S.g[0].mx * S.g[0].my * S.g[0].mz;
-  for (int q = threadIdx.x; q < n0; q += blockDim.x) {
+  for (int q = threadIdx.x; q < n0; q += kMgSmallThreads) {
     sf[q] = f_in[q];
     su[q] = 0.f;
   }
   __syncthreads();
-  // Down: smooth, residual, restrict.
-  for (int l = 0; l < S.levels - 1; ++l) {
-    const MgGrid &g = S.g[l];
-    float *u = su + S.off[l], *f = sf + S.off[l], *r = sr + S.off[l];
+  // Down: smooth, then the averaged residual as the next level's f.
+#pragma unroll
+  for (int l = 0; l < kMgSmallLevels - 1; ++l) {
+    if (l >= S.levels - 1) break;
+    const MgGrid &g = S.g[l], &c = S.g[l + 1];
+    float *u = su + S.off[l];
+    const float *f = sf + S.off[l];
     lds_smooth(u, f, g, S.nu1);
-    const int n = g.mx * g.my * g.mz;
-    const float diag = -2.f * (g.cx + g.cy + g.cz);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
-      r[q] = f[q] - (mg_lap_off(u, g, i, j, k) + diag * u[q]);
-    }
-    __syncthreads();
-    const MgGrid &c = S.g[l + 1];
-    float *uc = su + S.off[l + 1], *fc = sf + S.off[l + 1];
     const int nc = c.mx * c.my * c.mz;
-    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
-      const int I = q % c.mx, J = (q / c.mx) % c.my, K = q / (c.mx * c.my);
-      float t = 0.f;
-      for (int e = 0; e < 8; ++e) {
-        const int i = 2 * I + (e & 1), j = 2 * J + ((e >> 1) & 1), k = 2 * K + (e >> 2);
-        t += r[i + g.mx * (j + g.my * k)];
-      }
-      fc[q] = 0.125f * t;
-      uc[q] = 0.f;
+    for (int q = threadIdx.x; q < nc; q += kMgSmallThreads) {
+      const int crow = q / c.mx, I = q - crow * c.mx;
+      const int K = crow / c.my, J = crow - K * c.my;
+      sf[S.off[l + 1] + q] = mg_child_mean(u, f, g, I, J, K);
+      su[S.off[l + 1] + q] = 0.f;
     }
     __syncthreads();
   }
   // Coarsest.
-  {
-    const int l = S.levels - 1;
-    lds_smooth(su + S.off[l], sf + S.off[l], S.g[l], S.coarse_sweeps);
-  }
+#pragma unroll
+  for (int l = 1; l < kMgSmallLevels; ++l)
+    if (l == S.levels - 1) lds_smooth(su + S.off[l], sf + S.off[l], S.g[l], S.coarse_sweeps);
   // Up: prolong, smooth.
-  for (int l = S.levels - 2; l >= 0; --l) {
-    const MgGrid &g = S.g[l];
-    const MgGrid &c = S.g[l + 1];
+#pragma unroll
+  for (int l = kMgSmallLevels - 2; l >= 0; --l) {
+    if (l >= S.levels - 1) continue;
+    const MgGrid &g = S.g[l], &c = S.g[l + 1];
     float *u = su + S.off[l];
     const float *uc = su + S.off[l + 1];
-    const int n = g.mx * g.my * g.mz;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      const int i = q % g.mx, j = (q / g.mx) % g.my, k = q / (g.mx * g.my);
-      int x0, x1, y0, y1, z0, z1;
-      float wx, wy, wz;
-      mg_axis(i, c.mx, g.px, x0, x1, wx);
-      mg_axis(j, c.my, g.py, y0, y1, wy);
-      mg_axis(k, c.mz, g.pz, z0, z1, wz);
-      auto cv = [&](int a, int b, int d) { return uc[a + c.mx * (b + c.my * d)]; };
-      u[q] += (1.f - wz) * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z0) + wx * cv(x1, y0, z0)) +
-                            wy * ((1.f - wx) * cv(x0, y1, z0) + wx * cv(x1, y1, z0))) +
-              wz * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z1) + wx * cv(x1, y0, z1)) +
-                    wy * ((1.f - wx) * cv(x0, y1, z1) + wx * cv(x1, y1, z1)));
+    const int hx = g.mx >> 1, np = hx * g.my * g.mz;
+    for (int q = threadIdx.x; q < np; q += kMgSmallThreads) {
+      const int row = q / hx, I = q - row * hx;
+      const int k = row / g.my, j = row - k * g.my;
+      const float2 v = mg_interp_pair(uc, I, j, k, c.mx, c.my, c.mz, g.px, g.py, g.pz);
+      u[g.mx * row + 2 * I] += v.x;
+      u[g.mx * row + 2 * I + 1] += v.y;
     }
     __syncthreads();
     lds_smooth(u, sf + S.off[l], g, S.nu2);
   }
-  for (int q = threadIdx.x; q < n0; q += blockDim.x) u_out[q] = su[q];
+  for (int q = threadIdx.x; q < n0; q += kMgSmallThreads) u_out[q] = su[q];
 }
 
 MgGrid grid_of(const MgHierarchy &H, int l) {
@@ -284,21 +337,29 @@ MgGrid grid_of(const MgHierarchy &H, int l) {
   g.mx = H.m[l][0]; g.my = H.m[l][1]; g.mz = H.m[l][2];
   g.px = H.per[0]; g.py = H.per[1]; g.pz = H.per[2];
   g.cx = (float)H.c[l][0]; g.cy = (float)H.c[l][1]; g.cz = (float)H.c[l][2];
+  g.diag = -2.f * (g.cx + g.cy + g.cz);
+  g.invd = 1.f / g.diag;
   return g;
 }
 
-int blocks_of(const MgGrid &g) { return (g.mx * g.my * g.mz + 255) / 256; }
+unsigned blocks_for(long long threads) { return (unsigned)((threads + 255) / 256); }
 
 hipError_t smooth(const MgHierarchy &H, int l, int sweeps, hipStream_t s) {
   const MgGrid g = grid_of(H, l);
+  const long long n = (long long)g.mx * g.my * g.mz;
   for (int q = 0; q < sweeps; ++q)
-    for (int colour = 0; colour < 2; ++colour)
-      mg_smooth_kernel<<<blocks_of(g), 256, 0, s>>>(H.u[l], H.f[l], g, colour);
+    for (int colour = 0; colour < 2; ++colour) {
+      if (g.mx & 1)
+        mg_smooth_any_kernel<<<blocks_for(n), 256, 0, s>>>(H.u[l], H.f[l], g, colour);
+      else
+        mg_smooth_kernel<<<blocks_for(n / 2), 256, 0, s>>>(H.u[l], H.f[l], g, colour);
+    }
   return hipGetLastError();
 }
 
 // The levels from l down fit one block's LDS (every level at most
-// kMgCoarsestCells cells, the sum at most 4096 + 512 + 64 + 8).
+// kMgCoarsestCells cells, the sum at most kMgSmallCells), and every level
+// above the last is halvable (mx even).
 bool small_from(const MgHierarchy &H, int l) {
   if (H.levels - l > kMgSmallLevels) return false;
   size_t tot = 0;
@@ -307,9 +368,8 @@ bool small_from(const MgHierarchy &H, int l) {
     if (c > (size_t)kMgCoarsestCells) return false;
     tot += c;
   }
-  return tot <= 4096 + 512 + 64 + 8;
+  return tot <= (size_t)kMgSmallCells;
 }
-
 hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) {
   const MgGrid g = grid_of(H, l);
   const int n = g.mx * g.my * g.mz;
@@ -325,7 +385,7 @@ hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) 
     S.nu1 = nu1;
     S.nu2 = nu2;
     S.coarse_sweeps = 30;
-    mg_small_vcycle_kernel<<<1, 1024, 0, s>>>(H.u[l], H.f[l], S);
+    mg_small_vcycle_kernel<<<1, kMgSmallThreads, 0, s>>>(H.u[l], H.f[l], S);
     return hipGetLastError();
   }
   if (l == H.levels - 1) {
@@ -339,14 +399,11 @@ hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) 
   }
   hipError_t e = smooth(H, l, nu1, s);
   if (e != hipSuccess) return e;
-  mg_residual_kernel<<<blocks_of(g), 256, 0, s>>>(H.r[l], H.u[l], H.f[l], g);
-  const int nc = n / 8;
-  mg_restrict_kernel<<<(nc + 255) / 256, 256, 0, s>>>(H.f[l + 1], H.u[l + 1], H.r[l], g.mx,
-                                                      g.my, g.mz);
+  mg_restrict_kernel<<<blocks_for(n / 8), 256, 0, s>>>(H.f[l + 1], H.u[l + 1], H.u[l], H.f[l], g);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = vcycle(H, l + 1, nu1, nu2, s)) != hipSuccess) return e;
-  mg_prolong_kernel<<<blocks_of(g), 256, 0, s>>>(H.u[l], H.u[l + 1], g.mx, g.my, g.mz, g.px,
-                                                 g.py, g.pz);
+  mg_prolong_kernel<<<blocks_for(n / 2), 256, 0, s>>>(H.u[l], H.u[l + 1], g.mx, g.my, g.mz, g.px,
+                                                      g.py, g.pz);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return smooth(H, l, nu2, s);
 }
@@ -427,14 +484,15 @@ hipError_t mg_seed_from_level1(const MgHierarchy &H, hipStream_t s) {
   const MgGrid g = grid_of(H, 0);
   const int n0 = g.mx * g.my * g.mz;
   mg_zero_kernel<<<(n0 + 255) / 256, 256, 0, s>>>(H.u[0], n0);
-  mg_prolong_kernel<<<blocks_of(g), 256, 0, s>>>(H.u[0], H.u[1], g.mx, g.my, g.mz, g.px, g.py,
-                                                 g.pz);
+  mg_prolong_kernel<<<blocks_for(n0 / 2), 256, 0, s>>>(H.u[0], H.u[1], g.mx, g.my, g.mz, g.px,
+                                                       g.py, g.pz);
   return hipGetLastError();
 }
 
 hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStream_t s,
                     bool from_zero) {
   if (H.levels < 1) return hipErrorInvalidValue;
+  if ((long long)H.m[0][0] * H.m[0][1] * H.m[0][2] >= (1LL << 31)) return hipErrorInvalidValue;
   const int n0 = H.m[0][0] * H.m[0][1] * H.m[0][2];
   if (from_zero) mg_zero_kernel<<<(n0 + 255) / 256, 256, 0, s>>>(H.u[0], n0);
   hipError_t e = hipGetLastError();
