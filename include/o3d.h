@@ -107,6 +107,22 @@ int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz
 int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx, double hy,
                   double hz, const int *per, int cycles);
 
+/* solve_poisson_multigrid (src/poisson_multigrid.f90:10-18, called from
+ * correct_pression when multigrid = 1, src/integration.f90:242-244; that
+ * solver does not run as written: its arrays are declared with ghost layers
+ * the caller does not pass and its level count is kmax).  A working
+ * multigrid solve of the problem the SOR solvers solve (same operator and
+ * neighbour rules, solver as for o3d_poisson): cycles of FP64 residual, stop
+ * when max |rhs - L pp| < tol (the reference's stated criterion), coarse-grid
+ * correction on the 2^3 block averages by one V(npre, npost) cycle over at
+ * most nlevels - 1 coarser levels, npre + npost red-black Gauss-Seidel sweeps
+ * on the full grid, pp += correction.  At most max_cycles cycles; cycles and
+ * resmax (the final max |r|) may be NULL.  Needs nx a multiple of 8 and ny,
+ * nz even. */
+int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, double dy, double dz,
+                          int nx, int ny, int nz, int solver, int nlevels, int npre, int npost,
+                          double tol, int max_cycles, int *cycles, double *resmax);
+
 /* correct_pression, src/integration.f90:199-255 (SOR branch): divergence of
  * the predicted velocity (odd variants), rhs = div/dt, then the Poisson solver
  * that schemes() binds for bc (src/initialization.f90:283-301). */

@@ -81,7 +81,7 @@ EXPORTS = [
     "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
-    "o3d_warm_features", "o3d_solver_history", "o3d_multigrid",
+    "o3d_warm_features", "o3d_solver_history", "o3d_multigrid", "o3d_poisson_multigrid",
 ]
 NN_EXPORTS = ["o3d_nn_create", "o3d_nn_destroy", "o3d_nn_resize", "o3d_nn_num_params",
               "o3d_nn_get_params", "o3d_nn_set_params", "o3d_nn_get_grads", "o3d_nn_zero_grad",
@@ -203,6 +203,22 @@ def poisson(pp, rhs, d, omega, eps, kmax, idyn, solver):
                              int(kmax), int(idyn), int(code), C.byref(it), C.byref(dm),
                              C.byref(rs)))
     return it.value, dm.value, rs.value, om.value
+
+
+def poisson_multigrid(pp, rhs, d, solver, npre=5, npost=4, tol=1e-8, max_cycles=100,
+                      nlevels=100):
+    """Multigrid solve of the SOR solvers' Poisson problem (pp in place,
+    from its current value) until max |rhs - L pp| < tol or max_cycles
+    cycles; the interface of solve_poisson_multigrid
+    (src/poisson_multigrid.f90:10-18).  Returns (cycles, max |r|)."""
+    nx, ny, nz = _dims(pp)
+    cy = C.c_int(0)
+    rm = C.c_double(0)
+    code = {"0000": 0, "0011": 1, "111111": 2}.get(solver, solver)
+    check(load().o3d_poisson_multigrid(ptr(pp), ptr(rhs), *map(_d, d), nx, ny, nz, int(code),
+                                       int(nlevels), int(npre), int(npost), _d(tol),
+                                       int(max_cycles), C.byref(cy), C.byref(rm)))
+    return cy.value, rm.value
 
 
 def multigrid(u, f, h, per, cycles):

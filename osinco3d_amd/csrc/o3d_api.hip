@@ -634,6 +634,126 @@ extern "C" int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, d
   return s.finish();
 }
 
+// Multigrid solve of the pressure Poisson problem of the SOR solvers (same
+// 7-point operator and neighbour rules).  A cycle: the FP64 residual of pp
+// (stored colour-split in FP32), stop when its maximum is below tol, the
+// coarse-grid correction (residual averaged over 2^3 blocks, one V(npre,
+// npost) cycle on the block grid and its coarser levels, trilinear
+// interpolation), npre + npost red-black Gauss-Seidel sweeps of the
+// correction equation on the full grid from that correction, and pp += the
+// correction in FP64.  The residual is formed in FP64 every cycle, so the
+// FP32 corrections do not limit the attainable accuracy.
+extern "C" int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, double dy,
+                                     double dz, int nx, int ny, int nz, int solver, int nlevels,
+                                     int npre, int npost, double tol, int max_cycles,
+                                     int *cycles, double *resmax) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
+  CHK(check_red_black(solver, nx, ny, nz));
+  if (!sor_split_supported<float>(nx, ny, nz) || ny % 2 || nz % 2)
+    return fail("multigrid needs nx a multiple of 8 and ny, nz even (grid %dx%dx%d)", nx, ny,
+                nz);
+  if (npre < 0 || npost < 0 || npre + npost < 1)
+    return fail("multigrid: npre, npost must be >= 0 with npre + npost >= 1");
+  if (max_cycles < 1) return fail("multigrid: max_cycles must be >= 1");
+  if (!(tol >= 0.0)) return fail("multigrid: tol must be >= 0");
+  const size_t n = (size_t)nx * ny * nz;
+  Staged s;
+  double *p;
+  const double *rhs_d;
+  CHK(s.out(pp, n, &p, true));
+  CHK(s.in(rhs, n, &rhs_d));
+  int per[3];
+  solver_periodic(solver, per);
+  const SorConst c = sor_const(dx, dy, dz);
+  // The coarse levels: the grid of 2^3 blocks and the levels below it, at
+  // most nlevels - 1 of them (nlevels counts the full grid).
+  MgHierarchy H;
+  const int m0[3] = {nx / 2, ny / 2, nz / 2};
+  const double h[3] = {2.0 * dx, 2.0 * dy, 2.0 * dz};
+  float *mgbuf;
+  CHK(s.scratch(mg_plan(H, m0, h, per), &mgbuf));
+  mg_bind(H, mgbuf);
+  H.levels = std::min(H.levels, std::max(nlevels - 1, 0));
+  float *R, *E;
+  CHK(s.scratch(n, &R));
+  CHK(s.scratch(n, &E));
+  const int nb = residual_blocks(nx, ny, nz);
+  double *part;
+  CHK(s.scratch((size_t)4 * nb + 4, &part));
+  double *fin = part + 4 * (size_t)nb;
+  // Gauss-Seidel sweeps of A e = r (omega = 1 for all of them, ending on
+  // the sweep count only).
+  SorRun<float, float> run;
+  CHK(s.scratch(1, &run.st));
+  unsigned long long *dslot;
+  CHK(s.scratch(sor_split_slot_words(), &dslot));
+  HIPCHK(hipHostMalloc((void **)&run.hst, sizeof(SorState)));
+  double *hfin = nullptr;
+  if (hipHostMalloc((void **)&hfin, 4 * sizeof(double)) != hipSuccess) {
+    (void)hipHostFree(run.hst);
+    return fail("multigrid: pinned allocation failed");
+  }
+  std::vector<hipEvent_t> ev(2);
+  for (auto &e : ev) (void)hipEventCreate(&e);
+  run.events = &ev;
+  SorArgs<float, float> &a = run.a;
+  a.rhs = nullptr;
+  a.st = run.st;
+  a.log = nullptr;
+  a.log_len = 0;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  for (int q = 0; q < 3; ++q) a.per[q] = per[q];
+  a.ox = c.ox; a.oy = c.oy; a.oz = c.oz; a.A = c.A;
+  a.eps = 0.0;
+  a.factor = 1.0;
+  a.kmax = npre + npost;
+  a.idyn = 0;
+  a.it = 0;
+  a.gs = npre + npost;
+  const size_t half = n / 2;
+  run.split = true;
+  run.sa = split_args<float>(a, E, E + half, R, R + half, dslot);
+  int rc = 0, done = 0;
+  double rmax = 0.0;
+  while (rc == 0) {
+    // r = rhs - L pp (FP64 arithmetic), colour-split in FP32, and e = 0.
+    if (launch_residual_norm_pair<double>(p, p, rhs_d, part, fin, nx, ny, nz, per, c.ox, c.oy,
+                                          c.oz, c.A, g_stream, R, E) != hipSuccess ||
+        hipMemcpyAsync(hfin, fin, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(g_stream) != hipSuccess) {
+      rc = fail("multigrid: residual failed: %s", hipGetErrorString(hipGetLastError()));
+      break;
+    }
+    rmax = hfin[2];
+    if (!(rmax >= tol) || done >= max_cycles) break;  // converged (or NaN), or out of cycles
+    if (H.levels > 0) {
+      if (mg_pool_split(R, H.f[0], nx, ny, nz, g_stream) != hipSuccess ||
+          mg_solve(H, 1, std::max(npre, 1), std::max(npost, 1), g_stream) != hipSuccess ||
+          mg_prolong_split(E, H.u[0], nx, ny, nz, per, g_stream) != hipSuccess) {
+        rc = fail("multigrid: coarse-grid correction failed");
+        break;
+      }
+    }
+    int sweeps = 0;
+    if ((rc = run.run(1.0, 16, &sweeps)) != 0) break;
+    if (launch_fold_split(p, E, nx, ny, nz, g_stream) != hipSuccess) {
+      rc = fail("multigrid: fold failed");
+      break;
+    }
+    ++done;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipHostFree(run.hst);
+  (void)hipHostFree(hfin);
+  if (rc != 0) return rc;
+  if (cycles) *cycles = done;
+  if (resmax) *resmax = rmax;
+  return s.finish();
+}
+
 extern "C" int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz,
                                int nx, int ny, int nz, double *omega, double eps, int kmax,
                                int idyn, int solver, int variant, int *exit_iter, double *dmax,

@@ -4,7 +4,8 @@
 ! reference's names and dummy-argument lists (src/integration.f90:14-16,
 ! 199-200, 257-258) and run the fused HIP kernels of libo3d.so:
 !   predict_velocity  -> o3d_predict_velocity (9 D1 + 9 D2 + nu_t + AB update)
-!   correct_pression  -> o3d_correct_pression (divergence/dt + red-black SOR)
+!   correct_pression  -> o3d_correct_pression (divergence/dt + red-black SOR), or
+!                        o3d_divergence + o3d_poisson_multigrid with multigrid = 1
 !   correct_velocity  -> o3d_correct_velocity (grad p, update, NaN/max guard)
 ! Host arrays are staged through device memory on every call (the reference
 ! owns all state), so results come back in the caller's arrays exactly as
@@ -19,7 +20,7 @@ module integration
   use IOfunctions, only: print_in_outflow_rate, write_velocity_diverged
   use o3d_capi
   implicit none
-  private :: predict_x, pression_x, correct_x
+  private :: predict_x, pression_x, pression_mg, correct_x
 
 contains
 
@@ -84,13 +85,43 @@ contains
 
     print *, "* Correction pression"
     if (multigrid == 1) then
-       print *, "multigrid = 1 is not supported by the GPU path; use the SOR solver"
-       stop
+       ! src/integration.f90:242-244: npre = 5, npost = 4, tol = eps, kmax as
+       ! the level count (here also the cycle limit).
+       call pression_mg(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, nx, ny, nz, dt, eps, kmax)
+    else
+       call pression_x(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, nx, ny, nz, dt, omega, eps, &
+            kmax, idyn)
     end if
-    call pression_x(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, nx, ny, nz, dt, omega, eps, &
-         kmax, idyn)
     print *, ""
   end subroutine correct_pression
+
+  subroutine pression_mg(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, nx, ny, nz, dt, eps, kmax)
+    integer, intent(in) :: nx, ny, nz, kmax
+    real(kind=8), intent(inout) :: pp(nx, ny, nz)
+    real(kind=8), intent(in) :: ux_pred(nx, ny, nz), uy_pred(nx, ny, nz), uz_pred(nx, ny, nz)
+    real(kind=8), intent(in) :: dx, dy, dz, dt, eps
+    real(kind=8), allocatable :: rhs(:,:,:)
+    integer(c_int) :: bc(3), solver, cycles
+    real(c_double) :: stats(6), resmax
+    bc = [int(nbcx1, c_int), int(nbcy1, c_int), int(nbcz1, c_int)]
+    if (bc(1) == 0 .and. bc(2) == 0) then
+       solver = 0
+    else if (bc(1) == 0 .and. bc(2) == 1) then
+       solver = 1
+    else if (bc(1) == 1 .and. bc(2) == 0) then
+       solver = -1
+    else
+       solver = 2
+    end if
+    allocate(rhs(nx, ny, nz))
+    call o3d_check(o3d_divergence(rhs, ux_pred, uy_pred, uz_pred, dx, dy, dz, int(nx, c_int), &
+         int(ny, c_int), int(nz, c_int), bc, 1_c_int, dt, stats), 'correct_pression')
+    call o3d_check(o3d_poisson_multigrid(pp, rhs, dx, dy, dz, int(nx, c_int), int(ny, c_int), &
+         int(nz, c_int), solver, int(kmax, c_int), 5_c_int, 4_c_int, eps, int(kmax, c_int), &
+         cycles, resmax), 'solve_poisson_multigrid')
+    deallocate(rhs)
+    write(*, '(A20,I4,A1,E13.6)') ' Multigrid cycles : ', cycles, ',', resmax
+  end subroutine pression_mg
 
   subroutine pression_x(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, nx, ny, nz, dt, omega, &
        eps, kmax, idyn)

@@ -92,6 +92,18 @@ module o3d_capi
        integer(c_int) :: exit_iter, reason
      end function o3d_poisson
 
+     integer(c_int) function o3d_poisson_multigrid(pp, rhs, dx, dy, dz, nx, ny, nz, solver, &
+          nlevels, npre, npost, tol, max_cycles, cycles, resmax) &
+          bind(C, name="o3d_poisson_multigrid")
+       import :: c_int, c_double
+       real(c_double) :: pp(*)
+       real(c_double), intent(in) :: rhs(*)
+       real(c_double), value :: dx, dy, dz, tol
+       integer(c_int), value :: nx, ny, nz, solver, nlevels, npre, npost, max_cycles
+       integer(c_int) :: cycles
+       real(c_double) :: resmax
+     end function o3d_poisson_multigrid
+
      integer(c_int) function o3d_correct_pression(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, &
           nx, ny, nz, dt, omega, eps, kmax, idyn, bc, exit_iter, dmax, reason) &
           bind(C, name="o3d_correct_pression")

@@ -65,3 +65,26 @@ def test_dropin_matches_reference_run(name, dump):
             assert np.abs(a - g[f"{k}{itime}"]).max() <= tol_u, (itime, k)
         pr = g[f"pp{itime}"]
         assert np.abs((pp - pp.mean()) - (pr - pr.mean())).max() <= tol_p
+
+
+def test_dropin_multigrid_switch():
+    """multigrid = 1 in parameters.o3d: correct_pression runs the multigrid
+    solver (o3d_poisson_multigrid) to max |r| < eps; the run follows the
+    reference's SOR run of the same case within the eps-tied tolerance."""
+    if not os.path.exists(EXE):
+        pytest.skip("oracle/_ref/o3d_dropin_run not built (needs the reference tree)")
+    g = np.load(os.path.join(ROOT, "tests", "golden", "run_tgv16_ab2.npz"), allow_pickle=False)
+    cfg = Config.from_namelist(bytes(g["params"]).decode())
+    cfg.multigrid = 1
+    print("[gpu] drop-in run_tgv16_ab2.npz with multigrid = 1", flush=True)
+    recs, iters, out = _run(cfg, 10)
+    cyc = [int(m.group(1)) for m in re.finditer(r"Multigrid cycles\s*:\s*(\d+)", out)]
+    assert len(cyc) == cfg.itstop and not iters
+    print(f"[gpu] multigrid cycles per step: {cyc}", flush=True)
+    for itime, t, ux, uy, uz, pp in recs:
+        if itime == 0:
+            continue
+        for a, k in ((ux, "ux"), (uy, "uy"), (uz, "uz")):
+            assert np.abs(a - g[f"{k}{itime}"]).max() <= 2e-6, (itime, k)
+        pr = g[f"pp{itime}"]
+        assert np.abs((pp - pp.mean()) - (pr - pr.mean())).max() <= 2e-5
