@@ -1414,26 +1414,37 @@ __global__ void lw_coarse_init_kernel(float *__restrict__ u0, const float *__res
 }
 
 // q += trilinear interpolation of the coarse correction c (the multigrid
-// solution on the grid of F^3 block averages).
-__global__ void mg_prolong_fine_kernel(double *__restrict__ q, const float *__restrict__ c,
-                                       int nx, int ny, int nz, int F, int px, int py, int pz) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= nx) return;
+// solution on the grid of F^3 block averages), one thread per pair of cells
+// (2t, 2t + 1) of a row (nx even), 16-byte accesses.
+__global__ void __launch_bounds__(256) mg_prolong_fine_kernel(double *__restrict__ q,
+                                                              const float *__restrict__ c,
+                                                              int nx, int ny, int nz, int F,
+                                                              int px, int py, int pz) {
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (2 * t >= nx || j >= ny) return;
   const int mx = nx / F, my = ny / F, mz = nz / F;
-  int x0, x1, y0, y1, z0, z1;
-  float wx, wy, wz;
-  lw_axis(i, F, mx, px, x0, x1, wx);
+  int y0, y1, z0, z1;
+  float wy, wz;
   lw_axis(j, F, my, py, y0, y1, wy);
   lw_axis(k, F, mz, pz, z0, z1, wz);
   auto v = [&](int a, int b, int d) { return c[(size_t)a + (size_t)mx * (b + (size_t)my * d)]; };
-  const float t =
-      (1.f - wz) * ((1.f - wy) * ((1.f - wx) * v(x0, y0, z0) + wx * v(x1, y0, z0)) +
-                    wy * ((1.f - wx) * v(x0, y1, z0) + wx * v(x1, y1, z0))) +
-      wz * ((1.f - wy) * ((1.f - wx) * v(x0, y0, z1) + wx * v(x1, y0, z1)) +
-            wy * ((1.f - wx) * v(x0, y1, z1) + wx * v(x1, y1, z1)));
-  const size_t o = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
-  q[o] = q[o] + (double)t;
+  float tv[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    int x0, x1;
+    float wx;
+    lw_axis(2 * t + h, F, mx, px, x0, x1, wx);
+    tv[h] = (1.f - wz) * ((1.f - wy) * ((1.f - wx) * v(x0, y0, z0) + wx * v(x1, y0, z0)) +
+                          wy * ((1.f - wx) * v(x0, y1, z0) + wx * v(x1, y1, z0))) +
+            wz * ((1.f - wy) * ((1.f - wx) * v(x0, y0, z1) + wx * v(x1, y0, z1)) +
+                  wy * ((1.f - wx) * v(x0, y1, z1) + wx * v(x1, y1, z1)));
+  }
+  double2 *o = reinterpret_cast<double2 *>(q + (size_t)nx * (j + (size_t)ny * k)) + t;
+  double2 w = *o;
+  w.x = w.x + (double)tv[0];
+  w.y = w.y + (double)tv[1];
+  *o = w;
 }
 
 __global__ void lw_guess2_kernel(const double *p, const float *e1, const float *e2, double *q,
@@ -1441,14 +1452,29 @@ __global__ void lw_guess2_kernel(const double *p, const float *e1, const float *
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) q[i] = p[i] + (2.0 * (double)e1[i] - (double)e2[i]);
 }
-// e = p - p0 (float32) and the per-block sum of squares of e.
-__global__ void __launch_bounds__(256) lw_increment_kernel(const double *p, const double *p0,
-                                                           float *e, int n, double *part) {
+// e = p - p0 (float32) and the per-block sum of squares of e; pairs of
+// cells per thread (16-byte loads), the odd last cell by thread 0.
+constexpr int kIncBlocks = 2048;
+__global__ void __launch_bounds__(256) lw_increment_kernel(const double *__restrict__ p,
+                                                           const double *__restrict__ p0,
+                                                           float *__restrict__ e, int n,
+                                                           double *__restrict__ part) {
   __shared__ double sc[4];
   double s = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float v = (float)(p[i] - p0[i]);
-    e[i] = v;
+  const int np = n / 2;
+  const double2 *p2 = reinterpret_cast<const double2 *>(p);
+  const double2 *q2 = reinterpret_cast<const double2 *>(p0);
+  float2 *e2 = reinterpret_cast<float2 *>(e);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    const double2 a = p2[i], b = q2[i];
+    const float2 v = make_float2((float)(a.x - b.x), (float)(a.y - b.y));
+    e2[i] = v;
+    s += (double)v.x * v.x;
+    s += (double)v.y * v.y;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const float v = (float)(p[n - 1] - p0[n - 1]);
+    e[n - 1] = v;
     s += (double)v * v;
   }
   s = block_sum<4>(s, sc);
@@ -1547,8 +1573,8 @@ int lw_alloc(Solver &S) {
   const int np = std::max(4 * residual_blocks(S.p.nx, S.p.ny, S.p.nz) + 4, 2 * kLwBlocks);
   CHK(S.alloc(&S.npart, np));
   HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
-  CHK(S.alloc(&S.epart, kLwBlocks));
-  HIPCHK(hipHostMalloc((void **)&S.hepart, kLwBlocks * sizeof(double)));
+  CHK(S.alloc(&S.epart, kIncBlocks));
+  HIPCHK(hipHostMalloc((void **)&S.hepart, kIncBlocks * sizeof(double)));
   return 0;
 }
 
@@ -1642,7 +1668,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     // Sum of squares of e_prev: reduced when e_prev was formed, read back by
     // the previous step's final synchronisation.
     double se2 = 0.0;
-    for (int b = 0; b < kLwBlocks; ++b) se2 += S.hepart[b];
+    for (int b = 0; b < kIncBlocks; ++b) se2 += S.hepart[b];
     const int rc = lw_features(w, S.Rl, S.Eprev, nullptr, S.p.nx, S.p.ny, S.p.nz, h2, S.Feat,
                                nullptr, &re, se2);
     if (rc < 0) return rc;
@@ -1690,9 +1716,9 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     HIPCHK(hipGetLastError());
     if (nn_coarse) HIPCHK(mg_seed_from_level1(S.mg, g_stream));
     HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 2, 2, g_stream, !nn_coarse));
-    dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
-    mg_prolong_fine_kernel<<<grid, 256, 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny, S.p.nz, F,
-                                                       per[0], per[1], per[2]);
+    dim3 grid((S.p.nx / 2 + 63) / 64, (S.p.ny + 3) / 4, S.p.nz);
+    mg_prolong_fine_kernel<<<grid, dim3(64, 4), 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny,
+                                                               S.p.nz, F, per[0], per[1], per[2]);
     HIPCHK(hipGetLastError());
   }
   CHK(lw_residual_l2_pair(S, S.P[S.cur], q, warm));
@@ -1709,10 +1735,10 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
 int solver_learning_finish(Solver &S) {
   const int n = S.n;
   std::swap(S.Eprev, S.Eprev2);
-  lw_increment_kernel<<<kLwBlocks, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n,
-                                                      S.epart);
+  lw_increment_kernel<<<kIncBlocks, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n,
+                                                       S.epart);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(S.hepart, S.epart, kLwBlocks * sizeof(double), hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(S.hepart, S.epart, kIncBlocks * sizeof(double), hipMemcpyDeviceToHost,
                         g_stream));
   if (S.cap_e)
     HIPCHK(hipMemcpyAsync(S.cap_e, S.Eprev, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));

@@ -384,7 +384,7 @@ hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) 
     }
     S.nu1 = nu1;
     S.nu2 = nu2;
-    S.coarse_sweeps = 30;
+    S.coarse_sweeps = 10;
     mg_small_vcycle_kernel<<<1, kMgSmallThreads, 0, s>>>(H.u[l], H.f[l], S);
     return hipGetLastError();
   }
