@@ -1007,8 +1007,10 @@ int residual_blocks(int nx, int ny, int nz) {
   return (int)(g.x * g.y * g.z);
 }
 
-// f(offset of (i, j, k), i, j, k, residual) for the planes of this
-// thread's column; the iterate's value at offset o is ld(o).
+// f(offset of (i, j, k), i, j, k, p(i, j, k), residual) for the planes of
+// this thread's column; the iterate's value at offset o is ld(o).  Every
+// load of the column is issued before the first f (planes past nz are
+// loaded clamped and dropped), so no store of f sits between loads.
 template <typename TR, typename LD, typename F>
 __device__ __forceinline__ void residual_column_ld(LD &&ld, const TR *__restrict__ rhs, int nx,
                                                    int ny, int nz, int px, int py, int pz,
@@ -1023,19 +1025,27 @@ __device__ __forceinline__ void residual_column_ld(LD &&ld, const TR *__restrict
   const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
   const int c = i + nx * j;
   const int cim = im + nx * j, cip = ip + nx * j, cjm = i + nx * jm, cjp = i + nx * jp;
-  double pm = ld(c + plane * sor_ghost(k0 - 1, nz, pz));
-  double pc = ld(c + plane * k0);
+  double zc[RZC + 2], xm[RZC], xp[RZC], ym[RZC], yp[RZC], rh[RZC];
+#pragma unroll
+  for (int kk = 0; kk < RZC + 2; ++kk)
+    zc[kk] = ld(c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz));
+#pragma unroll
+  for (int kk = 0; kk < RZC; ++kk) {
+    const size_t o = plane * min(k0 + kk, nz - 1);
+    xm[kk] = ld(cim + o);
+    xp[kk] = ld(cip + o);
+    ym[kk] = ld(cjm + o);
+    yp[kk] = ld(cjp + o);
+    rh[kk] = (double)rhs[c + o];
+  }
 #pragma unroll
   for (int kk = 0; kk < RZC; ++kk) {
     const int k = k0 + kk;
     if (k < nz) {
-      const double pp = ld(c + plane * sor_ghost(k + 1, nz, pz));
       const size_t o = plane * k;
-      const double lap = ox * (ld(cim + o) + ld(cip + o)) + oy * (ld(cjm + o) + ld(cjp + o)) +
-                         oz * (pm + pp) + A * pc;
-      f(c + o, i, j, k, pc, (double)rhs[c + o] - lap);
-      pm = pc;
-      pc = pp;
+      const double lap = ox * (xm[kk] + xp[kk]) + oy * (ym[kk] + yp[kk]) +
+                         oz * (zc[kk] + zc[kk + 2]) + A * zc[kk + 1];
+      f(c + o, i, j, k, zc[kk + 1], rh[kk] - lap);
     }
   }
 }
