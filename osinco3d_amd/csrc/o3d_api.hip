@@ -220,6 +220,10 @@ struct SorRun {
   int sweeps_full = 0;
   int launches_full = 0;
   std::vector<hipEvent_t> *events = nullptr;
+  // Optional: recorded after iteration a.gs of the first batch.  Those
+  // iterations always run in full (they end the solve only on kmax), so a
+  // solve that ends inside its first batch still times them.
+  hipEvent_t mid = nullptr;
 
   hipError_t launch() {
     if constexpr (std::is_same<T, TR>::value) {
@@ -244,6 +248,7 @@ struct SorRun {
     int it = 1, L = 0;
     int batch = std::max(1, first_batch / step);
     int done_before = 0, launches_before = 0;
+    bool mid_set = false;
     while (true) {
       hipEvent_t e0 = (*events)[0], e1 = (*events)[1];
       HIPCHK(hipEventRecord(e0, g_stream));
@@ -254,6 +259,10 @@ struct SorRun {
         HIPCHK(launch());
         ++L;
         it += step;
+        if (mid && !pair && L == a.gs && a.gs <= a.kmax && done_before == 0) {
+          HIPCHK(hipEventRecord(mid, g_stream));
+          mid_set = true;
+        }
         if (it - step > a.kmax) break;  // that launch closes the state
       }
       HIPCHK(hipEventRecord(e1, g_stream));
@@ -271,6 +280,13 @@ struct SorRun {
         if (launched > a.kmax) return fail("SOR state did not close after kmax");
         batch = std::max(1, 8 / step);
         continue;
+      }
+      if (mid_set && done_before == 0 && hst->sweeps >= a.gs) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, mid));
+        ms_full += ms;
+        sweeps_full += a.gs;
+        launches_full += a.gs;
       }
       break;
     }
@@ -1048,6 +1064,7 @@ struct Solver {
   SorState *st = nullptr, *st64 = nullptr, *hst = nullptr;
   std::vector<hipEvent_t> ev;
   int last_sweeps[2] = {0, 0};
+  int last_first = 0;          // sweeps of the previous step's first FP32 correction solve
   double sor_ms[2] = {0, 0};
   int sor_sweeps[2] = {0, 0};
   int sor_launches[2] = {0, 0};
@@ -1277,6 +1294,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     r.st = S.st;
     r.hst = S.hst;
     r.events = &S.ev;
+    r.mid = S.ev[6];
     r.buf[0] = S.E[0];
     r.buf[1] = S.E[1];
     const int kin = std::min(inner_max, p.kmax - total);
@@ -1292,9 +1310,11 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
       r.sa = split_args<float>(r.a, S.E[0], S.E[0] + h, S.R, S.R + h, S.dslot);
     }
     int sweeps = 0;
-    const int first = cycles == 0 && S.last_sweeps[1] > 0
-                          ? std::min(kin, std::max(4, S.last_sweeps[1] - 2))
-                          : std::min(kin, 16);
+    // The first solve of a step usually takes as many sweeps as the
+    // previous step's: launch that many plus the launch that sees the stop,
+    // so the solve ends without a second batch or unused launches.
+    const int first = cycles == 0 && S.last_first > 0 ? std::min(kin + 1, S.last_first + 1)
+                                                      : std::min(kin, 16);
     CHK(r.run(S.omega, first, &sweeps));
     S.sor_ms[1] += r.ms_full;
     S.sor_sweeps[1] += r.sweeps_full;
@@ -1303,6 +1323,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     S.omega = S.hst->omega_final;
     total += sweeps;
     sw32 += sweeps;
+    if (cycles == 0) S.last_first = sweeps;
     ++cycles;
     if (split) HIPCHK(launch_fold_split(S.P[S.cur], S.E[0], p.nx, p.ny, p.nz, g_stream));
     else HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
@@ -1902,7 +1923,7 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     rc = fail("hipHostMalloc failed");
   if (!rc && hipHostMalloc((void **)&S->hst, sizeof(SorState)) != hipSuccess)
     rc = fail("hipHostMalloc failed");
-  S->ev.resize(6);
+  S->ev.resize(7);
   for (auto &e : S->ev)
     if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail("hipEventCreate failed");
   if (rc) { delete S; return rc; }
