@@ -1282,7 +1282,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
         // The start of this correction solve: the coarse-grid correction of
         // its residual (cycle 0 starts from a guess that already had one).
         HIPCHK(mg_pool_split(S.R, S.mg.f[0], p.nx, p.ny, p.nz, g_stream));
-        HIPCHK(mg_solve(S.mg, p.coarse_cycles, 2, 2, g_stream));
+        HIPCHK(mg_solve(S.mg, p.coarse_cycles, 1, 1, g_stream));
         HIPCHK(mg_prolong_split(S.E[0], S.mg.u[0], p.nx, p.ny, p.nz, per, g_stream));
       }
     }
@@ -1736,7 +1736,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
                                                            S.p.nz, F, 1.0f);
     HIPCHK(hipGetLastError());
     if (nn_coarse) HIPCHK(mg_seed_from_level1(S.mg, g_stream));
-    HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 2, 2, g_stream, !nn_coarse));
+    HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 1, 1, g_stream, !nn_coarse));
     dim3 grid((S.p.nx / 2 + 63) / 64, (S.p.ny + 3) / 4, S.p.nz);
     mg_prolong_fine_kernel<<<grid, dim3(64, 4), 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny,
                                                                S.p.nz, F, per[0], per[1], per[2]);
