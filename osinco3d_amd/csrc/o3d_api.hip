@@ -1081,6 +1081,8 @@ struct Solver {
   MgHierarchy mg;              // coarse-grid correction (coarse_cycles > 0)
   float *mgbuf = nullptr;
   double *npart = nullptr;     // reduction partials (residual norms, sums of squares)
+  double *p0part = nullptr;    // per-block norms of the residual of p0_of (guess pass)
+  const double *p0_of = nullptr;  // the pressure those norms belong to (null: none)
   double *hnpart = nullptr;    // pinned copy
   double *epart = nullptr;     // sum-of-squares partials of e_prev
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
@@ -1593,6 +1595,7 @@ int lw_alloc(Solver &S) {
   // (kLwBlocks partials, copied whole to the host).
   const int np = std::max(4 * residual_blocks(S.p.nx, S.p.ny, S.p.nz) + 4, 2 * kLwBlocks);
   CHK(S.alloc(&S.npart, np));
+  CHK(S.alloc(&S.p0part, 2 * residual_blocks(S.p.nx, S.p.ny, S.p.nz)));
   HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
   CHK(S.alloc(&S.epart, kIncBlocks));
   HIPCHK(hipHostMalloc((void **)&S.hepart, kIncBlocks * sizeof(double)));
@@ -1615,15 +1618,20 @@ int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *o
   // In the mixed form with colour-split sweeps, the residual of p1 (the
   // guess) is kept for the first correction solve (S.r_ready).
   S.r_ready = S.mixed && S.dslot != nullptr;
+  // The norms of p0's residual, when the guess pass of this step formed them.
+  const bool have0 = S.p0_of != nullptr && S.p0_of == p0;
+  S.p0_of = nullptr;
   if (S.mixed)
-    HIPCHK(launch_residual_norm_pair<float>(p0, p1, (const float *)S.rhs, S.npart, fin, S.p.nx,
-                                            S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz,
-                                            S.sc.A, g_stream, S.r_ready ? S.R : nullptr,
-                                            S.r_ready ? S.E[0] : nullptr));
+    HIPCHK(launch_residual_norm_pair<float>(have0 ? nullptr : p0, p1, (const float *)S.rhs,
+                                            S.npart, fin, S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox,
+                                            S.sc.oy, S.sc.oz, S.sc.A, g_stream,
+                                            S.r_ready ? S.R : nullptr,
+                                            S.r_ready ? S.E[0] : nullptr, S.p0part));
   else
-    HIPCHK(launch_residual_norm_pair<double>(p0, p1, (const double *)S.rhs, S.npart, fin,
-                                             S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox, S.sc.oy,
-                                             S.sc.oz, S.sc.A, g_stream));
+    HIPCHK(launch_residual_norm_pair<double>(have0 ? nullptr : p0, p1, (const double *)S.rhs,
+                                             S.npart, fin, S.p.nx, S.p.ny, S.p.nz, per, S.sc.ox,
+                                             S.sc.oy, S.sc.oz, S.sc.A, g_stream, nullptr,
+                                             nullptr, S.p0part));
   HIPCHK(hipMemcpyAsync(S.hnpart, fin, 4 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
   out[0] = sqrt(S.hnpart[1]);
@@ -1637,19 +1645,24 @@ int lw_residual_l2_pair(Solver &S, const double *p0, const double *p1, double *o
 // the residual of p_prev.
 int solver_learning_prepare(Solver &S) {
   CHK(lw_alloc(S));
+  S.p0_of = nullptr;
   const int n = S.n;
   int per[3];
   solver_periodic(S.kind, per);
   if (S.hist >= 1) {
-    // The extrapolation into the spare buffer and its residual, one pass.
+    // The extrapolation into the spare buffer and its residual, one pass,
+    // with the residual norms of p_prev for the fallback check.
     if (S.mixed)
       HIPCHK(launch_guess_residual<float>(S.P[S.cur], S.Eprev, (const float *)S.rhs,
                                           S.P[1 - S.cur], S.Rl, S.p.nx, S.p.ny, S.p.nz, per,
-                                          S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
+                                          S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream,
+                                          S.p0part));
     else
       HIPCHK(launch_guess_residual<double>(S.P[S.cur], S.Eprev, (const double *)S.rhs,
                                            S.P[1 - S.cur], S.Rl, S.p.nx, S.p.ny, S.p.nz, per,
-                                           S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream));
+                                           S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream,
+                                           S.p0part));
+    S.p0_of = S.P[S.cur];
   } else if (S.mixed) {
     // The residual kernel also zeroes its second output; Out is scratch here.
     HIPCHK(launch_residual_f32<float>(S.P[S.cur], (const float *)S.rhs, S.Rl, S.Out, S.p.nx,

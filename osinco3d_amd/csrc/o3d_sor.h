@@ -116,10 +116,13 @@ hipError_t launch_residual_f32(const double *p, const TR *rhs, float *r, float *
 hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s);
 // q = p + e and r = rhs - L q (FP32) in one pass (the extrapolated guess and
 // its residual), equal to forming q and then calling launch_residual_f32.
+// With part0 (2 * residual_blocks doubles), also the per-block (max |r0|,
+// sum r0^2) of r0 = rhs - L p, for launch_residual_norm_pair without p0.
 template <typename TR>
 hipError_t launch_guess_residual(const double *p, const float *e, const TR *rhs, double *q,
                                  float *r, int nx, int ny, int nz, const int per[3], double ox,
-                                 double oy, double oz, double A, hipStream_t s);
+                                 double oy, double oz, double A, hipStream_t s,
+                                 double *part0 = nullptr);
 // The same with r and e colour-split (red half-array, then black).
 template <typename TR>
 hipError_t launch_residual_split(const double *p, const TR *rhs, float *r, float *e, int nx,
@@ -130,13 +133,15 @@ hipError_t launch_fold_split(double *p, const float *e, int nx, int ny, int nz, 
 // out (device, 4 doubles) = (max |r0|, sum r0^2, max |r1|, sum r1^2).  part
 // holds 4 * residual_blocks(nx, ny, nz) doubles.  With r1s (nx even), the
 // residual of p1 is also stored colour-split in FP32 and e1s is zeroed, as
-// launch_residual_split would do for p1.
+// launch_residual_split would do for p1.  With p0 null, the first pair
+// comes from the part0 records launch_guess_residual wrote for the same p0
+// and rhs (bit-identical to computing them here).
 template <typename TR>
 hipError_t launch_residual_norm_pair(const double *p0, const double *p1, const TR *rhs,
                                      double *part, double *out, int nx, int ny, int nz,
                                      const int per[3], double ox, double oy, double oz, double A,
                                      hipStream_t s, float *r1s = nullptr,
-                                     float *e1s = nullptr);
+                                     float *e1s = nullptr, const double *part0 = nullptr);
 int residual_blocks(int nx, int ny, int nz);
 
 template <typename T>

@@ -1063,30 +1063,88 @@ __device__ __forceinline__ void residual_column(const double *__restrict__ p,
 
 // q = p + e (the extrapolated guess) and r = rhs - L q stored in FP32, in one
 // pass: the residual is formed from p + e as loaded, with the operation
-// order of residual_f32_kernel, so both equal the two-kernel result.
+// order of residual_f32_kernel, so both equal the two-kernel result.  With
+// part0, the block's (max |r0|, sum r0^2) of r0 = rhs - L p as well (from the
+// same loads), with the arithmetic of residual_norm_pair_kernel's p0 part.
 template <typename TR>
 __global__ void __launch_bounds__(RNT) guess_residual_kernel(const double *__restrict__ p,
                                                              const float *__restrict__ e,
                                                              const TR *__restrict__ rhs,
                                                              double *__restrict__ q,
-                                                             float *__restrict__ r, int nx,
+                                                             float *__restrict__ r,
+                                                             double *__restrict__ part0, int nx,
                                                              int ny, int nz, int px, int py,
                                                              int pz, double ox, double oy,
                                                              double oz, double A) {
-  residual_column_ld([&](size_t o) { return p[o] + 1.0 * (double)e[o]; }, rhs, nx, ny, nz, px,
-                     py, pz, ox, oy, oz, A,
-                     [&](size_t o, int, int, int, double qc, double res) {
-                       q[o] = qc;
-                       r[o] = (float)res;
-                     });
+  __shared__ double sc[RNT / O3D_WAVE];
+  double m0 = 0.0, s0 = 0.0;
+  const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
+  const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
+  if (i < nx && j < ny) {
+    const int k0 = blockIdx.z * RZC;
+    const size_t plane = (size_t)nx * ny;
+    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+    const int c = i + nx * j;
+    const int cl[4] = {im + nx * j, ip + nx * j, i + nx * jm, i + nx * jp};
+    double zp[RZC + 2], lp[4][RZC], rh[RZC];
+    float ze[RZC + 2], le[4][RZC];
+#pragma unroll
+    for (int kk = 0; kk < RZC + 2; ++kk) {
+      const size_t o = c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz);
+      zp[kk] = p[o];
+      ze[kk] = e[o];
+    }
+#pragma unroll
+    for (int kk = 0; kk < RZC; ++kk) {
+      const size_t o = plane * min(k0 + kk, nz - 1);
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        lp[d][kk] = p[cl[d] + o];
+        le[d][kk] = e[cl[d] + o];
+      }
+      rh[kk] = (double)rhs[c + o];
+    }
+    auto qv = [](double a, float b) { return a + 1.0 * (double)b; };
+#pragma unroll
+    for (int kk = 0; kk < RZC; ++kk) {
+      const int k = k0 + kk;
+      if (k < nz) {
+        const size_t o = c + plane * k;
+        const double lapq = ox * (qv(lp[0][kk], le[0][kk]) + qv(lp[1][kk], le[1][kk])) +
+                            oy * (qv(lp[2][kk], le[2][kk]) + qv(lp[3][kk], le[3][kk])) +
+                            oz * (qv(zp[kk], ze[kk]) + qv(zp[kk + 2], ze[kk + 2])) +
+                            A * qv(zp[kk + 1], ze[kk + 1]);
+        q[o] = qv(zp[kk + 1], ze[kk + 1]);
+        r[o] = (float)(rh[kk] - lapq);
+        if (part0) {
+          const double lapp = ox * (lp[0][kk] + lp[1][kk]) + oy * (lp[2][kk] + lp[3][kk]) +
+                              oz * (zp[kk] + zp[kk + 2]) + A * zp[kk + 1];
+          const double r0 = rh[kk] - lapp;
+          m0 = fmax(m0, fabs(r0));
+          s0 += r0 * r0;
+        }
+      }
+    }
+  }
+  if (part0) {
+    m0 = block_max<RNT / O3D_WAVE>(m0, sc);
+    s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
+    if (threadIdx.x == 0) {
+      const size_t b =
+          blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+      part0[2 * b] = m0;
+      part0[2 * b + 1] = s0;
+    }
+  }
 }
 
 template <typename TR>
 hipError_t launch_guess_residual(const double *p, const float *e, const TR *rhs, double *q,
                                  float *r, int nx, int ny, int nz, const int per[3], double ox,
-                                 double oy, double oz, double A, hipStream_t s) {
+                                 double oy, double oz, double A, hipStream_t s, double *part0) {
   guess_residual_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
-      p, e, rhs, q, r, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
+      p, e, rhs, q, r, part0, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
   return hipGetLastError();
 }
 
@@ -1185,11 +1243,12 @@ __global__ void __launch_bounds__(RNT) residual_norm_pair_kernel(
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0, m1 = 0.0, s1 = 0.0;
   const size_t half = (size_t)nx * ny * nz / 2;
-  residual_column(p0, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
-                  [&](size_t, int, int, int, double r) {
-                    m0 = fmax(m0, fabs(r));
-                    s0 += r * r;
-                  });
+  if (p0)
+    residual_column(p0, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
+                    [&](size_t, int, int, int, double r) {
+                      m0 = fmax(m0, fabs(r));
+                      s0 += r * r;
+                    });
   residual_column(p1, rhs, nx, ny, nz, px, py, pz, ox, oy, oz, A,
                   [&](size_t, int i, int j, int k, double r) {
                     m1 = fmax(m1, fabs(r));
@@ -1215,14 +1274,16 @@ __global__ void __launch_bounds__(RNT) residual_norm_pair_kernel(
   }
 }
 
-// out[0..3] = (max, sum, max, sum) over the block records, in a fixed order.
+// out[0..3] = (max, sum, max, sum) over the block records, in a fixed order;
+// with part0 the first pair comes from its (max, sum) records instead.
 __global__ void __launch_bounds__(1024) residual_norm_final_kernel(const double *__restrict__ part,
+                                                                  const double *__restrict__ part0,
                                                                   int nb, double *out) {
   __shared__ double sc[1024 / O3D_WAVE];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    v[0] = fmax(v[0], part[4 * b]);
-    v[1] += part[4 * b + 1];
+    v[0] = fmax(v[0], part0 ? part0[2 * b] : part[4 * b]);
+    v[1] += part0 ? part0[2 * b + 1] : part[4 * b + 1];
     v[2] = fmax(v[2], part[4 * b + 2]);
     v[3] += part[4 * b + 3];
   }
@@ -1238,11 +1299,14 @@ template <typename TR>
 hipError_t launch_residual_norm_pair(const double *p0, const double *p1, const TR *rhs,
                                      double *part, double *out, int nx, int ny, int nz,
                                      const int per[3], double ox, double oy, double oz, double A,
-                                     hipStream_t s, float *r1s, float *e1s) {
+                                     hipStream_t s, float *r1s, float *e1s,
+                                     const double *part0) {
   if (r1s && (nx % 2 != 0 || !e1s)) return hipErrorInvalidValue;
+  if (!p0 && !part0) return hipErrorInvalidValue;
   residual_norm_pair_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
       p0, p1, rhs, part, r1s, e1s, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz, A);
-  residual_norm_final_kernel<<<1, 1024, 0, s>>>(part, residual_blocks(nx, ny, nz), out);
+  residual_norm_final_kernel<<<1, 1024, 0, s>>>(part, p0 ? nullptr : part0,
+                                                residual_blocks(nx, ny, nz), out);
   return hipGetLastError();
 }
 
@@ -1304,10 +1368,12 @@ template hipError_t launch_residual_f32<float>(const double *, const float *, fl
                                                double, hipStream_t);
 template hipError_t launch_guess_residual<float>(const double *, const float *, const float *,
                                                  double *, float *, int, int, int, const int *,
-                                                 double, double, double, double, hipStream_t);
+                                                 double, double, double, double, hipStream_t,
+                                                 double *);
 template hipError_t launch_guess_residual<double>(const double *, const float *, const double *,
                                                   double *, float *, int, int, int, const int *,
-                                                  double, double, double, double, hipStream_t);
+                                                  double, double, double, double, hipStream_t,
+                                                  double *);
 template hipError_t launch_residual_split<float>(const double *, const float *, float *, float *,
                                                  int, int, int, const int *, double, double,
                                                  double, double, hipStream_t);
@@ -1315,11 +1381,12 @@ template hipError_t launch_residual_norm_pair<double>(const double *, const doub
                                                       const double *, double *, double *, int,
                                                       int, int, const int *, double, double,
                                                       double, double, hipStream_t, float *,
-                                                      float *);
+                                                      float *, const double *);
 template hipError_t launch_residual_norm_pair<float>(const double *, const double *,
                                                      const float *, double *, double *, int, int,
                                                      int, const int *, double, double, double,
-                                                     double, hipStream_t, float *, float *);
+                                                     double, hipStream_t, float *, float *,
+                                                     const double *);
 template hipError_t launch_copy<double>(double *, const double *, int, hipStream_t);
 template hipError_t launch_copy<float>(float *, const float *, int, hipStream_t);
 template hipError_t launch_convert<float, double>(float *, const double *, int, hipStream_t);
