@@ -1786,9 +1786,6 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   HIPCHK(hipEventRecord(S.ev[2], g_stream));
   CHK(solver_predict<T>(S, itime));
   CHK(solver_divergence<T>(S, S.up, (T *)S.rhs, p.dt, S.part_div));
-  HIPCHK(launch_finalize_partials(S.part_div, S.tl_div.blocks(), 1, 0, S.fin + 0, g_stream));
-  if (p.iles == 1)
-    HIPCHK(launch_finalize_partials(S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5, g_stream));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
   const bool guess =
       S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE || p.warm_start == O3D_WARM_EXTRAPOLATE2;
@@ -1811,11 +1808,19 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   HIPCHK(hipEventRecord(S.ev[5], g_stream));
   if (learning) CHK(solver_learning_finish(S));
   CHK(solver_correct<T>(S));
-  for (int c = 0; c < 3; ++c)
-    HIPCHK(launch_finalize_partials(S.part_cor, S.tl_cor.blocks(), 3, c, S.fin + 2 + c,
-                                    g_stream));
   CHK(solver_divergence<T>(S, S.u, nullptr, 1.0, S.part_div2));
-  HIPCHK(launch_finalize_partials(S.part_div2, S.tl_div.blocks(), 1, 0, S.fin + 1, g_stream));
+  // The step's statistics (their partials stay untouched until here): the
+  // rhs divergence, the diagnostic divergence, the three corrected
+  // components and, with LES, nu_t.
+  {
+    FinalizeSet fs[6];
+    int nf = 0;
+    fs[nf++] = {S.part_div, S.tl_div.blocks(), 1, 0, S.fin + 0};
+    fs[nf++] = {S.part_div2, S.tl_div.blocks(), 1, 0, S.fin + 1};
+    for (int c = 0; c < 3; ++c) fs[nf++] = {S.part_cor, S.tl_cor.blocks(), 3, c, S.fin + 2 + c};
+    if (p.iles == 1) fs[nf++] = {S.part_nut, S.tl.blocks(), 1, 0, S.fin + 5};
+    HIPCHK(launch_finalize_sets(fs, nf, g_stream));
+  }
   HIPCHK(hipEventRecord(S.ev[3], g_stream));
   HIPCHK(hipMemcpyAsync(S.hfin, S.fin, 6 * sizeof(FieldPartial), hipMemcpyDeviceToHost,
                         g_stream));

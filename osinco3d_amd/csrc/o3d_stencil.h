@@ -136,6 +136,15 @@ hipError_t launch_field_stats(const T *f, int n, FieldPartial *part, int nblocks
 // block) starting at `first` into out[0].
 hipError_t launch_finalize_partials(const FieldPartial *part, int nparts, int stride, int first,
                                     FieldPartial *out, hipStream_t s);
+// Several record sets finalized in one launch (one block each), each as
+// launch_finalize_partials does.
+struct FinalizeSet {
+  const FieldPartial *part;
+  int nparts, stride, first;
+  FieldPartial *out;
+};
+constexpr int kMaxFinalizeSets = 8;
+hipError_t launch_finalize_sets(const FinalizeSet *sets, int n, hipStream_t s);
 hipError_t launch_finalize_sums(const double *part, int nparts, int width, double *out,
                                 hipStream_t s);
 

@@ -1051,9 +1051,8 @@ hipError_t launch_field_stats(const T *f, int n, FieldPartial *part, int nblocks
 }
 
 // ---------------------------------------------------------------- finalize
-__global__ void __launch_bounds__(1024) finalize_partials_kernel(const FieldPartial *part,
-                                                                 int nparts, int stride,
-                                                                 int first, FieldPartial *out) {
+__device__ __forceinline__ void finalize_one(const FieldPartial *part, int nparts, int stride,
+                                             int first, FieldPartial *out) {
   __shared__ double sv[16];
   __shared__ int si[16];
   FieldPartial st;
@@ -1081,9 +1080,33 @@ __global__ void __launch_bounds__(1024) finalize_partials_kernel(const FieldPart
   }
 }
 
+__global__ void __launch_bounds__(1024) finalize_partials_kernel(const FieldPartial *part,
+                                                                 int nparts, int stride,
+                                                                 int first, FieldPartial *out) {
+  finalize_one(part, nparts, stride, first, out);
+}
+
 hipError_t launch_finalize_partials(const FieldPartial *part, int nparts, int stride, int first,
                                     FieldPartial *out, hipStream_t s) {
   finalize_partials_kernel<<<1, 1024, 0, s>>>(part, nparts, stride, first, out);
+  return hipGetLastError();
+}
+
+struct FinalizeSets {
+  FinalizeSet s[kMaxFinalizeSets];
+};
+
+// One block per set.
+__global__ void __launch_bounds__(1024) finalize_sets_kernel(FinalizeSets f) {
+  const FinalizeSet &t = f.s[blockIdx.x];
+  finalize_one(t.part, t.nparts, t.stride, t.first, t.out);
+}
+
+hipError_t launch_finalize_sets(const FinalizeSet *sets, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxFinalizeSets) return hipErrorInvalidValue;
+  FinalizeSets f{};
+  for (int q = 0; q < n; ++q) f.s[q] = sets[q];
+  finalize_sets_kernel<<<n, 1024, 0, s>>>(f);
   return hipGetLastError();
 }
 
