@@ -186,8 +186,9 @@ inline int conv_splits(int V, int co, int kp) {
 
 template <typename TO, int NT>
 hipError_t launch_conv_nt(ConvArgs<TO> a, int V, float *ws, size_t ws_n, hipStream_t s) {
-  // Four voxel tiles per wave once the grid still has >= 2048 blocks; on
-  // coarse grids, split K instead.
+  // Four voxel tiles per wave (four independent gathers per K chunk) once
+  // the grid still has >= 1024 blocks (4096 waves); on coarse grids, split K
+  // instead.
   constexpr int kMV = 4;
   const int gy = a.co / (16 * NT);
   const int split = ws ? conv_splits(V, a.co, a.kp) : 1;
@@ -197,7 +198,7 @@ hipError_t launch_conv_nt(ConvArgs<TO> a, int V, float *ws, size_t ws_n, hipStre
     dim3 grid((V + 63) / 64, gy, split);
     conv_kernel<TO, NT, 1><<<grid, 256, 0, s>>>(a);
     conv_reduce_kernel<TO><<<(int)(((size_t)V * a.co + 255) / 256), 256, 0, s>>>(a, V);
-  } else if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 2048) {
+  } else if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 1024) {
     dim3 grid((V + 64 * kMV - 1) / (64 * kMV), gy);
     conv_kernel<TO, NT, kMV><<<grid, 256, 0, s>>>(a);
   } else {

@@ -25,3 +25,14 @@ def ref():
     if r is None:
         pytest.skip("oracle/_ref not built (reference tree unavailable at build time)")
     return r
+
+
+@pytest.hookimpl(trylast=True)
+def pytest_collection_modifyitems(config, items):
+    """torch has to initialise the HIP runtime before libo3d is loaded into
+    the process (loaded first, libo3d leaves torch seeing no device), so a
+    run that selects GPU tests initialises it once, before any test runs,
+    whatever the selection and order."""
+    if any(item.get_closest_marker("gpu") for item in items):
+        import torch
+        torch.cuda.is_available()
