@@ -70,9 +70,11 @@ def parse_args():
                     help="V-cycles of the coarse-grid correction (forms mg, nn)")
     ap.add_argument("--smooth-sweeps", type=int, default=8,
                     help="Gauss-Seidel sweeps opening the FP32 solve (forms mg, nn)")
-    ap.add_argument("--workload", choices=("tgv", "mixing"), default="tgv",
+    ap.add_argument("--workload", choices=("tgv", "mixing", "poisson"), default="tgv",
                     help="tgv: the headline Taylor-Green case; mixing: the 384x256x128 LES "
-                         "temporal mixing layer (free-slip y, AB2), no CPU baseline")
+                         "temporal mixing layer (free-slip y, AB2), no CPU baseline; "
+                         "poisson: one n^3 pressure solve from zero, FP64 SOR against the "
+                         "multigrid solver (multigrid = 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-kmax", type=int, default=10)
     ap.add_argument("--model", default=os.path.join(ROOT, "osinco3d_amd", "warmstart_weights.npy"))
@@ -306,6 +308,60 @@ def cpu_baseline(n, kmax, iters_per_step, ratio):
     }
 
 
+def run_poisson(args):
+    """One periodic n^3 pressure solve from zero by the FP64 red-black SOR
+    (o3d_poisson, eps 1e-8, omega 2/(1+sin(pi/n))) and by the multigrid solver
+    (o3d_poisson_multigrid, npre 5, npost 4, max |r| < 1e-6).  The exact
+    solution is the Taylor-Green pressure (cos 2x + cos 2y)(cos 2z + 2)/16 on
+    [0, 2pi)^3; its rhs is the 7-point Laplacian in closed form (each cosine
+    of wavenumber 2 is an eigenfunction with eigenvalue -(2 - 2 cos 2h)/h^2).
+    Fields are device tensors; the errors are max |p - q| after removing the
+    mean (the periodic problem fixes p up to a constant).  Each solver runs
+    twice and the second run is reported (the first includes one-time set-up)."""
+    import math
+
+    import torch
+    from osinco3d_amd import capi
+    n = args.n
+    h = 2 * math.pi / n
+    lam = (2 - 2 * math.cos(2 * h)) / h ** 2
+    x = torch.arange(n, dtype=torch.float64, device="cuda") * h
+    cx = torch.cos(2 * x)
+    A = cx[None, None, :] + cx[None, :, None]
+    B = cx[:, None, None] + 2.0
+    q = A * B / 16
+    rhs = ((-lam * A * B - lam * A * cx[:, None, None]) / 16).contiguous()
+    q = q - q.mean()
+    d = (h, h, h)
+    out = {}
+    for name in ("sor", "multigrid", "sor", "multigrid"):  # the second pair is timed
+        p = torch.zeros((n, n, n), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if name == "sor":
+            it, dm, rs, _ = capi.poisson(p, rhs, d, 2 / (1 + math.sin(math.pi / n)), 1e-8, 50000,
+                                         0, "0000")
+            info = {"sweeps": it, "dmax": dm, "reason": rs}
+        else:
+            cyc, rmax = capi.poisson_multigrid(p, rhs, d, "0000", npre=5, npost=4, tol=1e-6,
+                                               max_cycles=200)
+            info = {"cycles": cyc, "max_residual": rmax}
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        err = float((p - p.mean() - q).abs().max())
+        info.update({"ms": ms, "max_error": err})
+        out[name] = info
+        print(f"[bench] poisson {name}: {ms:.1f} ms, {info}", file=sys.stderr, flush=True)
+    line = {"metric": f"periodic {n}^3 pressure solve from zero, ms (FP64 SOR vs multigrid)",
+            "value": out["multigrid"]["ms"], "unit": "ms", "n_gpus": 1, "steps": 1,
+            "warmup": 0, "ms_per_step": out["multigrid"]["ms"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+            "data": "synthetic (Taylor-Green pressure, closed-form rhs)",
+            "config": {"workload": f"Poisson solve {n}^3 periodic from zero"},
+            "speedup_vs_sor": out["sor"]["ms"] / out["multigrid"]["ms"], "solvers": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -319,6 +375,9 @@ def main():
         sys.exit(1)
     from osinco3d_amd.config import mixing_layer_config, tgv_config
 
+    if args.workload == "poisson":
+        run_poisson(args)
+        return
     init = None
     if args.workload == "mixing":
         from osinco3d_amd.solver import mixing_layer
