@@ -1266,6 +1266,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   int total = 0, sw32 = 0, sw64 = 0, cycles = 0;
   int reason = 2;
   double judge_prev = 1609.0, dlast = 0.0;
+  int capped = 0;  // consecutive correction solves that ended on their sweep cap
   while (total < p.kmax) {
     int per[3];
     solver_periodic(S.kind, per);
@@ -1329,7 +1330,12 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     ++cycles;
     if (split) HIPCHK(launch_fold_split(S.P[S.cur], S.E[0], p.nx, p.ny, p.nz, g_stream));
     else HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
-    if (rin == 2 && total < p.kmax) continue;  // cycle cap: refresh the residual
+    // A solve that ended on its sweep cap continues from a refreshed
+    // residual; every fourth such solve in a row is followed by the judge
+    // sweep anyway, so a problem that cannot converge (an rhs incompatible
+    // with the free-slip mirror) still reaches the stagnation rule.
+    if (rin == 2 && total < p.kmax && ++capped < 4) continue;
+    capped = 0;
     // FP64 judge sweep (applied, like the reference's last sweep).
     SorRun<double, float> j;
     j.st = S.st64;

@@ -408,6 +408,28 @@ hipError_t vcycle(const MgHierarchy &H, int l, int nu1, int nu2, hipStream_t s) 
   return smooth(H, l, nu2, s);
 }
 
+// Per-block sums of f (at most kMgMeanBlocks blocks), then f -= their total / n:
+// the coarse problem made compatible with its null space (the constants).
+constexpr int kMgMeanBlocks = 256;
+__global__ void __launch_bounds__(256) mg_sum_kernel(const float *__restrict__ f, int n,
+                                                     double *__restrict__ part) {
+  __shared__ double sc[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += (double)f[i];
+  s = block_sum<4>(s, sc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) mg_center_kernel(float *__restrict__ f, int n,
+                                                        const double *__restrict__ part,
+                                                        int nparts) {
+  __shared__ double sc[4];
+  double s = threadIdx.x < nparts ? part[threadIdx.x] : 0.0;
+  s = block_sum<4>(s, sc);
+  const float m = (float)(s / n);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) f[i] -= m;
+}
+
 __global__ void mg_zero_kernel(float *u, int n) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < n) u[idx] = 0.f;
@@ -494,6 +516,17 @@ hipError_t mg_solve(const MgHierarchy &H, int cycles, int nu1, int nu2, hipStrea
   if (H.levels < 1) return hipErrorInvalidValue;
   if ((long long)H.m[0][0] * H.m[0][1] * H.m[0][2] >= (1LL << 31)) return hipErrorInvalidValue;
   const int n0 = H.m[0][0] * H.m[0][1] * H.m[0][2];
+  if (!(H.per[0] && H.per[1] && H.per[2]) && n0 >= 2) {
+    // With a mirrored axis the fine residual need not have zero mean (an
+    // rhs the mirror cannot satisfy); only its zero-mean part is solvable
+    // here.  On all-periodic grids the mean is zero up to rounding.  The
+    // level-0 residual array is free: restriction forms residuals on the fly.
+    // At most n0 / 2 partials: they fit the n0 floats of that array.
+    double *part = reinterpret_cast<double *>(H.r[0]);
+    const int nb = std::max(1, std::min(kMgMeanBlocks, n0 / 2));
+    mg_sum_kernel<<<nb, 256, 0, s>>>(H.f[0], n0, part);
+    mg_center_kernel<<<nb, 256, 0, s>>>(H.f[0], n0, part, nb);
+  }
   if (from_zero) mg_zero_kernel<<<(n0 + 255) / 256, 256, 0, s>>>(H.u[0], n0);
   hipError_t e = hipGetLastError();
   for (int c = 0; c < cycles && e == hipSuccess; ++c) e = vcycle(H, 0, nu1, nu2, s);
