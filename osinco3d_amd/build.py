@@ -83,8 +83,9 @@ def build_fortran(verbose: bool = False) -> str:
 
 
 def build_oracle(verbose: bool = False, reference: str = "/root/reference") -> None:
-    """The checker: the C restatement always, the reference modules when the
-    reference tree is present (this container, not the GPU box)."""
+    """The oracle: the C restatement always, the reference modules (and the
+    Fortran drop-in built against them) when the reference sources are
+    present at build time."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "oracle"], verbose)
     if os.path.isdir(os.path.join(reference, "src")) and shutil.which("make"):
