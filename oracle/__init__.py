@@ -1,4 +1,4 @@
-"""Test infrastructure only: the checker for the osinco3d time-step path.
+"""Test infrastructure only: the oracle of the osinco3d time-step path.
 
 Two CPU implementations live here, both used only by ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``:

@@ -1,5 +1,5 @@
 /* Test infrastructure only: CPU restatement of the reference osinco3d
- * time-step path (the checker).  Nothing in the product links or calls it;
+ * time-step path (the oracle).  Nothing in the product links or calls it;
  * the tests, smoke() and bench.py's cpu_baseline leg load it via ctypes.
  *
  * Every expression keeps the reference's operation order, so that without

@@ -1,6 +1,6 @@
 /* Test infrastructure only: a CPU restatement of the reference osinco3d
- * time-step path, used by the tests, smoke() and the CPU baseline as the
- * checker.  It is never linked into, called by or shipped with the product.
+ * time-step path (the oracle), used by the tests, smoke() and the CPU
+ * baseline.  It is never linked into, called by or shipped with the product.
  *
  * Arrays are float64, column-major (x fastest), 0-based: f[i + nx*(j + ny*k)].
  * bc[axis]: 0 = periodic, 1 = free-slip (src/initialization.f90:76-78).

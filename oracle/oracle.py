@@ -1,4 +1,4 @@
-"""ctypes bindings for the checker libraries (test infrastructure only).
+"""ctypes bindings for the oracle libraries (test infrastructure only).
 
 Fields are numpy float64 arrays of shape (nz, ny, nx), C-contiguous, so that
 memory order is the reference's column-major (nx, ny, nz) with x fastest.

@@ -1,4 +1,4 @@
-! Test infrastructure only (checker, never the product path).
+! Test infrastructure only (the oracle, never the product path).
 !
 ! bind(C) wrappers around the reference osinco3d Fortran modules, compiled from
 ! the reference sources in place by oracle/Makefile into oracle/_ref/.  They let

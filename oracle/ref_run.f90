@@ -1,4 +1,4 @@
-! Test infrastructure only (checker and CPU baseline, never the product path).
+! Test infrastructure only (the oracle and the CPU baseline, never the product path).
 !
 ! Runs the reference osinco3d time loop on the CPU with the reference's own
 ! modules, compiled from the reference sources in place by oracle/Makefile.

@@ -5,7 +5,7 @@
 // guess is averaged over 2^3 blocks, this module solves the 7-point Poisson
 // problem of those averages (spacing 2h) approximately, and the solver adds
 // the trilinear interpolation of the result to the guess.  The SOR solve
-// that follows is still the judge of convergence (same stop rule), so the
+// that follows still decides convergence (same stop rule), so the
 // correction only changes where SOR starts.  The grids are small (128^3 and
 // below at 256^3): the per-level kernels give each thread one cell of the
 // colour being relaxed (or a pair of cells), and the levels of at most 16^3

@@ -1,5 +1,5 @@
 """Shared test helpers: seeded fields and CPU time steppers built from the
-oracle (checker only)."""
+oracle (used only to check results)."""
 from __future__ import annotations
 
 import numpy as np
