@@ -1388,6 +1388,52 @@ __global__ void lw_pool_kernel(const float *__restrict__ in, float *__restrict__
   out[I] = s * (scale / (float)(F * F * F));
 }
 
+// The same for F = 2 and 4: unrolled, each row segment one 8- or 16-byte
+// load (i0 is a multiple of F), summed in the same order (bit-identical).
+template <int F>
+__global__ void __launch_bounds__(256) lw_pool_f_kernel(const float *__restrict__ in,
+                                                        float *__restrict__ out, int nx, int ny,
+                                                        int nz, float scale) {
+  const int mx = nx / F, my = ny / F, mz = nz / F;
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= mx * my * mz) return;
+  const int i0 = (I % mx) * F, j0 = ((I / mx) % my) * F, k0 = (I / (mx * my)) * F;
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < F; ++k)
+#pragma unroll
+    for (int j = 0; j < F; ++j) {
+      const float *row = in + (size_t)i0 + (size_t)nx * ((j0 + j) + (size_t)ny * (k0 + k));
+      if constexpr (F == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(row);
+        s += v.x;
+        s += v.y;
+        s += v.z;
+        s += v.w;
+      } else {
+        const float2 v = *reinterpret_cast<const float2 *>(row);
+        s += v.x;
+        s += v.y;
+      }
+    }
+  out[I] = s * (scale / (float)(F * F * F));
+}
+
+// out = the F^3 block averages of in, times scale.
+hipError_t launch_lw_pool(const float *in, float *out, int nx, int ny, int nz, int F,
+                          float scale) {
+  const int nc = (nx / F) * (ny / F) * (nz / F);
+  const int blocks = (nc + 255) / 256;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(in);
+  if (F == 4 && nx % 4 == 0 && a % 16 == 0)
+    lw_pool_f_kernel<4><<<blocks, 256, 0, g_stream>>>(in, out, nx, ny, nz, scale);
+  else if (F == 2 && nx % 2 == 0 && a % 8 == 0)
+    lw_pool_f_kernel<2><<<blocks, 256, 0, g_stream>>>(in, out, nx, ny, nz, scale);
+  else
+    lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(in, out, nx, ny, nz, F, scale);
+  return hipGetLastError();
+}
+
 // Position of fine node i on the coarse grid of cell averages: lower cell and
 // weight of the upper one; periodic axes wrap, others clamp.
 __device__ __forceinline__ void lw_axis(int i, int F, int m, int per, int &c0, int &c1, float &w) {
@@ -1566,13 +1612,9 @@ int lw_features(LwSums &w, const float *r, const float *ep, const float *e, int 
   const double re = sqrt(se2 / n);
   *scale = re;
   if (!(re > 0.0)) return 1;
-  const int blocks = (nc + 255) / 256;
-  lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(r, inp, nx, ny, nz, F,
-                                               (float)(h2 * F * F / re));
-  lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(ep, inp + nc, nx, ny, nz, F, (float)(1.0 / re));
-  if (e && tgt)
-    lw_pool_kernel<<<blocks, 256, 0, g_stream>>>(e, tgt, nx, ny, nz, F, (float)(1.0 / re));
-  HIPCHK(hipGetLastError());
+  HIPCHK(launch_lw_pool(r, inp, nx, ny, nz, F, (float)(h2 * F * F / re)));
+  HIPCHK(launch_lw_pool(ep, inp + nc, nx, ny, nz, F, (float)(1.0 / re)));
+  if (e && tgt) HIPCHK(launch_lw_pool(e, tgt, nx, ny, nz, F, (float)(1.0 / re)));
   return 0;
 }
 
@@ -1751,9 +1793,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
                                         S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
                                         g_stream));
     const int nc = n / (F * F * F);
-    lw_pool_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.Rl, S.mg.f[0], S.p.nx, S.p.ny,
-                                                           S.p.nz, F, 1.0f);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_lw_pool(S.Rl, S.mg.f[0], S.p.nx, S.p.ny, S.p.nz, F, 1.0f));
     if (nn_coarse) HIPCHK(mg_seed_from_level1(S.mg, g_stream));
     HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 1, 1, g_stream, !nn_coarse));
     dim3 grid((S.p.nx / 2 + 63) / 64, (S.p.ny + 3) / 4, S.p.nz);
