@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(1024) mg_coarsest_kernel(float *__restrict__ u
 // of its cells of one colour in registers for the sweeps of a level.
 constexpr int kMgSmallLevels = 4;
 constexpr int kMgSmallCells = 4096 + 512 + 64 + 8;
-constexpr int kMgSmallThreads = 512;
+constexpr int kMgSmallThreads = 1024;
 constexpr int kMgSmallPer = kMgCoarsestCells / 2 / kMgSmallThreads;
 struct MgSmall {
   int levels;
