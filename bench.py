@@ -22,21 +22,30 @@ Each requested form is timed over the same steps:
 The JSON line's value is the steps/s of the headline form (the last of
 --forms); every form's figures are included under "forms".
 
+late_window: the forms fp64, mg and nn run the first 975 steps of the
+1000-step run untimed by events (wall-clocked as a block), then steps
+976-1000 are timed: steps/s of that late window, the 1000-step mean steps/s,
+sweeps per step, and E_k / enstrophy at step 1000 against the FP64 form.
+
 roofline: the dominant work of the headline form is the red-black SOR
 iteration (in the mixed forms the FP32 colour-split red and black launches,
-sor_split_kernel<0> and <1>).  Its algorithmic bytes per iteration are
-(read p, read rhs, write p) = 3 values per cell: 24 B/cell in FP64, 12 B/cell
-in FP32, times the cell count.  Its average duration is measured with HIP
-events on the launch stream over complete batches of iterations inside the
-timed steps; `traffic` is the measured HBM bytes of one iteration (both
-launches) from the committed rocprofv3 --pmc summary.
+sor_split_kernel<0> and <1>).  Its algorithmic bytes per iteration are those
+of a red-black sweep that reads the whole iterate once per colour: (2 reads
+of p, 1 read of rhs, 1 write of p) = 4 values per cell, 16 B/cell in FP32 and
+32 B/cell in FP64, times the cell count; the floor of one read of p and rhs
+and one write of p (12 / 24 B/cell) is reported beside it.  Its average
+duration is measured with HIP events on the launch stream over complete
+batches of iterations inside the timed steps; `traffic` is the measured HBM
+bytes of one iteration (both launches) from the committed rocprofv3 --pmc
+summary.  The ceiling beside the nominal 8 TB/s is a hand-written streaming
+triad over three 1 GiB buffers (o3d_stream_probe).
 
 cpu_baseline: the reference Fortran (oracle/_ref/o3d_ref_run, built from the
-reference sources at -O3, single-threaded) on one host core: a bounded sample
-of one 256^3 step with kmax capped, whose phases are timed; the per-step
-time is extrapolated with the measured per-sweep SOR cost and the number of
-lexicographic sweeps the reference needs (the FP64 form's red-black count
-times the lexicographic/red-black ratio measured on config 1).
+reference sources at -O3, single-threaded) on one host core: one full step
+of the same 256^3 Taylor-Green case (kmax = 5000, the SOR to eps = 1e-8),
+wall-clocked, with its phases timed by the harness.  The former model (a
+kmax-capped step extrapolated by the lexicographic/red-black sweep ratio
+of config 1) is kept beside it as a cross-check.
 """
 from __future__ import annotations
 
@@ -76,7 +85,12 @@ def parse_args():
                          "poisson: one n^3 pressure solve from zero, FP64 SOR against the "
                          "multigrid solver (multigrid = 1)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-kmax", type=int, default=10)
+    ap.add_argument("--cpu-kmax", type=int, default=5000,
+                    help="kmax of the reference's measured 256^3 step (5000: a full step)")
+    ap.add_argument("--late-forms", default="fp64,mg,nn",
+                    help="forms run through the 1000-step late window ('' to skip)")
+    ap.add_argument("--late-total", type=int, default=1000)
+    ap.add_argument("--late-window", type=int, default=25)
     ap.add_argument("--model", default=os.path.join(ROOT, "osinco3d_amd", "warmstart_weights.npy"))
     return ap.parse_args()
 
@@ -178,19 +192,22 @@ def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
         "t_end": t_end,
     }
     # SOR iterations: average duration from events over complete batches of
-    # launches.  The algorithmic bytes of one iteration are one read of the
-    # iterate and the rhs and one write of the iterate (24 B/cell FP64,
-    # 12 B/cell FP32).  An FP64 iteration is one sor_kernel launch; an FP32
-    # iteration of the mixed forms is the red and the black colour-split
-    # launch (sor_split_kernel<0>, <1>), which together move 16 B/cell.
-    for q, name, bpc in ((0, "sor_f64", 24), (1, "sor_f32", 12)):
+    # launches (one launch = one red-black iteration: the red and the black
+    # colour-split launch, sor_split_kernel<0> and <1>).  Algorithmic bytes
+    # of an iteration: a red-black sweep reading the whole iterate once per
+    # colour, the rhs once and writing the iterate once, 4 values per cell
+    # (32 B/cell FP64, 16 B/cell FP32); the floor of one read of p and rhs
+    # and one write of p (24 / 12 B/cell) beside it.
+    for q, name, bpc, floor in ((0, "sor_f64", 32, 24), (1, "sor_f32", 16, 12)):
         if sor_ln[q] > 0:
             avg_ms = sor_ms[q] / sor_ln[q]
             gbs = bpc * n / (avg_ms * 1e-3) / 1e9
             out[name] = {"avg_launch_us": 1e3 * avg_ms, "launches_timed": sor_ln[q],
                          "iterations_per_launch": sor_sw[q] / sor_ln[q],
                          "us_per_iteration": 1e3 * sor_ms[q] / sor_sw[q],
-                         "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs}
+                         "algorithmic_bytes_per_launch": bpc * n, "achieved_GBps": gbs,
+                         "floor_bytes_per_launch": floor * n,
+                         "floor_GBps": floor * n / (avg_ms * 1e-3) / 1e9}
     if cc or gs:
         out["coarse_cycles"] = cc
         out["smooth_sweeps"] = gs
@@ -207,24 +224,72 @@ def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
     return out
 
 
-def copy_bandwidth(nbytes=1 << 30, reps=20):
-    """Achievable HBM bandwidth: a 1 GiB device-to-device copy (read + write
-    counted), timed with events on the current stream."""
+def stream_bandwidth(nbytes=1 << 30, reps=20):
+    """HBM ceiling: the library's hand-written streaming triad (two reads and
+    one write per element) over three 1 GiB device buffers, far beyond the
+    256 MB Infinity Cache, timed with HIP events."""
+    from osinco3d_amd import capi
+    return capi.stream_probe(nbytes, reps)[0]
+
+
+def run_series(form, cfg, total, window, model_path, coarse, init=None):
+    """The first total - window steps of a run (wall-clocked as one block),
+    then the last `window` steps timed on their own: the late-window figures
+    and the whole-run mean."""
     import torch
-    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    src.fill_(1.0)
-    for _ in range(3):
-        dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
+    from osinco3d_amd import solver as S
+
+    prec = S.FP64 if form == "fp64" else S.MIXED
+    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE,
+            "mg": S.WARM_EXTRAPOLATE}.get(form, S.WARM_PREVIOUS)
+    cc, gs = coarse if form in ("mg", "nn") else (0, 0)
+    net = None
+    if form in ("nn", "nn0"):
+        from osinco3d_amd import warmstart
+        net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
+        if net is None:
+            return None
+    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs)
+    if net is not None:
+        net.attach(s)
+    if init is None:
+        s.init_taylor_green()
+    else:
+        s.upload(*init)
+    first = total - window
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del src, dst
-    return 2 * nbytes / (ms * 1e-3) / 1e9
+    t0 = time.perf_counter()
+    sweeps_all = []
+    for it in range(1, first + 1):
+        sweeps_all.append(s.step(it)["sor_exit_iter"])
+        if it % 250 == 0:
+            print(f"[bench] late window {form}: step {it}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    infos = [s.step(it) for it in range(first + 1, total + 1)]
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    st = s.statistics(total * s.dt)
+    s.close()
+    if net is not None:
+        net.close()
+    sweeps_all += [i["sor_exit_iter"] for i in infos]
+    late = [i["sor_exit_iter"] for i in infos]
+    q = len(sweeps_all) // 4
+    return {
+        "late_steps": [first + 1, total],
+        "late_steps_per_s": window / (t2 - t1),
+        "late_ms_per_step": 1e3 * (t2 - t1) / window,
+        "late_sweeps_mean": statistics.mean(late),
+        "late_poisson_ms_mean": statistics.mean(i["poisson_ms"] for i in infos),
+        "mean_steps_per_s": total / (t2 - t0),
+        "mean_sweeps": statistics.mean(sweeps_all),
+        "sweeps_by_quarter": [statistics.mean(sweeps_all[k * q:(k + 1) * q]) for k in range(4)],
+        "sor_reasons": sorted({i["sor_reason"] for i in infos}),
+        "kinetic_energy": st[1],
+        "enstrophy": st[4],
+        "t_end": total * s.dt,
+    }
 
 
 def lex_over_red_black():
@@ -244,7 +309,10 @@ def lex_over_red_black():
 
 
 def cpu_baseline(n, kmax, iters_per_step, ratio):
-    """Reference Fortran, one core, one n^3 step with kmax capped."""
+    """Reference Fortran, one core, one full n^3 step (kmax = 5000 by
+    default: the SOR runs to eps), wall-clocked; the older model (a capped
+    step extrapolated with the measured lexicographic/red-black ratio) is
+    returned beside it as a cross-check."""
     from oracle import ref_run_path
     from osinco3d_amd.config import tgv_config
 
@@ -270,20 +338,26 @@ def cpu_baseline(n, kmax, iters_per_step, ratio):
         return {"error": f"reference exited with {r.returncode}"}
     ph = None
     iters = None
+    t_meas = None
+    reason = None
     for line in r.stdout.splitlines():
         if line.startswith("O3D_PHASE_SECONDS"):
             ph = [float(x) for x in line.split()[2:]]
+        if line.startswith("O3D_STEP_SECONDS"):
+            t_meas = float(line.split()[2])
         m = re.match(r"\s*Exit iteration\s*:\s*(\d+)", line)
         if m:
             iters = int(m.group(1))
-    if ph is None or iters is None:
+        if "No converged" in line:
+            reason = "kmax or stagnation"
+    if ph is None or iters is None or t_meas is None:
         return {"error": "could not parse the reference output"}
     predict, pression, correct, div, diag = ph
     sweeps = min(iters, kmax)
     t_iter = (pression - div) / sweeps
     t_other = predict + div + correct + div + diag
     lex_iters = iters_per_step * ratio
-    t_step = t_other + lex_iters * t_iter
+    t_model = t_other + lex_iters * t_iter
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as fh:
@@ -293,18 +367,26 @@ def cpu_baseline(n, kmax, iters_per_step, ratio):
                     break
     except OSError:
         pass
+    full = iters <= kmax and reason is None
     return {
-        "value": 1.0 / t_step, "unit": "steps/s", "cores": 1, "kind": "reference",
+        "value": 1.0 / t_meas, "unit": "steps/s", "cores": 1, "kind": "reference",
         "sample": (f"reference Fortran (amdflang -O3) on 1 core of '{cpu_model}' "
-                   f"({os.cpu_count()} visible): one {n}^3 TGV step with kmax={kmax} "
-                   f"({wall:.1f} s wall); phases predict {predict:.2f} s, correct_pression "
-                   f"{pression:.2f} s ({sweeps} sweeps, {t_iter:.3f} s/sweep), correct "
-                   f"{correct:.2f} s, diagnostics {div + diag:.2f} s; step time = "
-                   f"{t_other:.2f} s + {lex_iters:.1f} sweeps x {t_iter:.3f} s, where "
-                   f"{lex_iters:.1f} = the FP64 GPU path's {iters_per_step:.1f} red-black "
-                   f"sweeps x {ratio:.3f}, the reference's lexicographic/red-black sweep "
-                   f"ratio measured on config 1 (TGV 64^3, 100 steps)"),
-        "seconds_per_step": t_step,
+                   f"({os.cpu_count()} visible): one {'full ' if full else ''}{n}^3 TGV step "
+                   f"(step 1, kmax={kmax}: {iters} lexicographic SOR sweeps"
+                   f"{' to eps=1e-8' if full else ', capped'}) measured at {t_meas:.2f} s "
+                   f"({wall:.1f} s wall for the process); phases predict {predict:.2f} s, "
+                   f"correct_pression {pression:.2f} s ({t_iter:.3f} s/sweep), correct "
+                   f"{correct:.2f} s, diagnostics {div + diag:.2f} s"),
+        "seconds_per_step": t_meas,
+        "sweeps": iters,
+        "full_step": full,
+        "model_cross_check": {
+            "seconds_per_step": t_model,
+            "rule": (f"{t_other:.2f} s + {lex_iters:.1f} sweeps x {t_iter:.3f} s, where "
+                     f"{lex_iters:.1f} = the FP64 GPU path's {iters_per_step:.1f} red-black "
+                     f"sweeps per step (timed window) x {ratio:.3f}, the reference's "
+                     f"lexicographic/red-black sweep ratio measured on config 1"),
+        },
     }
 
 
@@ -392,7 +474,7 @@ def main():
         workload = (f"Taylor-Green vortex Re=1600, {args.n}^3 periodic, AB3, CFL 0.25, "
                     f"SOR eps=1e-8")
         data = "synthetic (analytic Taylor-Green initial field)"
-    copy_gbs = copy_bandwidth()
+    copy_gbs = stream_bandwidth()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     results = {}
     for form in forms:
@@ -416,15 +498,42 @@ def main():
     dom = head.get("sor_" + which)
     roofline = None
     if dom:
-        kname = ("sor_split_kernel<0> + <1> (one colour-split red-black iteration, f32)"
-                 if which == "f32" else "sor_kernel (one red-black iteration, f64)")
+        kname = ("sor_split_kernel<float,0> + <float,1> (one colour-split red-black "
+                 "iteration, f32)" if which == "f32" else
+                 "sor_split_kernel<double,0> + <double,1> (one red-black iteration, f64)")
         roofline = {"bound": "hbm", "kernel": kname,
                     "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": dom["achieved_GBps"] / HBM_PEAK_GBS,
                     "traffic": measured_traffic(which),
                     "algorithmic_bytes": dom["algorithmic_bytes_per_launch"],
-                    "frac_of_measured_copy": dom["achieved_GBps"] / copy_gbs,
-                    "measured_copy_GBps": copy_gbs}
+                    "avg_launch_us": dom["avg_launch_us"],
+                    "floor_bytes": dom["floor_bytes_per_launch"],
+                    "frac_floor": dom["floor_GBps"] / HBM_PEAK_GBS,
+                    "frac_of_stream_probe": dom["achieved_GBps"] / copy_gbs,
+                    "stream_probe_GBps": copy_gbs}
+    late = None
+    late_forms = [f.strip() for f in args.late_forms.split(",") if f.strip()]
+    if late_forms and args.workload == "tgv":
+        late = {}
+        for form in late_forms:
+            print(f"[bench] late window {form}: {args.late_total} steps, last "
+                  f"{args.late_window} timed", file=sys.stderr, flush=True)
+            r = run_series(form, cfg, args.late_total, args.late_window, args.model,
+                           (args.coarse_cycles, args.smooth_sweeps), init)
+            if r is not None:
+                late[form] = r
+                print(f"[bench] late window {form}: {r['late_steps_per_s']:.2f} steps/s late, "
+                      f"{r['mean_steps_per_s']:.2f} mean, {r['late_sweeps_mean']:.1f} sweeps",
+                      file=sys.stderr, flush=True)
+        ref64 = late.get("fp64")
+        if ref64 is not None:
+            for f, r in late.items():
+                r["late_speedup_vs_fp64"] = r["late_steps_per_s"] / ref64["late_steps_per_s"]
+                r["mean_speedup_vs_fp64"] = r["mean_steps_per_s"] / ref64["mean_steps_per_s"]
+                r["kinetic_energy_rel_err_vs_fp64"] = (
+                    abs(r["kinetic_energy"] - ref64["kinetic_energy"]) / abs(ref64["kinetic_energy"]))
+                r["enstrophy_rel_err_vs_fp64"] = (
+                    abs(r["enstrophy"] - ref64["enstrophy"]) / abs(ref64["enstrophy"]))
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline (reference Fortran, 1 core)", file=sys.stderr, flush=True)
@@ -449,6 +558,7 @@ def main():
                                   "smooth_sweeps": head.get("smooth_sweeps", 0)}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "late_window": late,
         "forms": results,
     }
     print(json.dumps(line))

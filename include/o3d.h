@@ -14,7 +14,14 @@
  *    device buffers (correct, slower); device arrays are used in place.
  *  - bc[3]: per-axis boundary type, 0 = periodic, 1 = free-slip
  *    (src/initialization.f90:77-78).  Both faces of an axis share one type,
- *    as schemes() requires.
+ *    as schemes() requires.  2 = x-Dirichlet inflow / outflow (x axis only;
+ *    the reference's README lists it, its schemes() stops on it,
+ *    src/initialization.f90:238-241): accepted by o3d_der (which 20, 21),
+ *    o3d_divergence, o3d_nu_t, o3d_predict_velocity, o3d_correct_velocity,
+ *    o3d_statistics and the device-resident solver.  Velocity derivatives
+ *    along x use the cubic-extrapolated ends of o3d_der 20 / 21; the
+ *    pressure is mirrored in x (homogeneous Neumann), its x derivative zero
+ *    on the end nodes; o3d_correct_velocity leaves the x-end planes at u*.
  *  - Status: 0 = ok; 1 = velocity diverged (the reference prints
  *    "Velocity has diverged" and stops, src/integration.f90:309-325);
  *    < 0 = runtime or argument error, text from o3d_last_error().
@@ -23,6 +30,7 @@
  */
 #ifndef O3D_H
 #define O3D_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -35,6 +43,14 @@ const char *o3d_last_error(void);
 int o3d_sync(void);
 /* Library version string. */
 const char *o3d_version(void);
+/* Measurement: a hand-written streaming triad c = a + s b over three device
+ * buffers of bytes_per_buffer each (two reads and one write per element; at
+ * 1 GiB per buffer the working set is far beyond the 256 MB Infinity Cache),
+ * timed with HIP events over `reps` launches.  gbs = 3 * bytes_per_buffer /
+ * time per launch (GB/s), ms = time per launch; either may be NULL.  No
+ * counterpart in the reference: the HBM ceiling the kernels are compared
+ * with. */
+int o3d_stream_probe(size_t bytes_per_buffer, int reps, double *gbs, double *ms);
 
 /* ------------------------------------------------------ operator entry points */
 
@@ -43,7 +59,10 @@ const char *o3d_version(void);
  * variant 0 = "_00" periodic, 1 = "p_11" free-slip even, 2 = "i_11" free-slip
  * odd; 18 = derz_2dsim, 19 = derzz_2dsim.  E.g. which 0 = derx_00
  * (src/derivation.f90:6-60), 8 = deryi_11 (:269-321), 17 = derzzi_11
- * (:836-883). */
+ * (:836-883).  20 / 21: the x first / second derivative with x-Dirichlet
+ * ends (no counterpart in the reference): the values 1..3 nodes beyond an
+ * end are the cubic through the four nodes nearest it, so the stencil is
+ * exact on cubics up to the end nodes. */
 int o3d_der(int which, double *df, const double *f, int nx, int ny, int nz, double d);
 int o3d_der_f32(int which, float *df, const float *f, int nx, int ny, int nz, double d);
 
@@ -77,7 +96,9 @@ int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz_pred, cons
 
 /* poisson_solver_0000 / _0011 / _111111 (interface poi_type,
  * src/initialization.f90:93-102; bodies src/poisson.f90:6-381), red-black
- * ordering.  solver: 0 = "0000", 1 = "0011", 2 = "111111".  omega is in/out
+ * ordering.  solver: 0 = "0000", 1 = "0011", 2 = "111111"; 3 + (y mirrored)
+ * + 2 (z mirrored) = the x-Dirichlet kinds (x mirrored; no counterpart in
+ * the reference).  omega is in/out
  * (dynamic omega persists, idyn = 1).  exit_iter = the iteration the
  * reference prints on exit (kmax+1 when kmax ran out); reason 0 = dmax < eps,
  * 1 = stagnation, 2 = kmax.  Any of the three out pointers may be NULL. */
@@ -86,24 +107,25 @@ int o3d_poisson(double *pp, const double *rhs, double dx, double dy, double dz, 
                 int *exit_iter, double *dmax, int *reason);
 
 /* The red-black SOR of o3d_poisson in FP32 (the sweeps of the mixed form),
- * same rules and outputs.  variant selects the kernels, all with
+ * same rules and outputs.  variant selects the kernels, both with
  * bit-identical iterates: 0 = one fused red+black iteration per launch;
- * 1 = two iterations per launch (every axis >= 8; the stop rules are checked
- * after every second iteration); 2 = colour-split storage, one in-place
- * launch per colour (nx a multiple of 8; the form the mixed solver uses).
- * No counterpart in the reference (its solver is FP64). */
+ * 2 = colour-split storage, one in-place launch per colour (nx a multiple
+ * of 8; the form the mixed solver uses).  No counterpart in the reference
+ * (its solver is FP64). */
 int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy, double dz, int nx, int ny,
                     int nz, double *omega, double eps, int kmax, int idyn, int solver,
                     int variant, int *exit_iter, double *dmax, int *reason);
 
-/* The coarse-grid solver of the warm starts on its own: `cycles` multigrid
- * V(2,2) cycles on L u = f from u = 0, L the 7-point Laplacian of a
+/* The coarse-grid solver of the warm starts, on its own: `cycles` multigrid
+ * V(2,2) cycles on L u = f from u = 0 (the solver's coarse-grid correction
+ * runs the same cycles as V(1,1)), L the 7-point Laplacian of a
  * cell-centred mx x my x mz grid with spacings hx, hy, hz; per[a] != 0 makes
  * axis a periodic, else the ghost cell equals the edge cell (homogeneous
  * Neumann).  Levels are halved while every axis is even and >= 4; the
  * coarsest (at most 64 cells, or where halving stops) is solved by
- * Gauss-Seidel sweeps (30, or 60 above 64 cells).  u, f: host or device float32.  No counterpart in the reference
- * (used by the mixed forms' coarse-grid correction, o3d_params.coarse_cycles). */
+ * Gauss-Seidel sweeps (30, or 60 above 64 cells).  u, f: host or device
+ * float32.  No counterpart in the reference (the mixed forms' coarse-grid
+ * correction, o3d_params.coarse_cycles). */
 int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx, double hy,
                   double hz, const int *per, int cycles);
 
@@ -116,12 +138,15 @@ int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx, d
  * when max |rhs - L pp| < tol (the reference's stated criterion), coarse-grid
  * correction on the 2^3 block averages by one V(npre, npost) cycle over at
  * most nlevels - 1 coarser levels, npre + npost red-black Gauss-Seidel sweeps
- * on the full grid, pp += correction.  At most max_cycles cycles; cycles and
- * resmax (the final max |r|) may be NULL.  Needs nx a multiple of 8 and ny,
- * nz even. */
+ * on the full grid, pp += correction.  Stop rules as the SOR solvers'
+ * (src/poisson.f90:95-122) on max |r|: reason 0 = max |r| < tol,
+ * 1 = stagnation (max |r| changed by less than tol / 1000 since the last
+ * cycle: an rhs the mirrored operator cannot satisfy), 2 = max_cycles cycles
+ * done.  cycles, resmax (the final max |r|) and reason may be NULL.  Needs
+ * nx a multiple of 8 and ny, nz even. */
 int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, double dy, double dz,
                           int nx, int ny, int nz, int solver, int nlevels, int npre, int npost,
-                          double tol, int max_cycles, int *cycles, double *resmax);
+                          double tol, int max_cycles, int *cycles, double *resmax, int *reason);
 
 /* correct_pression, src/integration.f90:199-255 (SOR branch): divergence of
  * the predicted velocity (odd variants), rhs = div/dt, then the Poisson solver
@@ -150,6 +175,7 @@ int o3d_function_stats(const double *f, int nx, int ny, int nz, double *stats6);
 int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx, int ny, int nz,
                    double dx, double dy, double dz, double re, double t, const int *bc,
                    double *out17);
+
 
 /* transeq, src/integration.f90:332-468: passive scalar transport.  fphi is
  * the (nx,ny,nz,3) history (slot 1 = newest, shifted by copy as the
@@ -233,6 +259,9 @@ typedef struct o3d_step_info {
   double vel_min[3], vel_max[3], vel_amax[3];
   double cfl[3];          /* max|u_c| dt / d_c, src/utils.f90:178-208 */
   double nu_t[6];         /* function_stats of nu_t (iles = 1) */
+  double flow_rate[2];    /* x-Dirichlet: mean inflow and outflow velocity (the weighted
+                             plane fluxes / plane weight, outflow before its flux shift), as
+                             print_in_outflow_rate prints them (src/IOfunctions.f90:342-350) */
 } o3d_step_info;
 
 int o3d_solver_create(const o3d_params *p, void **handle);
@@ -253,8 +282,9 @@ void *o3d_solver_field(void *handle, int which);
 /* Time of SOR sweeps measured with HIP events on the launch stream over the
  * last step: total ms of complete sweep batches, the number of sweeps
  * (iterations) and of kernel launches in them, per precision (index 0 FP64,
- * 1 FP32; the FP32 correction sweeps run two iterations per launch when
- * every axis has at least 8 nodes).  launches2 may be NULL. */
+ * 1 FP32; one iteration per launch: a fused red+black launch, or with
+ * colour-split storage the red and the black launch counted as one).
+ * launches2 may be NULL. */
 int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches2);
 /* Adams-Bashforth history for checkpoint / restart (the reference has none:
  * it restarts with an Euler step).  hist: host array of 9 n elements, float64
@@ -262,8 +292,12 @@ int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches
  * right-hand side F, as fux(:,:,:,1) after predict_velocity) then component
  * (x, y, z); state2 = {history depth (predict calls so far), omega}.
  * write = 0 reads the solver's history into hist / state2, write = 1 sets it.
- * With the fields (o3d_solver_upload) this restores a run exactly; the
- * learned / extrapolated warm start re-primes itself after a restart. */
+ * With the fields (o3d_solver_upload) this restores a run with the previous
+ * pressure as warm start (the reference's) exactly.  The extrapolated and
+ * learned warm starts keep no history across a restart: writing the history
+ * or uploading a pressure re-primes them (the first step after it starts
+ * from the previous pressure), so a restart continues exactly as any other
+ * restart from the same checkpoint, not as the uninterrupted run. */
 int o3d_solver_history(void *handle, int write, void *hist, double *state2);
 /* Attach the learned warm start (an o3d_nn network, include/o3d_nn.h), or
  * NULL to detach; the network is re-sized to the grid coarsened by
@@ -276,6 +310,25 @@ int o3d_solver_history(void *handle, int write, void *hist, double *state2);
  * initial guess; the previous pressure stays the initial guess unless the
  * guess has the smaller L2 residual. */
 int o3d_solver_set_warm_model(void *handle, void *nn);
+/* x-Dirichlet inflow (bc[0] == 2; no counterpart in the reference, which
+ * stops on this boundary, src/initialization.f90:238-241).  Every step,
+ * after predict:
+ *  - inflow plane i = 1: u*_c(j, k) = base3[c][j, k]
+ *      + (amp weight[j, k]) (x1 cos(phi) + x2 sin(phi)),
+ *    phi = 2 pi itime dt / period, x1, x2 uniform in [-1, 1) from SplitMix64
+ *    of seed + 2 (3 (j + ny k) + c) (+ 1);
+ *  - outflow plane i = nx: first-order upwind convection at velocity uc,
+ *    u*_c = u_c - (uc dt / dx) (u_c(nx) - u_c(nx - 1)), then u*_x shifted by
+ *    one constant so that the outflow flux equals the inflow flux (plane sums
+ *    with weight 1/2 on the end nodes of mirrored axes);
+ *  - the Poisson rhs minus its weighted mean (the same weights in 3D), the
+ *    part the x-mirrored operator cannot reach, so SOR converges to eps;
+ *  - o3d_correct_velocity's rule: the two planes keep u*.
+ * base3: 3 planes of ny nz values (x, y, z components, j fastest), weight one
+ * plane; host or device float64.  Must be called before the first step. */
+int o3d_solver_set_inflow(void *handle, const double *base3, const double *weight, double amp,
+                          double period, unsigned long long seed, double uc);
+
 /* Training data capture: when set, every step writes float32 planes of n
  * cells each: r_out = rhs - L (p_prev + e_prev) before the solve (rhs - L p_prev
  * while no increment exists), eprev_out = the previous step's pressure

@@ -21,6 +21,16 @@
 
 #define IDX(i, j, k) ((size_t)(i) + (size_t)nx * ((size_t)(j) + (size_t)ny * (size_t)(k)))
 
+/* x-Dirichlet ends (kind 3; no counterpart in the reference, which stops
+ * on this boundary type, src/initialization.f90:238-241): a ghost m nodes
+ * beyond an end (m = 1, 2, 3) is the cubic through the four nodes nearest
+ * that end, evaluated there: f(-m) = XW[m-1] . (f(0), f(1), f(2), f(3)),
+ * summed in that order, and mirrored at the far end.  Every stencil that
+ * reaches past the end is then exact on cubics. */
+static const double XW[3][4] = {{4.0, -6.0, 4.0, -1.0},
+                                {10.0, -20.0, 15.0, -4.0},
+                                {20.0, -45.0, 36.0, -10.0}};
+
 /* Neighbour index along an axis of n nodes, with the ghost value's sign.
  * Periodic: index n+1 wraps to 1 (src/derivation.f90:34-56), so the period
  * is n nodes.  Free-slip: mirror about the end node, ghost f(1-m) = +-f(1+m)
@@ -46,6 +56,18 @@ static inline double at(const double *f, int nx, int ny, int nz, int axis,
                         int i, int j, int k, int s, int kind) {
     double sg;
     (void)nz;
+    if (kind == 3) {
+        const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+        const int p = (axis == 0 ? i : (axis == 1 ? j : k)) + s;
+        const size_t st = axis == 0 ? 1 : (axis == 1 ? (size_t)nx : (size_t)nx * ny);
+        const size_t base = IDX(i, j, k) - (size_t)(axis == 0 ? i : (axis == 1 ? j : k)) * st;
+        if (p >= 0 && p < n) return f[base + (size_t)p * st];
+        const int m = p < 0 ? -p : p - (n - 1);
+        const size_t e = p < 0 ? base : base + (size_t)(n - 1) * st;
+        const long d = p < 0 ? (long)st : -(long)st;
+        const double *w = XW[m - 1];
+        return ((w[0] * f[e] + w[1] * f[e + d]) + w[2] * f[e + 2 * d]) + w[3] * f[e + 3 * d];
+    }
     if (axis == 0) { int q = ghost(i + s, nx, kind, &sg); return sg == 1.0 ? f[IDX(q, j, k)] : -f[IDX(q, j, k)]; }
     if (axis == 1) { int q = ghost(j + s, ny, kind, &sg); return sg == 1.0 ? f[IDX(i, q, k)] : -f[IDX(i, q, k)]; }
     { int q = ghost(k + s, nz, kind, &sg); return sg == 1.0 ? f[IDX(i, j, q)] : -f[IDX(i, j, q)]; }
@@ -92,9 +114,17 @@ void orc_der2(double *df, const double *f, int nx, int ny, int nz, int axis,
 }
 
 /* Variant bound for a field that is odd along `axis` (the "i" pointers) or
- * even (the "p" pointers): schemes(), src/initialization.f90:226-282. */
+ * even (the "p" pointers): schemes(), src/initialization.f90:226-282.  A
+ * velocity on an x-Dirichlet axis (bc 2) uses the extrapolated ends. */
 static inline int kind_of(const int *bc, int axis, int odd) {
+    if (bc[axis] == 2) return 3;
     return bc[axis] == 0 ? 0 : (odd ? 2 : 1);
+}
+/* The pressure's variant: the p pointers (src/integration.f90:298-300); on an
+ * x-Dirichlet axis the pressure is mirrored (homogeneous Neumann), its
+ * derivative zero on the end nodes. */
+static inline int kind_p(const int *bc, int axis) {
+    return bc[axis] == 0 ? 0 : 1;
 }
 
 /* src/differential_operators.f90:7-38 */
@@ -308,6 +338,13 @@ static inline void nbr(int q, int n, int periodic, int *m, int *p) {
     else { *m = q - 1; *p = q + 1; }
 }
 static inline void solver_bc(int solver, int *px, int *py, int *pz) {
+    if (solver >= 3) {
+        /* x-Dirichlet velocity (no reference counterpart): the pressure is
+         * mirrored in x; y and z periodic unless bit 0 / bit 1 of
+         * solver - 3 says mirrored. */
+        *px = 0; *py = !((solver - 3) & 1); *pz = !((solver - 3) & 2);
+        return;
+    }
     *px = solver != 2; *py = solver == 0; *pz = solver != 2;
 }
 
@@ -449,10 +486,19 @@ int orc_correct_velocity(double *ux, double *uy, double *uz,
     const double *up[3] = {ux_pred, uy_pred, uz_pred};
     const double h[3] = {dx, dy, dz};
     for (int a = 0; a < 3; ++a) {
-        orc_der1(g, pp, nx, ny, nz, a, kind_of(bc, a, 0), h[a]);
+        orc_der1(g, pp, nx, ny, nz, a, kind_p(bc, a), h[a]);
         for (size_t q = 0; q < N; ++q) u[a][q] = up[a][q] - dt * g[q];
     }
     free(g);
+    if (bc[0] == 2)
+        /* x-Dirichlet: the inflow and outflow planes keep u* (set by
+         * orc_jet_edges), no pressure correction there. */
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j) {
+                    u[a][IDX(0, j, k)] = up[a][IDX(0, j, k)];
+                    u[a][IDX(nx - 1, j, k)] = up[a][IDX(nx - 1, j, k)];
+                }
     for (int a = 0; a < 3; ++a) {
         double mx = -INFINITY;
         for (size_t q = 0; q < N; ++q) mx = u[a][q] > mx ? u[a][q] : mx;
@@ -527,4 +573,142 @@ void orc_statistics(const double *ux, const double *uy, const double *uz,
     }
     out[3] = eps2 / nn;
     free9(g);
+}
+
+/* ------------------------------------------------------------------------
+ * x-Dirichlet inflow / outflow (no counterpart in the reference).
+ * ---------------------------------------------------------------------- */
+
+/* SplitMix64 (Steele, Lea, Flood 2014), output number counter + 1 of the
+ * stream seeded with `seed` (state seed + (counter + 1) * golden gamma, then
+ * the finalizer): the counter-based generator of the inflow noise, as a
+ * uniform double in [-1, 1) from the top 53 bits. */
+unsigned long long orc_splitmix64(unsigned long long seed, long long counter) {
+    unsigned long long z = seed + (unsigned long long)(counter + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double orc_noise_uniform(unsigned long long seed, long long counter) {
+    const unsigned long long h = orc_splitmix64(seed, counter);
+    return (double)(h >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+}
+
+/* Weight of a plane node in the flux sums: 1/2 on the end nodes of a
+ * mirrored (non-periodic) axis, else 1 (the trapezoid rule the mirrored
+ * 7-point operator's compatibility condition uses). */
+static inline double wgt(int q, int n, int mirrored) {
+    return (mirrored && (q == 0 || q == n - 1)) ? 0.5 : 1.0;
+}
+
+/* Weighted sum over the (y, z) plane of the values v[j + ny k], in the order
+ * the device uses: per z row the sequential sum over j of wy * v, then the
+ * sequential sum over k of wz * row. */
+static double plane_sum(const double *v, int ny, int nz, int my, int mz) {
+    double q = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        double r = 0.0;
+        for (int j = 0; j < ny; ++j) r = r + wgt(j, ny, my) * v[j + (size_t)ny * k];
+        q = q + wgt(k, nz, mz) * r;
+    }
+    return q;
+}
+
+/* After predict (which leaves every node's u* from the interior formula):
+ *  - inflow plane i = 0: u*_c = base_c(j, k) + (amp w(j, k)) (x1 C + x2 S),
+ *    x1, x2 = orc_noise_uniform(seed, 2 (3 (j + ny k) + c) + 0 / 1),
+ *    C = cos, S = sin of the inflow phase 2 pi t / T (given);
+ *  - outflow plane i = nx-1: first-order upwind convection at Uc = the mean
+ *    inflow velocity of the base profile: u*_c = u_c - cu (u_c - u_c(i-1)),
+ *    cu = Uc dt / dx, from the velocity before the step;
+ *  - then u*_x on the outflow plane is shifted by (Q_in - Q_out) / W so that
+ *    the outflow flux equals the inflow flux (trapezoid-weighted plane sums,
+ *    plane_sum order).
+ * base: 3 planes of ny nz values; w: one plane.  flux[0..2] = Q_in, Q_out
+ * before the shift, W. */
+void orc_jet_edges(double *uxp, double *uyp, double *uzp, const double *ux,
+                   const double *uy, const double *uz, const double *base,
+                   const double *w, double amp, double cph, double sph,
+                   unsigned long long seed, double cu, int nx, int ny, int nz,
+                   const int *bc, double *flux) {
+    double *up[3] = {uxp, uyp, uzp};
+    const double *u[3] = {ux, uy, uz};
+    const size_t P = (size_t)ny * nz;
+    double *pin = malloc(P * sizeof(double)), *pout = malloc(P * sizeof(double));
+    const int my = bc[1] != 0, mz = bc[2] != 0;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                const size_t q = j + (size_t)ny * k;
+                const long long cnt = 2 * (3 * (long long)q + c);
+                const double x1 = orc_noise_uniform(seed, cnt);
+                const double x2 = orc_noise_uniform(seed, cnt + 1);
+                const double vin = base[c * P + q] + (amp * w[q]) * (x1 * cph + x2 * sph);
+                const double ub = u[c][IDX(nx - 1, j, k)], ub1 = u[c][IDX(nx - 2, j, k)];
+                const double vout = ub - cu * (ub - ub1);
+                up[c][IDX(0, j, k)] = vin;
+                up[c][IDX(nx - 1, j, k)] = vout;
+                if (c == 0) { pin[q] = vin; pout[q] = vout; }
+            }
+    for (size_t q = 0; q < P; ++q) pin[q] = 1.0;
+    const double W = plane_sum(pin, ny, nz, my, mz);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) pin[j + (size_t)ny * k] = up[0][IDX(0, j, k)];
+    const double qin = plane_sum(pin, ny, nz, my, mz);
+    const double qout = plane_sum(pout, ny, nz, my, mz);
+    const double shift = (qin - qout) / W;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) up[0][IDX(nx - 1, j, k)] = pout[j + (size_t)ny * k] + shift;
+    flux[0] = qin; flux[1] = qout; flux[2] = W;
+    free(pin); free(pout);
+}
+
+/* Weighted mean of a field with the trapezoid weights of the mirrored axes
+ * (mirror[a] != 0), in the device's order: 1024 chunks of ceil(n / 1024)
+ * cells; within a chunk 256 threads sum cells lo + t, lo + t + 256, ...;
+ * each wave of 64 combines its lanes by the xor butterfly (offsets 32, 16,
+ * 8, 4, 2, 1) and the chunk sum is ((0 + w0) + w1) + w2 + w3 over its four
+ * waves; the 1024 chunk sums are combined the same way by 1024 threads (one
+ * each, 16 waves).  Returns sum(w f) / sum(w). */
+static double wave_butterfly(double *s) {
+    double t[64];
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int l = 0; l < 64; ++l) t[l] = s[l] + s[l ^ o];
+        memcpy(s, t, sizeof(t));
+    }
+    return s[0];
+}
+static double block_tree(double *lanes, int nwaves) {
+    double r = 0.0;
+    for (int w = 0; w < nwaves; ++w) r = r + wave_butterfly(lanes + 64 * w);
+    return r;
+}
+static double weighted_sum_dev(const double *f, int nx, int ny, int nz, const int *mirror,
+                               int ones) {
+    const long long n = (long long)nx * ny * nz;
+    const int B = 1024, T = 256;
+    const long long per = (n + B - 1) / B;
+    double *parts = malloc(B * sizeof(double));
+    double lanes[256];
+    for (int b = 0; b < B; ++b) {
+        const long long lo = (long long)b * per, hi = lo + per < n ? lo + per : n;
+        for (int t = 0; t < T; ++t) {
+            double acc = 0.0;
+            for (long long q = lo + t; q < hi; q += T) {
+                const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long long)nx * ny));
+                const double wq = (wgt(i, nx, mirror[0]) * wgt(j, ny, mirror[1])) * wgt(k, nz, mirror[2]);
+                acc = acc + (ones ? wq : wq * f[q]);
+            }
+            lanes[t] = acc;
+        }
+        parts[b] = block_tree(lanes, 4);
+    }
+    double big[1024];
+    memcpy(big, parts, sizeof(big));
+    const double r = block_tree(big, 16);
+    free(parts);
+    return r;
+}
+double orc_weighted_mean(const double *f, int nx, int ny, int nz, const int *mirror) {
+    return weighted_sum_dev(f, nx, ny, nz, mirror, 0) / weighted_sum_dev(f, nx, ny, nz, mirror, 1);
 }

@@ -55,6 +55,17 @@ void orc_statistics(const double *ux, const double *uy, const double *uz,
                     double re, double t, const int *bc, double *out);
 void orc_residual(double *r, const double *pp, const double *rhs, double dx,
                   double dy, double dz, int nx, int ny, int nz, int solver);
+/* x-Dirichlet inflow / outflow (no reference counterpart).  kind 3 of
+ * orc_der1 / orc_der2 = ends by cubic extrapolation; solver 3..6 of the SOR
+ * routines = x mirrored, y / z periodic unless bit 0 / 1 of solver - 3. */
+unsigned long long orc_splitmix64(unsigned long long seed, long long counter);
+double orc_noise_uniform(unsigned long long seed, long long counter);
+void orc_jet_edges(double *uxp, double *uyp, double *uzp, const double *ux,
+                   const double *uy, const double *uz, const double *base,
+                   const double *w, double amp, double cph, double sph,
+                   unsigned long long seed, double cu, int nx, int ny, int nz,
+                   const int *bc, double *flux);
+double orc_weighted_mean(const double *f, int nx, int ny, int nz, const int *mirror);
 
 #ifdef __cplusplus
 }

@@ -145,6 +145,30 @@ class Oracle:
                                 C.c_double(re), C.c_double(t), _bc(bc), _p(out))
         return out
 
+    # ---- x-Dirichlet inflow / outflow (no reference counterpart)
+    def splitmix64(self, seed, counter):
+        self.lib.orc_splitmix64.restype = C.c_ulonglong
+        return self.lib.orc_splitmix64(C.c_ulonglong(seed), C.c_longlong(counter))
+
+    def noise_uniform(self, seed, counter):
+        self.lib.orc_noise_uniform.restype = C.c_double
+        return self.lib.orc_noise_uniform(C.c_ulonglong(seed), C.c_longlong(counter))
+
+    def jet_edges(self, up, u, base, w, amp, cph, sph, seed, cu, bc):
+        """In place on up = [ux*, uy*, uz*]; base (3, nz, ny), w (nz, ny).
+        Returns (Q_in, Q_out before the shift, W)."""
+        flux = np.zeros(3)
+        self.lib.orc_jet_edges(_p(up[0]), _p(up[1]), _p(up[2]), _p(u[0]), _p(u[1]), _p(u[2]),
+                               _p(np.ascontiguousarray(base)), _p(np.ascontiguousarray(w)),
+                               C.c_double(amp), C.c_double(cph), C.c_double(sph),
+                               C.c_ulonglong(seed), C.c_double(cu), *_dims(u[0]), _bc(bc),
+                               _p(flux))
+        return tuple(flux)
+
+    def weighted_mean(self, f, mirror):
+        self.lib.orc_weighted_mean.restype = C.c_double
+        return self.lib.orc_weighted_mean(_p(f), *_dims(f), _bc(mirror))
+
 
 class Reference:
     """The reference's own Fortran routines (oracle/_ref/libo3dref.so)."""

@@ -61,6 +61,7 @@ class O3DStepInfo(C.Structure):
         ("vel_min", C.c_double * 3), ("vel_max", C.c_double * 3), ("vel_amax", C.c_double * 3),
         ("cfl", C.c_double * 3),
         ("nu_t", C.c_double * 6),
+        ("flow_rate", C.c_double * 2),
     ]
 
     def to_dict(self) -> dict:
@@ -74,6 +75,7 @@ class O3DStepInfo(C.Structure):
 # Exported symbols of include/o3d.h (checked by the CPU test suite).
 EXPORTS = [
     "o3d_init", "o3d_set_stream", "o3d_last_error", "o3d_sync", "o3d_version",
+    "o3d_stream_probe",
     "o3d_der", "o3d_der_f32", "o3d_divergence", "o3d_nu_t", "o3d_predict_velocity",
     "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity",
     "o3d_function_stats",
@@ -82,6 +84,7 @@ EXPORTS = [
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
     "o3d_warm_features", "o3d_solver_history", "o3d_multigrid", "o3d_poisson_multigrid",
+    "o3d_solver_set_inflow",
 ]
 NN_EXPORTS = ["o3d_nn_create", "o3d_nn_destroy", "o3d_nn_resize", "o3d_nn_num_params",
               "o3d_nn_get_params", "o3d_nn_set_params", "o3d_nn_get_grads", "o3d_nn_zero_grad",
@@ -106,7 +109,7 @@ def load(path: str = LIB_PATH):
     for name in ("o3d_solver_destroy", "o3d_solver_upload", "o3d_solver_download",
                  "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_sor_timing",
                  "o3d_solver_set_warm_model", "o3d_solver_capture", "o3d_warm_features",
-                 "o3d_solver_history"):
+                 "o3d_solver_history", "o3d_solver_set_inflow"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -206,18 +209,23 @@ def poisson(pp, rhs, d, omega, eps, kmax, idyn, solver):
 
 
 def poisson_multigrid(pp, rhs, d, solver, npre=5, npost=4, tol=1e-8, max_cycles=100,
-                      nlevels=100):
+                      nlevels=100, with_reason=False):
     """Multigrid solve of the SOR solvers' Poisson problem (pp in place,
-    from its current value) until max |rhs - L pp| < tol or max_cycles
-    cycles; the interface of solve_poisson_multigrid
-    (src/poisson_multigrid.f90:10-18).  Returns (cycles, max |r|)."""
+    from its current value) until max |rhs - L pp| < tol, stagnation or
+    max_cycles cycles; the interface of solve_poisson_multigrid
+    (src/poisson_multigrid.f90:10-18).  Returns (cycles, max |r|), or
+    (cycles, max |r|, reason) with with_reason (0 tol, 1 stagnation,
+    2 max_cycles)."""
     nx, ny, nz = _dims(pp)
     cy = C.c_int(0)
     rm = C.c_double(0)
+    rs = C.c_int(-1)
     code = {"0000": 0, "0011": 1, "111111": 2}.get(solver, solver)
     check(load().o3d_poisson_multigrid(ptr(pp), ptr(rhs), *map(_d, d), nx, ny, nz, int(code),
                                        int(nlevels), int(npre), int(npost), _d(tol),
-                                       int(max_cycles), C.byref(cy), C.byref(rm)))
+                                       int(max_cycles), C.byref(cy), C.byref(rm), C.byref(rs)))
+    if with_reason:
+        return cy.value, rm.value, rs.value
     return cy.value, rm.value
 
 
@@ -306,9 +314,22 @@ def q_criterion(ux, uy, uz, d, bc):
     return q
 
 
+def stream_probe(bytes_per_buffer=1 << 30, reps=20):
+    """Hand-written streaming triad (2 reads + 1 write per element) over three
+    device buffers: (GB/s, ms per launch)."""
+    g = C.c_double(0)
+    m = C.c_double(0)
+    check(load().o3d_stream_probe(C.c_size_t(int(bytes_per_buffer)), int(reps), C.byref(g),
+                                  C.byref(m)))
+    return g.value, m.value
+
+
 def statistics(ux, uy, uz, d, re, t, bc):
+    """statistics_calc (src/utils.f90:243-375): the 17 values of a stats.dat
+    row."""
     nx, ny, nz = _dims(ux)
     out = np.zeros(17)
-    check(load().o3d_statistics(ptr(ux), ptr(uy), ptr(uz), nx, ny, nz, *map(_d, d), _d(re),
-                                _d(t), _bc(bc), out.ctypes.data_as(_dp)))
+    fn = load().o3d_statistics
+    check(fn(ptr(ux), ptr(uy), ptr(uz), nx, ny, nz, *map(_d, d), _d(re), _d(t), _bc(bc),
+             out.ctypes.data_as(_dp)))
     return out

@@ -5,9 +5,14 @@ written by fields_io.write_fields, so the reference driver can restart from
 it with typesim = 0) plus a sidecar `<path>.history.npz` holding what the
 reference does not save: the Adams-Bashforth right-hand sides of the last
 three steps, the history depth, omega, the step number and the time
-(o3d_solver_history).  Restoring both continues a run bit for bit; a
-restart from fields.bin alone starts again with an Euler step, as the
-reference's restart does.
+(o3d_solver_history).  With the previous pressure as warm start (the
+reference's behaviour) restoring both continues a run bit for bit.  The
+extrapolated and learned warm starts are not saved: they re-prime after a
+restart (the first step starts from the previous pressure), so such a run
+continues exactly as every restart from the same checkpoint does, and
+differs from the uninterrupted run only by where its solves start (all end
+on the same stop rule).  A restart from fields.bin alone starts again with
+an Euler step, as the reference's restart does.
 """
 from __future__ import annotations
 

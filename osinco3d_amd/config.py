@@ -29,6 +29,7 @@ NAMELIST_GROUPS: List[Tuple[str, Tuple[str, ...]]] = [
 
 PERIODIC = 0   # src/initialization.f90:77
 FREE_SLIP = 1  # src/initialization.f90:78
+DIRICHLET = 2  # x only; listed by the reference's README, refused by its schemes()
 
 
 @dataclasses.dataclass
@@ -115,7 +116,8 @@ class Config:
         out = []
         for a, b, name in ((self.nbcx1, self.nbcxn, "x"), (self.nbcy1, self.nbcyn, "y"),
                            (self.nbcz1, self.nbczn, "z")):
-            if a != b or a not in (PERIODIC, FREE_SLIP):
+            ok = (PERIODIC, FREE_SLIP, DIRICHLET) if name == "x" else (PERIODIC, FREE_SLIP)
+            if a != b or a not in ok:
                 raise ValueError(f"Unrecognized boundary types for {name}-axis: {a}, {b}")
             out.append(a)
         return tuple(out)
@@ -123,7 +125,11 @@ class Config:
     def poisson_kind(self) -> str:
         """Which SOR solver schemes() binds: it looks at x and y only
         (src/initialization.f90:283-301).  '' means no solver is bound."""
-        bx, by, _ = self.bc()
+        bx, by, bz = self.bc()
+        if bx == DIRICHLET:
+            # The x-Dirichlet kinds (no reference counterpart): 3 + (y
+            # mirrored) + 2 (z mirrored), pressure mirrored in x.
+            return 3 + (by != PERIODIC) + 2 * (bz != PERIODIC)
         if bx == PERIODIC and by == PERIODIC:
             return "0000"
         if bx == PERIODIC and by == FREE_SLIP:
@@ -242,6 +248,25 @@ def mixing_layer_config(nx: int = 384, ny: int = 256, nz: int = 128, steps: int 
                  x0=0.0, y0=-20.0, z0=0.0,
                  nbcy1=FREE_SLIP, nbcyn=FREE_SLIP,
                  itscheme=2, itstart=1, itstop=steps,
+                 omega=sor_omega(max(nx, ny, nz)), idyn=0, eps=eps, kmax=kmax,
+                 nfre=steps + 1000000, nsve=steps + 1000000, iles=1, cs=cs)
+    cfg.dt = cfl * min(cfg.spacing()) / cfg.u0
+    return cfg
+
+
+def jet_config(nx: int = 512, ny: int = 384, nz: int = 256, steps: int = 500,
+               cfl: float = 0.25, re: float = 3000.0, cs: float = 0.17, eps: float = 1e-8,
+               kmax: int = 5000) -> Config:
+    """Planar jet with x-Dirichlet inflow / outflow: jet width l0 = 1 and
+    centre velocity u0 = 1 on a 16 x 12 x 8 box (y in [-6, 6], z exactly
+    periodic, free-slip y), coflow u0 / 10 (the reference's planar-jet
+    profile with ratio = 0.1, typesim = 3), Smagorinsky LES, AB3."""
+    zl = exact_periodic_length(nz, 8.0)
+    cfg = Config(u0=1.0, l0=1.0, re=re, typesim=3, ratio=0.1,
+                 nx=nx, ny=ny, nz=nz, xlx=16.0, yly=12.0, zlz=zl,
+                 x0=0.0, y0=-6.0, z0=0.0,
+                 nbcx1=DIRICHLET, nbcxn=DIRICHLET, nbcy1=FREE_SLIP, nbcyn=FREE_SLIP,
+                 itscheme=3, itstart=1, itstop=steps,
                  omega=sor_omega(max(nx, ny, nz)), idyn=0, eps=eps, kmax=kmax,
                  nfre=steps + 1000000, nsve=steps + 1000000, iles=1, cs=cs)
     cfg.dt = cfl * min(cfg.spacing()) / cfg.u0

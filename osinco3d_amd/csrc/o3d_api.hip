@@ -91,11 +91,16 @@ int check_grid(int nx, int ny, int nz) {
   return 0;
 }
 
-int check_bc(const int *bc) {
+// xdir: the caller supports x-Dirichlet (DIRICHLET on axis 0 only).
+int check_bc(const int *bc, bool xdir = false) {
   if (!bc) return fail("bc is NULL");
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
+    if (bc[a] == DIRICHLET && a == 0 && xdir) continue;
     if (bc[a] != PERIODIC && bc[a] != FREE_SLIP)
-      return fail("Unrecognized boundary type %d on axis %d", bc[a], a);
+      return fail("Unrecognized boundary type %d on axis %d%s", bc[a], a,
+                  bc[a] == DIRICHLET ? (a == 0 ? " (x-Dirichlet is not supported by this entry "
+                                                 "point)" : " (Dirichlet only on x)") : "");
+  }
   return 0;
 }
 
@@ -171,8 +176,11 @@ void stats6_from(const FieldPartial &r, int nx, int ny, int nz, double *st) {
   st[5] = ix / (nx * ny) + 1;
 }
 
-// Solver kind that schemes() binds from the x and y boundary types.
+// Solver kind that schemes() binds from the x and y boundary types (the
+// x-Dirichlet kinds from y and z).
 int solver_from_bc(const int *bc) {
+  if (bc[0] == DIRICHLET)
+    return SOR_XDIR + (bc[1] != PERIODIC ? 1 : 0) + (bc[2] != PERIODIC ? 2 : 0);
   if (bc[0] == PERIODIC && bc[1] == PERIODIC) return SOR_0000;
   if (bc[0] == PERIODIC && bc[1] == FREE_SLIP) return SOR_0011;
   if (bc[0] == FREE_SLIP && bc[1] == FREE_SLIP) return SOR_111111;
@@ -203,16 +211,14 @@ SorConst sor_const(double dx, double dy, double dz) {
 
 // Batched launcher of red-black iterations with device-side stop control.
 // Buffers: buf[0] holds the start iterate; launch L reads buf[L & 1] and
-// writes buf[(L + 1) & 1].  With `pair` (FP32 only) each launch runs two
-// iterations (launch_sor2_iteration).  With `split` (T == TR) each launch
-// is one iteration of the colour-split in-place sweeps on `sa` (buf unused).
+// writes buf[(L + 1) & 1].  With `split` (T == TR) each launch is one
+// iteration of the colour-split in-place sweeps on `sa` (buf unused).
 template <typename T, typename TR>
 struct SorRun {
   SorArgs<T, TR> a;
   SorState *st = nullptr;    // device
   SorState *hst = nullptr;   // pinned host copy
   T *buf[2] = {nullptr, nullptr};
-  bool pair = false;
   bool split = false;
   SorSplitArgs<T> sa{};
   // Timing of complete batches (all launches did their iterations).
@@ -232,21 +238,17 @@ struct SorRun {
         return launch_sor_split_iteration<T>(sa, g_stream);
       }
     }
-    if constexpr (std::is_same<T, float>::value && std::is_same<TR, float>::value) {
-      if (pair) return launch_sor2_iteration(a, g_stream);
-    }
     return launch_sor_iteration<T, TR>(a, g_stream);
   }
 
   // Buffer holding the result after `sweeps` iterations.
-  T *result(int sweeps) const { return buf[(pair ? sweeps / 2 : sweeps) & 1]; }
+  T *result(int sweeps) const { return buf[sweeps & 1]; }
 
   int run(double omega, int first_batch, int *sweeps_out) {
     HIPCHK(launch_sor_init(st, omega, 0, g_stream));
     if (split) HIPCHK(launch_sor_split_init(sa.dslot, g_stream));
-    const int step = pair ? 2 : 1;
     int it = 1, L = 0;
-    int batch = std::max(1, first_batch / step);
+    int batch = std::max(1, first_batch);
     int done_before = 0, launches_before = 0;
     bool mid_set = false;
     while (true) {
@@ -258,12 +260,12 @@ struct SorRun {
         a.pout = buf[(L + 1) & 1];
         HIPCHK(launch());
         ++L;
-        it += step;
-        if (mid && !pair && L == a.gs && a.gs <= a.kmax && done_before == 0) {
+        ++it;
+        if (mid && L == a.gs && a.gs <= a.kmax && done_before == 0) {
           HIPCHK(hipEventRecord(mid, g_stream));
           mid_set = true;
         }
-        if (it - step > a.kmax) break;  // that launch closes the state
+        if (it - 1 > a.kmax) break;  // that launch closes the state
       }
       HIPCHK(hipEventRecord(e1, g_stream));
       HIPCHK(hipMemcpyAsync(hst, st, sizeof(SorState), hipMemcpyDeviceToHost, g_stream));
@@ -278,7 +280,7 @@ struct SorRun {
         done_before = launched;
         launches_before = L;
         if (launched > a.kmax) return fail("SOR state did not close after kmax");
-        batch = std::max(1, 8 / step);
+        batch = 8;
         continue;
       }
       if (mid_set && done_before == 0 && hst->sweeps >= a.gs) {
@@ -321,12 +323,66 @@ extern "C" int o3d_sync(void) {
 
 extern "C" const char *o3d_version(void) { return "osinco3d-amd 0.1 gfx950"; }
 
+// ============================================================ measurement
+namespace {
+// Streaming triad c = a + s b over float4 vectors, grid-stride: two reads
+// and one write per element, no reuse, the HBM ceiling the stencil and SOR
+// kernels are compared with.
+__global__ void __launch_bounds__(256) stream_triad_kernel(const float4 *__restrict__ a,
+                                                           const float4 *__restrict__ b,
+                                                           float4 *__restrict__ c, size_t n4,
+                                                           float s) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const float4 x = a[i], y = b[i];
+    c[i] = make_float4(x.x + s * y.x, x.y + s * y.y, x.z + s * y.z, x.w + s * y.w);
+  }
+}
+}  // namespace
+
+extern "C" int o3d_stream_probe(size_t bytes_per_buffer, int reps, double *gbs, double *ms) {
+  CHK(ensure_init());
+  if (reps < 1 || bytes_per_buffer < (1u << 20)) return fail("stream probe: bad arguments");
+  const size_t n4 = bytes_per_buffer / 16;
+  float4 *buf[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  for (int q = 0; q < 3 && rc == 0; ++q)
+    if (hipMalloc((void **)&buf[q], n4 * 16) != hipSuccess) rc = fail("stream probe: hipMalloc");
+  if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail("stream probe: events");
+  if (rc == 0) {
+    for (int q = 0; q < 3; ++q) (void)hipMemsetAsync(buf[q], 0, n4 * 16, g_stream);
+    const int blocks = 256 * 16;
+    for (int w = 0; w < 3; ++w)
+      stream_triad_kernel<<<blocks, 256, 0, g_stream>>>(buf[0], buf[1], buf[2], n4, 0.5f);
+    (void)hipEventRecord(e0, g_stream);
+    for (int r = 0; r < reps; ++r)
+      stream_triad_kernel<<<blocks, 256, 0, g_stream>>>(buf[r & 1], buf[1 - (r & 1)], buf[2],
+                                                        n4, 0.5f);
+    (void)hipEventRecord(e1, g_stream);
+    float t = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      rc = fail("stream probe: timing failed: %s", hipGetErrorString(hipGetLastError()));
+    else {
+      const double per = (double)t / reps;
+      if (ms) *ms = per;
+      if (gbs) *gbs = 3.0 * (double)n4 * 16.0 / (per * 1e-3) / 1e9;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (int q = 0; q < 3; ++q)
+    if (buf[q]) (void)hipFree(buf[q]);
+  return rc;
+}
+
 // ================================================================ operators
 extern "C" int o3d_der(int which, double *df, const double *f, int nx, int ny, int nz,
                        double d) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  if (which < 0 || which > 19) return fail("derivative selector %d out of range", which);
+  if (which < 0 || which > 21) return fail("derivative selector %d out of range", which);
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
   const double *fd;
@@ -341,7 +397,7 @@ extern "C" int o3d_der_f32(int which, float *df, const float *f, int nx, int ny,
                            double d) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  if (which < 0 || which > 19) return fail("derivative selector %d out of range", which);
+  if (which < 0 || which > 21) return fail("derivative selector %d out of range", which);
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
   const float *fd;
@@ -367,7 +423,7 @@ extern "C" int o3d_divergence(double *divf, const double *fx, const double *fy, 
                               const int *bc, int odd, double divisor, double *stats6) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  CHK(check_bc(bc));
+  CHK(check_bc(bc, true));
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
   DivArgs<double> a;
@@ -425,7 +481,7 @@ extern "C" int o3d_nu_t(double *nu_t, const double *ux, const double *uy, const 
                         int nz, const int *bc, double *stats6) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  CHK(check_bc(bc));
+  CHK(check_bc(bc, true));
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
   const double *u[3];
@@ -464,7 +520,7 @@ extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz
                                     double *nu_t, const int *bc) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  CHK(check_bc(bc));
+  CHK(check_bc(bc, true));
   // Coefficient choice, src/integration.f90:84-105.
   int ci;
   if (itscheme == 1 || itime == 1) ci = 0;
@@ -535,20 +591,22 @@ SorSplitArgs<T> split_args(const SorArgs<T, T> &a, T *red, T *black, const T *rh
 }
 
 namespace {
-// variant: 0 one fused iteration per launch, 1 two per launch (FP32),
-// 2 colour-split half sweeps, -1 the colour-split sweeps where the grid
-// allows them, else the fused kernel (identical iterates in every case).
+// variant: 0 one fused iteration per launch, 2 colour-split half sweeps,
+// -1 the colour-split sweeps where the grid allows them, else the fused
+// kernel (identical iterates in every case).
 template <typename T>
 int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, int ny, int nz,
                 double *omega, double eps, int kmax, int idyn, int solver, int *exit_iter,
                 double *dmax, int *reason, Staged &s, int variant = -1) {
-  if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
+  if (solver < 0 || solver >= SOR_KINDS)
+    return fail("no Poisson solver bound for these boundaries");
   CHK(check_red_black(solver, nx, ny, nz));
   if (kmax < 1) return fail("kmax must be >= 1");
+  if (!sor_buffer_fits<T, T>(nx, ny, nz))
+    return fail("SOR: a %dx%dx%d field exceeds the 2 GiB a buffer descriptor addresses", nx, ny,
+                nz);
   const int n = nx * ny * nz;
-  const bool pair = variant == 1;
   SorRun<T, T> r;
-  r.pair = pair;
   CHK(s.scratch(1, &r.st));
   HIPCHK(hipHostMalloc((void **)&r.hst, sizeof(SorState)));
   std::vector<hipEvent_t> ev(2);
@@ -565,7 +623,7 @@ int poisson_dev(T *pp, const T *rhs, double dx, double dy, double dz, int nx, in
   a.log_len = 0;
   a.nx = nx; a.ny = ny; a.nz = nz;
   solver_periodic(solver, a.per);
-  a.tl = pair ? sor2_tiling(nx, ny, nz) : sor_tiling<T, T>(nx, ny, nz);
+  a.tl = sor_tiling<T, T>(nx, ny, nz);
   const SorConst c = sor_const(dx, dy, dz);
   a.ox = c.ox; a.oy = c.oy; a.oz = c.oz; a.A = c.A;
   a.eps = eps;
@@ -662,10 +720,11 @@ extern "C" int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, d
 extern "C" int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, double dy,
                                      double dz, int nx, int ny, int nz, int solver, int nlevels,
                                      int npre, int npost, double tol, int max_cycles,
-                                     int *cycles, double *resmax) {
+                                     int *cycles, double *resmax, int *reason) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  if (solver < 0 || solver > 2) return fail("no Poisson solver bound for these boundaries");
+  if (solver < 0 || solver >= SOR_KINDS)
+    return fail("no Poisson solver bound for these boundaries");
   CHK(check_red_black(solver, nx, ny, nz));
   if (!sor_split_supported<float>(nx, ny, nz) || ny % 2 || nz % 2)
     return fail("multigrid needs nx a multiple of 8 and ny, nz even (grid %dx%dx%d)", nx, ny,
@@ -731,8 +790,8 @@ extern "C" int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, d
   const size_t half = n / 2;
   run.split = true;
   run.sa = split_args<float>(a, E, E + half, R, R + half, dslot);
-  int rc = 0, done = 0;
-  double rmax = 0.0;
+  int rc = 0, done = 0, why = 2;
+  double rmax = 0.0, rmax_old = 1609.0;
   while (rc == 0) {
     // r = rhs - L pp (FP64 arithmetic), colour-split in FP32, and e = 0.
     if (launch_residual_norm_pair<double>(p, p, rhs_d, part, fin, nx, ny, nz, per, c.ox, c.oy,
@@ -744,7 +803,15 @@ extern "C" int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, d
       break;
     }
     rmax = hfin[2];
-    if (!(rmax >= tol) || done >= max_cycles) break;  // converged (or NaN), or out of cycles
+    // The stop rules of the SOR solvers (src/poisson.f90:95-122) on max |r|:
+    // below tol (or NaN), stagnation (a change below tol / 1000 between
+    // cycles: on a mirrored grid the part of an incompatible rhs that the
+    // operator cannot reach keeps max |r| from falling below tol), or out
+    // of cycles.
+    if (!(rmax >= tol)) { why = 0; break; }
+    if (done > 0 && fabs(rmax - rmax_old) < tol / 1000.0) { why = 1; break; }
+    if (done >= max_cycles) { why = 2; break; }
+    rmax_old = rmax;
     if (H.levels > 0) {
       if (mg_pool_split(R, H.f[0], nx, ny, nz, g_stream) != hipSuccess ||
           mg_solve(H, 1, std::max(npre, 1), std::max(npost, 1), g_stream) != hipSuccess ||
@@ -767,6 +834,7 @@ extern "C" int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, d
   if (rc != 0) return rc;
   if (cycles) *cycles = done;
   if (resmax) *resmax = rmax;
+  if (reason) *reason = why;
   return s.finish();
 }
 
@@ -777,9 +845,7 @@ extern "C" int o3d_poisson_f32(float *pp, const float *rhs, double dx, double dy
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
   if (!omega) return fail("omega is NULL");
-  if (variant < 0 || variant > 2) return fail("SOR kernel variant %d unknown", variant);
-  if (variant == 1 && !sor2_supported(nx, ny, nz))
-    return fail("two iterations per launch need every axis >= 8 (grid %dx%dx%d)", nx, ny, nz);
+  if (variant != 0 && variant != 2) return fail("SOR kernel variant %d unknown", variant);
   if (variant == 2 && !sor_split_supported<float>(nx, ny, nz))
     return fail("colour-split sweeps need nx a multiple of 8 (nx = %d)", nx);
   const size_t n = (size_t)nx * ny * nz;
@@ -832,7 +898,7 @@ extern "C" int o3d_correct_velocity(double *ux, double *uy, double *uz, const do
                                     int nx, int ny, int nz, const int *bc, double *vstats) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  CHK(check_bc(bc));
+  CHK(check_bc(bc, true));
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
   CorrectArgs<double, double> a;
@@ -889,9 +955,13 @@ extern "C" int o3d_function_stats(const double *f, int nx, int ny, int nz, doubl
 }
 
 namespace {
+// statistics_calc on device fields with the per-block partial sums in
+// `part` (16 doubles per block of make_tiling) and the 16 finalized sums in
+// `fin`; out17 is host memory.
 template <typename T>
-int statistics_dev(const T *u[3], int nx, int ny, int nz, double dx, double dy, double dz,
-                   double re, double t, const int *bc, double *out17, Staged &s) {
+int statistics_run(const T *u[3], int nx, int ny, int nz, double dx, double dy, double dz,
+                   double re, double t, const int *bc, double *out17, double *part,
+                   double *fin) {
   StatsArgs<T> a;
   for (int c = 0; c < 3; ++c) a.u[c] = u[c];
   a.nx = nx; a.ny = ny; a.nz = nz;
@@ -901,14 +971,12 @@ int statistics_dev(const T *u[3], int nx, int ny, int nz, double dx, double dy, 
   fill_coef<T>(a.d1, h, false);
   fill_coef<T>(a.d2, h, true);
   a.xnu = 1.0 / re;
-  double *fin;
-  CHK(s.scratch((size_t)16 * a.tl.blocks(), &a.part));
-  CHK(s.scratch(16, &fin));
+  a.part = part;
   HIPCHK(launch_statistics<T>(a, g_stream));
   HIPCHK(launch_finalize_sums(a.part, a.tl.blocks(), 16, fin, g_stream));
   double h16[16];
   HIPCHK(hipMemcpyAsync(h16, fin, sizeof(h16), hipMemcpyDeviceToHost, g_stream));
-  CHK(s.finish());
+  HIPCHK(hipStreamSynchronize(g_stream));
   const double N = (double)((long long)nx * ny * nz);
   out17[0] = t;
   out17[1] = h16[0] / N;  // E_k
@@ -919,6 +987,16 @@ int statistics_dev(const T *u[3], int nx, int ny, int nz, double dx, double dy, 
   for (int m = 0; m < 9; ++m) out17[8 + m] = h16[7 + m] / N;
   return 0;
 }
+
+template <typename T>
+int statistics_dev(const T *u[3], int nx, int ny, int nz, double dx, double dy, double dz,
+                   double re, double t, const int *bc, double *out17, Staged &s) {
+  double *part, *fin;
+  CHK(s.scratch((size_t)16 * make_tiling(nx, ny, nz).blocks(), &part));
+  CHK(s.scratch(16, &fin));
+  CHK(statistics_run<T>(u, nx, ny, nz, dx, dy, dz, re, t, bc, out17, part, fin));
+  return s.finish();
+}
 }  // namespace
 
 extern "C" int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
@@ -926,7 +1004,7 @@ extern "C" int o3d_statistics(const double *ux, const double *uy, const double *
                               double t, const int *bc, double *out17) {
   CHK(ensure_init());
   CHK(check_grid(nx, ny, nz));
-  CHK(check_bc(bc));
+  CHK(check_bc(bc, true));
   if (!out17) return fail("out17 is NULL");
   const size_t n = (size_t)nx * ny * nz;
   Staged s;
@@ -1086,6 +1164,17 @@ struct Solver {
   double *hnpart = nullptr;    // pinned copy
   double *epart = nullptr;     // sum-of-squares partials of e_prev
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
+  double *mpart = nullptr;     // mean-pressure partials (mixed form, every axis periodic)
+  // x-Dirichlet inflow / outflow (bc[0] == DIRICHLET), set by o3d_solver_set_inflow.
+  bool xdir = false, in_set = false;
+  double *in_base = nullptr;   // 3 planes of ny nz values: the inflow profile
+  double *in_w = nullptr;      // plane: noise weight
+  double *in_pl = nullptr;     // 2 planes (inflow u, outflow u before the shift) + 2 nz rows
+  double *in_flux = nullptr;   // Q_in, Q_out, shift
+  double *hflux = nullptr;     // pinned copy, valid after the step's final sync
+  double *wpart = nullptr;     // weighted-sum partials (1024) + the sum of the weights + mean
+  double in_amp = 0.0, in_period = 1.0, in_cu = 0.0, in_W = 0.0;
+  unsigned long long in_seed = 0;
   int hist = 0;                // steps with an increment available
   std::vector<void *> allocs;
 
@@ -1114,8 +1203,196 @@ struct Solver {
     if (hst) (void)hipHostFree(hst);
     if (hnpart) (void)hipHostFree(hnpart);
     if (hepart) (void)hipHostFree(hepart);
+    if (hflux) (void)hipHostFree(hflux);
   }
 };
+
+// ---- x-Dirichlet inflow / outflow (no counterpart in the reference) --------
+// Flux weight of node q of an axis of n nodes: 1/2 on the end nodes of a
+// mirrored axis (the trapezoid weights that make the mirrored 7-point
+// operator's rows sum to zero), else 1.
+__host__ __device__ __forceinline__ double edge_weight(int q, int n, int mirrored) {
+  return (mirrored && (q == 0 || q == n - 1)) ? 0.5 : 1.0;
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014) as a counter-based generator: output
+// number counter + 1 of the stream seeded with `seed`, as a uniform double
+// in [-1, 1) from the top 53 bits.
+__host__ __device__ __forceinline__ double noise_uniform(unsigned long long seed,
+                                                         long long counter) {
+  unsigned long long z = seed + (unsigned long long)(counter + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+}
+
+// After predict, one thread per (j, k): the inflow plane i = 0 gets the
+// profile plus the noise (amp w)(x1 C + x2 S) with x1, x2 drawn per node and
+// component, C, S the cosine and sine of the inflow phase; the outflow plane
+// i = nx - 1 gets first-order upwind convection at the convective velocity
+// (cu = Uc dt / dx) from the velocity before the step.  The x components of
+// both planes go to pin / pout for the flux balance.
+template <typename T>
+__global__ void __launch_bounds__(256) jet_edges_kernel(
+    T *__restrict__ up0, T *__restrict__ up1, T *__restrict__ up2, const T *__restrict__ u0,
+    const T *__restrict__ u1, const T *__restrict__ u2, const double *__restrict__ base,
+    const double *__restrict__ w, double amp, double cph, double sph, unsigned long long seed,
+    double cu, int nx, int ny, int nz, double *__restrict__ pin, double *__restrict__ pout) {
+  const int P = ny * nz;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  const int j = q % ny, k = q / ny;
+  const int row = nx * (j + ny * k);
+  T *up[3] = {up0, up1, up2};
+  const T *u[3] = {u0, u1, u2};
+  const double aw = amp * w[q];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long long cnt = 2 * (3 * (long long)q + c);
+    const double x1 = noise_uniform(seed, cnt), x2 = noise_uniform(seed, cnt + 1);
+    const double vin = base[c * P + q] + aw * (x1 * cph + x2 * sph);
+    const double ub = (double)u[c][row + nx - 1], ub1 = (double)u[c][row + nx - 2];
+    const double vout = ub - cu * (ub - ub1);
+    up[c][row] = T(vin);
+    up[c][row + nx - 1] = T(vout);
+    if (c == 0) {
+      pin[q] = (double)T(vin);
+      pout[q] = (double)T(vout);
+    }
+  }
+}
+
+// The plane sums in a fixed order (per z row the sequential sum over j of
+// wy v, then one thread's sequential sum over k of wz row), and the shift
+// (Q_in - Q_out) / W that makes the outflow flux equal the inflow flux.
+__global__ void __launch_bounds__(256) jet_flux_kernel(const double *__restrict__ pin,
+                                                       const double *__restrict__ pout,
+                                                       double *__restrict__ rows, int ny,
+                                                       int nz, int my, int mz, double W,
+                                                       double *__restrict__ flux) {
+  for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+    double a = 0.0, b = 0.0;
+    for (int j = 0; j < ny; ++j) {
+      const double wy = edge_weight(j, ny, my);
+      a = a + wy * pin[j + ny * k];
+      b = b + wy * pout[j + ny * k];
+    }
+    rows[k] = a;
+    rows[nz + k] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double qi = 0.0, qo = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      const double wz = edge_weight(k, nz, mz);
+      qi = qi + wz * rows[k];
+      qo = qo + wz * rows[nz + k];
+    }
+    flux[0] = qi;
+    flux[1] = qo;
+    flux[2] = (qi - qo) / W;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) jet_shift_kernel(T *__restrict__ up0,
+                                                        const double *__restrict__ pout,
+                                                        const double *__restrict__ flux, int nx,
+                                                        int ny, int nz) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ny * nz) return;
+  const int j = q % ny, k = q / ny;
+  up0[nx * (j + ny * k) + nx - 1] = T(pout[q] + flux[2]);
+}
+
+// Weighted sum of a field (f null: of the weights alone) with the trapezoid
+// weights of the mirrored axes: kWsumBlocks chunks of ceil(n / kWsumBlocks)
+// cells, thread t of a chunk summing cells lo + t, lo + t + 256, ... in
+// order, the chunk total by block_sum<4>.
+constexpr int kWsumBlocks = 1024;
+template <typename T>
+__global__ void __launch_bounds__(256) wsum_kernel(const T *__restrict__ f, int nx, int ny,
+                                                   int nz, int mx, int my, int mz,
+                                                   double *__restrict__ part) {
+  __shared__ double sv[4];
+  const long long n = (long long)nx * ny * nz;
+  const long long per = (n + kWsumBlocks - 1) / kWsumBlocks;
+  const long long lo = (long long)blockIdx.x * per;
+  const long long hi = lo + per < n ? lo + per : n;
+  double acc = 0.0;
+  for (long long q = lo + threadIdx.x; q < hi; q += 256) {
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long long)nx * ny));
+    const double wq = (edge_weight(i, nx, mx) * edge_weight(j, ny, my)) * edge_weight(k, nz, mz);
+    acc = acc + (f ? wq * (double)f[q] : wq);
+  }
+  acc = block_sum<4>(acc, sv);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// The chunk totals summed by 1024 threads (one each) with block_sum<16>;
+// out = total, or total / *den when den is given.
+__global__ void __launch_bounds__(1024) wsum_final_kernel(const double *__restrict__ part,
+                                                          const double *__restrict__ den,
+                                                          double *__restrict__ out) {
+  __shared__ double sv[16];
+  double v = 0.0;
+  if (threadIdx.x < kWsumBlocks) v = v + part[threadIdx.x];
+  v = block_sum<16>(v, sv);
+  if (threadIdx.x == 0) *out = den ? v / *den : v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) sub_mean_kernel(T *__restrict__ f, long long n,
+                                                       const double *__restrict__ mean) {
+  const double m = *mean;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    f[q] = T((double)f[q] - m);
+}
+
+// The inflow and outflow planes of u* after predict, with the outflow flux
+// set equal to the inflow flux (x-Dirichlet).
+template <typename T>
+int solver_jet_edges(Solver &S, int itime) {
+  const o3d_params &p = S.p;
+  if (!S.in_set) return fail("x-Dirichlet: no inflow set (o3d_solver_set_inflow)");
+  const int P = p.ny * p.nz;
+  const double ph = 2.0 * M_PI * ((double)itime * p.dt) / S.in_period;
+  const double cph = cos(ph), sph = sin(ph);
+  double *pin = S.in_pl, *pout = S.in_pl + P, *rows = S.in_pl + 2 * (size_t)P;
+  const int g = (P + 255) / 256;
+  jet_edges_kernel<T><<<g, 256, 0, g_stream>>>(
+      (T *)S.up[0], (T *)S.up[1], (T *)S.up[2], (const T *)S.u[0], (const T *)S.u[1],
+      (const T *)S.u[2], S.in_base, S.in_w, S.in_amp, cph, sph, S.in_seed, S.in_cu, p.nx, p.ny,
+      p.nz, pin, pout);
+  HIPCHK(hipGetLastError());
+  jet_flux_kernel<<<1, 256, 0, g_stream>>>(pin, pout, rows, p.ny, p.nz, p.bc[1] != PERIODIC,
+                                           p.bc[2] != PERIODIC, S.in_W, S.in_flux);
+  HIPCHK(hipGetLastError());
+  jet_shift_kernel<T><<<g, 256, 0, g_stream>>>((T *)S.up[0], pout, S.in_flux, p.nx, p.ny, p.nz);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The compatibility projection of the x-Dirichlet Poisson rhs: minus its
+// weighted mean (trapezoid weights on the mirrored axes), the part of the
+// rhs the mirrored operator cannot reach, so SOR converges to eps.
+template <typename T>
+int solver_rhs_project(Solver &S) {
+  const o3d_params &p = S.p;
+  const int mx = 1, my = p.bc[1] != PERIODIC, mz = p.bc[2] != PERIODIC;
+  wsum_kernel<T><<<kWsumBlocks, 256, 0, g_stream>>>((const T *)S.rhs, p.nx, p.ny, p.nz, mx, my,
+                                                     mz, S.wpart);
+  HIPCHK(hipGetLastError());
+  wsum_final_kernel<<<1, 1024, 0, g_stream>>>(S.wpart, S.wpart + kWsumBlocks,
+                                              S.wpart + kWsumBlocks + 1);
+  HIPCHK(hipGetLastError());
+  sub_mean_kernel<T><<<2048, 256, 0, g_stream>>>((T *)S.rhs, (long long)S.n,
+                                                  S.wpart + kWsumBlocks + 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 
 template <typename T>
 int solver_predict(Solver &S, int itime) {
@@ -1253,6 +1530,55 @@ int solver_poisson_f64(Solver &S, o3d_step_info *info) {
 // Mixed precision: FP32 red-black sweeps on the correction e of A e = r,
 // r = rhs - A p formed in FP64; p += e in FP64; convergence judged by an FP64
 // red-black sweep of p with the reference's stop rules.
+// Periodic mean-pressure removal (mixed forms, every axis periodic: the
+// pressure is defined up to a constant, which the FP32 sweeps let drift).
+// The FP64 sum is formed in a fixed order: per block over a contiguous
+// chunk (2 doubles per thread and load), then the partials in index order.
+constexpr int kMeanBlocks = 1024;
+__global__ void __launch_bounds__(256) p_sum_kernel(const double *__restrict__ p, int n,
+                                                    double *__restrict__ part) {
+  __shared__ double sv[4];
+  const int n2 = n / 2;
+  const int per = (n2 + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(n2, lo + per);
+  const double2 *p2 = (const double2 *)p;
+  double s = 0.0;
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const double2 v = p2[i];
+    s += v.x;
+    s += v.y;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && (n & 1)) s += p[n - 1];
+  s = block_sum<4>(s, sv);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(256) p_sub_mean_kernel(double *__restrict__ p, int n,
+                                                         const double *__restrict__ part,
+                                                         int nparts) {
+  __shared__ double sv[4];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += part[b];
+  s = block_sum<4>(s, sv);  // the same sum in every block
+  const double mean = s / (double)n;
+  double2 *p2 = (double2 *)p;
+  const int n2 = n / 2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
+    double2 v = p2[i];
+    v.x -= mean;
+    v.y -= mean;
+    p2[i] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) p[n - 1] -= mean;
+}
+
+int remove_pressure_mean(double *p, int n, double *part) {
+  p_sum_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(p, n, part);
+  HIPCHK(hipGetLastError());
+  p_sub_mean_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(p, n, part, kMeanBlocks);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   const o3d_params &p = S.p;
   const int n = S.n;
@@ -1272,8 +1598,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     solver_periodic(S.kind, per);
     // Colour-split correction sweeps where the grid allows (nx a multiple
     // of 8), else one fused iteration per launch; both give the same
-    // iterates.  (Two iterations per launch, sor2_kernel, measured slower
-    // than either at 256^3.)
+    // iterates.
     const bool split = S.dslot != nullptr;
     if (split && cycles == 0 && S.r_ready) {
       // r and e = 0 already written by the warm-start acceptance check.
@@ -1306,7 +1631,6 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     // (smoothing the high-frequency error a warm start leaves), and so does
     // every one that starts from a coarse-grid correction.
     if (cycles == 0 || mgc) r.a.gs = p.smooth_sweeps;
-    r.pair = false;
     if (split) {
       const int h = n / 2;
       r.split = true;
@@ -1356,6 +1680,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   }
   S.last_sweeps[1] = sw32;
   S.r_ready = false;
+  if (S.mpart) CHK(remove_pressure_mean(S.P[S.cur], n, S.mpart));
   info->sor_exit_iter = reason == 2 ? p.kmax + 1 : total;
   info->sor_reason = reason;
   info->sweeps_f64 = sw64;
@@ -1831,7 +2156,9 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   const o3d_params &p = S.p;
   HIPCHK(hipEventRecord(S.ev[2], g_stream));
   CHK(solver_predict<T>(S, itime));
+  if (S.xdir) CHK(solver_jet_edges<T>(S, itime));
   CHK(solver_divergence<T>(S, S.up, (T *)S.rhs, p.dt, S.part_div));
+  if (S.xdir) CHK(solver_rhs_project<T>(S));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
   const bool guess =
       S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE || p.warm_start == O3D_WARM_EXTRAPOLATE2;
@@ -1870,7 +2197,16 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   HIPCHK(hipEventRecord(S.ev[3], g_stream));
   HIPCHK(hipMemcpyAsync(S.hfin, S.fin, 6 * sizeof(FieldPartial), hipMemcpyDeviceToHost,
                         g_stream));
+  if (S.xdir)
+    HIPCHK(hipMemcpyAsync(S.hflux, S.in_flux, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                          g_stream));
   HIPCHK(hipStreamSynchronize(g_stream));
+  if (S.xdir) {
+    info->flow_rate[0] = S.hflux[0] / S.in_W;
+    info->flow_rate[1] = S.hflux[1] / S.in_W;
+  } else {
+    info->flow_rate[0] = info->flow_rate[1] = 0.0;
+  }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
   info->step_ms = ms;
@@ -1922,7 +2258,15 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
   CHK(ensure_init());
   if (!p || !handle) return fail("NULL argument");
   CHK(check_grid(p->nx, p->ny, p->nz));
-  CHK(check_bc(p->bc));
+  CHK(check_bc(p->bc, true));
+  if (p->bc[0] == DIRICHLET && p->nx < 8)
+    return fail("x-Dirichlet needs at least 8 nodes in x (nx = %d)", p->nx);
+  if (p->precision == O3D_FP64 && !sor_buffer_fits<double, double>(p->nx, p->ny, p->nz))
+    return fail("grid %dx%dx%d: an FP64 field exceeds the 2 GiB the SOR kernels' buffer "
+                "descriptors address", p->nx, p->ny, p->nz);
+  if (p->precision == O3D_MIXED && !sor_buffer_fits<float, double>(p->nx, p->ny, p->nz))
+    return fail("grid %dx%dx%d: an FP64 field exceeds the 2 GiB the SOR kernels' buffer "
+                "descriptors address", p->nx, p->ny, p->nz);
   if (p->itscheme < 1 || p->itscheme > 3) {
     fail("itscheme: %d unrecognized", p->itscheme);
     return -2;
@@ -1970,10 +2314,32 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     rc |= S->alloc(&S->R, n);
     if (!rc && sor_split_supported<float>(p->nx, p->ny, p->nz))
       rc |= S->alloc(&S->dslot, sor_split_slot_words());
+    int per[3];
+    solver_periodic(kind, per);
+    if (!rc && per[0] && per[1] && per[2]) rc |= S->alloc(&S->mpart, kMeanBlocks);
   }
   if (!rc && !S->mixed && sor_split_supported<double>(p->nx, p->ny, p->nz)) {
     rc |= S->alloc(&S->dslot, sor_split_slot_words());
     rc |= S->alloc(&S->rhs_split, n);
+  }
+  if (!rc && p->bc[0] == DIRICHLET) {
+    S->xdir = true;
+    const size_t P = (size_t)p->ny * p->nz;
+    rc |= S->alloc(&S->in_base, 3 * P);
+    if (!rc) rc |= S->alloc(&S->in_w, P);
+    if (!rc) rc |= S->alloc(&S->in_pl, 2 * P + 2 * (size_t)p->nz);
+    if (!rc) rc |= S->alloc(&S->in_flux, 3);
+    if (!rc) rc |= S->alloc(&S->wpart, kWsumBlocks + 2);
+    if (!rc && hipHostMalloc((void **)&S->hflux, 3 * sizeof(double)) != hipSuccess)
+      rc = fail("hipHostMalloc failed");
+    if (!rc) {
+      // The sum of the weights, once, by the kernels that form the weighted sums.
+      wsum_kernel<double><<<kWsumBlocks, 256, 0, g_stream>>>(
+          nullptr, p->nx, p->ny, p->nz, 1, p->bc[1] != PERIODIC, p->bc[2] != PERIODIC, S->wpart);
+      wsum_final_kernel<<<1, 1024, 0, g_stream>>>(S->wpart, nullptr, S->wpart + kWsumBlocks);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_stream) != hipSuccess)
+        rc = fail("weight sum failed");
+    }
   }
   const int nb = std::max(S->tl.blocks(), std::max(S->tl_div.blocks(), S->tl_cor.blocks()));
   if (!rc) rc |= S->alloc(&S->part_div, nb);
@@ -2024,6 +2390,17 @@ int download_field(Solver &S, double *dst, const void *src) {
 }
 }  // namespace
 
+namespace {
+// A pressure written from outside (upload, restart) has no increment
+// history: the extrapolated / learned warm start re-primes from it, so a
+// fresh solver and one that has already stepped continue alike.
+void reset_warm_state(Solver &S) {
+  S.hist = 0;
+  S.p0_of = nullptr;
+  S.r_ready = false;
+}
+}  // namespace
+
 extern "C" int o3d_solver_upload(void *handle, const double *ux, const double *uy,
                                  const double *uz, const double *pp) {
   if (!handle) return fail("NULL handle");
@@ -2037,6 +2414,7 @@ extern "C" int o3d_solver_upload(void *handle, const double *ux, const double *u
     CHK(s.in(pp, S.n, &d));
     HIPCHK(hipMemcpyAsync(S.P[S.cur], d, (size_t)S.n * 8, hipMemcpyDeviceToDevice, g_stream));
     CHK(s.finish());
+    reset_warm_state(S);
   }
   return 0;
 }
@@ -2104,6 +2482,7 @@ extern "C" int o3d_solver_history(void *handle, int write, void *hist, double *s
       return fail("history depth %g is not a count", r);
     S.ring = (int)r;
     S.omega = state2[1];
+    reset_warm_state(S);
   }
   for (int age = 0; age < 3; ++age) {
     const int slot = ((S.ring - 1 - age) % 3 + 3) % 3;
@@ -2164,6 +2543,41 @@ extern "C" int o3d_warm_features(const float *r, const float *eprev, const float
   if (rc < 0) return rc;
   HIPCHK(hipStreamSynchronize(g_stream));
   return rc;
+}
+
+extern "C" int o3d_solver_set_inflow(void *handle, const double *base3, const double *weight,
+                                     double amp, double period, unsigned long long seed,
+                                     double uc) {
+  if (!handle) return fail("NULL handle");
+  Solver &S = *(Solver *)handle;
+  if (!S.xdir) return fail("o3d_solver_set_inflow: the solver has no x-Dirichlet axis");
+  if (!base3 || !weight) return fail("o3d_solver_set_inflow: NULL plane");
+  if (!(period > 0.0)) return fail("o3d_solver_set_inflow: period must be > 0");
+  const o3d_params &p = S.p;
+  const size_t P = (size_t)p.ny * p.nz;
+  Staged s;
+  const double *b, *w;
+  CHK(s.in(base3, 3 * P, &b));
+  CHK(s.in(weight, P, &w));
+  HIPCHK(hipMemcpyAsync(S.in_base, b, 3 * P * sizeof(double), hipMemcpyDeviceToDevice,
+                        g_stream));
+  HIPCHK(hipMemcpyAsync(S.in_w, w, P * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  CHK(s.finish());
+  // W = the weighted plane sum of ones, in the order jet_flux_kernel sums.
+  const int my = p.bc[1] != PERIODIC, mz = p.bc[2] != PERIODIC;
+  double W = 0.0;
+  for (int k = 0; k < p.nz; ++k) {
+    double r = 0.0;
+    for (int j = 0; j < p.ny; ++j) r = r + edge_weight(j, p.ny, my) * 1.0;
+    W = W + edge_weight(k, p.nz, mz) * r;
+  }
+  S.in_W = W;
+  S.in_amp = amp;
+  S.in_period = period;
+  S.in_seed = seed;
+  S.in_cu = uc * p.dt / p.dx;
+  S.in_set = true;
+  return 0;
 }
 
 extern "C" int o3d_solver_capture(void *handle, float *r_out, float *eprev_out, float *e_out) {

@@ -11,7 +11,9 @@
 
 namespace o3d {
 
-enum Bc : int { PERIODIC = 0, FREE_SLIP = 1 };
+// DIRICHLET: x-Dirichlet inflow / outflow (x axis only; no counterpart in
+// the reference, which stops on it, src/initialization.f90:238-241).
+enum Bc : int { PERIODIC = 0, FREE_SLIP = 1, DIRICHLET = 2 };
 
 struct Grid {
   int nx, ny, nz;
@@ -33,6 +35,32 @@ __device__ __forceinline__ int ghost_index(int q, int n, int bc, bool &flip) {
   if (q < 0) { flip = true; return -q; }
   if (q > n - 1) { flip = true; return 2 * (n - 1) - q; }
   return q;
+}
+
+// x-Dirichlet ends: the ghost m nodes beyond an end (m = 1, 2, 3) is the
+// cubic through the four nodes nearest that end (f0 at the end, f3 three
+// nodes in), evaluated there and summed in this order, so every stencil
+// that reaches past the end is exact on cubics.
+template <typename T>
+__device__ __forceinline__ T xdir_extrap(int m, T f0, T f1, T f2, T f3) {
+  if (m == 1) return ((T(4) * f0 + T(-6) * f1) + T(4) * f2) + T(-1) * f3;
+  if (m == 2) return ((T(10) * f0 + T(-20) * f1) + T(15) * f2) + T(-4) * f3;
+  return ((T(20) * f0 + T(-45) * f1) + T(36) * f2) + T(-10) * f3;
+}
+
+// Fill column c of an LDS row whose column c holds x = x0 + c - 3, when that
+// x lies 1..3 nodes beyond an x-Dirichlet end, from the row's nodes nearest
+// the end (interior columns, which the fill never writes).
+template <typename T>
+__device__ __forceinline__ void xdir_fill(T *row, int c, int x0, int nx) {
+  const int x = x0 + c - 3;
+  if (x < 0 && x >= -3) {
+    const int e = 3 - x0;
+    row[c] = xdir_extrap<T>(-x, row[e], row[e + 1], row[e + 2], row[e + 3]);
+  } else if (x >= nx && x <= nx + 2) {
+    const int e = nx + 2 - x0;
+    row[c] = xdir_extrap<T>(x - (nx - 1), row[e], row[e - 1], row[e - 2], row[e - 3]);
+  }
 }
 
 template <typename T>

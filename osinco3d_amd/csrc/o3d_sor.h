@@ -24,9 +24,19 @@ struct SorState {
 };
 
 // The three neighbour rules of src/poisson.f90 (chosen from x and y
-// boundary types only, src/initialization.f90:283-301).
-enum SolverKind : int { SOR_0000 = 0, SOR_0011 = 1, SOR_111111 = 2 };
+// boundary types only, src/initialization.f90:283-301), and the x-Dirichlet
+// kinds SOR_XDIR + (y mirrored) + 2 (z mirrored), which have no counterpart
+// in the reference: with the velocity imposed on the x ends the pressure is
+// mirrored there (homogeneous Neumann), y and z follow their own types.
+enum SolverKind : int { SOR_0000 = 0, SOR_0011 = 1, SOR_111111 = 2, SOR_XDIR = 3,
+                        SOR_KINDS = 7 };
 inline void solver_periodic(int kind, int per[3]) {
+  if (kind >= SOR_XDIR) {
+    per[0] = 0;
+    per[1] = !((kind - SOR_XDIR) & 1);
+    per[2] = !((kind - SOR_XDIR) & 2);
+    return;
+  }
   per[0] = kind != SOR_111111;
   per[1] = kind == SOR_0000;
   per[2] = kind != SOR_111111;
@@ -64,13 +74,15 @@ struct SorArgs {
 
 template <typename T, typename TR>
 hipError_t launch_sor_iteration(const SorArgs<T, TR> &a, hipStream_t s);
-// Two red-black iterations (a.it and a.it + 1) in one launch, FP32 only:
-// bit-identical to two launch_sor_iteration calls, except that the stop
-// rules (and the idyn omega update) are checked once per pair.  Needs every
-// axis >= 8 and its own tiling (sor2_tiling).
-hipError_t launch_sor2_iteration(const SorArgs<float, float> &a, hipStream_t s);
-SorTiling sor2_tiling(int nx, int ny, int nz);
-bool sor2_supported(int nx, int ny, int nz);
+// The fused SOR kernel addresses the iterate and the rhs through raw buffer
+// descriptors, whose record counts and offsets are 32-bit byte counts: every
+// field must fit in INT_MAX bytes.
+template <typename T, typename TR>
+inline bool sor_buffer_fits(int nx, int ny, int nz) {
+  const long long bytes = (long long)nx * ny * nz *
+                          (long long)(sizeof(T) > sizeof(TR) ? sizeof(T) : sizeof(TR));
+  return bytes <= 0x7fffffffLL;
+}
 hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s);
 
 // Colour-split sweeps: the iterate and the rhs stored per row as the red

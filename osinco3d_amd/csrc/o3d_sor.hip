@@ -194,7 +194,12 @@ constexpr int kBufWord3 = 0x00020000;
 
 template <typename T>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *p, int n) {
-  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, n * (int)sizeof(T), kBufWord3);
+  // The launcher admits only fields of at most INT_MAX bytes
+  // (sor_buffer_fits); the count is formed in 64 bits all the same.
+  const long long bytes = (long long)n * (long long)sizeof(T);
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0,
+                                           (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL),
+                                           kBufWord3);
 }
 
 template <typename T>
@@ -677,298 +682,11 @@ template hipError_t launch_unsplit<float>(float *, const float *, const float *,
 template hipError_t launch_unsplit<double>(double *, const double *, const double *, int, int,
                                            int, hipStream_t);
 
-// ------------------------------------------------- two iterations per launch
-// The FP32 correction sweeps of the mixed form run two red-black iterations
-// per launch, so the iterate and the rhs cross HBM once per two iterations.
-// Four half-sweeps march z together, each one plane behind the previous:
-// at step s the block publishes the old plane s+1, then computes red of
-// iteration 1 (R1) on plane s, black of iteration 1 (B1) on s-1, red of
-// iteration 2 (R2) on s-2 and black of iteration 2 (B2) on s-3, the last
-// written to HBM.  Every half-sweep recomputes the halo its successor needs
-// (R1 on the tile + 3, B1 + 2, R2 + 1), from inputs every block reads
-// unmodified, so neighbouring blocks agree bit for bit; the old iterate is
-// staged with a 4-wide halo.  The update formulas and their operands are
-// those of sor_kernel, so a pair launch equals two single launches.
-constexpr int Q4W = SX + 8, Q4H = SY + 8, Q4N = Q4W * Q4H;  // old iterate, halo 4
-constexpr int Q3W = SX + 6, Q3H = SY + 6, Q3N = Q3W * Q3H;  // R1 and rhs, halo 3
-constexpr int Q2W = SX + 4, Q2H = SY + 4, Q2N = Q2W * Q2H;  // B1, halo 2
-constexpr int Q1W = SX + 2, Q1H = SY + 2, Q1N = Q1W * Q1H;  // R2, halo 1
-static_assert(Q4N <= 3 * SNT && Q3N <= 2 * SNT, "cells per thread");
-static_assert((Q3W / 2) * Q3H <= SNT && (Q2W / 2) * Q2H <= SNT, "nodes per thread");
-constexpr int kPairRing = 4;  // every ring holds 4 planes; z chunks are multiples of 4
-
-// Neighbour index with a halo up to 4 deep (periodic wrap or free-slip
-// mirror, as sor_ghost); positions further out belong to idle lanes of a
-// partial tile and are clamped.
-__device__ __forceinline__ int sor_ghost4(int q, int n, int periodic) {
-  if (q > n + 3) return n - 1;
-  if (q < -4) return 0;
-  if (periodic) {
-    if (q < 0) return q + n;
-    if (q >= n) return q - n;
-    return q;
-  }
-  if (q < 0) return -q;
-  if (q > n - 1) return 2 * (n - 1) - q;
-  return q;
-}
-
-__global__ void __launch_bounds__(SNT) sor2_kernel(const SorArgs<float, float> a) {
-  using T = float;
-  constexpr int NR = kPairRing;
-  __shared__ T o4[NR][Q4N];
-  __shared__ T r1[NR][Q3N];
-  __shared__ T b1[NR][Q2N];
-  __shared__ T r2[NR][Q1N];
-  __shared__ T rh[NR][Q3N];
-  __shared__ double scr[SNW];
-  SorState *st = a.st;
-  if (st->done) return;
-  const int it = a.it + st->base;
-  double om, dlast;
-  int reason;
-  if (sor_decide(st, it, a.eps, a.factor, a.kmax, a.idyn, om, reason, dlast)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->done = 1;
-      st->reason = reason;
-      st->exit_iter = reason == 2 ? a.kmax + 1 : it - 1;
-      st->sweeps = it - 1;
-      st->dmax_final = dlast;
-      st->omega_final = om;
-      if (a.log && it >= 3 && it - 3 < a.log_len)
-        a.log[it - 3] = bits_to_double(st->dmax_bits[(it - 2) & 7]);
-      if (a.log && it - 2 < a.log_len) a.log[it - 2] = dlast;
-    }
-    return;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->omega[it & 7] = om;
-    st->omega[(it + 1) & 7] = om;
-    st->dmax_bits[(it + 2) & 7] = 0ull;
-    st->dmax_bits[(it + 3) & 7] = 0ull;
-    if (a.log && it >= 3 && it - 3 < a.log_len)
-      a.log[it - 3] = bits_to_double(st->dmax_bits[(it - 2) & 7]);
-    if (a.log && it >= 2 && it - 2 < a.log_len) a.log[it - 2] = dlast;
-  }
-
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int plane = nx * ny;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int tix = t % a.tl.ntx;
-  const int rr = t / a.tl.ntx;
-  const int tiy = rr % a.tl.nty;
-  const int tiz = rr / a.tl.nty;
-  const int x0 = tix * SX, y0 = tiy * SY, z0 = tiz * a.tl.zc;
-  const int z1 = min(z0 + a.tl.zc, nz);
-  const int tid = threadIdx.x;
-  const T ox = T(a.ox), oy = T(a.oy), oz = T(a.oz), A = T(a.A), invA = T(1.0 / a.A);
-  const T w = T(om);
-  const T w1 = T(1.0 - om);
-
-  // Staging: old-iterate cells tid, tid+SNT, tid+2*SNT of the halo-4 region
-  // and rhs cells tid, tid+SNT of the halo-3 region (byte column offsets).
-  unsigned oo[3], ro[2];
-  bool ov[3], rv[2];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int c = tid + q * SNT;
-    ov[q] = c < Q4N;
-    const int cc = ov[q] ? c : 0;
-    oo[q] = (unsigned)(sor_ghost4(x0 - 4 + cc % Q4W, nx, a.per[0]) +
-                       nx * sor_ghost4(y0 - 4 + cc / Q4W, ny, a.per[1])) * sizeof(T);
-  }
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int c = tid + q * SNT;
-    rv[q] = c < Q3N;
-    const int cc = rv[q] ? c : 0;
-    ro[q] = (unsigned)(sor_ghost4(x0 - 3 + cc % Q3W, nx, a.per[0]) +
-                       nx * sor_ghost4(y0 - 3 + cc / Q3W, ny, a.per[1])) * sizeof(T);
-  }
-  const int perz = a.per[2];
-  auto zq = [&](int zp) { return (unsigned)(plane * sor_ghost4(zp, nz, perz)) * sizeof(T); };
-  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(a.pin, plane * nz);
-  const __amdgpu_buffer_rsrc_t rrh = buf_rsrc(a.rhs, plane * nz);
-
-  // Half-sweep roles.  R1: red nodes of the halo-3 region (35 per row);
-  // B1: black nodes of the halo-2 region (34 per row); R2: red nodes of the
-  // halo-1 region (33 per row); B2: every tile node (branch-free select).
-  // Colour of global node (i, j, k) is (i + j + k) & 1, red = 0.
-  const int r1row = tid / (Q3W / 2), r1k = tid % (Q3W / 2);
-  const bool r1on = tid < (Q3W / 2) * Q3H;
-  const int r1par = (x0 - 3 + y0 - 3 + r1row) & 1;  // a = 2k + ((r1par + z) & 1) is red
-  const int b1row = tid / (Q2W / 2), b1k = tid % (Q2W / 2);
-  const bool b1on = tid < (Q2W / 2) * Q2H;
-  const int b1par = (x0 - 2 + y0 - 2 + b1row + 1) & 1;  // a = 2k + ((b1par + z) & 1) is black
-  const int r2row = tid / (Q1W / 2), r2k = tid % (Q1W / 2);
-  const bool r2on = tid < (Q1W / 2) * Q1H;
-  const int r2par = (x0 - 1 + y0 - 1 + r2row) & 1;
-  const int tx = tid % SX, ty = tid / SX;
-  const int bi = x0 + tx, bj = y0 + ty;
-  const bool bown = bi < nx && bj < ny;
-  const int bpar = (bi + bj) & 1;
-  auto i4 = [](int X, int Y) { return (Y + 4) * Q4W + X + 4; };
-  auto i3 = [](int X, int Y) { return (Y + 3) * Q3W + X + 3; };
-  auto i2 = [](int X, int Y) { return (Y + 2) * Q2W + X + 2; };
-  auto i1 = [](int X, int Y) { return (Y + 1) * Q1W + X + 1; };
-  auto tile_own = [&](int X, int Y) {
-    return X >= 0 && X < SX && Y >= 0 && Y < SY && x0 + X < nx && y0 + Y < ny;
-  };
-
-  // Prefetch registers: step k publishes old plane s_k + 1 and rhs plane s_k.
-  constexpr int PD = 2;
-  T po[PD][3], pr[PD][2];
-  const int s0 = z0 - 3;
-  {
-    const unsigned g4 = zq(z0 - 4), g3 = zq(z0 - 3);
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-      if (ov[q]) {
-        o4[slot<NR>(z0 - 4)][tid + q * SNT] = bload<T>(rin, oo[q], g4);
-        o4[slot<NR>(z0 - 3)][tid + q * SNT] = bload<T>(rin, oo[q], g3);
-      }
-#pragma unroll
-    for (int k = 0; k < PD; ++k) {
-      const unsigned go = zq(s0 + k + 1), gr = zq(s0 + k);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) po[k][q] = bload<T>(rin, oo[q], go);
-#pragma unroll
-      for (int q = 0; q < 2; ++q) pr[k][q] = bload<T>(rrh, ro[q], gr);
-    }
-  }
-
-  T dm1 = T(0), dm2 = T(0);
-  T rhs_b2 = T(0), rhs_b2n = T(0);  // rhs of this thread's tile node: plane s-3, s-2
-  // Step at plane s; K = position in the unrolled group of NR steps (the
-  // slot of plane s + j is slot<NR>(s0 + K + j): z0 is a multiple of NR).
-  auto step = [&](int s, auto K) {
-    constexpr int k = decltype(K)::value;
-    auto sl = [&](int j) { return slot<NR>(-3 + k + j); };
-    // Publish old(s+1) and rhs(s); prefetch PD steps ahead.
-    {
-      T *od = o4[sl(1)];
-      T *rd = rh[sl(0)];
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        if (ov[q]) od[tid + q * SNT] = po[k % PD][q];
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-        if (rv[q]) rd[tid + q * SNT] = pr[k % PD][q];
-      const unsigned go = zq(s + 1 + PD), gr = zq(s + PD);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) po[k % PD][q] = bload<T>(rin, oo[q], go);
-#pragma unroll
-      for (int q = 0; q < 2; ++q) pr[k % PD][q] = bload<T>(rrh, ro[q], gr);
-    }
-    __syncthreads();
-    // R1 on plane s (needed for planes z0-3 .. z1+2).
-    if (r1on) {
-      const int z = s;
-      const int aa = 2 * r1k + ((r1par + z) & 1);
-      const int X = aa - 3, Y = r1row - 3;
-      const T *c = o4[sl(0)];
-      const int q4 = i4(X, Y);
-      const T pc = c[q4];
-      const T pn = sor_gs<T, T>(c[q4 - 1], c[q4 + 1], c[q4 - Q4W], c[q4 + Q4W], o4[sl(-1)][q4],
-                                o4[sl(1)][q4], rh[sl(0)][i3(X, Y)], ox, oy, oz, A, invA);
-      r1[sl(0)][i3(X, Y)] = w1 * pc + w * pn;
-      if (z >= z0 && z < z1) dm1 = fmaxf(dm1, tile_own(X, Y) ? fabsf(pn - pc) : T(0));
-    }
-    __syncthreads();
-    // B1 on plane s-1 (planes z0-2 .. z1+1).
-    if (s - 1 >= z0 - 2 && b1on) {
-      const int z = s - 1;
-      const int aa = 2 * b1k + ((b1par + z) & 1);
-      const int X = aa - 2, Y = b1row - 2;
-      const T *c = r1[sl(-1)];
-      const int q3 = i3(X, Y);
-      const T pc = o4[sl(-1)][i4(X, Y)];
-      const T pn = sor_gs<T, T>(c[q3 - 1], c[q3 + 1], c[q3 - Q3W], c[q3 + Q3W], r1[sl(-2)][q3],
-                                r1[sl(0)][q3], rh[sl(-1)][q3], ox, oy, oz, A, invA);
-      b1[sl(-1)][i2(X, Y)] = w1 * pc + w * pn;
-      if (z >= z0 && z < z1) dm1 = fmaxf(dm1, tile_own(X, Y) ? fabsf(pn - pc) : T(0));
-    }
-    __syncthreads();
-    // R2 on plane s-2 (planes z0-1 .. z1); keep the rhs of plane s-2 at this
-    // thread's tile node for B2 at the next step.
-    if (s - 2 >= z0 - 1) {
-      if (r2on) {
-        const int z = s - 2;
-        const int aa = 2 * r2k + ((r2par + z) & 1);
-        const int X = aa - 1, Y = r2row - 1;
-        const T *c = b1[sl(-2)];
-        const int q2 = i2(X, Y);
-        const T pc = r1[sl(-2)][i3(X, Y)];
-        const T pn = sor_gs<T, T>(c[q2 - 1], c[q2 + 1], c[q2 - Q2W], c[q2 + Q2W], b1[sl(-3)][q2],
-                                  b1[sl(-1)][q2], rh[sl(-2)][i3(X, Y)], ox, oy, oz, A, invA);
-        r2[sl(-2)][i1(X, Y)] = w1 * pc + w * pn;
-        if (z >= z0 && z < z1) dm2 = fmaxf(dm2, tile_own(X, Y) ? fabsf(pn - pc) : T(0));
-      }
-      rhs_b2n = rh[sl(-2)][i3(tx, ty)];
-    }
-    __syncthreads();
-    // B2 on plane s-3 (planes z0 .. z1-1): the new iterate of the tile.
-    if (s - 3 >= z0 && s - 3 < z1) {
-      const int z = s - 3;
-      const T *c = r2[sl(-3)];
-      const int q1 = i1(tx, ty);
-      const T cr = c[q1];
-      const T pc = b1[sl(-3)][i2(tx, ty)];
-      const T pn = sor_gs<T, T>(c[q1 - 1], c[q1 + 1], c[q1 - Q1W], c[q1 + Q1W], r2[sl(-4)][q1],
-                                r2[sl(-2)][q1], rhs_b2, ox, oy, oz, A, invA);
-      const bool blk = ((bpar + z) & 1) != 0;
-      const T v = blk ? w1 * pc + w * pn : cr;
-      dm2 = fmaxf(dm2, blk && bown ? fabsf(pn - pc) : T(0));
-      if (bown) (a.pout + (size_t)plane * z)[(unsigned)(bi + nx * bj)] = v;
-    }
-    rhs_b2 = rhs_b2n;
-  };
-  for (int s = s0; s <= z1 + 2; s += NR)
-    unroll_steps(std::make_integer_sequence<int, NR>{}, [&](auto K) {
-      const int sk = s + decltype(K)::value;
-      if (decltype(K)::value == 0 || sk <= z1 + 2) step(sk, K);
-    });
-  const double d1 = block_max<SNW>((double)dm1, scr);
-  if (threadIdx.x == 0 && d1 != 0.0)
-    atomicMax(&st->dmax_bits[it & 7], (unsigned long long)__double_as_longlong(d1));
-  const double d2 = block_max<SNW>((double)dm2, scr);
-  if (threadIdx.x == 0 && d2 != 0.0)
-    atomicMax(&st->dmax_bits[(it + 1) & 7], (unsigned long long)__double_as_longlong(d2));
-}
-
-bool sor2_supported(int nx, int ny, int nz) { return nx >= 8 && ny >= 8 && nz >= 8; }
-
-SorTiling sor2_tiling(int nx, int ny, int nz) {
-  static int slots = 0;
-  if (slots == 0) {
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sor2_kernel, SNT, 0) !=
-            hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
-      (void)hipGetLastError();
-      per_cu = 2;
-      cus = 256;
-    }
-    slots = std::max(1, per_cu * cus);
-  }
-  return make_sor_tiling(nx, ny, nz, slots, kPairRing);
-}
-
-hipError_t launch_sor2_iteration(const SorArgs<float, float> &a, hipStream_t s) {
-  if (a.tl.zc % kPairRing != 0 || !sor2_supported(a.nx, a.ny, a.nz) || a.gs != 0 ||
-      (long long)a.nx * a.ny * a.nz * 4LL > 0x7fffffffLL)
-    return hipErrorInvalidValue;
-  sor2_kernel<<<dim3(a.tl.blocks()), dim3(SNT), 0, s>>>(a);
-  return hipGetLastError();
-}
-
 template <typename T, typename TR>
 hipError_t launch_sor_iteration(const SorArgs<T, TR> &a, hipStream_t s) {
   // The kernel's ring slots assume z chunks that are a multiple of the ring
   // depth, and its buffer descriptors address at most INT_MAX bytes.
-  if (a.tl.zc % SorShape<T>::NO != 0 ||
-      (long long)a.nx * a.ny * a.nz * (long long)sizeof(T) > 0x7fffffffLL)
+  if (a.tl.zc % SorShape<T>::NO != 0 || !sor_buffer_fits<T, TR>(a.nx, a.ny, a.nz))
     return hipErrorInvalidValue;
   sor_kernel<T, TR><<<dim3(a.tl.blocks()), dim3(SNT), 0, s>>>(a);
   return hipGetLastError();

@@ -149,6 +149,18 @@ __global__ void __launch_bounds__(256) der_kernel(T *__restrict__ df, const T *_
   const int bc = variant == 0 ? PERIODIC : FREE_SLIP;
   const bool odd = variant == 2;
   auto F = [&](int s) -> T {
+    if (variant == 3) {  // x-Dirichlet: extrapolated ends
+      const int q = p + s;
+      if (q < 0)
+        return xdir_extrap<T>(-q, f[base], f[base + stride], f[base + 2 * stride],
+                              f[base + 3 * stride]);
+      if (q > n - 1) {
+        const int e = base + (n - 1) * stride;
+        return xdir_extrap<T>(q - (n - 1), f[e], f[e - stride], f[e - 2 * stride],
+                              f[e - 3 * stride]);
+      }
+      return f[base + q * stride];
+    }
     bool fl;
     const int q = ghost_index(p + s, n, bc, fl);
     const T v = f[base + q * stride];
@@ -178,8 +190,11 @@ hipError_t launch_der(int which, T *df, const T *f, int nx, int ny, int nz, doub
     der2dsim_kernel<T><<<grd, blk, 0, s>>>(df, f, N, T(h));
     return hipGetLastError();
   }
-  if (which < 0 || which > 19) return hipErrorInvalidValue;
-  const int axis = which / 6, order = (which % 6) / 3 + 1, variant = which % 3;
+  if (which < 0 || which > 21) return hipErrorInvalidValue;
+  // 20 / 21: the x first / second derivative with x-Dirichlet ends.
+  const int axis = which >= 20 ? 0 : which / 6;
+  const int order = which >= 20 ? which - 19 : (which % 6) / 3 + 1;
+  const int variant = which >= 20 ? 3 : which % 3;
   double a, b, c;
   if (order == 1) { D1Coef k = d1coef(h); a = k.a; b = k.b; c = k.c; }
   else { D2Coef k = d2coef(h); a = k.a; b = k.b; c = k.c; }
@@ -263,6 +278,8 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
   // Forced zeros of the even free-slip first derivative (src/derivation.f90:87,105).
   const bool edge_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
   const bool edge_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  // Tiles whose plane reaches past an x-Dirichlet end (block-uniform).
+  const bool xdir = A.bc[0] == DIRICHLET && (P.x0 == 0 || P.x0 + kTX + 2 >= nx);
 
   FieldPartial nut_p;
   partial_init(nut_p);
@@ -322,6 +339,14 @@ __global__ void __launch_bounds__(kTX *kTY) predict_kernel(const PredictArgs<T> 
       if (hy_row >= 0) pl[c][hy_row][tx + 3] = sgn(hyr[c][sl], hneg_y[c]);
     }
     __syncthreads();
+    if (xdir) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xdir_fill<T>(&pl[c][ty + 3][0], tx + 3, P.x0, nx);
+        if (hx_col >= 0) xdir_fill<T>(&pl[c][ty + 3][0], hx_col, P.x0, nx);
+      }
+      __syncthreads();
+    }
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
       const int r = ty + 3, cc = tx + 3;
@@ -453,6 +478,7 @@ __global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
   const int hy_idx = P.gi + nx * hy_g;
   const bool zero_x = !odd && A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
   const bool zero_y = !odd && A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  const bool xdir = A.bc[0] == DIRICHLET && (P.x0 == 0 || P.x0 + kTX + 2 >= nx);
 
   FieldPartial st;
   partial_init(st);
@@ -505,6 +531,11 @@ __global__ void __launch_bounds__(kTX *kTY) div_kernel(const DivArgs<T> A) {
     if (hx_col >= 0) su[ty][hx_col] = sgn(nhx[sl], neg_hx);
     if (hy_row >= 0) sv[hy_row][tx] = sgn(nhy[sl], neg_hy);
     __syncthreads();
+    if (xdir) {
+      xdir_fill<T>(&su[ty][0], tx + 3, P.x0, nx);
+      if (hx_col >= 0) xdir_fill<T>(&su[ty][0], hx_col, P.x0, nx);
+      __syncthreads();
+    }
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
       const int cc = tx + 3, r = ty + 3;
@@ -580,8 +611,11 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
   else if (ty >= kTY - 3) { hy_row = ty + 6; hy_g = ghost_safe(P.y0 + ty + 3, ny, A.bc[1], hy_f); }
   const int hx_idx = hx_g + nx * P.gj;
   const int hy_idx = P.gi + nx * hy_g;
-  const bool zero_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
+  // The pressure is mirrored on an x-Dirichlet axis too (homogeneous
+  // Neumann), and the inflow / outflow planes keep u* (no correction).
+  const bool zero_x = A.bc[0] != PERIODIC && (P.i == 0 || P.i == nx - 1);
   const bool zero_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  const bool keep = A.bc[0] == DIRICHLET && (P.i == 0 || P.i == nx - 1);
 
   FieldPartial st[3];
   partial_init(st[0]); partial_init(st[1]); partial_init(st[2]);
@@ -641,7 +675,7 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
       if (A.bc[2] == FREE_SLIP && (k == 0 || k == nz - 1)) g[2] = TP(0);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T un = T(TP(upv[sl][c]) - A.dt * g[c]);
+        const T un = keep ? upv[sl][c] : T(TP(upv[sl][c]) - A.dt * g[c]);
         A.u[c][idx] = un;
         partial_add(st[c], (double)un, idx);
       }
@@ -705,6 +739,7 @@ __global__ void __launch_bounds__(kTX *kTY) stats_kernel(const StatsArgs<T> A) {
   const int hy_idx = P.gi + nx * hy_g;
   const bool edge_x = A.bc[0] == FREE_SLIP && (P.i == 0 || P.i == nx - 1);
   const bool edge_y = A.bc[1] == FREE_SLIP && (P.j == 0 || P.j == ny - 1);
+  const bool xdir = A.bc[0] == DIRICHLET && (P.x0 == 0 || P.x0 + kTX + 2 >= nx);
   double acc[16];
 #pragma unroll
   for (int m = 0; m < 16; ++m) acc[m] = 0.0;
@@ -744,6 +779,14 @@ __global__ void __launch_bounds__(kTX *kTY) stats_kernel(const StatsArgs<T> A) {
       if (hy_row >= 0) pl[c][hy_row][tx + 3] = hy[c];
     }
     __syncthreads();
+    if (xdir) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xdir_fill<T>(&pl[c][ty + 3][0], tx + 3, P.x0, nx);
+        if (hx_col >= 0) xdir_fill<T>(&pl[c][ty + 3][0], hx_col, P.x0, nx);
+      }
+      __syncthreads();
+    }
     if (P.valid) {
       const int r = ty + 3, cc = tx + 3;
       double g[3][3], l[3][3];

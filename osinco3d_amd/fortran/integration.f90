@@ -101,7 +101,7 @@ contains
     real(kind=8), intent(in) :: ux_pred(nx, ny, nz), uy_pred(nx, ny, nz), uz_pred(nx, ny, nz)
     real(kind=8), intent(in) :: dx, dy, dz, dt, eps
     real(kind=8), allocatable :: rhs(:,:,:)
-    integer(c_int) :: bc(3), solver, cycles
+    integer(c_int) :: bc(3), solver, cycles, reason
     real(c_double) :: stats(6), resmax
     bc = [int(nbcx1, c_int), int(nbcy1, c_int), int(nbcz1, c_int)]
     if (bc(1) == 0 .and. bc(2) == 0) then
@@ -118,9 +118,9 @@ contains
          int(ny, c_int), int(nz, c_int), bc, 1_c_int, dt, stats), 'correct_pression')
     call o3d_check(o3d_poisson_multigrid(pp, rhs, dx, dy, dz, int(nx, c_int), int(ny, c_int), &
          int(nz, c_int), solver, int(kmax, c_int), 5_c_int, 4_c_int, eps, int(kmax, c_int), &
-         cycles, resmax), 'solve_poisson_multigrid')
+         cycles, resmax, reason), 'solve_poisson_multigrid')
     deallocate(rhs)
-    write(*, '(A20,I4,A1,E13.6)') ' Multigrid cycles : ', cycles, ',', resmax
+    write(*, '(A20,I4,A1,E13.6,A1,I2)') ' Multigrid cycles : ', cycles, ',', resmax, ',', reason
   end subroutine pression_mg
 
   subroutine pression_x(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, nx, ny, nz, dt, omega, &

@@ -93,14 +93,14 @@ module o3d_capi
      end function o3d_poisson
 
      integer(c_int) function o3d_poisson_multigrid(pp, rhs, dx, dy, dz, nx, ny, nz, solver, &
-          nlevels, npre, npost, tol, max_cycles, cycles, resmax) &
+          nlevels, npre, npost, tol, max_cycles, cycles, resmax, reason) &
           bind(C, name="o3d_poisson_multigrid")
        import :: c_int, c_double
        real(c_double) :: pp(*)
        real(c_double), intent(in) :: rhs(*)
        real(c_double), value :: dx, dy, dz, tol
        integer(c_int), value :: nx, ny, nz, solver, nlevels, npre, npost, max_cycles
-       integer(c_int) :: cycles
+       integer(c_int) :: cycles, reason
        real(c_double) :: resmax
      end function o3d_poisson_multigrid
 

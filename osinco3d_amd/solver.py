@@ -82,6 +82,51 @@ def mixing_layer(cfg: Config, seed: int = 1234, amplitude: float = 0.05):
             np.zeros(sh))
 
 
+def jet_profile(cfg: Config, y: np.ndarray) -> np.ndarray:
+    """The planar-jet profile of src/initial_conditions.f90:176-242:
+    u = (u1 + u2)/2 + (u1 - u2)/2 tanh(t3 (1 - 2|y|/l0)), u1 = u0,
+    u2 = ratio u0, theta0 = l0/30, t3 = l0/(4 theta0)."""
+    u1 = cfg.u0
+    u2 = u1 * cfg.ratio
+    theta0 = 1.0 / 30.0 * cfg.l0
+    t3 = cfg.l0 / (4.0 * theta0)
+    return 0.5 * (u1 + u2) + 0.5 * (u1 - u2) * np.tanh(t3 * (1.0 - 2.0 * np.abs(y) / cfg.l0))
+
+
+def planar_jet(cfg: Config):
+    """Initial field of the x-Dirichlet jet: the inflow profile in the whole
+    box (x-independent, as the reference's initialize_planar_jet), v = w = 0,
+    p = 0."""
+    x, y, z = grid(cfg)
+    sh = (cfg.nz, cfg.ny, cfg.nx)
+    ux = np.broadcast_to(jet_profile(cfg, y)[None, :, None], sh)
+    return (np.ascontiguousarray(ux), np.zeros(sh), np.zeros(sh), np.zeros(sh))
+
+
+def jet_inflow(cfg: Config, amplitude: float = 0.05, seed: int = 1234):
+    """Inflow of the jet: (base3, weight, amp, period, seed, uc) for
+    o3d_solver_set_inflow.  base3 = the profile (u) and zeros (v, w); the
+    noise weight is the normalised shear 1 - tanh^2 of the profile's
+    argument (1 in the shear layers, 0 in the core and the coflow), as the
+    reference weights its initial noise by the shear profile
+    (src/initial_conditions.f90:515-545); amp = amplitude u0 (5 %); period
+    l0 / u0; uc = the trapezoid-weighted mean inflow velocity (the outflow's
+    convective velocity)."""
+    _, y, _ = grid(cfg)
+    P = (cfg.nz, cfg.ny)
+    u = jet_profile(cfg, y)
+    t3 = cfg.l0 / (4.0 * (cfg.l0 / 30.0))
+    w = 1.0 - np.tanh(t3 * (1.0 - 2.0 * np.abs(y) / cfg.l0)) ** 2
+    base = np.zeros((3,) + P)
+    base[0] = u[None, :]
+    weight = np.ascontiguousarray(np.broadcast_to(w[None, :], P))
+    wy = np.ones(cfg.ny)
+    if cfg.bc()[1] != 0:
+        wy[0] = wy[-1] = 0.5
+    uc = float((wy * u).sum() / wy.sum())
+    return base, weight, amplitude * cfg.u0, cfg.l0 / cfg.u0, seed, uc
+
+
 def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
                 inner_max: int = 0, coarse_cycles: int = 0,
                 smooth_sweeps: int = 0) -> capi.O3DParams:
@@ -151,6 +196,19 @@ class Solver:
     def init_taylor_green(self):
         self.upload(*taylor_green(self.cfg))
 
+    def set_inflow(self, base3, weight, amp, period, seed, uc):
+        """x-Dirichlet inflow (o3d_solver_set_inflow)."""
+        b = np.ascontiguousarray(base3, dtype=np.float64)
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        capi.check(self.lib.o3d_solver_set_inflow(self.handle, capi.ptr(b), capi.ptr(w),
+                                                  C.c_double(amp), C.c_double(period),
+                                                  C.c_ulonglong(seed), C.c_double(uc)))
+
+    def init_jet(self, amplitude: float = 0.05, seed: int = 1234):
+        """The planar jet's initial field and inflow."""
+        self.upload(*planar_jet(self.cfg))
+        self.set_inflow(*jet_inflow(self.cfg, amplitude, seed))
+
     def step(self, itime: int) -> dict:
         info = capi.O3DStepInfo()
         rc = capi.check(self.lib.o3d_solver_step(self.handle, int(itime), C.byref(info)))
@@ -166,6 +224,15 @@ class Solver:
 
     def field_ptr(self, which: int) -> int:
         return int(self.lib.o3d_solver_field(self.handle, int(which)) or 0)
+
+    def pressure_stats(self) -> np.ndarray:
+        """function_stats of the solver's pressure on the device (min, max,
+        mean, then the 1-based position of the first maximum)."""
+        out = np.zeros(6)
+        capi.check(self.lib.o3d_function_stats(C.c_void_p(self.field_ptr(6)), self.cfg.nx,
+                                               self.cfg.ny, self.cfg.nz,
+                                               out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def sor_timing(self, launches: bool = False):
         """(ms, sweeps) of the last step's complete SOR batches per precision

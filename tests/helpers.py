@@ -2,6 +2,8 @@
 oracle (used only to check results)."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from osinco3d_amd.config import Config
@@ -13,7 +15,9 @@ def rand_fields(shape, n=3, seed=0, scale=1.0):
 
 
 def solver_code(bc):
-    bx, by, _ = bc
+    bx, by, bz = bc
+    if bx == 2:  # x-Dirichlet: 3 + (y mirrored) + 2 (z mirrored)
+        return 3 + (by != 0) + 2 * (bz != 0)
     if bx == 0 and by == 0:
         return "0000"
     if bx == 0 and by == 1:
@@ -43,13 +47,28 @@ class OracleStepper:
         self.omega = cfg.omega
         self.solver = solver_code(self.bc)
 
+    def set_inflow(self, base3, weight, amp, period, seed, uc):
+        """x-Dirichlet inflow, as o3d_solver_set_inflow takes it."""
+        self.inflow = (np.ascontiguousarray(base3), np.ascontiguousarray(weight), amp, period,
+                       seed, uc)
+
     def step(self, itime, ux, uy, uz, pp):
         c = self.cfg
         up = self.o.predict(ux, uy, uz, *self.F, c.re, self.adt, self.bdt, self.cdt, itime,
                             c.itscheme, self.d, c.iles, c.cs, c.delta(), self.nut, self.bc)
+        xdir = self.bc[0] == 2
+        flux = None
+        if xdir:
+            base, w, amp, period, seed, uc = self.inflow
+            ph = 2.0 * math.pi * (itime * self.dt) / period
+            cu = uc * self.dt / self.d[0]
+            flux = self.o.jet_edges(up, [ux, uy, uz], base, w, amp, math.cos(ph), math.sin(ph),
+                                    seed, cu, self.bc)
         rhs = self.o.divergence(*up, self.d, self.bc, 1) / self.dt
+        if xdir:
+            rhs = rhs - self.o.weighted_mean(rhs, [1, self.bc[1] != 0, self.bc[2] != 0])
         it, dm, reason, self.omega, _ = self.o.sor(pp, rhs, self.d, self.solver, self.omega,
                                                    c.eps, c.kmax, c.idyn, red_black=self.rb)
         u = [np.empty_like(ux) for _ in range(3)]
         rc = self.o.correct_velocity(*u, *up, pp, self.dt, self.d, self.bc)
-        return u, pp, up, dict(exit_iter=it, dmax=dm, reason=reason, diverged=rc)
+        return u, pp, up, dict(exit_iter=it, dmax=dm, reason=reason, diverged=rc, flux=flux)

@@ -1,0 +1,176 @@
+"""x-Dirichlet inflow / outflow and the planar jet on the GPU (no
+counterpart in the reference, which stops on this boundary type,
+src/initialization.f90:238-241; the oracle is pinned by the mathematics in
+tests/test_xdirichlet_cpu.py).  FP64 operators and whole FP64 steps are
+bit-identical to the oracle; the mixed form tracks the FP64 form; the full
+512 x 384 x 256 jet runs 20 steps in both forms."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from osinco3d_amd import capi
+from osinco3d_amd.config import jet_config
+from osinco3d_amd.solver import FP64, MIXED, WARM_EXTRAPOLATE, Solver, jet_inflow, planar_jet
+from tests.helpers import OracleStepper, rand_fields
+
+pytestmark = pytest.mark.gpu
+
+# Partial x tiles: 66 leaves two nodes in the last 64-wide tile (its ghosts
+# fall in the previous tile's halo), 72 eight.
+SHAPES = [(6, 8, 16), (10, 12, 66), (8, 9, 72)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_xdir_derivatives(orc, shape):
+    f, = rand_fields(shape, 1, seed=11)
+    h = 0.17
+    for which, order in ((20, 1), (21, 2)):
+        df = np.empty_like(f)
+        capi.der(which, df, f, h)
+        want = (orc.der1 if order == 1 else orc.der2)(f, 0, 3, h)
+        assert np.array_equal(df, want), (which, np.abs(df - want).max())
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("bc", [(2, 1, 0), (2, 0, 1)])
+def test_xdir_divergence_predict_correct(orc, shape, bc):
+    print(f"[gpu] x-Dirichlet operators {shape} bc {bc}", flush=True)
+    u = rand_fields(shape, 3, seed=12)
+    d = (0.11, 0.13, 0.17)
+    div = np.empty_like(u[0])
+    capi.divergence(div, *u, d, bc, 1, 0.5)
+    assert np.array_equal(div, orc.divergence(*u, d, bc, 1) / 0.5)
+    for itscheme, iles in ((2, 0), (3, 1)):
+        F = [np.ascontiguousarray(a) for a in
+             [np.stack(rand_fields(shape, 3, seed=20 + c)) for c in range(3)]]
+        Fo = [a.copy() for a in F]
+        up = [np.empty_like(u[0]) for _ in range(3)]
+        nut = np.zeros_like(u[0])
+        nuto = np.zeros_like(u[0])
+        adt, bdt, cdt = [0.01, 0.015, 23 / 1200], [0.0, -0.005, -16 / 1200], [0.0, 0.0, 5 / 1200]
+        capi.predict_velocity(up, u, F, 3000.0, adt, bdt, cdt, 5, itscheme, d, iles, 0.17, 0.13,
+                              nut, bc)
+        upo = orc.predict(*u, *Fo, 3000.0, adt, bdt, cdt, 5, itscheme, d, iles, 0.17, 0.13, nuto,
+                          bc)
+        for a, b in zip(up, upo):
+            assert np.array_equal(a, b), (itscheme, iles, np.abs(a - b).max())
+        for a, b in zip(F, Fo):
+            assert np.array_equal(a, b)
+    pp, = rand_fields(shape, 1, seed=13)
+    ug = [np.empty_like(u[0]) for _ in range(3)]
+    uo = [np.empty_like(u[0]) for _ in range(3)]
+    capi.correct_velocity(ug, up, pp, 0.01, d, bc)
+    orc.correct_velocity(*uo, *up, pp, 0.01, d, bc)
+    for a, b in zip(ug, uo):
+        assert np.array_equal(a, b)
+    for c in range(3):  # the x-end planes keep u*
+        assert np.array_equal(ug[c][:, :, 0], up[c][:, :, 0])
+        assert np.array_equal(ug[c][:, :, -1], up[c][:, :, -1])
+    st = capi.statistics(*u, d, 3000.0, 0.25, bc)
+    so = orc.statistics(*u, d, 3000.0, 0.25, bc)
+    assert np.allclose(st, so, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("code", [3, 4, 6])
+def test_xdir_sor_matches_oracle(orc, code):
+    shape = (10, 12, 16)
+    rhs, p0 = rand_fields(shape, 2, seed=14)
+    d = (0.2, 0.25, 0.3)
+    a = p0.copy()
+    b = p0.copy()
+    r1 = capi.poisson(a, rhs, d, 1.7, 1e-30, 25, 0, code)
+    r2 = orc.sor(b, rhs, d, code, 1.7, 1e-30, 25, red_black=True)
+    assert r1[0] == r2[0] and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(12, 16, 48), (10, 12, 72)])
+def test_jet_fp64_steps_bit_identical(orc, shape):
+    """Whole FP64 jet steps (inflow, outflow with its flux balance, projected
+    rhs, red-black SOR to eps, correction) against the oracle."""
+    nz, ny, nx = shape
+    cfg = jet_config(nx, ny, nz, steps=6)
+    cfg.kmax = 3000
+    print(f"[gpu] jet FP64 steps {shape}", flush=True)
+    s = Solver(cfg, FP64)
+    s.init_jet()
+    ref = OracleStepper(orc, cfg, red_black=True)
+    ref.set_inflow(*jet_inflow(cfg))
+    u = list(planar_jet(cfg))
+    p = u.pop()
+    for it in range(1, cfg.itstop + 1):
+        info = s.step(it)
+        u, p, _, oi = ref.step(it, *u, p)
+        g = s.download()
+        for c in range(3):
+            assert np.array_equal(g[c], u[c]), (it, c, np.abs(g[c] - u[c]).max())
+        assert np.array_equal(g[3], p), (it, np.abs(g[3] - p).max())
+        assert info["sor_exit_iter"] == oi["exit_iter"] and info["sor_reason"] == 0
+        qin, qout, W = oi["flux"]
+        assert info["flow_rate"][0] == qin / W and info["flow_rate"][1] == qout / W
+    s.close()
+
+
+def _jet_run(cfg, precision, steps, warm=0, coarse=(0, 0)):
+    s = Solver(cfg, precision, warm, coarse_cycles=coarse[0], smooth_sweeps=coarse[1])
+    s.init_jet()
+    ek, en, iters = [], [], []
+    for it in range(1, steps + 1):
+        info = s.step(it)
+        assert info["sor_reason"] == 0 and not info["diverged"], (it, info["sor_reason"])
+        iters.append(info["sor_exit_iter"])
+        st = s.statistics(it * s.dt)
+        ek.append(st[1])
+        en.append(st[4])
+    u = s.download()
+    s.close()
+    return np.array(ek), np.array(en), np.array(iters), u
+
+
+def _balance(cfg, u):
+    wy = np.ones(cfg.ny)
+    wy[[0, -1]] = 0.5
+    pw = np.ones(cfg.nz)[:, None] * wy[None, :]
+    qin = (pw * u[0][:, :, 0]).sum()
+    qout = (pw * u[0][:, :, -1]).sum()
+    return abs(qout - qin) / abs(qin)
+
+
+def test_jet_mixed_tracks_fp64_small():
+    cfg = jet_config(64, 48, 32, steps=40)
+    e0, z0, i0, u0 = _jet_run(cfg, FP64, 40)
+    e1, z1, i1, u1 = _jet_run(cfg, MIXED, 40)
+    e2, z2, i2, u2 = _jet_run(cfg, MIXED, 40, WARM_EXTRAPOLATE, (2, 8))
+    for e, z, name in ((e1, z1, "mixed"), (e2, z2, "mg")):
+        re_, rz = np.abs(e - e0).max() / e0.max(), np.abs(z - z0).max() / z0.max()
+        print(f"[gpu] jet 64x48x32 {name}: E_k {re_:.2e}, enstrophy {rz:.2e} vs FP64; sweeps "
+              f"{np.mean(i1 if name == 'mixed' else i2):.1f} (FP64 {np.mean(i0):.1f})",
+              flush=True)
+        assert re_ <= 1e-6 and rz <= 1e-6
+    for u in (u0, u1, u2):
+        assert _balance(cfg, u) <= 1e-6
+
+
+def test_jet_full_size_fp64_and_mixed():
+    """The 512 x 384 x 256 planar jet, 20 AB3 steps with LES: both forms end
+    every solve on eps, balance the outflow against the inflow, and agree on
+    E_k and enstrophy."""
+    cfg = jet_config(steps=20)
+    print("[gpu] jet 512x384x256 FP64, 20 steps", flush=True)
+    e0, z0, i0, u0 = _jet_run(cfg, FP64, 20)
+    b0 = _balance(cfg, u0)
+    del u0
+    print(f"[gpu] jet FP64 sweeps/step {i0.mean():.1f}; mixed + coarse-grid correction",
+          flush=True)
+    e1, z1, i1, u1 = _jet_run(cfg, MIXED, 20, WARM_EXTRAPOLATE, (2, 8))
+    b1 = _balance(cfg, u1)
+    del u1
+    re_, rz = np.abs(e1 - e0).max() / e0.max(), np.abs(z1 - z0).max() / z0.max()
+    print(f"[gpu] jet full size: mixed+mg E_k {re_:.2e}, enstrophy {rz:.2e}; sweeps/step "
+          f"{i1.mean():.1f} (FP64 {i0.mean():.1f}); flux imbalance {b0:.1e} / {b1:.1e}",
+          flush=True)
+    assert re_ <= 1e-6 and rz <= 1e-6
+    assert b0 <= 1e-12 and b1 <= 1e-6
+    assert np.all(np.diff(e0) < 0.05 * e0[0])

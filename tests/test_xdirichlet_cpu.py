@@ -1,0 +1,183 @@
+"""The oracle's x-Dirichlet inflow / outflow (no counterpart in the
+reference: its README lists the boundary type, its schemes() stops on it,
+src/initialization.f90:238-241).  With no reference behaviour to match, the
+oracle is pinned by what the mathematics fixes:
+  * the extrapolated ends make the 6th-order D1 and 4th-order D2 exact on
+    cubics at every node (and only on cubics at the ends), with third- and
+    second-order convergence there;
+  * the x-mirrored 7-point operator has the Neumann eigenfunctions
+    cos(pi a i / (n - 1)) with the closed-form symbol, and its columns are
+    annihilated by the trapezoid weights, which is why the projected rhs is
+    compatible;
+  * the outflow flux equals the inflow flux after the edge rule;
+  * the inflow noise is SplitMix64, checked against its published test
+    vector and the package's independent numpy generator;
+  * a short jet on the oracle converges every Poisson solve on eps.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from osinco3d_amd.config import jet_config
+from osinco3d_amd.solver import jet_inflow, planar_jet, splitmix64_uniform
+from tests.helpers import OracleStepper
+
+
+def _poly(x, c):
+    return sum(ck * x ** k for k, ck in enumerate(c))
+
+
+def _xfield(v, ny=3, nz=2):
+    return np.ascontiguousarray(np.broadcast_to(v[None, None, :], (nz, ny, v.size)))
+
+
+def test_xdir_closures_exact_on_cubics(orc):
+    rng = np.random.default_rng(3)
+    n, h = 17, 0.13
+    x = 0.4 + h * np.arange(n)
+    c = rng.standard_normal(4)
+    f = _xfield(_poly(x, c))
+    d1 = orc.der1(f, 0, 3, h)[0, 0]
+    d2 = orc.der2(f, 0, 3, h)[0, 0]
+    e1 = c[1] + 2 * c[2] * x + 3 * c[3] * x ** 2
+    e2 = 2 * c[2] + 6 * c[3] * x
+    assert np.abs(d1 - e1).max() <= 1e-12 * np.abs(e1).max()
+    assert np.abs(d2 - e2).max() <= 1e-11 * np.abs(e2).max()
+    # A quartic: exact in the interior (the central stencils are exact to
+    # degree 6 and 5), not at the ends, where the cubic ghosts cut it.
+    q = _xfield(x ** 4)
+    g1 = orc.der1(q, 0, 3, h)[0, 0] - 4 * x ** 3
+    g2 = orc.der2(q, 0, 3, h)[0, 0] - 12 * x ** 2
+    assert np.abs(g1[3:-3]).max() <= 1e-11 and np.abs(g2[2:-2]).max() <= 1e-10
+    assert abs(g1[0]) > 1e-4 and abs(g1[-1]) > 1e-4 and abs(g2[0]) > 1e-3
+
+
+def test_xdir_closure_convergence_orders(orc):
+    errs1, errs2 = [], []
+    for n in (33, 65, 129):
+        h = 2.0 / (n - 1)
+        x = h * np.arange(n)
+        f = _xfield(np.sin(x + 0.3))
+        errs1.append(abs(orc.der1(f, 0, 3, h)[0, 0, 0] - math.cos(0.3)))
+        errs2.append(abs(orc.der2(f, 0, 3, h)[0, 0, 0] + math.sin(0.3)))
+    r1 = [errs1[i] / errs1[i + 1] for i in range(2)]
+    r2 = [errs2[i] / errs2[i + 1] for i in range(2)]
+    assert all(6.5 < r < 9.5 for r in r1), r1   # third order at the end node
+    assert all(3.2 < r < 4.8 for r in r2), r2   # second order
+
+
+@pytest.mark.parametrize("code,per", [(3, (0, 1, 1)), (4, (0, 0, 1)), (6, (0, 0, 0))])
+def test_xdir_poisson_neumann_eigenfunction(orc, code, per):
+    nx, ny, nz = 14, 12, 10
+    d = (0.21, 0.25, 0.3)
+
+    def mode(n, a, periodic):
+        i = np.arange(n)
+        return np.cos(2 * np.pi * a * i / n) if periodic else np.cos(np.pi * a * i / (n - 1))
+
+    def sym(n, a, periodic, h):
+        th = 2 * np.pi * a / n if periodic else np.pi * a / (n - 1)
+        return (2 - 2 * np.cos(th)) / h ** 2
+
+    A = (2, 3, 1)
+    p = (mode(nz, A[2], per[2])[:, None, None] * mode(ny, A[1], per[1])[None, :, None] *
+         mode(nx, A[0], per[0])[None, None, :])
+    lam = sym(nx, A[0], per[0], d[0]) + sym(ny, A[1], per[1], d[1]) + sym(nz, A[2], per[2], d[2])
+    rhs = np.ascontiguousarray(-lam * p)
+    r = orc.residual(np.ascontiguousarray(p), rhs, d, code)
+    assert np.abs(r).max() <= 1e-12 * lam
+    q = np.zeros_like(p)
+    it, dm, reason, _, _ = orc.sor(q, rhs, d, code, 1.6, 1e-13, 20000, red_black=True)
+    assert reason == 0, (it, dm)
+    assert np.abs((q - q.mean()) - (p - p.mean())).max() <= 1e-9
+
+
+def test_xdir_trapezoid_weights_annihilate_the_operator(orc):
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 13, 11, 8
+    d = (0.2, 0.3, 0.25)
+    p = np.ascontiguousarray(rng.standard_normal((nz, ny, nx)))
+    lp = -orc.residual(p, np.zeros_like(p), d, 4)   # x, y mirrored, z periodic
+    wx = np.ones(nx); wx[[0, -1]] = 0.5
+    wy = np.ones(ny); wy[[0, -1]] = 0.5
+    W = np.ones(nz)[:, None, None] * wy[None, :, None] * wx[None, None, :]
+    assert abs((W * lp).sum()) <= 1e-12 * np.abs(lp).sum()
+    # The oracle's weighted mean (device summation order) against numpy.
+    f = np.ascontiguousarray(rng.standard_normal((nz, ny, nx)) + 0.3)
+    m = orc.weighted_mean(f, [1, 1, 0])
+    assert abs(m - (W * f).sum() / W.sum()) <= 1e-14
+    g = f - m
+    assert abs((W * g).sum()) <= 1e-13 * np.abs(g).sum()
+    # Dropping a weight (all ones) gives a different mean: the weights matter.
+    assert abs(m - f.mean()) > 1e-6
+
+
+def test_splitmix64_known_answer(orc):
+    # Published test vector of SplitMix64 for seed 1234567.
+    want = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+            4593380528125082431, 16408922859458223821]
+    assert [orc.splitmix64(1234567, c) for c in range(5)] == want
+    u = splitmix64_uniform(10, 99)
+    assert all(orc.noise_uniform(99, c) == 2.0 * u[c] - 1.0 for c in range(10))
+
+
+def test_jet_edges_mass_balance_and_inflow(orc):
+    cfg = jet_config(24, 16, 12, steps=5)
+    rng = np.random.default_rng(7)
+    sh = (cfg.nz, cfg.ny, cfg.nx)
+    u = [np.ascontiguousarray(rng.standard_normal(sh) * 0.1 + (1.0 if c == 0 else 0.0))
+         for c in range(3)]
+    up = [a.copy() for a in u]
+    base, w, amp, period, seed, uc = jet_inflow(cfg)
+    dt, dx = cfg.dt, cfg.spacing()[0]
+    ph = 2.0 * math.pi * (3 * dt) / period
+    cph, sph = math.cos(ph), math.sin(ph)
+    cu = uc * dt / dx
+    qin, qout, W = orc.jet_edges(up, u, base, w, amp, cph, sph, seed, cu, cfg.bc())
+    wy = np.ones(cfg.ny); wy[[0, -1]] = 0.5
+    plane_w = np.ones(cfg.nz)[:, None] * wy[None, :]
+    fin = (plane_w * up[0][:, :, 0]).sum()
+    fout = (plane_w * up[0][:, :, -1]).sum()
+    assert abs(W - plane_w.sum()) <= 1e-12
+    assert abs(fin - qin) <= 1e-12 * abs(qin) and abs(fout - fin) <= 1e-12 * abs(fin)
+    # Inflow: profile + (amp w) (x1 C + x2 S), the noise from SplitMix64.
+    n = 6 * cfg.ny * cfg.nz
+    xs = 2.0 * splitmix64_uniform(n, seed) - 1.0
+    for c in range(3):
+        q = np.arange(cfg.ny * cfg.nz)
+        x1 = xs[2 * (3 * q + c)].reshape(cfg.nz, cfg.ny)
+        x2 = xs[2 * (3 * q + c) + 1].reshape(cfg.nz, cfg.ny)
+        want = base[c] + (amp * w) * (x1 * cph + x2 * sph)
+        assert np.array_equal(up[c][:, :, 0], want)
+        if c > 0:
+            ub, ub1 = u[c][:, :, -1], u[c][:, :, -2]
+            assert np.array_equal(up[c][:, :, -1], ub - cu * (ub - ub1))
+    # The interior is untouched.
+    for c in range(3):
+        assert np.array_equal(up[c][:, :, 1:-1], u[c][:, :, 1:-1])
+
+
+def test_jet_on_the_oracle_converges_and_balances(orc):
+    """A small jet: every Poisson solve ends on eps (the projected rhs is
+    compatible), the outflow flux equals the inflow flux every step, the
+    planes keep the imposed values and nothing blows up."""
+    cfg = jet_config(32, 24, 16, steps=12)
+    cfg.kmax = 4000
+    st = OracleStepper(orc, cfg, red_black=True)
+    st.set_inflow(*jet_inflow(cfg))
+    u = list(planar_jet(cfg))
+    pp = u.pop()
+    for it in range(1, cfg.itstop + 1):
+        u, pp, up, info = st.step(it, *u, pp)
+        assert info["reason"] == 0 and info["exit_iter"] < cfg.kmax, (it, info)
+        qin, qout, W = info["flux"]
+        wy = np.ones(cfg.ny); wy[[0, -1]] = 0.5
+        pw = np.ones(cfg.nz)[:, None] * wy[None, :]
+        assert abs((pw * u[0][:, :, -1]).sum() - qin) <= 1e-12 * abs(qin)
+        for c in range(3):
+            assert np.array_equal(u[c][:, :, 0], up[c][:, :, 0])
+        assert all(np.isfinite(a).all() for a in u)
+    assert np.abs(u[0]).max() < 2.0
