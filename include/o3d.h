@@ -52,6 +52,27 @@ const char *o3d_version(void);
  * with. */
 int o3d_stream_probe(size_t bytes_per_buffer, int reps, double *gbs, double *ms);
 
+/* ----------------------------------------------- device-field registry
+ * Host arrays the caller keeps for the whole run (the driver's fields,
+ * src/initialization.f90:50-59) can be registered: the library then keeps a
+ * device copy across calls (the SURVEY's "registry keyed by host address
+ * plus size, with host-valid / device-valid flags").  A registered input
+ * whose device copy is current is not copied again; a registered output
+ * stays on the device and is copied back to the host at the end of the call
+ * only when `mirror` is set (host code reads it).  Other host arrays, and
+ * any part of a registered array passed on its own, are staged per call as
+ * before (a stale host copy is refreshed first).  Contract: host code writes
+ * a registered array only before its first use by the library, or calls
+ * o3d_mark_host_dirty after writing it.  No counterpart in the reference
+ * (its arrays are host memory only). */
+int o3d_register(const void *host, size_t bytes, int mirror);
+int o3d_unregister(const void *host);      /* copies a stale host copy back */
+int o3d_mark_host_dirty(const void *host); /* the host copy changed */
+int o3d_sync_host(const void *host);       /* make the host copy current now */
+/* Bytes moved host -> device and device -> host by the staging and the
+ * registry so far; reset != 0 zeroes the counters. */
+int o3d_transfer_stats(double *h2d_bytes, double *d2h_bytes, int reset);
+
 /* ------------------------------------------------------ operator entry points */
 
 /* One routine of src/derivation.f90 (interface der_type(df, f, d),

@@ -4,9 +4,11 @@
 ! modules, compiled from the reference sources in place by oracle/Makefile.
 ! It follows the step sequence of src/osinco3d_main.f90:97-188 (old_values,
 ! predict_velocity, correct_pression, correct_velocity, the two diagnostic
-! divergences with function_stats, print_velocity_values, compute_cfl) but
-! leaves out the file output of the driver (slices, XDMF, checkpoints,
-! gnuplot), which needs no timing and writes outside the run directory.
+! divergences with function_stats, print_velocity_values, compute_cfl, and
+! every 25 steps calculate_residuals and statistics_calc, which appends to
+! outputs/stats.dat when the run directory has outputs/) but leaves out the
+! rest of the driver's file output (slices, XDMF, checkpoints, gnuplot),
+! which needs no timing and writes outside the run directory.
 ! The full driver cannot be built here: src/initial_conditions.f90 needs
 ! src/noise_module.f90, which needs FFTW3, absent from this image.
 !
@@ -23,7 +25,10 @@
 ! every step that is a multiple of dump_every, and the last step.
 ! Per-step wall time goes to stdout as "O3D_STEP_SECONDS <itime> <seconds>",
 ! and per phase as "O3D_PHASE_SECONDS <itime> <predict> <correct_pression>
-! <correct_velocity> <one divergence> <remaining diagnostics>".
+! <correct_velocity> <one divergence> <remaining diagnostics>".  Built as
+! the drop-in (O3D_DROPIN: the same loop over this package's modules), it
+! also prints the host <-> device bytes of each step as
+! "O3D_STEP_TRANSFER <itime> <h2d> <d2h>".
 program o3d_ref_run
   use initialization
   use integration
@@ -31,7 +36,15 @@ program o3d_ref_run
   use functions
   use utils
   use IOfunctions
+#ifdef O3D_DROPIN
+  use o3d_capi, only: o3d_transfer_stats
+  use iso_c_binding, only: c_double, c_int
+#endif
   implicit none
+#ifdef O3D_DROPIN
+  real(c_double) :: h2d, d2h
+  integer(c_int) :: trc
+#endif
   character(len=256) :: outname, arg
   integer :: dump_every, i, j, k, iunit
   integer(8) :: c0, c1, crate, cp(6)
@@ -105,6 +118,14 @@ program o3d_ref_run
      call print_velocity_values(ux, uy, uz)
      call compute_cfl(cflx, cfly, cflz, ux, uy, uz, dx, dy, dz, dt)
      call print_cfl(cflx, cfly, cflz)
+     if (modulo(itime, 25) == 0) then
+        ! src/osinco3d_main.f90:167-181 (without the profile files)
+        call calculate_residuals(ux, uy, uz, old_ux, old_uy, old_uz, dt, &
+             t_ref, u_ref, nx, ny, nz, itime)
+        if (time > initstat) then
+           call statistics_calc(ux, uy, uz, nx, ny, nz, dx, dy, dz, re, time)
+        end if
+     end if
      call system_clock(c1)
      cp(6) = c1
      t_step = real(c1 - c0, kind=8) / real(crate, kind=8)
@@ -115,6 +136,10 @@ program o3d_ref_run
      ! predict, correct_pression, correct_velocity, one divergence, rest of
      ! the diagnostics
      write(*, '(A18,I8,5ES16.8)') 'O3D_PHASE_SECONDS ', itime, tph
+#ifdef O3D_DROPIN
+     trc = o3d_transfer_stats(h2d, d2h, 1_c_int)
+     write(*, '(A18,I8,2ES16.8)') 'O3D_STEP_TRANSFER ', itime, h2d, d2h
+#endif
      if (dump .and. ((dump_every > 0 .and. modulo(itime, max(dump_every, 1)) == 0) .or. &
           itime == itstop)) then
         write(iunit) int(itime, 4), int(nx, 4), int(ny, 4), int(nz, 4)

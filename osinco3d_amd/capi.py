@@ -75,7 +75,8 @@ class O3DStepInfo(C.Structure):
 # Exported symbols of include/o3d.h (checked by the CPU test suite).
 EXPORTS = [
     "o3d_init", "o3d_set_stream", "o3d_last_error", "o3d_sync", "o3d_version",
-    "o3d_stream_probe",
+    "o3d_stream_probe", "o3d_register", "o3d_unregister", "o3d_mark_host_dirty",
+    "o3d_sync_host", "o3d_transfer_stats",
     "o3d_der", "o3d_der_f32", "o3d_divergence", "o3d_nu_t", "o3d_predict_velocity",
     "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity",
     "o3d_function_stats",

@@ -239,13 +239,14 @@ def tgv_config(n: int, steps: int, itscheme: int, cfl: float = 0.25,
 
 def mixing_layer_config(nx: int = 384, ny: int = 256, nz: int = 128, steps: int = 500,
                         cfl: float = 0.25, re: float = 3000.0, cs: float = 0.17,
-                        eps: float = 1e-8, kmax: int = 5000) -> Config:
-    """Temporal mixing layer, 60x40x20 (delta = 1), periodic x and z,
-    free-slip y, Smagorinsky LES; the initial field comes from fields.bin
-    (typesim = 0)."""
+                        eps: float = 1e-8, kmax: int = 5000,
+                        lengths: Tuple[float, float, float] = (60.0, 40.0, 20.0)) -> Config:
+    """Temporal mixing layer, 60x40x20 (delta = 1) unless `lengths`, periodic
+    x and z, free-slip y (y centred on 0), Smagorinsky LES; the initial field
+    comes from fields.bin (typesim = 0)."""
     cfg = Config(u0=1.0, l0=1.0, re=re, typesim=0, ratio=2.0,
-                 nx=nx, ny=ny, nz=nz, xlx=60.0, yly=40.0, zlz=20.0,
-                 x0=0.0, y0=-20.0, z0=0.0,
+                 nx=nx, ny=ny, nz=nz, xlx=lengths[0], yly=lengths[1], zlz=lengths[2],
+                 x0=0.0, y0=-0.5 * lengths[1], z0=0.0,
                  nbcy1=FREE_SLIP, nbcyn=FREE_SLIP,
                  itscheme=2, itstart=1, itstop=steps,
                  omega=sor_omega(max(nx, ny, nz)), idyn=0, eps=eps, kmax=kmax,

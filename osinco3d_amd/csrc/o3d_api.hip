@@ -104,11 +104,54 @@ int check_bc(const int *bc, bool xdir = false) {
   return 0;
 }
 
+// Device-field registry of the drop-in (o3d_register): host arrays the
+// caller keeps for the run (the driver's velocity, pressure, predicted
+// velocity and Adams-Bashforth history) get a device mirror that outlives
+// the call.  A registered input whose device copy is valid is not copied
+// again; a registered output stays on the device, and is copied back at the
+// end of the call only when the entry mirrors its host copy (host code reads
+// it).  Contract: the host writes a registered array only before the
+// library first uses it or after o3d_mark_host_dirty.  Host arrays that are
+// not registered are staged through temporary device buffers per call.
+struct RegEntry {
+  char *host;
+  size_t bytes;
+  void *dev;
+  bool dev_valid, host_valid, mirror;
+};
+std::vector<RegEntry> g_reg;
+double g_h2d = 0.0, g_d2h = 0.0;  // bytes moved between host and device
+
+RegEntry *reg_exact(const void *p) {
+  for (RegEntry &e : g_reg)
+    if (e.host == (const char *)p) return &e;
+  return nullptr;
+}
+
+// Before host memory [p, p + bytes) is read or written outside its own entry:
+// entries overlapping it whose host copy is stale are copied back, and when
+// the range is about to be written, their device copies are invalidated.
+int reg_touch(const void *p, size_t bytes, bool write) {
+  const char *lo = (const char *)p, *hi = lo + bytes;
+  for (RegEntry &e : g_reg) {
+    if (e.host == lo || e.host + e.bytes <= lo || hi <= e.host) continue;
+    if (!e.host_valid) {
+      HIPCHK(hipMemcpyAsync(e.host, e.dev, e.bytes, hipMemcpyDeviceToHost, g_stream));
+      HIPCHK(hipStreamSynchronize(g_stream));
+      g_d2h += (double)e.bytes;
+      e.host_valid = true;
+    }
+    if (write) e.dev_valid = false;
+  }
+  return 0;
+}
+
 // Staging of host arrays through device buffers for one call.
 struct Staged {
   struct Out { void *dev; void *host; size_t bytes; };
   std::vector<void *> allocs;
   std::vector<Out> outs;
+  std::vector<RegEntry *> written;
   ~Staged() {
     for (void *p : allocs) (void)hipFree(p);
   }
@@ -116,10 +159,23 @@ struct Staged {
   int in(const T *p, size_t n, const T **dev) {
     if (!p) { *dev = nullptr; return 0; }
     if (is_device_ptr(p)) { *dev = p; return 0; }
+    const size_t bytes = n * sizeof(T);
+    RegEntry *e = reg_exact(p);
+    if (e && e->bytes >= bytes) {
+      if (!e->dev_valid) {
+        HIPCHK(hipMemcpyAsync(e->dev, e->host, e->bytes, hipMemcpyHostToDevice, g_stream));
+        g_h2d += (double)e->bytes;
+        e->dev_valid = true;
+      }
+      *dev = (const T *)e->dev;
+      return 0;
+    }
+    CHK(reg_touch(p, bytes, false));
     void *d = nullptr;
-    HIPCHK(hipMalloc(&d, n * sizeof(T)));
+    HIPCHK(hipMalloc(&d, bytes));
     allocs.push_back(d);
-    HIPCHK(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, g_stream));
+    g_h2d += (double)bytes;
     *dev = (const T *)d;
     return 0;
   }
@@ -128,11 +184,28 @@ struct Staged {
   int out(T *p, size_t n, T **dev, bool copy_in) {
     if (!p) { *dev = nullptr; return 0; }
     if (is_device_ptr(p)) { *dev = p; return 0; }
+    const size_t bytes = n * sizeof(T);
+    RegEntry *e = reg_exact(p);
+    if (e && e->bytes >= bytes) {
+      if (copy_in && !e->dev_valid) {
+        HIPCHK(hipMemcpyAsync(e->dev, e->host, e->bytes, hipMemcpyHostToDevice, g_stream));
+        g_h2d += (double)e->bytes;
+      }
+      e->dev_valid = true;
+      e->host_valid = false;
+      written.push_back(e);
+      *dev = (T *)e->dev;
+      return 0;
+    }
+    CHK(reg_touch(p, bytes, true));
     void *d = nullptr;
-    HIPCHK(hipMalloc(&d, n * sizeof(T)));
+    HIPCHK(hipMalloc(&d, bytes));
     allocs.push_back(d);
-    if (copy_in) HIPCHK(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
-    outs.push_back({d, (void *)p, n * sizeof(T)});
+    if (copy_in) {
+      HIPCHK(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, g_stream));
+      g_h2d += (double)bytes;
+    }
+    outs.push_back({d, (void *)p, bytes});
     *dev = (T *)d;
     return 0;
   }
@@ -145,8 +218,16 @@ struct Staged {
     return 0;
   }
   int finish() {
-    for (const Out &o : outs)
+    for (const Out &o : outs) {
       HIPCHK(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, g_stream));
+      g_d2h += (double)o.bytes;
+    }
+    for (RegEntry *e : written)
+      if (e->mirror && !e->host_valid) {
+        HIPCHK(hipMemcpyAsync(e->host, e->dev, e->bytes, hipMemcpyDeviceToHost, g_stream));
+        g_d2h += (double)e->bytes;
+        e->host_valid = true;
+      }
     HIPCHK(hipStreamSynchronize(g_stream));
     return 0;
   }
@@ -322,6 +403,71 @@ extern "C" int o3d_sync(void) {
 }
 
 extern "C" const char *o3d_version(void) { return "osinco3d-amd 0.1 gfx950"; }
+
+// ============================================================ registry
+extern "C" int o3d_register(const void *host, size_t bytes, int mirror) {
+  CHK(ensure_init());
+  if (!host || bytes == 0) return fail("o3d_register: NULL or empty array");
+  if (is_device_ptr(host)) return 0;  // device arrays are used in place anyway
+  if (RegEntry *e = reg_exact(host)) {
+    if (e->bytes == bytes) {
+      e->mirror = e->mirror || mirror != 0;
+      return 0;
+    }
+    return fail("o3d_register: %p is registered with %zu bytes, not %zu", host, e->bytes, bytes);
+  }
+  const char *lo = (const char *)host;
+  for (const RegEntry &e : g_reg)
+    if (!(e.host + e.bytes <= lo || lo + bytes <= e.host))
+      return fail("o3d_register: %p overlaps a registered array", host);
+  void *d = nullptr;
+  HIPCHK(hipMalloc(&d, bytes));
+  g_reg.push_back({(char *)host, bytes, d, false, true, mirror != 0});
+  return 0;
+}
+
+extern "C" int o3d_unregister(const void *host) {
+  for (size_t q = 0; q < g_reg.size(); ++q)
+    if (g_reg[q].host == (const char *)host) {
+      RegEntry &e = g_reg[q];
+      if (!e.host_valid) {
+        HIPCHK(hipMemcpyAsync(e.host, e.dev, e.bytes, hipMemcpyDeviceToHost, g_stream));
+        g_d2h += (double)e.bytes;
+      }
+      HIPCHK(hipStreamSynchronize(g_stream));
+      HIPCHK(hipFree(e.dev));
+      g_reg.erase(g_reg.begin() + q);
+      return 0;
+    }
+  return fail("o3d_unregister: %p is not registered", host);
+}
+
+extern "C" int o3d_mark_host_dirty(const void *host) {
+  RegEntry *e = reg_exact(host);
+  if (!e) return fail("o3d_mark_host_dirty: %p is not registered", host);
+  e->dev_valid = false;
+  e->host_valid = true;
+  return 0;
+}
+
+extern "C" int o3d_sync_host(const void *host) {
+  RegEntry *e = reg_exact(host);
+  if (!e) return fail("o3d_sync_host: %p is not registered", host);
+  if (!e->host_valid) {
+    HIPCHK(hipMemcpyAsync(e->host, e->dev, e->bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    g_d2h += (double)e->bytes;
+    e->host_valid = true;
+  }
+  return 0;
+}
+
+extern "C" int o3d_transfer_stats(double *h2d_bytes, double *d2h_bytes, int reset) {
+  if (h2d_bytes) *h2d_bytes = g_h2d;
+  if (d2h_bytes) *d2h_bytes = g_d2h;
+  if (reset) g_h2d = g_d2h = 0.0;
+  return 0;
+}
 
 // ============================================================ measurement
 namespace {
@@ -540,7 +686,24 @@ extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz
   CHK(s.out(fux, 3 * n, &F[0], true));
   CHK(s.out(fuy, 3 * n, &F[1], true));
   CHK(s.out(fuz, 3 * n, &F[2], true));
-  CHK(s.out(nu_t, n, &nut, false));
+  const bool les = iles == 1;
+  nut = nullptr;
+  if (les) {
+    CHK(s.out(nu_t, n, &nut, false));
+  } else if (nu_t) {
+    // Without LES the reference sets nu_t = 0 (src/integration.f90:106-113):
+    // on the device for a device array, on both copies of a registered one,
+    // on the host otherwise (no transfer either way).
+    RegEntry *e = is_device_ptr(nu_t) ? nullptr : reg_exact(nu_t);
+    if (is_device_ptr(nu_t) || e) {
+      HIPCHK(launch_fill_f64(e ? (double *)e->dev : nu_t, 0.0, (int)n, g_stream));
+      if (e) { e->dev_valid = true; e->host_valid = true; }
+    }
+    if (!is_device_ptr(nu_t)) {
+      CHK(reg_touch(nu_t, n * sizeof(double), true));
+      memset(nu_t, 0, n * sizeof(double));
+    }
+  }
   double *f1[3];
   const double *f2[3], *f3[3];
   for (int c = 0; c < 3; ++c) { f1[c] = F[c]; f2[c] = F[c] + n; f3[c] = F[c] + 2 * n; }
@@ -548,8 +711,6 @@ extern "C" int o3d_predict_velocity(double *ux_pred, double *uy_pred, double *uz
   FieldPartial *part;
   CHK(s.scratch(tl.blocks(), &part));
   const double h[3] = {dx, dy, dz};
-  const bool les = iles == 1;
-  if (!les && nut) HIPCHK(launch_fill_f64(nut, 0.0, (int)n, g_stream));
   CHK(predict_common(up, u, f1, f2, f3, les ? nut : nullptr, ci + 1, les, re, adt[ci], bdt[ci],
                      cdt[ci], cs, delta, h, nx, ny, nz, bc, part));
   // History shift by copy, src/integration.f90:176-188.
@@ -1164,7 +1325,8 @@ struct Solver {
   double *hnpart = nullptr;    // pinned copy
   double *epart = nullptr;     // sum-of-squares partials of e_prev
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
-  double *mpart = nullptr;     // mean-pressure partials (mixed form, every axis periodic)
+  double *mpart = nullptr;     // mean-pressure partials + the mean (mixed, every axis periodic)
+  bool mean_pending = false;   // the mean is formed, its subtraction not yet done
   // x-Dirichlet inflow / outflow (bc[0] == DIRICHLET), set by o3d_solver_set_inflow.
   bool xdir = false, in_set = false;
   double *in_base = nullptr;   // 3 planes of ny nz values: the inflow profile
@@ -1552,14 +1714,19 @@ __global__ void __launch_bounds__(256) p_sum_kernel(const double *__restrict__ p
   s = block_sum<4>(s, sv);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-__global__ void __launch_bounds__(256) p_sub_mean_kernel(double *__restrict__ p, int n,
-                                                         const double *__restrict__ part,
-                                                         int nparts) {
-  __shared__ double sv[4];
+// The mean from the partials (1024 threads, one partial each, the same
+// order every time): mp[0] = sum / n.
+__global__ void __launch_bounds__(1024) p_mean_kernel(const double *__restrict__ part,
+                                                      int nparts, int n, double *__restrict__ mp) {
+  __shared__ double sv[16];
   double s = 0.0;
   for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += part[b];
-  s = block_sum<4>(s, sv);  // the same sum in every block
-  const double mean = s / (double)n;
+  s = block_sum<16>(s, sv);
+  if (threadIdx.x == 0) mp[0] = s / (double)n;
+}
+__global__ void __launch_bounds__(256) p_sub_mean_kernel(double *__restrict__ p, int n,
+                                                         const double *__restrict__ mp) {
+  const double mean = mp[0];
   double2 *p2 = (double2 *)p;
   const int n2 = n / 2;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
@@ -1571,10 +1738,12 @@ __global__ void __launch_bounds__(256) p_sub_mean_kernel(double *__restrict__ p,
   if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) p[n - 1] -= mean;
 }
 
-int remove_pressure_mean(double *p, int n, double *part) {
+// The mean of the pressure into part[kMeanBlocks] (the subtraction follows,
+// fused into the increment pass of the warm-start forms, else on its own).
+int pressure_mean(double *p, int n, double *part) {
   p_sum_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(p, n, part);
   HIPCHK(hipGetLastError());
-  p_sub_mean_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(p, n, part, kMeanBlocks);
+  p_mean_kernel<<<1, 1024, 0, g_stream>>>(part, kMeanBlocks, n, part + kMeanBlocks);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1680,7 +1849,10 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   }
   S.last_sweeps[1] = sw32;
   S.r_ready = false;
-  if (S.mpart) CHK(remove_pressure_mean(S.P[S.cur], n, S.mpart));
+  if (S.mpart) {
+    CHK(pressure_mean(S.P[S.cur], n, S.mpart));
+    S.mean_pending = true;
+  }
   info->sor_exit_iter = reason == 2 ? p.kmax + 1 : total;
   info->sor_reason = reason;
   info->sweeps_f64 = sw64;
@@ -1855,24 +2027,35 @@ __global__ void lw_guess2_kernel(const double *p, const float *e1, const float *
 // e = p - p0 (float32) and the per-block sum of squares of e; pairs of
 // cells per thread (16-byte loads), the odd last cell by thread 0.
 constexpr int kIncBlocks = 2048;
-__global__ void __launch_bounds__(256) lw_increment_kernel(const double *__restrict__ p,
+// With mp (the periodic mean-pressure removal pending), p -= mp[0] first,
+// written back, and e is formed from the shifted p.
+__global__ void __launch_bounds__(256) lw_increment_kernel(double *__restrict__ p,
                                                            const double *__restrict__ p0,
                                                            float *__restrict__ e, int n,
-                                                           double *__restrict__ part) {
+                                                           double *__restrict__ part,
+                                                           const double *__restrict__ mp) {
   __shared__ double sc[4];
   double s = 0.0;
   const int np = n / 2;
-  const double2 *p2 = reinterpret_cast<const double2 *>(p);
+  const double m = mp ? mp[0] : 0.0;
+  double2 *p2 = reinterpret_cast<double2 *>(p);
   const double2 *q2 = reinterpret_cast<const double2 *>(p0);
   float2 *e2 = reinterpret_cast<float2 *>(e);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
-    const double2 a = p2[i], b = q2[i];
+    double2 a = p2[i];
+    const double2 b = q2[i];
+    if (mp) {
+      a.x -= m;
+      a.y -= m;
+      p2[i] = a;
+    }
     const float2 v = make_float2((float)(a.x - b.x), (float)(a.y - b.y));
     e2[i] = v;
     s += (double)v.x * v.x;
     s += (double)v.y * v.y;
   }
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (mp) p[n - 1] -= m;
     const float v = (float)(p[n - 1] - p0[n - 1]);
     e[n - 1] = v;
     s += (double)v * v;
@@ -2140,8 +2323,10 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
 int solver_learning_finish(Solver &S) {
   const int n = S.n;
   std::swap(S.Eprev, S.Eprev2);
-  lw_increment_kernel<<<kIncBlocks, 256, 0, g_stream>>>(S.P[S.cur], S.Pstart, S.Eprev, n,
-                                                       S.epart);
+  lw_increment_kernel<<<kIncBlocks, 256, 0, g_stream>>>(
+      S.P[S.cur], S.Pstart, S.Eprev, n, S.epart,
+      S.mean_pending ? S.mpart + kMeanBlocks : nullptr);
+  S.mean_pending = false;
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(S.hepart, S.epart, kIncBlocks * sizeof(double), hipMemcpyDeviceToHost,
                         g_stream));
@@ -2180,6 +2365,12 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   info->warm_used = warm;
   HIPCHK(hipEventRecord(S.ev[5], g_stream));
   if (learning) CHK(solver_learning_finish(S));
+  if (S.mean_pending) {
+    p_sub_mean_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(S.P[S.cur], S.n,
+                                                         S.mpart + kMeanBlocks);
+    HIPCHK(hipGetLastError());
+    S.mean_pending = false;
+  }
   CHK(solver_correct<T>(S));
   CHK(solver_divergence<T>(S, S.u, nullptr, 1.0, S.part_div2));
   // The step's statistics (their partials stay untouched until here): the
@@ -2316,7 +2507,7 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
       rc |= S->alloc(&S->dslot, sor_split_slot_words());
     int per[3];
     solver_periodic(kind, per);
-    if (!rc && per[0] && per[1] && per[2]) rc |= S->alloc(&S->mpart, kMeanBlocks);
+    if (!rc && per[0] && per[1] && per[2]) rc |= S->alloc(&S->mpart, kMeanBlocks + 1);
   }
   if (!rc && !S->mixed && sor_split_supported<double>(p->nx, p->ny, p->nz)) {
     rc |= S->alloc(&S->dslot, sor_split_slot_words());

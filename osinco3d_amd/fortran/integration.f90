@@ -7,9 +7,12 @@
 !   correct_pression  -> o3d_correct_pression (divergence/dt + red-black SOR), or
 !                        o3d_divergence + o3d_poisson_multigrid with multigrid = 1
 !   correct_velocity  -> o3d_correct_velocity (grad p, update, NaN/max guard)
-! Host arrays are staged through device memory on every call (the reference
-! owns all state), so results come back in the caller's arrays exactly as
-! the reference leaves them.  The passive scalar transeq runs o3d_transeq
+! The run's fields (velocity, predicted velocity, history, pressure, nu_t)
+! are registered with the library's device registry on first use, so they
+! stay on the device between calls; the ones host code reads (velocity,
+! pressure, nu_t) are copied back after every call that writes them, so the
+! caller's arrays hold what the reference leaves in them.  Other host arrays
+! are staged through device memory per call.  The passive scalar transeq runs o3d_transeq
 ! (derivatives, update, conservative clipping in one kernel chain).
 module integration
   use functions
@@ -32,10 +35,10 @@ contains
     real(kind=8), intent(in) :: re, cs, delta
     real(kind=8), intent(in) :: dx, dy, dz
     real(kind=8), dimension(3), intent(in) :: adt, bdt, cdt
-    real(kind=8), dimension(:,:,:), intent(inout) :: ux, uy, uz
-    real(kind=8), dimension(:,:,:), intent(inout) :: nu_t
-    real(kind=8), dimension(:,:,:,:), intent(inout) :: fux, fuy, fuz
-    real(kind=8), dimension(:,:,:), intent(out) :: ux_pred, uy_pred, uz_pred
+    real(kind=8), dimension(:,:,:), intent(inout), target :: ux, uy, uz
+    real(kind=8), dimension(:,:,:), intent(inout), target :: nu_t
+    real(kind=8), dimension(:,:,:,:), intent(inout), target :: fux, fuy, fuz
+    real(kind=8), dimension(:,:,:), intent(out), target :: ux_pred, uy_pred, uz_pred
 
     print *, "* Predict velocity"
     if (itscheme == 1 .or. itime == 1) then
@@ -49,6 +52,19 @@ contains
        stop
     end if
     if (iles == 1) print *, " LES model calculation"
+    ! The run's fields stay on the device between calls: the velocity (read
+    ! by the driver's host diagnostics every step, so mirrored back), the
+    ! predicted velocity and the history (device only), nu_t (mirrored).
+    call o3d_keep3(ux, 1)
+    call o3d_keep3(uy, 1)
+    call o3d_keep3(uz, 1)
+    call o3d_keep3(ux_pred, 0)
+    call o3d_keep3(uy_pred, 0)
+    call o3d_keep3(uz_pred, 0)
+    call o3d_keep4(fux, 0)
+    call o3d_keep4(fuy, 0)
+    call o3d_keep4(fuz, 0)
+    if (iles == 1) call o3d_keep3(nu_t, 1)
     call predict_x(ux_pred, uy_pred, uz_pred, ux, uy, uz, fux, fuy, fuz, re, adt, bdt, &
          cdt, itime, itscheme, dx, dy, dz, nx, ny, nz, iles, cs, delta, nu_t)
     print *, ""
@@ -78,12 +94,14 @@ contains
 
   subroutine correct_pression(pp, ux_pred, uy_pred, uz_pred, dx, dy, dz, &
        nx, ny, nz, dt, omega, eps, kmax, idyn, multigrid)
-    real(kind=8), intent(inout) :: pp(:,:,:), omega
+    real(kind=8), intent(inout), target :: pp(:,:,:)
+    real(kind=8), intent(inout) :: omega
     real(kind=8), intent(in) :: ux_pred(:,:,:), uy_pred(:,:,:), uz_pred(:,:,:)
     real(kind=8), intent(in) :: dx, dy, dz, dt, eps
     integer, intent(in) :: nx, ny, nz, kmax, idyn, multigrid
 
     print *, "* Correction pression"
+    call o3d_keep3(pp, 1)  ! the driver writes the pressure out on I/O steps
     if (multigrid == 1) then
        ! src/integration.f90:242-244: npre = 5, npost = 4, tol = eps, kmax as
        ! the level count (here also the cycle limit).
@@ -149,7 +167,7 @@ contains
 
   subroutine correct_velocity(ux, uy, uz, ux_pred, uy_pred, uz_pred, &
        pp, dt, dx, dy, dz, nx, ny, nz)
-    real(kind=8), intent(inout) :: ux(:,:,:), uy(:,:,:), uz(:,:,:)
+    real(kind=8), intent(inout), target :: ux(:,:,:), uy(:,:,:), uz(:,:,:)
     real(kind=8), intent(in) :: ux_pred(:,:,:), uy_pred(:,:,:), uz_pred(:,:,:)
     real(kind=8), intent(in) :: pp(:,:,:)
     real(kind=8), intent(in) :: dt, dx, dy, dz
@@ -157,6 +175,9 @@ contains
     integer :: rc
 
     print *, "* Correct velocity"
+    call o3d_keep3(ux, 1)
+    call o3d_keep3(uy, 1)
+    call o3d_keep3(uz, 1)
     call correct_x(ux, uy, uz, ux_pred, uy_pred, uz_pred, pp, dt, dx, dy, dz, nx, ny, nz, rc)
     if (rc == 1) then
        call write_velocity_diverged()

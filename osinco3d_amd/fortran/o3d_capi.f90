@@ -131,9 +131,45 @@ module o3d_capi
      type(c_ptr) function o3d_last_error() bind(C, name="o3d_last_error")
        import :: c_ptr
      end function o3d_last_error
+
+     integer(c_int) function o3d_register(host, bytes, mirror) bind(C, name="o3d_register")
+       import :: c_int, c_ptr, c_size_t
+       type(c_ptr), value :: host
+       integer(c_size_t), value :: bytes
+       integer(c_int), value :: mirror
+     end function o3d_register
+
+     integer(c_int) function o3d_transfer_stats(h2d, d2h, reset) &
+          bind(C, name="o3d_transfer_stats")
+       import :: c_int, c_double
+       real(c_double) :: h2d, d2h
+       integer(c_int), value :: reset
+     end function o3d_transfer_stats
   end interface
 
 contains
+
+  ! Register a field the driver keeps for the whole run (a contiguous
+  ! allocatable of src/initialization.f90:50-59) with the library's device
+  ! registry: its device copy then lives across calls.  mirror = 1 when host
+  ! code reads the field between steps (the library copies it back after
+  ! each call that writes it).  A non-contiguous section is left alone (it
+  ! is staged per call).
+  subroutine o3d_keep3(a, mirror)
+    real(c_double), dimension(:,:,:), intent(in), target :: a
+    integer, intent(in) :: mirror
+    if (.not. is_contiguous(a) .or. size(a) == 0) return
+    call o3d_check(o3d_register(c_loc(a), int(8, c_size_t) * size(a, kind=c_size_t), &
+         int(mirror, c_int)), 'o3d_register')
+  end subroutine o3d_keep3
+
+  subroutine o3d_keep4(a, mirror)
+    real(c_double), dimension(:,:,:,:), intent(in), target :: a
+    integer, intent(in) :: mirror
+    if (.not. is_contiguous(a) .or. size(a) == 0) return
+    call o3d_check(o3d_register(c_loc(a), int(8, c_size_t) * size(a, kind=c_size_t), &
+         int(mirror, c_int)), 'o3d_register')
+  end subroutine o3d_keep4
 
   ! Stop with the library's message after a runtime error (status < 0).
   subroutine o3d_check(rc, where)

@@ -4,10 +4,13 @@
         [--epochs 12] [--out osinco3d_amd/warmstart_weights.npy] [--eval-n 256]
 
 Training pairs (inputs: the residual of the extrapolated pressure and e_prev;
-target: the increment e; see warmstart.py) come from
-Taylor-Green runs at n^3 (periodic, AB3, CFL 0.25, SOR eps 1e-8, mixed
-precision form): Re 1600 over the vortex breakdown, Re 400, and Re 1600 with
-the axes permuted so the network does not learn one orientation.  Every
+target: the increment e; see warmstart.py) come from seeded runs at n^3
+(SOR eps 1e-8, mixed precision form, CFL 0.25): Taylor-Green (periodic, AB3)
+at Re 1600 over the vortex breakdown, at Re 400, and at Re 1600 with the
+axes permuted so the network does not learn one orientation; and two
+temporal mixing layers (20^3 delta box, free-slip y, LES, AB2, Re 3000,
+the C3 initial field with SplitMix64 seeds 1234 and 4321), 2000 pairs in
+all by default.  Every
 sample stays resident in device memory: the network's input and target, 3
 float32 planes on the grid coarsened by capi.WARM_COARSEN (384 KiB per
 sample at 128^3).  Every tenth step of each run is held out to report the
@@ -28,9 +31,9 @@ import time
 import numpy as np
 
 from . import capi, warmstart
-from .config import tgv_config
-from .solver import (MIXED, WARM_EXTRAPOLATE, WARM_EXTRAPOLATE2, WARM_LEARNED, WARM_PREVIOUS,
-                     Solver, taylor_green)
+from .config import jet_config, mixing_layer_config, tgv_config
+from .solver import (MIXED, WARM_EXTRAPOLATE, WARM_LEARNED, WARM_PREVIOUS, Solver, mixing_layer,
+                     taylor_green)
 
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npy")
 
@@ -50,8 +53,8 @@ def permuted(fields):
 
 def runs(n: int, total: int):
     """(label, config, initial fields, steps) with about `total` steps."""
-    specs = [("tgv Re1600", 1600.0, False, 0.45), ("tgv Re400", 400.0, False, 0.25),
-             ("tgv Re1600 permuted", 1600.0, True, 0.30)]
+    specs = [("tgv Re1600", 1600.0, False, 0.35), ("tgv Re400", 400.0, False, 0.15),
+             ("tgv Re1600 permuted", 1600.0, True, 0.20)]
     out = []
     for label, re, perm, frac in specs:
         steps = max(8, int(round(total * frac)))
@@ -60,12 +63,28 @@ def runs(n: int, total: int):
         if perm:
             init = permuted(init)
         out.append((label, cfg, init, steps))
+    for seed in (1234, 4321):
+        steps = max(8, int(round(total * 0.15)))
+        cfg = mixing_layer_config(n, n, n, steps=steps, lengths=(20.0, 20.0, 20.0))
+        out.append((f"mixing layer seed {seed}", cfg, mixing_layer(cfg, seed=seed), steps))
     return out
 
 
-def collect(n: int, total: int):
+def jet_runs(total: int):
+    """The planar jet with x-Dirichlet inflow at 128x96x64 (the C5 box at a
+    quarter of its resolution), seeds 1234 and 4321 of the inflow noise: the
+    fine-tuning data of the jet's network."""
+    out = []
+    for seed in (1234, 4321):
+        steps = max(8, total // 2)
+        cfg = jet_config(128, 96, 64, steps=steps)
+        out.append((f"jet seed {seed}", cfg, (lambda s, seed=seed: s.init_jet(seed=seed)), steps))
+    return out
+
+
+def collect(n: int, total: int, config: str = "tgv+mixing"):
     train, held = [], []
-    for label, cfg, init, steps in runs(n, total):
+    for label, cfg, init, steps in (jet_runs(total) if config == "jet" else runs(n, total)):
         t0 = time.perf_counter()
         samples, iters = warmstart.capture_run(cfg, steps + 2, MIXED, init=init)
         train += [x for i, x in enumerate(samples) if i % 10 != 9]
@@ -96,7 +115,6 @@ def evaluate(net, n: int, steps: int, re: float = 1600.0):
     res = {}
     for name, warm, nn in (("previous", WARM_PREVIOUS, None),
                            ("extrapolation", WARM_EXTRAPOLATE, None),
-                           ("quadratic extrapolation", WARM_EXTRAPOLATE2, None),
                            ("learned", WARM_LEARNED, net)):
         res[name] = sweeps_with(cfg, init, steps, warm, nn)
         log(f"eval {n}^3 Re{re:g}: {name}: {res[name][0]:.1f} sweeps/step "
@@ -116,15 +134,22 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--eval-n", type=int, default=256)
     ap.add_argument("--eval-steps", type=int, default=16)
+    ap.add_argument("--config", choices=("tgv+mixing", "jet"), default="tgv+mixing",
+                    help="training data: Taylor-Green and mixing-layer runs at n^3 (C4), or "
+                         "the x-Dirichlet planar jet at 128x96x64 (fine-tuning for C5, "
+                         "usually with --init)")
     a = ap.parse_args(argv)
 
     import torch
     if not torch.cuda.is_available():
         sys.exit("train_warmstart needs a HIP device")
-    train, held = collect(a.n, a.samples)
-    log(f"{len(train)} training and {len(held)} held-out samples at {a.n}^3")
+    t_start = time.perf_counter()
+    train, held = collect(a.n, a.samples, a.config)
+    grid = (128, 96, 64) if a.config == "jet" else (a.n, a.n, a.n)
+    log(f"{len(train)} training and {len(held)} held-out samples on {grid} "
+        f"({time.perf_counter() - t_start:.1f} s to capture)")
     F = capi.WARM_COARSEN
-    net = warmstart.Network(a.n // F, a.n // F, a.n // F, seed=a.seed)
+    net = warmstart.Network(grid[0] // F, grid[1] // F, grid[2] // F, seed=a.seed)
     if a.init:
         net.load(a.init)
     log(f"network: {net.nparam} parameters")
@@ -152,7 +177,8 @@ def main(argv=None):
         f"extrapolation {before['extrapolation']:.4f})")
     del train, held
     torch.cuda.empty_cache()
-    if a.eval_n > 0:
+    log(f"training wall time {time.perf_counter() - t0:.1f} s")
+    if a.eval_n > 0 and a.config != "jet":
         evaluate(net, a.eval_n, a.eval_steps)
     net.close()
 

@@ -159,12 +159,16 @@ def normalize(r, eprev, e, dx: float):
 
 def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 2,
                 keep_every: int = 1, store_bf16: bool = False):
-    """Run the solver and keep (inp, target) samples of every kept step."""
+    """Run the solver and keep (inp, target) samples of every kept step.
+    init: initial fields, or a callable that initialises the solver (the
+    x-Dirichlet jet also sets its inflow)."""
     import torch
     from .solver import Solver, taylor_green
     s = Solver(cfg, precision)
     if init is None:
         s.upload(*taylor_green(cfg))
+    elif callable(init):
+        init(s)
     else:
         s.upload(*init)
     r = torch.empty((cfg.nz, cfg.ny, cfg.nx), dtype=torch.float32, device="cuda")

@@ -117,3 +117,35 @@ def test_mixing_layer_steps_mixed_tracks_fp64():
     assert abs(sb[1] - sa[1]) <= 1e-8 * abs(sa[1])
     for c in range(3):
         assert np.abs(fa[c] - fb[c]).max() <= 1e-4
+
+
+def test_mixing_layer_full_size_fp64_and_coarse_grid_forms_50_steps():
+    """Config 3 at full size (384x256x128, LES, free-slip y, AB2) for 50
+    steps in the FP64 form and the mixed form with the extrapolated warm
+    start and the coarse-grid correction: every solve ends on the
+    reference's rules (eps, or stagnation for this incompatible free-slip
+    rhs; never kmax), the coarse-grid form needs fewer sweeps than FP64 and
+    E_k agrees within 1e-8."""
+    from osinco3d_amd.solver import FP64, MIXED, WARM_EXTRAPOLATE, Solver, mixing_layer
+    cfg = mixing_layer_config(steps=50)
+    init = mixing_layer(cfg)
+    res = {}
+    for name, prec, warm, coarse in (("fp64", FP64, 0, (0, 0)),
+                                     ("mg", MIXED, WARM_EXTRAPOLATE, (2, 8))):
+        print(f"[fullsize] mixing layer 50 steps, form {name}", flush=True)
+        s = Solver(cfg, prec, warm, coarse_cycles=coarse[0], smooth_sweeps=coarse[1])
+        s.upload(*init)
+        sweeps, ek = [], []
+        for it in range(1, 51):
+            info = s.step(it)
+            assert info["sor_reason"] in (0, 1) and not info["diverged"], (name, it, info)
+            sweeps.append(info["sor_exit_iter"])
+            if it % 10 == 0:
+                ek.append(s.statistics(it * s.dt)[1])
+        s.close()
+        res[name] = (np.array(sweeps), np.array(ek))
+        print(f"[fullsize] {name}: sweeps/step {np.mean(sweeps):.1f} (max {max(sweeps)})",
+              flush=True)
+    (s64, e64), (smg, emg) = res["fp64"], res["mg"]
+    assert smg.mean() < s64.mean()
+    assert np.abs(emg - e64).max() <= 1e-8 * np.abs(e64).max()

@@ -2,14 +2,18 @@
 Taylor-Green Re 1600 at 256^3, mixed-precision form, SOR to eps = 1e-8.
 
 The method without them starts each solve from the previous step's pressure
-(the reference's behaviour).  The learned form starts from the linear
-extrapolation corrected by the trained network (the previous pressure stays
-the fallback); the plain extrapolation is measured beside it so the
-network's own share of the gain is visible.  The coarse-grid correction
-(multigrid on the 2^3 block averages of the guess's residual) with
-Gauss-Seidel sweeps opening the FP32 solve is measured on its own and
-after the network (the headline form).  The solve's stop rule is the same
-in every form, so the fields agree to a tolerance tied to eps.
+(the reference's behaviour).  The extrapolated guess p_prev + e_prev is
+measured on its own, and the learned form (the network's coarse correction
+of that extrapolation, the previous pressure as fallback) beside it, so the
+network's own share is visible: the network may not cost sweeps against the
+extrapolation it corrects, and on held-out solves of a run it was not
+trained on its prediction of the increment must beat the extrapolation's.
+The coarse-grid correction (multigrid on the 2^3 block averages of the
+guess's residual) with Gauss-Seidel sweeps opening the FP32 solve is
+measured after each.  No test here claims that the network beats the
+coarse-grid correction: it does not (measured figures in the design notes).
+The solve's stop rule is the same in every form, so the fields agree to a
+tolerance tied to eps.
 """
 from __future__ import annotations
 
@@ -54,7 +58,7 @@ def test_warm_starts_reduce_sor_sweeps():
     net = warmstart.load_or_none(WEIGHTS, (N, N, N))
     try:
         it_prev, _, f_prev = run(WARM_PREVIOUS)
-        it_ex, _, _ = run(WARM_EXTRAPOLATE)
+        it_ex, used_ex, _ = run(WARM_EXTRAPOLATE)
         it_nn, used, f_nn = run(WARM_LEARNED, net)
         it_mg, used_mg, f_mg = run(WARM_EXTRAPOLATE, coarse=COARSE)
         it_nm, used_nm, f_nm = run(WARM_LEARNED, net, COARSE)
@@ -62,17 +66,40 @@ def test_warm_starts_reduce_sor_sweeps():
         net.close()
     print(f"[warm] SOR sweeps/step at {N}^3: previous {it_prev.mean():.1f}, "
           f"extrapolation {it_ex.mean():.1f}, learned {it_nn.mean():.1f} (accepted on "
-          f"{used:.2f} of the steps), extrapolation + coarse-grid correction "
-          f"{it_mg.mean():.1f}, learned + coarse-grid correction {it_nm.mean():.1f}",
-          flush=True)
-    assert used == 1.0 and used_mg == 1.0 and used_nm == 1.0
-    assert it_nn.mean() <= 0.8 * it_prev.mean()
-    assert it_nn.mean() <= 1.02 * it_ex.mean()
-    assert it_mg.mean() <= 0.5 * it_ex.mean()
+          f"{used:.2f} of the steps; network's own gain "
+          f"{100 * (1 - it_nn.mean() / it_ex.mean()):.1f} %), extrapolation + coarse-grid "
+          f"correction {it_mg.mean():.1f}, learned + coarse-grid correction "
+          f"{it_nm.mean():.1f}", flush=True)
+    assert used_ex == 1.0 and used == 1.0 and used_mg == 1.0 and used_nm == 1.0
+    assert it_ex.mean() <= 0.8 * it_prev.mean()      # the extrapolation's gain
+    assert it_nn.mean() <= it_ex.mean()              # the network may not lose sweeps
+    assert it_mg.mean() <= 0.5 * it_ex.mean()        # the coarse-grid correction's gain
     assert it_nm.mean() <= 0.5 * it_ex.mean()
     for f in (f_nn, f_mg, f_nm):
         for c in range(3):
             assert np.abs(f[c] - f_prev[c]).max() <= 1e-6
+
+
+def test_network_beats_extrapolation_on_unseen_solves():
+    """The network's own target: the increment e = p_new - p_prev of the
+    solves (block averages on its grid, divided by RMS(e_prev)).  On solves
+    of a run it was not trained on (Taylor-Green Re 1000 at 128^3), its
+    relative L2 error is below that of the linear extrapolation e = e_prev
+    it corrects."""
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained warm-start weights in the package")
+    from osinco3d_amd import warmstart
+    cfg = tgv_config(128, 60, 3, re=1000.0)
+    samples, iters = warmstart.capture_run(cfg, 60, MIXED, skip=10)
+    net = warmstart.load_or_none(WEIGHTS, (128, 128, 128))
+    try:
+        err = warmstart.relative_errors(net, samples)
+    finally:
+        net.close()
+    print(f"[warm] unseen TGV Re 1000 128^3, {len(samples)} solves: relative error of the "
+          f"increment: network {err['nn']:.4f}, extrapolation {err['extrapolation']:.4f}, "
+          f"previous pressure {err.get('previous', float('nan')):.4f}", flush=True)
+    assert err["nn"] < err["extrapolation"]
 
 
 def test_coarse_grid_correction_on_incompatible_mixing_layer():
