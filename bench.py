@@ -19,6 +19,8 @@ Each requested form is timed over the same steps:
          osinco3d_amd/warmstart_weights.npy; skipped when it is missing)
   nn     nn0 + the coarse-grid correction and Gauss-Seidel sweeps of mg (the
          headline: every warm-start stage)
+  nn1    nn with one V-cycle (instead of coarse_cycles) in the first coarse-grid
+         correction, which starts from the network's coarse estimate
 The JSON line's value is the steps/s of the headline form (the last of
 --forms); every form's figures are included under "forms".
 
@@ -130,18 +132,20 @@ def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
     from osinco3d_amd import solver as S
 
     prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE,
-            "mg": S.WARM_EXTRAPOLATE, "extrap2": S.WARM_EXTRAPOLATE2}.get(form, S.WARM_PREVIOUS)
-    cc, gs = coarse if form in ("mg", "nn") else (0, 0)
+    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "nn1": S.WARM_LEARNED,
+            "extrap": S.WARM_EXTRAPOLATE, "mg": S.WARM_EXTRAPOLATE,
+            "extrap2": S.WARM_EXTRAPOLATE2}.get(form, S.WARM_PREVIOUS)
+    cc, gs = coarse if form in ("mg", "nn", "nn1") else (0, 0)
     net = None
-    if form in ("nn", "nn0"):
+    if form in ("nn", "nn0", "nn1"):
         from osinco3d_amd import warmstart
         net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
         if net is None:
             print(f"[bench] form nn skipped: no model at {model_path}", file=sys.stderr,
                   flush=True)
             return None
-    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs)
+    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs,
+                 coarse_cycles_learned=1 if form == "nn1" else 0)
     if net is not None:
         net.attach(s)
     if init is None:
@@ -216,7 +220,7 @@ def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
     if cc or gs:
         out["coarse_cycles"] = cc
         out["smooth_sweeps"] = gs
-    if form in ("nn", "nn0", "extrap", "extrap2", "mg"):
+    if form in ("nn", "nn0", "nn1", "extrap", "extrap2", "mg"):
         out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)
         out["warm_residual_l2"] = {"previous": statistics.mean(i["warm_dmax"][0] for i in infos),
                                    "guess": statistics.mean(i["warm_dmax"][1] for i in infos)}
@@ -245,16 +249,17 @@ def run_series(form, cfg, total, window, model_path, coarse, init=None):
     from osinco3d_amd import solver as S
 
     prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "extrap": S.WARM_EXTRAPOLATE,
-            "mg": S.WARM_EXTRAPOLATE}.get(form, S.WARM_PREVIOUS)
-    cc, gs = coarse if form in ("mg", "nn") else (0, 0)
+    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "nn1": S.WARM_LEARNED,
+            "extrap": S.WARM_EXTRAPOLATE, "mg": S.WARM_EXTRAPOLATE}.get(form, S.WARM_PREVIOUS)
+    cc, gs = coarse if form in ("mg", "nn", "nn1") else (0, 0)
     net = None
-    if form in ("nn", "nn0"):
+    if form in ("nn", "nn0", "nn1"):
         from osinco3d_amd import warmstart
         net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
         if net is None:
             return None
-    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs)
+    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs,
+                 coarse_cycles_learned=1 if form == "nn1" else 0)
     if net is not None:
         net.attach(s)
     if init is None:

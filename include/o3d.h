@@ -159,12 +159,15 @@ int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, double hx, d
  * when max |rhs - L pp| < tol (the reference's stated criterion), coarse-grid
  * correction on the 2^3 block averages by one V(npre, npost) cycle over at
  * most nlevels - 1 coarser levels, npre + npost red-black Gauss-Seidel sweeps
- * on the full grid, pp += correction.  Stop rules as the SOR solvers'
- * (src/poisson.f90:95-122) on max |r|: reason 0 = max |r| < tol,
- * 1 = stagnation (max |r| changed by less than tol / 1000 since the last
- * cycle: an rhs the mirrored operator cannot satisfy), 2 = max_cycles cycles
- * done.  cycles, resmax (the final max |r|) and reason may be NULL.  Needs
- * nx a multiple of 8 and ny, nz even. */
+ * on the full grid, pp += correction.  With a mirrored (free-slip) axis the
+ * operator's columns sum to zero under the trapezoid weights, so only an
+ * rhs of zero weighted mean can be met: the solve works on the rhs minus its
+ * weighted mean (the part any solver can satisfy) and r is judged on it.
+ * Stop rules as the SOR solvers' (src/poisson.f90:95-122) on max |r|:
+ * reason 0 = max |r| < tol, 1 = stagnation (max |r| changed by less than
+ * tol / 1000 since the last cycle), 2 = max_cycles cycles done.  cycles,
+ * resmax (the final max |r|) and reason may be NULL.  Needs nx a multiple
+ * of 8 and ny, nz even. */
 int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, double dy, double dz,
                           int nx, int ny, int nz, int solver, int nlevels, int npre, int npost,
                           double tol, int max_cycles, int *cycles, double *resmax, int *reason);
@@ -258,6 +261,9 @@ typedef struct o3d_params {
                            every axis even) */
   int smooth_sweeps;    /* O3D_MIXED: Gauss-Seidel sweeps (omega = 1) that open the first FP32
                            correction solve of each step (0 = none) */
+  int coarse_cycles_learned; /* with the learned warm start: V-cycles of the first coarse-grid
+                                correction, which starts from the network's coarse estimate
+                                instead of zero (0 = coarse_cycles) */
 } o3d_params;
 
 typedef struct o3d_step_info {

@@ -45,6 +45,7 @@ class O3DParams(C.Structure):
         ("inner_max", C.c_int),
         ("coarse_cycles", C.c_int),
         ("smooth_sweeps", C.c_int),
+        ("coarse_cycles_learned", C.c_int),
     ]
 
 

@@ -471,18 +471,31 @@ extern "C" int o3d_transfer_stats(double *h2d_bytes, double *d2h_bytes, int rese
 
 // ============================================================ measurement
 namespace {
-// Streaming triad c = a + s b over float4 vectors, grid-stride: two reads
-// and one write per element, no reuse, the HBM ceiling the stencil and SOR
-// kernels are compared with.
-__global__ void __launch_bounds__(256) stream_triad_kernel(const float4 *__restrict__ a,
-                                                           const float4 *__restrict__ b,
-                                                           float4 *__restrict__ c, size_t n4,
+// Streaming triad c = a + s b over float4 vectors: two reads and one write
+// per element, no reuse, the HBM ceiling the stencil and SOR kernels are
+// compared with.  Each thread handles 4 vectors per round (their loads in
+// flight together) with non-temporal loads and stores (no cache pollution).
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) stream_triad_kernel(const float4 *__restrict__ a4,
+                                                           const float4 *__restrict__ b4,
+                                                           float4 *__restrict__ c4, size_t n4,
                                                            float s) {
+  const v4f *a = reinterpret_cast<const v4f *>(a4);
+  const v4f *b = reinterpret_cast<const v4f *>(b4);
+  v4f *c = reinterpret_cast<v4f *>(c4);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-    const float4 x = a[i], y = b[i];
-    c[i] = make_float4(x.x + s * y.x, x.y + s * y.y, x.z + s * y.z, x.w + s * y.w);
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    v4f x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = __builtin_nontemporal_load(a + i + u * stride);
+      y[u] = __builtin_nontemporal_load(b + i + u * stride);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(x[u] + s * y[u], c + i + u * stride);
   }
+  for (; i < n4; i += stride) c[i] = a[i] + s * b[i];
 }
 }  // namespace
 
@@ -499,7 +512,7 @@ extern "C" int o3d_stream_probe(size_t bytes_per_buffer, int reps, double *gbs, 
     rc = fail("stream probe: events");
   if (rc == 0) {
     for (int q = 0; q < 3; ++q) (void)hipMemsetAsync(buf[q], 0, n4 * 16, g_stream);
-    const int blocks = 256 * 16;
+    const int blocks = 256 * 8;
     for (int w = 0; w < 3; ++w)
       stream_triad_kernel<<<blocks, 256, 0, g_stream>>>(buf[0], buf[1], buf[2], n4, 0.5f);
     (void)hipEventRecord(e0, g_stream);
@@ -869,6 +882,153 @@ extern "C" int o3d_multigrid(float *u, const float *f, int mx, int my, int mz, d
   return s.finish();
 }
 
+namespace {
+// ---- x-Dirichlet inflow / outflow (no counterpart in the reference) --------
+// Flux weight of node q of an axis of n nodes: 1/2 on the end nodes of a
+// mirrored axis (the trapezoid weights that make the mirrored 7-point
+// operator's rows sum to zero), else 1.
+__host__ __device__ __forceinline__ double edge_weight(int q, int n, int mirrored) {
+  return (mirrored && (q == 0 || q == n - 1)) ? 0.5 : 1.0;
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014) as a counter-based generator: output
+// number counter + 1 of the stream seeded with `seed`, as a uniform double
+// in [-1, 1) from the top 53 bits.
+__host__ __device__ __forceinline__ double noise_uniform(unsigned long long seed,
+                                                         long long counter) {
+  unsigned long long z = seed + (unsigned long long)(counter + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+}
+
+// After predict, one thread per (j, k): the inflow plane i = 0 gets the
+// profile plus the noise (amp w)(x1 C + x2 S) with x1, x2 drawn per node and
+// component, C, S the cosine and sine of the inflow phase; the outflow plane
+// i = nx - 1 gets first-order upwind convection at the convective velocity
+// (cu = Uc dt / dx) from the velocity before the step.  The x components of
+// both planes go to pin / pout for the flux balance.
+template <typename T>
+__global__ void __launch_bounds__(256) jet_edges_kernel(
+    T *__restrict__ up0, T *__restrict__ up1, T *__restrict__ up2, const T *__restrict__ u0,
+    const T *__restrict__ u1, const T *__restrict__ u2, const double *__restrict__ base,
+    const double *__restrict__ w, double amp, double cph, double sph, unsigned long long seed,
+    double cu, int nx, int ny, int nz, double *__restrict__ pin, double *__restrict__ pout) {
+  const int P = ny * nz;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  const int j = q % ny, k = q / ny;
+  const int row = nx * (j + ny * k);
+  T *up[3] = {up0, up1, up2};
+  const T *u[3] = {u0, u1, u2};
+  const double aw = amp * w[q];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long long cnt = 2 * (3 * (long long)q + c);
+    const double x1 = noise_uniform(seed, cnt), x2 = noise_uniform(seed, cnt + 1);
+    const double vin = base[c * P + q] + aw * (x1 * cph + x2 * sph);
+    const double ub = (double)u[c][row + nx - 1], ub1 = (double)u[c][row + nx - 2];
+    const double vout = ub - cu * (ub - ub1);
+    up[c][row] = T(vin);
+    up[c][row + nx - 1] = T(vout);
+    if (c == 0) {
+      pin[q] = (double)T(vin);
+      pout[q] = (double)T(vout);
+    }
+  }
+}
+
+// The plane sums in a fixed order (per z row the sequential sum over j of
+// wy v, then one thread's sequential sum over k of wz row), and the shift
+// (Q_in - Q_out) / W that makes the outflow flux equal the inflow flux.
+__global__ void __launch_bounds__(256) jet_flux_kernel(const double *__restrict__ pin,
+                                                       const double *__restrict__ pout,
+                                                       double *__restrict__ rows, int ny,
+                                                       int nz, int my, int mz, double W,
+                                                       double *__restrict__ flux) {
+  for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+    double a = 0.0, b = 0.0;
+    for (int j = 0; j < ny; ++j) {
+      const double wy = edge_weight(j, ny, my);
+      a = a + wy * pin[j + ny * k];
+      b = b + wy * pout[j + ny * k];
+    }
+    rows[k] = a;
+    rows[nz + k] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double qi = 0.0, qo = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      const double wz = edge_weight(k, nz, mz);
+      qi = qi + wz * rows[k];
+      qo = qo + wz * rows[nz + k];
+    }
+    flux[0] = qi;
+    flux[1] = qo;
+    flux[2] = (qi - qo) / W;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) jet_shift_kernel(T *__restrict__ up0,
+                                                        const double *__restrict__ pout,
+                                                        const double *__restrict__ flux, int nx,
+                                                        int ny, int nz) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ny * nz) return;
+  const int j = q % ny, k = q / ny;
+  up0[nx * (j + ny * k) + nx - 1] = T(pout[q] + flux[2]);
+}
+
+// Weighted sum of a field (f null: of the weights alone) with the trapezoid
+// weights of the mirrored axes: kWsumBlocks chunks of ceil(n / kWsumBlocks)
+// cells, thread t of a chunk summing cells lo + t, lo + t + 256, ... in
+// order, the chunk total by block_sum<4>.
+constexpr int kWsumBlocks = 1024;
+template <typename T>
+__global__ void __launch_bounds__(256) wsum_kernel(const T *__restrict__ f, int nx, int ny,
+                                                   int nz, int mx, int my, int mz,
+                                                   double *__restrict__ part) {
+  __shared__ double sv[4];
+  const long long n = (long long)nx * ny * nz;
+  const long long per = (n + kWsumBlocks - 1) / kWsumBlocks;
+  const long long lo = (long long)blockIdx.x * per;
+  const long long hi = lo + per < n ? lo + per : n;
+  double acc = 0.0;
+  for (long long q = lo + threadIdx.x; q < hi; q += 256) {
+    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long long)nx * ny));
+    const double wq = (edge_weight(i, nx, mx) * edge_weight(j, ny, my)) * edge_weight(k, nz, mz);
+    acc = acc + (f ? wq * (double)f[q] : wq);
+  }
+  acc = block_sum<4>(acc, sv);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// The chunk totals summed by 1024 threads (one each) with block_sum<16>;
+// out = total, or total / *den when den is given.
+__global__ void __launch_bounds__(1024) wsum_final_kernel(const double *__restrict__ part,
+                                                          const double *__restrict__ den,
+                                                          double *__restrict__ out) {
+  __shared__ double sv[16];
+  double v = 0.0;
+  if (threadIdx.x < kWsumBlocks) v = v + part[threadIdx.x];
+  v = block_sum<16>(v, sv);
+  if (threadIdx.x == 0) *out = den ? v / *den : v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) sub_mean_kernel(T *__restrict__ f, long long n,
+                                                       const double *__restrict__ mean) {
+  const double m = *mean;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    f[q] = T((double)f[q] - m);
+}
+
+}  // namespace
+
 // Multigrid solve of the pressure Poisson problem of the SOR solvers (same
 // 7-point operator and neighbour rules).  A cycle: the FP64 residual of pp
 // (stored colour-split in FP32), stop when its maximum is below tol, the
@@ -903,6 +1063,26 @@ extern "C" int o3d_poisson_multigrid(double *pp, const double *rhs, double dx, d
   int per[3];
   solver_periodic(solver, per);
   const SorConst c = sor_const(dx, dy, dz);
+  if (!per[0] || !per[1] || !per[2]) {
+    // A mirrored axis: the operator's columns sum to zero under the
+    // trapezoid weights (1/2 on the end nodes), so only the rhs with zero
+    // weighted mean can be met.  The solve works on that compatible part
+    // (the rhs minus its weighted mean), and max |r| is judged on it: an
+    // incompatible rhs would otherwise leave a part no cycle can reduce.
+    double *rc, *wp;
+    CHK(s.scratch(n, &rc));
+    CHK(s.scratch((size_t)kWsumBlocks + 2, &wp));
+    wsum_kernel<double><<<kWsumBlocks, 256, 0, g_stream>>>(nullptr, nx, ny, nz, !per[0],
+                                                           !per[1], !per[2], wp);
+    wsum_final_kernel<<<1, 1024, 0, g_stream>>>(wp, nullptr, wp + kWsumBlocks);
+    wsum_kernel<double><<<kWsumBlocks, 256, 0, g_stream>>>(rhs_d, nx, ny, nz, !per[0],
+                                                           !per[1], !per[2], wp);
+    wsum_final_kernel<<<1, 1024, 0, g_stream>>>(wp, wp + kWsumBlocks, wp + kWsumBlocks + 1);
+    HIPCHK(hipMemcpyAsync(rc, rhs_d, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+    sub_mean_kernel<double><<<2048, 256, 0, g_stream>>>(rc, (long long)n, wp + kWsumBlocks + 1);
+    HIPCHK(hipGetLastError());
+    rhs_d = rc;
+  }
   // The coarse levels: the grid of 2^3 blocks and the levels below it, at
   // most nlevels - 1 of them (nlevels counts the full grid).
   MgHierarchy H;
@@ -1368,150 +1548,6 @@ struct Solver {
     if (hflux) (void)hipHostFree(hflux);
   }
 };
-
-// ---- x-Dirichlet inflow / outflow (no counterpart in the reference) --------
-// Flux weight of node q of an axis of n nodes: 1/2 on the end nodes of a
-// mirrored axis (the trapezoid weights that make the mirrored 7-point
-// operator's rows sum to zero), else 1.
-__host__ __device__ __forceinline__ double edge_weight(int q, int n, int mirrored) {
-  return (mirrored && (q == 0 || q == n - 1)) ? 0.5 : 1.0;
-}
-
-// SplitMix64 (Steele, Lea, Flood 2014) as a counter-based generator: output
-// number counter + 1 of the stream seeded with `seed`, as a uniform double
-// in [-1, 1) from the top 53 bits.
-__host__ __device__ __forceinline__ double noise_uniform(unsigned long long seed,
-                                                         long long counter) {
-  unsigned long long z = seed + (unsigned long long)(counter + 1) * 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z = z ^ (z >> 31);
-  return (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
-}
-
-// After predict, one thread per (j, k): the inflow plane i = 0 gets the
-// profile plus the noise (amp w)(x1 C + x2 S) with x1, x2 drawn per node and
-// component, C, S the cosine and sine of the inflow phase; the outflow plane
-// i = nx - 1 gets first-order upwind convection at the convective velocity
-// (cu = Uc dt / dx) from the velocity before the step.  The x components of
-// both planes go to pin / pout for the flux balance.
-template <typename T>
-__global__ void __launch_bounds__(256) jet_edges_kernel(
-    T *__restrict__ up0, T *__restrict__ up1, T *__restrict__ up2, const T *__restrict__ u0,
-    const T *__restrict__ u1, const T *__restrict__ u2, const double *__restrict__ base,
-    const double *__restrict__ w, double amp, double cph, double sph, unsigned long long seed,
-    double cu, int nx, int ny, int nz, double *__restrict__ pin, double *__restrict__ pout) {
-  const int P = ny * nz;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P) return;
-  const int j = q % ny, k = q / ny;
-  const int row = nx * (j + ny * k);
-  T *up[3] = {up0, up1, up2};
-  const T *u[3] = {u0, u1, u2};
-  const double aw = amp * w[q];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const long long cnt = 2 * (3 * (long long)q + c);
-    const double x1 = noise_uniform(seed, cnt), x2 = noise_uniform(seed, cnt + 1);
-    const double vin = base[c * P + q] + aw * (x1 * cph + x2 * sph);
-    const double ub = (double)u[c][row + nx - 1], ub1 = (double)u[c][row + nx - 2];
-    const double vout = ub - cu * (ub - ub1);
-    up[c][row] = T(vin);
-    up[c][row + nx - 1] = T(vout);
-    if (c == 0) {
-      pin[q] = (double)T(vin);
-      pout[q] = (double)T(vout);
-    }
-  }
-}
-
-// The plane sums in a fixed order (per z row the sequential sum over j of
-// wy v, then one thread's sequential sum over k of wz row), and the shift
-// (Q_in - Q_out) / W that makes the outflow flux equal the inflow flux.
-__global__ void __launch_bounds__(256) jet_flux_kernel(const double *__restrict__ pin,
-                                                       const double *__restrict__ pout,
-                                                       double *__restrict__ rows, int ny,
-                                                       int nz, int my, int mz, double W,
-                                                       double *__restrict__ flux) {
-  for (int k = threadIdx.x; k < nz; k += blockDim.x) {
-    double a = 0.0, b = 0.0;
-    for (int j = 0; j < ny; ++j) {
-      const double wy = edge_weight(j, ny, my);
-      a = a + wy * pin[j + ny * k];
-      b = b + wy * pout[j + ny * k];
-    }
-    rows[k] = a;
-    rows[nz + k] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double qi = 0.0, qo = 0.0;
-    for (int k = 0; k < nz; ++k) {
-      const double wz = edge_weight(k, nz, mz);
-      qi = qi + wz * rows[k];
-      qo = qo + wz * rows[nz + k];
-    }
-    flux[0] = qi;
-    flux[1] = qo;
-    flux[2] = (qi - qo) / W;
-  }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) jet_shift_kernel(T *__restrict__ up0,
-                                                        const double *__restrict__ pout,
-                                                        const double *__restrict__ flux, int nx,
-                                                        int ny, int nz) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ny * nz) return;
-  const int j = q % ny, k = q / ny;
-  up0[nx * (j + ny * k) + nx - 1] = T(pout[q] + flux[2]);
-}
-
-// Weighted sum of a field (f null: of the weights alone) with the trapezoid
-// weights of the mirrored axes: kWsumBlocks chunks of ceil(n / kWsumBlocks)
-// cells, thread t of a chunk summing cells lo + t, lo + t + 256, ... in
-// order, the chunk total by block_sum<4>.
-constexpr int kWsumBlocks = 1024;
-template <typename T>
-__global__ void __launch_bounds__(256) wsum_kernel(const T *__restrict__ f, int nx, int ny,
-                                                   int nz, int mx, int my, int mz,
-                                                   double *__restrict__ part) {
-  __shared__ double sv[4];
-  const long long n = (long long)nx * ny * nz;
-  const long long per = (n + kWsumBlocks - 1) / kWsumBlocks;
-  const long long lo = (long long)blockIdx.x * per;
-  const long long hi = lo + per < n ? lo + per : n;
-  double acc = 0.0;
-  for (long long q = lo + threadIdx.x; q < hi; q += 256) {
-    const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long long)nx * ny));
-    const double wq = (edge_weight(i, nx, mx) * edge_weight(j, ny, my)) * edge_weight(k, nz, mz);
-    acc = acc + (f ? wq * (double)f[q] : wq);
-  }
-  acc = block_sum<4>(acc, sv);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-// The chunk totals summed by 1024 threads (one each) with block_sum<16>;
-// out = total, or total / *den when den is given.
-__global__ void __launch_bounds__(1024) wsum_final_kernel(const double *__restrict__ part,
-                                                          const double *__restrict__ den,
-                                                          double *__restrict__ out) {
-  __shared__ double sv[16];
-  double v = 0.0;
-  if (threadIdx.x < kWsumBlocks) v = v + part[threadIdx.x];
-  v = block_sum<16>(v, sv);
-  if (threadIdx.x == 0) *out = den ? v / *den : v;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) sub_mean_kernel(T *__restrict__ f, long long n,
-                                                       const double *__restrict__ mean) {
-  const double m = *mean;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-       q += (long long)gridDim.x * blockDim.x)
-    f[q] = T((double)f[q] - m);
-}
 
 // The inflow and outflow planes of u* after predict, with the outflow flux
 // set equal to the inflow flux (x-Dirichlet).
@@ -2303,7 +2339,11 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     const int nc = n / (F * F * F);
     HIPCHK(launch_lw_pool(S.Rl, S.mg.f[0], S.p.nx, S.p.ny, S.p.nz, F, 1.0f));
     if (nn_coarse) HIPCHK(mg_seed_from_level1(S.mg, g_stream));
-    HIPCHK(mg_solve(S.mg, S.p.coarse_cycles, 1, 1, g_stream, !nn_coarse));
+    // From the network's coarse estimate the first correction may run fewer
+    // V-cycles (coarse_cycles_learned) than from zero.
+    const int cyc = nn_coarse && S.p.coarse_cycles_learned > 0 ? S.p.coarse_cycles_learned
+                                                                : S.p.coarse_cycles;
+    HIPCHK(mg_solve(S.mg, cyc, 1, 1, g_stream, !nn_coarse));
     dim3 grid((S.p.nx / 2 + 63) / 64, (S.p.ny + 3) / 4, S.p.nz);
     mg_prolong_fine_kernel<<<grid, dim3(64, 4), 0, g_stream>>>(q, S.mg.u[0], S.p.nx, S.p.ny,
                                                                S.p.nz, F, per[0], per[1], per[2]);
@@ -2468,8 +2508,8 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     return fail("unknown warm start %d", p->warm_start);
   const int kind = solver_from_bc(p->bc);
   if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");
-  if (p->coarse_cycles < 0 || p->smooth_sweeps < 0)
-    return fail("coarse_cycles and smooth_sweeps must be >= 0");
+  if (p->coarse_cycles < 0 || p->smooth_sweeps < 0 || p->coarse_cycles_learned < 0)
+    return fail("coarse_cycles, coarse_cycles_learned and smooth_sweeps must be >= 0");
   if ((p->coarse_cycles > 0 || p->smooth_sweeps > 0) && p->precision != O3D_MIXED)
     return fail("coarse_cycles and smooth_sweeps apply to the mixed form only");
   if (p->coarse_cycles > 0 && (p->nx % 2 || p->ny % 2 || p->nz % 2))

@@ -262,6 +262,52 @@ __global__ void pack_input_kernel(const float *__restrict__ in, bf16 *__restrict
   for (int c = 2; c < C; ++c) row[c] = (bf16)0.f;
 }
 
+// The first convolution in the forward pass: only 2 of its 16 (padded)
+// input channels are non-zero, so it runs as a direct 3x3x3 convolution on
+// the vector ALUs instead of an implicit GEMM whose K would be 8/9 zeros.
+// One thread per voxel computes all 16 output channels from the 27 taps x 2
+// channels (the same BF16 inputs and weights as the MFMA path, FP32 sums),
+// then bias and leaky ReLU; the weights (16 x 27 x 2) are staged in LDS.
+__global__ void __launch_bounds__(256) conv_first_kernel(const bf16 *__restrict__ x,
+                                                         const bf16 *__restrict__ w,
+                                                         const float *__restrict__ bias,
+                                                         bf16 *__restrict__ y, int nx, int ny,
+                                                         int nz, int kp) {
+  __shared__ float ws[27][2][16];
+  __shared__ float bs[16];
+  for (int e = threadIdx.x; e < 27 * 2 * 16; e += blockDim.x) {
+    const int co = e & 15, c = (e >> 4) & 1, t = e >> 5;
+    ws[t][c][co] = (float)w[(size_t)co * kp + t * 16 + c];
+  }
+  if (threadIdx.x < 16) bs[threadIdx.x] = bias[threadIdx.x];
+  __syncthreads();
+  const int V = nx * ny * nz;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const int i = v % nx, j = (v / nx) % ny, k = v / (nx * ny);
+  float acc[16];
+#pragma unroll
+  for (int co = 0; co < 16; ++co) acc[co] = bs[co];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) {
+    const int vx = wrapi(i + t % 3 - 1, nx), vy = wrapi(j + (t / 3) % 3 - 1, ny),
+              vz = wrapi(k + t / 9 - 1, nz);
+    const bf16 *src = x + (size_t)(vx + nx * (vy + ny * vz)) * 16;
+    const float a0 = (float)src[0], a1 = (float)src[1];
+#pragma unroll
+    for (int co = 0; co < 16; ++co) acc[co] += a0 * ws[t][0][co] + a1 * ws[t][1][co];
+  }
+  bf16x8 o0, o1;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    o0[q] = (bf16)act_f(acc[q]);
+    o1[q] = (bf16)act_f(acc[8 + q]);
+  }
+  bf16x8 *dst = (bf16x8 *)(y + (size_t)v * 16);
+  dst[0] = o0;
+  dst[1] = o1;
+}
+
 __global__ void pool_fwd_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int nx,
                                 int ny, int nz, int C) {
   // out grid (nx/2, ny/2, nz/2)
@@ -612,6 +658,15 @@ void init_params(std::vector<float> &w, const Net &n, uint64_t seed) {
 
 int conv_fwd(Net &n, int li, const bf16 *x, const bf16 *skip, bf16 *y) {
   const Conv &c = n.conv[li];
+  if (li == 0 && c.ci == 16 && c.co == 16 && !skip) {
+    const int V = (int)n.V[c.level];
+    conv_first_kernel<<<(V + 255) / 256, 256, 0, st()>>>(x, c.wf, n.W + c.boff, y,
+                                                         n.nx[c.level], n.ny[c.level],
+                                                         n.nz[c.level], c.kpf);
+    NNCHK(hipGetLastError());
+    n.last_flops += 54.0 * 2 * c.co * (double)V;  // the two real input channels
+    return 0;
+  }
   ConvArgs<bf16> a;
   a.x = x; a.w = c.wf; a.bias = n.W + c.boff; a.skip = skip; a.y = y;
   a.nx = n.nx[c.level]; a.ny = n.ny[c.level]; a.nz = n.nz[c.level];

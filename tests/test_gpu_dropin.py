@@ -97,12 +97,13 @@ def test_dropin_multigrid_switch():
         assert np.abs((pp - pp.mean()) - (pr - pr.mean())).max() <= 2e-5
 
 
-def test_dropin_multigrid_incompatible_rhs_stagnates():
+def test_dropin_multigrid_incompatible_rhs_converges():
     """multigrid = 1 on the mixing-layer run (free-slip y, whose rhs the
-    mirror cannot satisfy, so max |r| never reaches eps): every step's solve
-    ends on the stagnation rule, far below kmax = 5000 cycles, and the run
-    follows the reference's SOR run (which stagnates as well) within the
-    stagnation-tied tolerance."""
+    mirror cannot satisfy, so the reference's SOR ends every solve on its
+    stagnation rule): the multigrid solve works on the compatible part of
+    the rhs and ends every step on eps (or stagnation) far below kmax, and
+    the run follows the reference's SOR run within the stagnation-tied
+    tolerance."""
     if not os.path.exists(EXE):
         pytest.skip("oracle/_ref/o3d_dropin_run not built (needs the reference tree)")
     g = np.load(os.path.join(ROOT, "tests", "golden", "run_mixing24_ab2_les.npz"),
@@ -116,7 +117,7 @@ def test_dropin_multigrid_incompatible_rhs_stagnates():
          re.findall(r"Multigrid cycles\s*:\s*(\d+),\s*\S+,\s*(\d+)", out)]
     print(f"[gpu] multigrid (cycles, reason) per step: {m}", flush=True)
     assert len(m) == cfg.itstop
-    assert all(why == 1 and cyc < 200 for cyc, why in m)
+    assert all(why in (0, 1) and cyc < 200 for cyc, why in m)
     for itime, t, ux, uy, uz, pp in recs:
         if itime == 0:
             continue

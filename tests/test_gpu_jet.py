@@ -107,7 +107,7 @@ def test_jet_fp64_steps_bit_identical(orc, shape):
         for c in range(3):
             assert np.array_equal(g[c], u[c]), (it, c, np.abs(g[c] - u[c]).max())
         assert np.array_equal(g[3], p), (it, np.abs(g[3] - p).max())
-        assert info["sor_exit_iter"] == oi["exit_iter"] and info["sor_reason"] == 0
+        assert info["sor_exit_iter"] == oi["exit_iter"] and info["sor_reason"] == oi["reason"]
         qin, qout, W = oi["flux"]
         assert info["flow_rate"][0] == qin / W and info["flow_rate"][1] == qout / W
     s.close()
@@ -119,7 +119,9 @@ def _jet_run(cfg, precision, steps, warm=0, coarse=(0, 0)):
     ek, en, iters = [], [], []
     for it in range(1, steps + 1):
         info = s.step(it)
-        assert info["sor_reason"] == 0 and not info["diverged"], (it, info["sor_reason"])
+        # eps, or the reference's stagnation rule (|dmax - dmax_old| < eps/1000,
+        # which over-relaxed sweeps can meet by coincidence); never kmax.
+        assert info["sor_reason"] in (0, 1) and not info["diverged"], (it, info["sor_reason"])
         iters.append(info["sor_exit_iter"])
         st = s.statistics(it * s.dt)
         ek.append(st[1])
@@ -139,6 +141,11 @@ def _balance(cfg, u):
 
 
 def test_jet_mixed_tracks_fp64_small():
+    """64 x 48 x 32 jet, 40 steps: the mixed forms against FP64.  On this
+    coarse grid some FP64 solves end on the stagnation rule with dmax near
+    5e-7 instead of 1e-8 (the rule compares consecutive over-relaxed dmax
+    values), so the pressures differ by more than at eps; E_k and enstrophy
+    agree to 5e-6 (at full size, 3e-7)."""
     cfg = jet_config(64, 48, 32, steps=40)
     e0, z0, i0, u0 = _jet_run(cfg, FP64, 40)
     e1, z1, i1, u1 = _jet_run(cfg, MIXED, 40)
@@ -148,15 +155,15 @@ def test_jet_mixed_tracks_fp64_small():
         print(f"[gpu] jet 64x48x32 {name}: E_k {re_:.2e}, enstrophy {rz:.2e} vs FP64; sweeps "
               f"{np.mean(i1 if name == 'mixed' else i2):.1f} (FP64 {np.mean(i0):.1f})",
               flush=True)
-        assert re_ <= 1e-6 and rz <= 1e-6
+        assert re_ <= 5e-6 and rz <= 5e-6
     for u in (u0, u1, u2):
         assert _balance(cfg, u) <= 1e-6
 
 
 def test_jet_full_size_fp64_and_mixed():
     """The 512 x 384 x 256 planar jet, 20 AB3 steps with LES: both forms end
-    every solve on eps, balance the outflow against the inflow, and agree on
-    E_k and enstrophy."""
+    every solve on the reference's rules (never kmax), balance the outflow
+    against the inflow, and agree on E_k and enstrophy."""
     cfg = jet_config(steps=20)
     print("[gpu] jet 512x384x256 FP64, 20 steps", flush=True)
     e0, z0, i0, u0 = _jet_run(cfg, FP64, 20)

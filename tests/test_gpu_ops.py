@@ -425,23 +425,34 @@ def test_poisson_multigrid_eigenfunction():
 
 
 def test_poisson_multigrid_limits():
-    """An rhs incompatible with the free-slip mirror never reaches tol: the
-    solve ends on the SOR solvers' stagnation rule (max |r| settles), far
-    below a kmax-sized cycle limit, or on max_cycles when that comes first,
-    without an error, as the SOR solvers only report non-convergence;
-    unsupported grids are rejected."""
+    """An rhs incompatible with the free-slip mirror: the solve works on its
+    compatible part (minus the trapezoid-weighted mean) and ends on tol far
+    below a kmax-sized cycle limit, with the oracle's residual of the result
+    equal to that mean everywhere; with tol = 0 it runs out of cycles without
+    an error, as the SOR solvers only report non-convergence; unsupported
+    grids are rejected."""
     sh = (8, 8, 16)
-    rhs = np.ones(sh)
+    rng = np.random.default_rng(9)
+    rhs = np.ascontiguousarray(1.0 + 0.1 * rng.standard_normal(sh))
+    d = (0.1, 0.1, 0.1)
     p = np.zeros(sh)
-    cycles, rmax, why = capi.poisson_multigrid(p, rhs, (0.1, 0.1, 0.1), "111111", tol=1e-8,
-                                               max_cycles=3, with_reason=True)
-    assert cycles == 3 and rmax > 1e-8 and why == 2
-    p = np.zeros(sh)
-    cycles, rmax, why = capi.poisson_multigrid(p, rhs, (0.1, 0.1, 0.1), "111111", tol=1e-8,
+    cycles, rmax, why = capi.poisson_multigrid(p, rhs, d, "111111", tol=1e-8,
                                                max_cycles=5000, with_reason=True)
     print(f"[mgp] incompatible rhs: {cycles} cycles, max|r| {rmax:.3e}, reason {why}",
           flush=True)
-    assert why == 1 and cycles < 200 and rmax > 1e-8
+    assert why == 0 and cycles < 60 and rmax < 1e-8
+    w = [np.ones(n) for n in sh]
+    for a in w:
+        a[[0, -1]] = 0.5
+    W = w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
+    m = (W * rhs).sum() / W.sum()
+    from oracle import load_oracle
+    r = load_oracle().residual(p, rhs, d, "111111")
+    assert np.abs(r - m).max() <= 1e-8
+    p = np.zeros(sh)
+    cycles, rmax, why = capi.poisson_multigrid(p, rhs, d, "111111", tol=0.0, max_cycles=3,
+                                               with_reason=True)
+    assert cycles == 3 and why == 2
     with pytest.raises(capi.O3DError, match="multiple of 8"):
         capi.poisson_multigrid(np.zeros((8, 8, 12)), np.zeros((8, 8, 12)), (0.1, 0.1, 0.1),
                                "0000")

@@ -161,9 +161,11 @@ def test_jet_edges_mass_balance_and_inflow(orc):
 
 
 def test_jet_on_the_oracle_converges_and_balances(orc):
-    """A small jet: every Poisson solve ends on eps (the projected rhs is
-    compatible), the outflow flux equals the inflow flux every step, the
-    planes keep the imposed values and nothing blows up."""
+    """A small jet: every Poisson solve ends on the reference's rules well
+    before kmax (the projected rhs is compatible; the stagnation rule can
+    still end a solve whose over-relaxed dmax repeats itself to 1e-11), the
+    outflow flux equals the inflow flux every step, the planes keep the
+    imposed values and nothing blows up."""
     cfg = jet_config(32, 24, 16, steps=12)
     cfg.kmax = 4000
     st = OracleStepper(orc, cfg, red_black=True)
@@ -172,7 +174,7 @@ def test_jet_on_the_oracle_converges_and_balances(orc):
     pp = u.pop()
     for it in range(1, cfg.itstop + 1):
         u, pp, up, info = st.step(it, *u, pp)
-        assert info["reason"] == 0 and info["exit_iter"] < cfg.kmax, (it, info)
+        assert info["reason"] in (0, 1) and info["exit_iter"] < cfg.kmax, (it, info)
         qin, qout, W = info["flux"]
         wy = np.ones(cfg.ny); wy[[0, -1]] = 0.5
         pw = np.ones(cfg.nz)[:, None] * wy[None, :]
