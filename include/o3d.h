@@ -200,6 +200,19 @@ int o3d_statistics(const double *ux, const double *uy, const double *uz, int nx,
                    double dx, double dy, double dz, double re, double t, const int *bc,
                    double *out17);
 
+/* o3d_statistics without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments
+ * and results (bit for bit) as o3d_statistics, but ux, uy, uz must be device
+ * arrays, and the per-block partial sums live in a fixed 2 MiB device array
+ * of the library (one per GPU, held until the process ends): grids needing
+ * more than 16384 partial records (the blocks of the stencil tiling, e.g.
+ * 256^3 needs 1024) are rejected with an error.  Calls on one GPU share
+ * that array, so calls that overlap on one GPU (other threads or streams)
+ * race. */
+int o3d_statistics_noalloc(const double *ux, const double *uy, const double *uz, int nx,
+                           int ny, int nz, double dx, double dy, double dz, double re, double t,
+                           const int *bc, double *out17);
 
 /* transeq, src/integration.f90:332-468: passive scalar transport.  fphi is
  * the (nx,ny,nz,3) history (slot 1 = newest, shifted by copy as the

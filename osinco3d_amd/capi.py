@@ -81,7 +81,7 @@ EXPORTS = [
     "o3d_der", "o3d_der_f32", "o3d_divergence", "o3d_nu_t", "o3d_predict_velocity",
     "o3d_poisson", "o3d_poisson_f32", "o3d_correct_pression", "o3d_correct_velocity",
     "o3d_function_stats",
-    "o3d_statistics", "o3d_transeq", "o3d_rotational", "o3d_q_criterion", "o3d_solver_create",
+    "o3d_statistics", "o3d_statistics_noalloc", "o3d_transeq", "o3d_rotational", "o3d_q_criterion", "o3d_solver_create",
     "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
     "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
@@ -326,12 +326,13 @@ def stream_probe(bytes_per_buffer=1 << 30, reps=20):
     return g.value, m.value
 
 
-def statistics(ux, uy, uz, d, re, t, bc):
+def statistics(ux, uy, uz, d, re, t, bc, noalloc=False):
     """statistics_calc (src/utils.f90:243-375): the 17 values of a stats.dat
-    row."""
+    row.  noalloc=True calls o3d_statistics_noalloc (device tensors only;
+    partial sums in the library's fixed device array)."""
     nx, ny, nz = _dims(ux)
     out = np.zeros(17)
-    fn = load().o3d_statistics
+    fn = load().o3d_statistics_noalloc if noalloc else load().o3d_statistics
     check(fn(ptr(ux), ptr(uy), ptr(uz), nx, ny, nz, *map(_d, d), _d(re), _d(t), _bc(bc),
              out.ctypes.data_as(_dp)))
     return out

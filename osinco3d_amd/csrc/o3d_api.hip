@@ -1356,6 +1356,58 @@ extern "C" int o3d_statistics(const double *ux, const double *uy, const double *
   return statistics_dev<double>(u, nx, ny, nz, dx, dy, dz, re, t, bc, out17, s);
 }
 
+// ---------------------------------------------- statistics without allocation
+// Device memory of o3d_statistics_noalloc: the per-block partial sums (16
+// doubles for each of at most kNoallocStatBlocks blocks) and the 16
+// finalized sums, 2 MiB + 128 B per device for the life of the process.
+// Host code never passes `scratch` itself to a kernel: its device address
+// on the current device comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr int kNoallocStatBlocks = 16384;
+constexpr size_t kNoallocScratchDoubles = (size_t)16 * kNoallocStatBlocks + 16;
+static __device__ __align__(256) double scratch[kNoallocScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current).
+int noalloc_scratch(double **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("device %d beyond the scratch table", dev);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" int o3d_statistics_noalloc(const double *ux, const double *uy, const double *uz,
+                                      int nx, int ny, int nz, double dx, double dy, double dz,
+                                      double re, double t, const int *bc, double *out17) {
+  CHK(ensure_init());
+  CHK(check_grid(nx, ny, nz));
+  CHK(check_bc(bc, true));
+  if (!out17) return fail("out17 is NULL");
+  if (!is_device_ptr(ux) || !is_device_ptr(uy) || !is_device_ptr(uz))
+    return fail("o3d_statistics_noalloc: ux, uy and uz must be device arrays");
+  const int blocks = make_tiling(nx, ny, nz).blocks();
+  if (blocks > kNoallocStatBlocks)
+    return fail("o3d_statistics_noalloc: grid %dx%dx%d needs %d partial records, the "
+                "scratch holds %d", nx, ny, nz, blocks, kNoallocStatBlocks);
+  double *sp;
+  CHK(noalloc_scratch(&sp));
+  const double *u[3] = {ux, uy, uz};
+  // Every block writes its own record and the finalizing launch writes the
+  // 16 sums, so nothing of a previous call is read.
+  return statistics_run<double>(u, nx, ny, nz, dx, dy, dz, re, t, bc, out17, sp,
+                                sp + (size_t)16 * kNoallocStatBlocks);
+}
+
 namespace {
 int diag_dev(int mode, double *out0, double *out1, double *out2, const double *ux,
              const double *uy, const double *uz, double dx, double dy, double dz, int nx, int ny,
