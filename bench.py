@@ -12,13 +12,19 @@ Each requested form is timed over the same steps:
   mixed  FP32 stencils / AB update / SOR sweeps, FP64-refined Poisson
   extrap mixed + warm start by linear extrapolation p_prev + (p_prev - p_prev2)
   extrap2 mixed + quadratic extrapolation p_prev + 2 e_prev - e_prev2
-  mg     extrap + the coarse-grid correction of the guess (multigrid V-cycles
-         on the 2^3 block averages of its residual) and Gauss-Seidel sweeps
-         opening the FP32 correction solve
+  mg     the adaptive extrapolation p_prev + e_prev + beta (e_prev - e_prev2)
+         (beta the least-squares weight of the previous step), then the
+         coarse-grid correction of the guess (multigrid V-cycles on the 2^3
+         block averages of its residual) and Gauss-Seidel sweeps opening the
+         FP32 correction solve
   nn0    mixed + learned warm start (the trained network of
          osinco3d_amd/warmstart_weights.npy; skipped when it is missing)
-  nn     nn0 + the coarse-grid correction and Gauss-Seidel sweeps of mg (the
-         headline: every warm-start stage)
+  nn     the learned warm start with every stage of mg: adaptive extrapolation,
+         the network's estimate as the start of the coarse-grid correction,
+         Gauss-Seidel opening (the headline)
+  mgl    mg with the linear extrapolation (beta = 0)
+  nnl    nn with the linear extrapolation
+  mg2    mg with the quadratic extrapolation (beta = 1)
   nn1    nn with one V-cycle (instead of coarse_cycles) in the first coarse-grid
          correction, which starts from the network's coarse estimate
 The JSON line's value is the steps/s of the headline form (the last of
@@ -127,27 +133,48 @@ def pct(v, q):
     return v[k]
 
 
-def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
-    import torch
+LEARNED_FORMS = ("nn", "nn0", "nn1", "nnl")
+COARSE_FORMS = ("mg", "mgl", "mg2", "nn", "nn1", "nnl")
+WARM_FORMS = LEARNED_FORMS + ("extrap", "extrap2", "mg", "mgl", "mg2")
+
+
+def make_solver(form, cfg, model_path, coarse):
+    """The solver of a bench form and its network (None for the forms
+    without one); (None, None) when a learned form has no model."""
     from osinco3d_amd import solver as S
 
     prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "nn1": S.WARM_LEARNED,
-            "extrap": S.WARM_EXTRAPOLATE, "mg": S.WARM_EXTRAPOLATE,
-            "extrap2": S.WARM_EXTRAPOLATE2}.get(form, S.WARM_PREVIOUS)
-    cc, gs = coarse if form in ("mg", "nn", "nn1") else (0, 0)
+    warm = S.WARM_PREVIOUS
+    if form in LEARNED_FORMS:
+        warm = S.WARM_LEARNED
+    elif form in ("extrap", "mg", "mgl"):
+        warm = S.WARM_EXTRAPOLATE
+    elif form in ("extrap2", "mg2"):
+        warm = S.WARM_EXTRAPOLATE2
+    cc, gs = coarse if form in COARSE_FORMS else (0, 0)
     net = None
-    if form in ("nn", "nn0", "nn1"):
+    if form in LEARNED_FORMS:
         from osinco3d_amd import warmstart
         net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
         if net is None:
-            print(f"[bench] form nn skipped: no model at {model_path}", file=sys.stderr,
-                  flush=True)
-            return None
+            return None, None
     s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs,
-                 coarse_cycles_learned=1 if form == "nn1" else 0)
+                 coarse_cycles_learned=1 if form == "nn1" else 0,
+                 adaptive=1 if form in ("mg", "nn", "nn1") else 0)
     if net is not None:
         net.attach(s)
+    return s, net
+
+
+def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
+    import torch
+
+    s, net = make_solver(form, cfg, model_path, coarse)
+    if s is None:
+        print(f"[bench] form {form} skipped: no model at {model_path}", file=sys.stderr,
+              flush=True)
+        return None
+    cc, gs = coarse if form in COARSE_FORMS else (0, 0)
     if init is None:
         s.init_taylor_green()
     elif callable(init):
@@ -220,7 +247,8 @@ def run_form(form, cfg, steps, warmup, model_path, init=None, coarse=(0, 0)):
     if cc or gs:
         out["coarse_cycles"] = cc
         out["smooth_sweeps"] = gs
-    if form in ("nn", "nn0", "nn1", "extrap", "extrap2", "mg"):
+    if form in WARM_FORMS:
+        out["extrap_beta_mean"] = statistics.mean(i["extrap_beta"] for i in infos)
         out["warm_used_frac"] = statistics.mean(i["warm_used"] for i in infos)
         out["warm_residual_l2"] = {"previous": statistics.mean(i["warm_dmax"][0] for i in infos),
                                    "guess": statistics.mean(i["warm_dmax"][1] for i in infos)}
@@ -246,22 +274,10 @@ def run_series(form, cfg, total, window, model_path, coarse, init=None):
     then the last `window` steps timed on their own: the late-window figures
     and the whole-run mean."""
     import torch
-    from osinco3d_amd import solver as S
 
-    prec = S.FP64 if form == "fp64" else S.MIXED
-    warm = {"nn": S.WARM_LEARNED, "nn0": S.WARM_LEARNED, "nn1": S.WARM_LEARNED,
-            "extrap": S.WARM_EXTRAPOLATE, "mg": S.WARM_EXTRAPOLATE}.get(form, S.WARM_PREVIOUS)
-    cc, gs = coarse if form in ("mg", "nn", "nn1") else (0, 0)
-    net = None
-    if form in ("nn", "nn0", "nn1"):
-        from osinco3d_amd import warmstart
-        net = warmstart.load_or_none(model_path, (cfg.nx, cfg.ny, cfg.nz))
-        if net is None:
-            return None
-    s = S.Solver(cfg, prec, warm, coarse_cycles=cc, smooth_sweeps=gs,
-                 coarse_cycles_learned=1 if form == "nn1" else 0)
-    if net is not None:
-        net.attach(s)
+    s, net = make_solver(form, cfg, model_path, coarse)
+    if s is None:
+        return None
     if init is None:
         s.init_taylor_green()
     elif callable(init):
@@ -286,6 +302,7 @@ def run_series(form, cfg, total, window, model_path, coarse, init=None):
     if net is not None:
         net.close()
     sweeps_all += [i["sor_exit_iter"] for i in infos]
+    beta = [i["extrap_beta"] for i in infos]
     late = [i["sor_exit_iter"] for i in infos]
     q = len(sweeps_all) // 4
     return {
@@ -298,6 +315,7 @@ def run_series(form, cfg, total, window, model_path, coarse, init=None):
         "mean_sweeps": statistics.mean(sweeps_all),
         "sweeps_by_quarter": [statistics.mean(sweeps_all[k * q:(k + 1) * q]) for k in range(4)],
         "sor_reasons": sorted({i["sor_reason"] for i in infos}),
+        "late_extrap_beta_mean": statistics.mean(beta),
         "kinetic_energy": st[1],
         "enstrophy": st[4],
         "t_end": total * s.dt,

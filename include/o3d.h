@@ -46,8 +46,10 @@ const char *o3d_version(void);
 /* Measurement: a hand-written streaming triad c = a + s b over three device
  * buffers of bytes_per_buffer each (two reads and one write per element; at
  * 1 GiB per buffer the working set is far beyond the 256 MB Infinity Cache),
- * timed with HIP events over `reps` launches.  gbs = 3 * bytes_per_buffer /
- * time per launch (GB/s), ms = time per launch; either may be NULL.  No
+ * timed with HIP events over `reps` launches, once with plain and once with
+ * nontemporal loads and stores; the faster is reported.  gbs = 3 *
+ * bytes_per_buffer / time per launch (GB/s), ms = time per launch; either may
+ * be NULL.  No
  * counterpart in the reference: the HBM ceiling the kernels are compared
  * with. */
 int o3d_stream_probe(size_t bytes_per_buffer, int reps, double *gbs, double *ms);
@@ -277,6 +279,14 @@ typedef struct o3d_params {
   int coarse_cycles_learned; /* with the learned warm start: V-cycles of the first coarse-grid
                                 correction, which starts from the network's coarse estimate
                                 instead of zero (0 = coarse_cycles) */
+  int adaptive_extrapolation; /* O3D_WARM_EXTRAPOLATE / _LEARNED: extrapolate
+                                 p_prev + e_prev + beta (e_prev - e_prev2) (beta 0: linear, 1:
+                                 quadratic) with beta the least-squares weight of the previous
+                                 step's rhs (the L2 residual of the guess minimised over beta),
+                                 kept in [0, 1.5]; 0 = linear.  With a network attached it
+                                 applies only when coarse_cycles > 0: the network's input is
+                                 then the residual of this guess, and its estimate, taken
+                                 relative to the guess, starts the coarse-grid solve */
 } o3d_params;
 
 typedef struct o3d_step_info {
@@ -299,9 +309,12 @@ typedef struct o3d_step_info {
   double vel_min[3], vel_max[3], vel_amax[3];
   double cfl[3];          /* max|u_c| dt / d_c, src/utils.f90:178-208 */
   double nu_t[6];         /* function_stats of nu_t (iles = 1) */
+  double extrap_beta;     /* adaptive extrapolation: the weight beta used this step */
   double flow_rate[2];    /* x-Dirichlet: mean inflow and outflow velocity (the weighted
                              plane fluxes / plane weight, outflow before its flux shift), as
                              print_in_outflow_rate prints them (src/IOfunctions.f90:342-350) */
+  double extrap_beta_ls;  /* adaptive extrapolation: the unclamped least-squares weight
+                             that extrap_beta was clamped from (0 before the first) */
 } o3d_step_info;
 
 int o3d_solver_create(const o3d_params *p, void **handle);

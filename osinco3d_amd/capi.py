@@ -46,6 +46,7 @@ class O3DParams(C.Structure):
         ("coarse_cycles", C.c_int),
         ("smooth_sweeps", C.c_int),
         ("coarse_cycles_learned", C.c_int),
+        ("adaptive_extrapolation", C.c_int),
     ]
 
 
@@ -62,7 +63,9 @@ class O3DStepInfo(C.Structure):
         ("vel_min", C.c_double * 3), ("vel_max", C.c_double * 3), ("vel_amax", C.c_double * 3),
         ("cfl", C.c_double * 3),
         ("nu_t", C.c_double * 6),
+        ("extrap_beta", C.c_double),
         ("flow_rate", C.c_double * 2),
+        ("extrap_beta_ls", C.c_double),
     ]
 
     def to_dict(self) -> dict:

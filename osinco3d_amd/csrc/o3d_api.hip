@@ -476,6 +476,12 @@ namespace {
 // compared with.  Each thread handles 4 vectors per round (their loads in
 // flight together) with non-temporal loads and stores (no cache pollution).
 typedef float v4f __attribute__((ext_vector_type(4)));
+// Streaming triad c = a + s b: each thread moves TRIAD_VEC 16-byte vectors
+// spaced a block apart (every load instruction of a wave covers 1 KiB
+// contiguously), one pass over the grid, no loop.  NT selects nontemporal
+// loads and stores.
+constexpr int TRIAD_VEC = 4;
+template <bool NT>
 __global__ void __launch_bounds__(256) stream_triad_kernel(const float4 *__restrict__ a4,
                                                            const float4 *__restrict__ b4,
                                                            float4 *__restrict__ c4, size_t n4,
@@ -483,19 +489,26 @@ __global__ void __launch_bounds__(256) stream_triad_kernel(const float4 *__restr
   const v4f *a = reinterpret_cast<const v4f *>(a4);
   const v4f *b = reinterpret_cast<const v4f *>(b4);
   v4f *c = reinterpret_cast<v4f *>(c4);
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n4; i += 4 * stride) {
-    v4f x[4], y[4];
+  const size_t base = (size_t)blockIdx.x * (blockDim.x * TRIAD_VEC) + threadIdx.x;
+  v4f x[TRIAD_VEC], y[TRIAD_VEC];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      x[u] = __builtin_nontemporal_load(a + i + u * stride);
-      y[u] = __builtin_nontemporal_load(b + i + u * stride);
+  for (int u = 0; u < TRIAD_VEC; ++u) {
+    const size_t i = base + (size_t)u * blockDim.x;
+    if (i < n4) {
+      x[u] = NT ? __builtin_nontemporal_load(a + i) : a[i];
+      y[u] = NT ? __builtin_nontemporal_load(b + i) : b[i];
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(x[u] + s * y[u], c + i + u * stride);
   }
-  for (; i < n4; i += stride) c[i] = a[i] + s * b[i];
+#pragma unroll
+  for (int u = 0; u < TRIAD_VEC; ++u) {
+    const size_t i = base + (size_t)u * blockDim.x;
+    if (i < n4) {
+      if (NT)
+        __builtin_nontemporal_store(x[u] + s * y[u], c + i);
+      else
+        c[i] = x[u] + s * y[u];
+    }
+  }
 }
 }  // namespace
 
@@ -512,21 +525,33 @@ extern "C" int o3d_stream_probe(size_t bytes_per_buffer, int reps, double *gbs, 
     rc = fail("stream probe: events");
   if (rc == 0) {
     for (int q = 0; q < 3; ++q) (void)hipMemsetAsync(buf[q], 0, n4 * 16, g_stream);
-    const int blocks = 256 * 8;
-    for (int w = 0; w < 3; ++w)
-      stream_triad_kernel<<<blocks, 256, 0, g_stream>>>(buf[0], buf[1], buf[2], n4, 0.5f);
-    (void)hipEventRecord(e0, g_stream);
-    for (int r = 0; r < reps; ++r)
-      stream_triad_kernel<<<blocks, 256, 0, g_stream>>>(buf[r & 1], buf[1 - (r & 1)], buf[2],
-                                                        n4, 0.5f);
-    (void)hipEventRecord(e1, g_stream);
-    float t = 0.f;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
-      rc = fail("stream probe: timing failed: %s", hipGetErrorString(hipGetLastError()));
-    else {
+    const unsigned blocks = (unsigned)((n4 + 256 * TRIAD_VEC - 1) / (256 * TRIAD_VEC));
+    double best = 0.0;
+    for (int nt = 0; nt < 2 && rc == 0; ++nt) {
+      auto launch = [&](int r) {
+        if (nt)
+          stream_triad_kernel<true><<<blocks, 256, 0, g_stream>>>(buf[r & 1], buf[1 - (r & 1)],
+                                                                  buf[2], n4, 0.5f);
+        else
+          stream_triad_kernel<false><<<blocks, 256, 0, g_stream>>>(buf[r & 1], buf[1 - (r & 1)],
+                                                                   buf[2], n4, 0.5f);
+      };
+      for (int w = 0; w < 3; ++w) launch(w);
+      (void)hipEventRecord(e0, g_stream);
+      for (int r = 0; r < reps; ++r) launch(r);
+      (void)hipEventRecord(e1, g_stream);
+      float t = 0.f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        rc = fail("stream probe: timing failed: %s", hipGetErrorString(hipGetLastError()));
+        break;
+      }
       const double per = (double)t / reps;
-      if (ms) *ms = per;
-      if (gbs) *gbs = 3.0 * (double)n4 * 16.0 / (per * 1e-3) / 1e9;
+      const double g = 3.0 * (double)n4 * 16.0 / (per * 1e-3) / 1e9;
+      if (g > best) {
+        best = g;
+        if (ms) *ms = per;
+        if (gbs) *gbs = g;
+      }
     }
   }
   if (e0) (void)hipEventDestroy(e0);
@@ -1546,7 +1571,7 @@ struct Solver {
   float *Eprev = nullptr;      // previous step's increment
   float *Eprev2 = nullptr;     // the one before
   float *Rl = nullptr;         // residual of p_prev (float32)
-  float *Feat = nullptr;       // network input, 2 coarse planes
+  float *Feat = nullptr;       // network input, 2 coarse planes (+ e_prev2's, adaptive)
   float *Out = nullptr;        // fine-grid scratch
   float *Outc = nullptr;       // network output (coarse)
   MgHierarchy mg;              // coarse-grid correction (coarse_cycles > 0)
@@ -1558,6 +1583,13 @@ struct Solver {
   double *epart = nullptr;     // sum-of-squares partials of e_prev
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
   double *mpart = nullptr;     // mean-pressure partials + the mean (mixed, every axis periodic)
+  // Adaptive extrapolation (o3d_params.adaptive_extrapolation): the weight of
+  // e_prev - e_prev2 and the device sums behind the next one.
+  double beta = 0.0;
+  double *bpart = nullptr, *bsum = nullptr, *hbsum = nullptr;
+  double beta_ls = 0.0;        // the unclamped least-squares weight beta came from
+  bool beta_pending = false;
+  bool adapt_now = false;      // this step's guess is the adaptive extrapolation
   bool mean_pending = false;   // the mean is formed, its subtraction not yet done
   // x-Dirichlet inflow / outflow (bc[0] == DIRICHLET), set by o3d_solver_set_inflow.
   bool xdir = false, in_set = false;
@@ -1598,6 +1630,7 @@ struct Solver {
     if (hnpart) (void)hipHostFree(hnpart);
     if (hepart) (void)hipHostFree(hepart);
     if (hflux) (void)hipHostFree(hflux);
+    if (hbsum) (void)hipHostFree(hbsum);
   }
 };
 
@@ -2067,10 +2100,19 @@ __global__ void lw_guess_coarse_kernel(const double *__restrict__ p, const float
 
 // u0 = s * (out - in1) on the coarse grid: the network's estimate of the
 // coarse correction of the extrapolation, the multigrid's starting iterate.
+// u0 = s (out - in1 - beta (in1 - in2)): the network's increment estimate
+// relative to the guess p_prev + e_prev + beta (e_prev - e_prev2), on the
+// block grid (in1, in2: block averages of e_prev, e_prev2 over s; in2 null
+// when beta = 0).
 __global__ void lw_coarse_init_kernel(float *__restrict__ u0, const float *__restrict__ out,
-                                      const float *__restrict__ in1, int nc, float s) {
+                                      const float *__restrict__ in1,
+                                      const float *__restrict__ in2, float beta, int nc,
+                                      float s) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I < nc) u0[I] = s * (out[I] - in1[I]);
+  if (I >= nc) return;
+  float v = out[I] - in1[I];
+  if (in2) v -= beta * (in1[I] - in2[I]);
+  u0[I] = s * v;
 }
 
 // q += trilinear interpolation of the coarse correction c (the multigrid
@@ -2223,7 +2265,7 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.Eprev2, n));
   CHK(S.alloc(&S.Rl, n));
   CHK(S.alloc(&S.Out, n));
-  CHK(S.alloc(&S.Feat, 2 * nc));
+  CHK(S.alloc(&S.Feat, 3 * nc));
   CHK(S.alloc(&S.Outc, nc));
   if (S.p.coarse_cycles > 0) {
     const int F = kMgCoarsen;
@@ -2293,7 +2335,29 @@ int solver_learning_prepare(Solver &S) {
   const int n = S.n;
   int per[3];
   solver_periodic(S.kind, per);
-  if (S.hist >= 1) {
+  // With a network, the adaptive guess is used only under the coarse-grid
+  // correction, whose start takes the network's estimate relative to it.
+  S.adapt_now = S.hist >= 2 && S.p.adaptive_extrapolation &&
+                S.p.warm_start != O3D_WARM_EXTRAPOLATE2 && (!S.nn || S.p.coarse_cycles > 0);
+  if (S.adapt_now) {
+    // p_prev + e_prev + beta (e_prev - e_prev2), beta the least-squares
+    // weight of the previous step's rhs, and its residual, with the
+    // residual norms of p_prev and the sums for the next beta.
+    if (S.mixed)
+      HIPCHK(launch_guess_adapt<float>(S.P[S.cur], S.Eprev, S.Eprev2, (const float *)S.rhs,
+                                       S.beta, S.P[1 - S.cur], S.Rl, S.p.nx, S.p.ny, S.p.nz,
+                                       per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream,
+                                       S.p0part, S.bpart, S.bsum));
+    else
+      HIPCHK(launch_guess_adapt<double>(S.P[S.cur], S.Eprev, S.Eprev2, (const double *)S.rhs,
+                                        S.beta, S.P[1 - S.cur], S.Rl, S.p.nx, S.p.ny, S.p.nz,
+                                        per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A, g_stream,
+                                        S.p0part, S.bpart, S.bsum));
+    HIPCHK(hipMemcpyAsync(S.hbsum, S.bsum, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          g_stream));
+    S.beta_pending = true;
+    S.p0_of = S.P[S.cur];
+  } else if (S.hist >= 1) {
     // The extrapolation into the spare buffer and its residual, one pass,
     // with the residual norms of p_prev for the fallback check.
     if (S.mixed)
@@ -2360,8 +2424,14 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
       // The network's coarse correction (on the multigrid's level 1, the
       // grid of F^3 blocks) starts the multigrid solve of the coarse-grid
       // correction below; q keeps the extrapolation.
-      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.mg.u[1], S.Outc,
-                                                                    S.Feat + nc, nc, (float)re);
+      const float *in2 = nullptr;
+      if (S.adapt_now) {
+        in2 = S.Feat + 2 * nc;
+        HIPCHK(launch_lw_pool(S.Eprev2, S.Feat + 2 * nc, S.p.nx, S.p.ny, S.p.nz, F,
+                              (float)(1.0 / re)));
+      }
+      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(
+          S.mg.u[1], S.Outc, S.Feat + nc, in2, (float)S.beta, nc, (float)re);
       nn_coarse = true;
     } else if (S.p.coarse_cycles == 0) {
       dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
@@ -2377,8 +2447,8 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
   if (S.p.coarse_cycles > 0) {
     // Coarse-grid correction: the residual of q averaged over F^3 blocks,
     // the coarse Poisson problem solved by multigrid V-cycles, its trilinear
-    // interpolation added to q.  S.Rl holds the residual of the plain
-    // extrapolation; other guesses need their own.
+    // interpolation added to q.  S.Rl holds the residual of the extrapolation
+    // (linear or adaptive); other guesses need their own.
     const int F = kMgCoarsen;
     int per[3];
     solver_periodic(S.kind, per);
@@ -2489,6 +2559,18 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
     info->flow_rate[1] = S.hflux[1] / S.in_W;
   } else {
     info->flow_rate[0] = info->flow_rate[1] = 0.0;
+  }
+  info->extrap_beta = S.beta;
+  info->extrap_beta_ls = S.beta_ls;
+  if (S.beta_pending) {
+    // The least-squares weight of this step's rhs, for the next step's
+    // guess, kept within [0, 1.5] (between the linear and somewhat beyond
+    // the quadratic extrapolation).
+    if (S.hbsum[1] > 0.0) {
+      S.beta_ls = S.hbsum[0] / S.hbsum[1];
+      S.beta = std::min(1.5, std::max(0.0, S.beta_ls));
+    }
+    S.beta_pending = false;
   }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, S.ev[2], S.ev[3]));
@@ -2605,6 +2687,12 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     rc |= S->alloc(&S->dslot, sor_split_slot_words());
     rc |= S->alloc(&S->rhs_split, n);
   }
+  if (!rc && p->adaptive_extrapolation) {
+    rc |= S->alloc(&S->bpart, 2 * (size_t)residual_blocks(p->nx, p->ny, p->nz));
+    if (!rc) rc |= S->alloc(&S->bsum, 2);
+    if (!rc && hipHostMalloc((void **)&S->hbsum, 2 * sizeof(double)) != hipSuccess)
+      rc = fail("hipHostMalloc failed");
+  }
   if (!rc && p->bc[0] == DIRICHLET) {
     S->xdir = true;
     const size_t P = (size_t)p->ny * p->nz;
@@ -2679,6 +2767,8 @@ namespace {
 // fresh solver and one that has already stepped continue alike.
 void reset_warm_state(Solver &S) {
   S.hist = 0;
+  S.beta = 0.0;
+  S.beta_ls = 0.0;
   S.p0_of = nullptr;
   S.r_ready = false;
 }

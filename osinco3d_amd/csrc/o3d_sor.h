@@ -130,6 +130,15 @@ hipError_t launch_fold(double *p, const float *e, int n, hipStream_t s);
 // its residual), equal to forming q and then calling launch_residual_f32.
 // With part0 (2 * residual_blocks doubles), also the per-block (max |r0|,
 // sum r0^2) of r0 = rhs - L p, for launch_residual_norm_pair without p0.
+// q = p + e1 + beta (e1 - e2) and r = rhs - L q (FP32) in one pass, with
+// part0 as launch_guess_residual; sums[0..1] (device) receive
+// (sum (r + beta Ld) Ld, sum Ld Ld), d = e1 - e2, whose ratio is the
+// least-squares beta of this rhs; partb holds 2 * residual_blocks doubles.
+template <typename TR>
+hipError_t launch_guess_adapt(const double *p, const float *e1, const float *e2, const TR *rhs,
+                              double beta, double *q, float *r, int nx, int ny, int nz,
+                              const int per[3], double ox, double oy, double oz, double A,
+                              hipStream_t s, double *part0, double *partb, double *sums);
 template <typename TR>
 hipError_t launch_guess_residual(const double *p, const float *e, const TR *rhs, double *q,
                                  float *r, int nx, int ny, int nz, const int per[3], double ox,

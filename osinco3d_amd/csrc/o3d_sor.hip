@@ -866,6 +866,131 @@ hipError_t launch_guess_residual(const double *p, const float *e, const TR *rhs,
   return hipGetLastError();
 }
 
+// The adaptive extrapolation q = p + e1 + beta (e1 - e2) (e1, e2 the last
+// two pressure increments: beta = 0 is the linear, beta = 1 the quadratic
+// extrapolation) and r = rhs - L q stored in FP32, in one pass; with part0
+// the block's (max |r0|, sum r0^2) of r0 = rhs - L p as in
+// guess_residual_kernel; and per block the two sums that give the
+// least-squares beta of this rhs, (sum (r + beta Ld) Ld, sum Ld Ld) with
+// d = e1 - e2: the weight that minimises the L2 residual of p + e1 + beta d,
+// which the next step uses.
+template <typename TR>
+__global__ void __launch_bounds__(RNT) guess_adapt_kernel(
+    const double *__restrict__ p, const float *__restrict__ e1, const float *__restrict__ e2,
+    const TR *__restrict__ rhs, double beta, double *__restrict__ q, float *__restrict__ r,
+    double *__restrict__ part0, double *__restrict__ partb, int nx, int ny, int nz, int px,
+    int py, int pz, double ox, double oy, double oz, double A) {
+  __shared__ double sc[RNT / O3D_WAVE];
+  double m0 = 0.0, s0 = 0.0, sb = 0.0, sd = 0.0;
+  const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
+  const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
+  if (i < nx && j < ny) {
+    const int k0 = blockIdx.z * RZC;
+    const size_t plane = (size_t)nx * ny;
+    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+    const int c = i + nx * j;
+    const int cl[4] = {im + nx * j, ip + nx * j, i + nx * jm, i + nx * jp};
+    // The column (planes k0-1 .. k0+RZC) of p, e1 and d = e1 - e2, then the
+    // lateral neighbours plane by plane.
+    double zp[RZC + 2];
+    float ze[RZC + 2], zd[RZC + 2];
+#pragma unroll
+    for (int kk = 0; kk < RZC + 2; ++kk) {
+      const size_t o = c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz);
+      zp[kk] = p[o];
+      const float a = e1[o], b = e2[o];
+      ze[kk] = a;
+      zd[kk] = a - b;
+    }
+#pragma unroll
+    for (int kk = 0; kk < RZC; ++kk) {
+      const int k = k0 + kk;
+      const size_t ok = plane * min(k, nz - 1);
+      double lp[4];
+      float le[4], ld[4];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        lp[d] = p[cl[d] + ok];
+        const float a = e1[cl[d] + ok], b = e2[cl[d] + ok];
+        le[d] = a;
+        ld[d] = a - b;
+      }
+      const double rh = (double)rhs[c + ok];
+      if (k < nz) {
+        auto qv = [&](double pv, float ev, float dv) {
+          return pv + (double)ev + beta * (double)dv;
+        };
+        const double lapq =
+            ox * (qv(lp[0], le[0], ld[0]) + qv(lp[1], le[1], ld[1])) +
+            oy * (qv(lp[2], le[2], ld[2]) + qv(lp[3], le[3], ld[3])) +
+            oz * (qv(zp[kk], ze[kk], zd[kk]) + qv(zp[kk + 2], ze[kk + 2], zd[kk + 2])) +
+            A * qv(zp[kk + 1], ze[kk + 1], zd[kk + 1]);
+        const double rq = rh - lapq;
+        q[c + plane * k] = qv(zp[kk + 1], ze[kk + 1], zd[kk + 1]);
+        r[c + plane * k] = (float)rq;
+        const double lapd = ox * ((double)ld[0] + (double)ld[1]) +
+                            oy * ((double)ld[2] + (double)ld[3]) +
+                            oz * ((double)zd[kk] + (double)zd[kk + 2]) + A * (double)zd[kk + 1];
+        sb += (rq + beta * lapd) * lapd;
+        sd += lapd * lapd;
+        if (part0) {
+          const double lapp = ox * (lp[0] + lp[1]) + oy * (lp[2] + lp[3]) +
+                              oz * (zp[kk] + zp[kk + 2]) + A * zp[kk + 1];
+          const double r0 = rh - lapp;
+          m0 = fmax(m0, fabs(r0));
+          s0 += r0 * r0;
+        }
+      }
+    }
+  }
+  const size_t b =
+      blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+  if (part0) {
+    m0 = block_max<RNT / O3D_WAVE>(m0, sc);
+    s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
+    if (threadIdx.x == 0) {
+      part0[2 * b] = m0;
+      part0[2 * b + 1] = s0;
+    }
+  }
+  sb = block_sum<RNT / O3D_WAVE>(sb, sc);
+  sd = block_sum<RNT / O3D_WAVE>(sd, sc);
+  if (threadIdx.x == 0) {
+    partb[2 * b] = sb;
+    partb[2 * b + 1] = sd;
+  }
+}
+
+// The two beta sums over the blocks (fixed order), out[0..1].
+__global__ void __launch_bounds__(1024) beta_final_kernel(const double *__restrict__ partb,
+                                                          int nb, double *__restrict__ out) {
+  __shared__ double sc[1024 / O3D_WAVE];
+  double a = 0.0, d = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    a += partb[2 * b];
+    d += partb[2 * b + 1];
+  }
+  a = block_sum<1024 / O3D_WAVE>(a, sc);
+  d = block_sum<1024 / O3D_WAVE>(d, sc);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = d;
+  }
+}
+
+template <typename TR>
+hipError_t launch_guess_adapt(const double *p, const float *e1, const float *e2, const TR *rhs,
+                              double beta, double *q, float *r, int nx, int ny, int nz,
+                              const int per[3], double ox, double oy, double oz, double A,
+                              hipStream_t s, double *part0, double *partb, double *sums) {
+  guess_adapt_kernel<TR><<<residual_grid(nx, ny, nz), RNT, 0, s>>>(
+      p, e1, e2, rhs, beta, q, r, part0, partb, nx, ny, nz, per[0], per[1], per[2], ox, oy, oz,
+      A);
+  beta_final_kernel<<<1, 1024, 0, s>>>(partb, residual_blocks(nx, ny, nz), sums);
+  return hipGetLastError();
+}
+
 template <typename TR>
 __global__ void __launch_bounds__(RNT) residual_f32_kernel(const double *__restrict__ p,
                                                            const TR *__restrict__ rhs,
@@ -1084,6 +1209,16 @@ template hipError_t launch_residual_f32<double>(const double *, const double *, 
 template hipError_t launch_residual_f32<float>(const double *, const float *, float *, float *,
                                                int, int, int, const int *, double, double, double,
                                                double, hipStream_t);
+template hipError_t launch_guess_adapt<float>(const double *, const float *, const float *,
+                                              const float *, double, double *, float *, int,
+                                              int, int, const int *, double, double, double,
+                                              double, hipStream_t, double *, double *,
+                                              double *);
+template hipError_t launch_guess_adapt<double>(const double *, const float *, const float *,
+                                               const double *, double, double *, float *, int,
+                                               int, int, const int *, double, double, double,
+                                               double, hipStream_t, double *, double *,
+                                               double *);
 template hipError_t launch_guess_residual<float>(const double *, const float *, const float *,
                                                  double *, float *, int, int, int, const int *,
                                                  double, double, double, double, hipStream_t,

@@ -2,6 +2,8 @@
 
     python -m osinco3d_amd.train_warmstart [--n 128] [--samples 2000]
         [--epochs 12] [--out osinco3d_amd/warmstart_weights.npy] [--eval-n 256]
+    python -m osinco3d_amd.train_warmstart --config jet [--grid 512,384,256]
+        [--init weights.npy] [--out osinco3d_amd/warmstart_weights_jet.npy]
 
 Training pairs (inputs: the residual of the extrapolated pressure and e_prev;
 target: the increment e; see warmstart.py) come from seeded runs at n^3
@@ -19,7 +21,8 @@ e = e_prev, and the saved weights are those of the epoch with the smallest
 held-out error.
 
 Evaluation: the same solver steps with the previous-pressure, the
-extrapolated and the learned warm starts, reporting SOR sweeps per step.
+extrapolated and the learned warm starts, reporting SOR sweeps per step; for
+the jet, a run with an inflow seed absent from the training runs.
 """
 from __future__ import annotations
 
@@ -70,35 +73,44 @@ def runs(n: int, total: int):
     return out
 
 
-def jet_runs(total: int):
-    """The planar jet with x-Dirichlet inflow at 128x96x64 (the C5 box at a
-    quarter of its resolution), seeds 1234 and 4321 of the inflow noise: the
-    fine-tuning data of the jet's network."""
+JET_GRID = (128, 96, 64)
+
+
+def jet_runs(total: int, grid=JET_GRID):
+    """The planar jet with x-Dirichlet inflow on `grid` (by default the
+    512x384x256 box at a quarter of its resolution), seeds 1234 and 4321 of
+    the inflow noise: the fine-tuning data of the jet's network."""
     out = []
     for seed in (1234, 4321):
         steps = max(8, total // 2)
-        cfg = jet_config(128, 96, 64, steps=steps)
-        out.append((f"jet seed {seed}", cfg, (lambda s, seed=seed: s.init_jet(seed=seed)), steps))
+        cfg = jet_config(*grid, steps=steps)
+        out.append((f"jet {grid} seed {seed}", cfg,
+                    (lambda s, seed=seed: s.init_jet(seed=seed)), steps))
     return out
 
 
-def collect(n: int, total: int, config: str = "tgv+mixing"):
+def collect(n: int, total: int, config: str = "tgv+mixing", grid=JET_GRID):
     train, held = [], []
-    for label, cfg, init, steps in (jet_runs(total) if config == "jet" else runs(n, total)):
+    for label, cfg, init, steps in (jet_runs(total, grid) if config == "jet" else runs(n, total)):
         t0 = time.perf_counter()
-        samples, iters = warmstart.capture_run(cfg, steps + 2, MIXED, init=init)
+        reasons = []
+        samples, iters = warmstart.capture_run(cfg, steps + 2, MIXED, init=init, reasons=reasons)
         train += [x for i, x in enumerate(samples) if i % 10 != 9]
         held += samples[9::10]
-        log(f"{label}: {len(samples)} samples in {time.perf_counter() - t0:.1f} s, "
-            f"SOR sweeps/step mean {np.mean(iters):.1f}")
+        log(f"{label}: {len(samples)} samples of {len(iters)} steps in "
+            f"{time.perf_counter() - t0:.1f} s, SOR sweeps/step mean {np.mean(iters):.1f}, "
+            f"stop reasons eps/stagnation/kmax {[reasons.count(k) for k in (0, 1, 2)]}")
     return train, held
 
 
-def sweeps_with(cfg, init, steps, warm, net=None, skip=3):
-    s = Solver(cfg, MIXED, warm)
+def sweeps_with(cfg, init, steps, warm, net=None, skip=3, coarse=(0, 0)):
+    s = Solver(cfg, MIXED, warm, coarse_cycles=coarse[0], smooth_sweeps=coarse[1])
     if net is not None:
         net.attach(s)
-    s.upload(*init)
+    if callable(init):
+        init(s)
+    else:
+        s.upload(*init)
     its, used = [], []
     for it in range(1, steps + 1):
         info = s.step(it)
@@ -122,6 +134,27 @@ def evaluate(net, n: int, steps: int, re: float = 1600.0):
     return res
 
 
+def evaluate_jet(net, grid, steps: int, seed: int = 777):
+    """The jet with an inflow seed absent from the training runs: relative
+    error of the predicted increment on its solves, then SOR sweeps per step
+    with each warm start, alone and under the coarse-grid correction."""
+    cfg = jet_config(*grid, steps=steps)
+    init = lambda s: s.init_jet(seed=seed)  # noqa: E731
+    samples, _ = warmstart.capture_run(cfg, steps, MIXED, init=init)
+    err = warmstart.relative_errors(net, samples)
+    log(f"eval jet {grid} unseen seed {seed}: {len(samples)} solves, relative error of the "
+        f"increment: network {err['nn']:.4f}, extrapolation {err['extrapolation']:.4f}")
+    res = {"relative_error": err}
+    for name, warm, nn, coarse in (("extrapolation", WARM_EXTRAPOLATE, None, (0, 0)),
+                                   ("learned", WARM_LEARNED, net, (0, 0)),
+                                   ("extrapolation + coarse-grid", WARM_EXTRAPOLATE, None, (2, 8)),
+                                   ("learned + coarse-grid", WARM_LEARNED, net, (2, 8))):
+        res[name] = sweeps_with(cfg, init, steps, warm, nn, coarse=coarse)
+        log(f"eval jet {grid} unseen seed {seed}: {name}: {res[name][0]:.1f} sweeps/step "
+            f"(guess accepted {res[name][1]:.2f})")
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--n", type=int, default=128)
@@ -131,21 +164,27 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--init", default="", help="start from these weights (.npy)")
-    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--out", default="",
+                    help="default osinco3d_amd/warmstart_weights.npy (_jet.npy for --config jet)")
     ap.add_argument("--eval-n", type=int, default=256)
     ap.add_argument("--eval-steps", type=int, default=16)
     ap.add_argument("--config", choices=("tgv+mixing", "jet"), default="tgv+mixing",
                     help="training data: Taylor-Green and mixing-layer runs at n^3 (C4), or "
                          "the x-Dirichlet planar jet at 128x96x64 (fine-tuning for C5, "
                          "usually with --init)")
+    ap.add_argument("--grid", default="",
+                    help="jet grid NX,NY,NZ (default 128,96,64)")
     a = ap.parse_args(argv)
+    jgrid = tuple(int(v) for v in a.grid.split(",")) if a.grid else JET_GRID
+    if not a.out:
+        a.out = DEFAULT_OUT.replace(".npy", "_jet.npy") if a.config == "jet" else DEFAULT_OUT
 
     import torch
     if not torch.cuda.is_available():
         sys.exit("train_warmstart needs a HIP device")
     t_start = time.perf_counter()
-    train, held = collect(a.n, a.samples, a.config)
-    grid = (128, 96, 64) if a.config == "jet" else (a.n, a.n, a.n)
+    train, held = collect(a.n, a.samples, a.config, jgrid)
+    grid = jgrid if a.config == "jet" else (a.n, a.n, a.n)
     log(f"{len(train)} training and {len(held)} held-out samples on {grid} "
         f"({time.perf_counter() - t_start:.1f} s to capture)")
     F = capi.WARM_COARSEN
@@ -178,7 +217,10 @@ def main(argv=None):
     del train, held
     torch.cuda.empty_cache()
     log(f"training wall time {time.perf_counter() - t0:.1f} s")
-    if a.eval_n > 0 and a.config != "jet":
+    if a.config == "jet":
+        if a.eval_steps > 0:
+            evaluate_jet(net, grid, a.eval_steps)
+    elif a.eval_n > 0:
         evaluate(net, a.eval_n, a.eval_steps)
     net.close()
 

@@ -158,10 +158,11 @@ def normalize(r, eprev, e, dx: float):
 
 
 def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 2,
-                keep_every: int = 1, store_bf16: bool = False):
+                keep_every: int = 1, store_bf16: bool = False, reasons=None):
     """Run the solver and keep (inp, target) samples of every kept step.
     init: initial fields, or a callable that initialises the solver (the
-    x-Dirichlet jet also sets its inflow)."""
+    x-Dirichlet jet also sets its inflow).  reasons: a list that receives
+    each step's stop reason."""
     import torch
     from .solver import Solver, taylor_green
     s = Solver(cfg, precision)
@@ -178,12 +179,20 @@ def capture_run(cfg: Config, steps: int, precision: int, init=None, skip: int = 
     dx = cfg.spacing()[0]
     samples = []
     iters = []
+    last_kmax = -10
     for it in range(1, steps + 1):
         info = s.step(it)
-        if info["diverged"] or info["has_nan"] or info["sor_reason"] == 2:
+        if info["diverged"] or info["has_nan"]:
             break   # the run left the regime the warm start is meant for
         iters.append(info["sor_exit_iter"])
-        if it > skip and (it - skip) % keep_every == 0:
+        if reasons is not None:
+            reasons.append(info["sor_reason"])
+        if info["sor_reason"] == 2:
+            # A solve stopped by kmax leaves an unconverged pressure: neither
+            # its increment nor the next one (whose input it is) is a sample.
+            last_kmax = it
+            continue
+        if it > skip and it > last_kmax + 1 and (it - skip) % keep_every == 0:
             z = normalize(r, ep, e, dx)
             if z is not None:
                 inp, tgt, _ = z

@@ -131,3 +131,51 @@ def test_coarse_grid_correction_on_incompatible_mixing_layer():
     assert max(i["sor_exit_iter"] for i in info_mg) < cfg.kmax // 2
     assert all(i["sor_reason"] == 1 for i in info_64)
     assert abs(st_mg[1] - st_64[1]) <= 1e-8 * abs(st_64[1])
+
+
+def _lap7(f, d):
+    """The solver's 7-point periodic Laplacian in FP64, arrays (nz, ny, nx)."""
+    ox, oy, oz = (1.0 / h ** 2 for h in d)
+    return (ox * (np.roll(f, 1, 2) + np.roll(f, -1, 2)) + oy * (np.roll(f, 1, 1) + np.roll(f, -1, 1))
+            + oz * (np.roll(f, 1, 0) + np.roll(f, -1, 0)) - 2.0 * (ox + oy + oz) * f)
+
+
+def test_adaptive_extrapolation_weight_is_least_squares():
+    """The adaptive guess p_prev + e_prev + beta (e_prev - e_prev2): the
+    weight of step k + 1 is the one minimising the L2 residual of the guess
+    at step k, || r_lin - beta L d || with d = e_prev - e_prev2, i.e.
+    beta* = (r_q + beta_k L d) . L d / || L d ||^2 from the residual r_q of
+    the guess step k used.  Recomputed here in numpy from the captured r_q
+    and increments, then clamped to [0, 1.5] as the solver does."""
+    import torch
+
+    from osinco3d_amd import capi
+    n, steps = 64, 14
+    cfg = tgv_config(n, steps, 3)
+    d = cfg.spacing()
+    s = Solver(cfg, MIXED, WARM_EXTRAPOLATE, adaptive=1)
+    s.init_taylor_green()
+    r = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+    ep = torch.empty_like(r)
+    e = torch.empty_like(r)
+    capi.check(capi.load().o3d_solver_capture(s.handle, capi.ptr(r), capi.ptr(ep), capi.ptr(e)))
+    infos, rs, eps_ = [], [], []
+    for it in range(1, steps + 1):
+        infos.append(s.step(it))
+        rs.append(r.cpu().numpy().astype(np.float64))
+        eps_.append(ep.cpu().numpy())
+    capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
+    s.close()
+    checked = 0
+    for k in range(3, steps - 1):     # 0-based step index; two increments exist from k = 2
+        dd = (eps_[k] - eps_[k - 1]).astype(np.float64)   # FP32 difference, as the kernel forms it
+        ld = _lap7(dd, d)
+        beta_k = infos[k]["extrap_beta"]
+        want = float(((rs[k] + beta_k * ld) * ld).sum() / (ld * ld).sum())
+        got = infos[k + 1]["extrap_beta_ls"]
+        print(f"[adapt] step {k + 2}: beta used {infos[k + 1]['extrap_beta']:.4f}, least squares "
+              f"{got:.6f} (numpy {want:.6f})", flush=True)
+        assert abs(got - want) <= 1e-6 * max(1.0, abs(want)), (k, got, want)
+        assert infos[k + 1]["extrap_beta"] == min(1.5, max(0.0, got))
+        checked += 1
+    assert checked >= 8
