@@ -43,7 +43,9 @@ of p, 1 read of rhs, 1 write of p) = 4 values per cell, 16 B/cell in FP32 and
 32 B/cell in FP64, times the cell count; the floor of one read of p and rhs
 and one write of p (12 / 24 B/cell) is reported beside it.  Its average
 duration is measured with HIP events on the launch stream over complete
-batches of iterations inside the timed steps; `traffic` is the measured HBM
+batches of iterations inside the timed steps (the start event follows each
+batch's first iteration, so the interval holds back-to-back iterations and
+matches the iteration period of a kernel trace); `traffic` is the measured HBM
 bytes of one iteration (both launches) from the committed rocprofv3 --pmc
 summary.  The ceiling beside the nominal 8 TB/s is a hand-written streaming
 triad over three 1 GiB buffers (o3d_stream_probe).

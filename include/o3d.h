@@ -285,8 +285,16 @@ typedef struct o3d_params {
                                  step's rhs (the L2 residual of the guess minimised over beta),
                                  kept in [0, 1.5]; 0 = linear.  With a network attached it
                                  applies only when coarse_cycles > 0: the network's input is
-                                 then the residual of this guess, and its estimate, taken
-                                 relative to the guess, starts the coarse-grid solve */
+                                 then the residual of this guess, and the correction it
+                                 estimates from it starts the coarse-grid solve */
+  int rhs_as_given;           /* mixed form on a grid with a mirrored axis (free-slip y or z,
+                                 x-Dirichlet excepted): 0 = remove the rhs's weighted mean
+                                 (trapezoid weights on the mirrored axes), the part the
+                                 mirrored operator cannot reach, so every solve can end on eps;
+                                 1 = solve the rhs as the divergence gives it, as the FP64 form
+                                 does (its solves then end on the stagnation rule).  The FP64
+                                 form always keeps the reference's rhs; x-Dirichlet grids
+                                 always project */
 } o3d_params;
 
 typedef struct o3d_step_info {

@@ -47,6 +47,7 @@ class O3DParams(C.Structure):
         ("smooth_sweeps", C.c_int),
         ("coarse_cycles_learned", C.c_int),
         ("adaptive_extrapolation", C.c_int),
+        ("rhs_as_given", C.c_int),
     ]
 
 

@@ -302,14 +302,18 @@ struct SorRun {
   T *buf[2] = {nullptr, nullptr};
   bool split = false;
   SorSplitArgs<T> sa{};
-  // Timing of complete batches (all launches did their iterations).
+  // Timing of complete batches (all launches did their iterations).  The
+  // start event is recorded after a batch's first iteration, so it completes
+  // while the queue is busy and the interval holds back-to-back iterations
+  // only (recorded before it, on the queue left idle by the host's read of
+  // the state, it would also hold the host's submission of the batch).
   double ms_full = 0.0;
   int sweeps_full = 0;
   int launches_full = 0;
   std::vector<hipEvent_t> *events = nullptr;
   // Optional: recorded after iteration a.gs of the first batch.  Those
   // iterations always run in full (they end the solve only on kmax), so a
-  // solve that ends inside its first batch still times them.
+  // solve that ends inside its first batch still times iterations 2..a.gs.
   hipEvent_t mid = nullptr;
 
   hipError_t launch() {
@@ -330,11 +334,11 @@ struct SorRun {
     if (split) HIPCHK(launch_sor_split_init(sa.dslot, g_stream));
     int it = 1, L = 0;
     int batch = std::max(1, first_batch);
-    int done_before = 0, launches_before = 0;
-    bool mid_set = false;
+    bool first = true, mid_set = false;
     while (true) {
       hipEvent_t e0 = (*events)[0], e1 = (*events)[1];
-      HIPCHK(hipEventRecord(e0, g_stream));
+      bool e0_set = false;
+      int it0 = 0, L0 = 0;  // iterations and launches before e0
       for (int b = 0; b < batch; ++b) {
         a.it = it;
         a.pin = buf[L & 1];
@@ -342,7 +346,13 @@ struct SorRun {
         HIPCHK(launch());
         ++L;
         ++it;
-        if (mid && L == a.gs && a.gs <= a.kmax && done_before == 0) {
+        if (b == 0 && batch > 1) {
+          HIPCHK(hipEventRecord(e0, g_stream));
+          e0_set = true;
+          it0 = it - 1;
+          L0 = L;
+        }
+        if (mid && first && L == a.gs && a.gs >= 2 && a.gs <= a.kmax && e0_set) {
           HIPCHK(hipEventRecord(mid, g_stream));
           mid_set = true;
         }
@@ -353,23 +363,24 @@ struct SorRun {
       HIPCHK(hipStreamSynchronize(g_stream));
       const int launched = it - 1;  // iterations numbered 1..launched were launched
       if (!hst->done) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        ms_full += ms;
-        sweeps_full += launched - done_before;
-        launches_full += L - launches_before;
-        done_before = launched;
-        launches_before = L;
+        if (e0_set && launched > it0) {
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          ms_full += ms;
+          sweeps_full += launched - it0;
+          launches_full += L - L0;
+        }
         if (launched > a.kmax) return fail("SOR state did not close after kmax");
         batch = 8;
+        first = false;
         continue;
       }
-      if (mid_set && done_before == 0 && hst->sweeps >= a.gs) {
+      if (mid_set && first && hst->sweeps >= a.gs) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e0, mid));
         ms_full += ms;
-        sweeps_full += a.gs;
-        launches_full += a.gs;
+        sweeps_full += a.gs - it0;
+        launches_full += a.gs - L0;
       }
       break;
     }
@@ -1571,7 +1582,7 @@ struct Solver {
   float *Eprev = nullptr;      // previous step's increment
   float *Eprev2 = nullptr;     // the one before
   float *Rl = nullptr;         // residual of p_prev (float32)
-  float *Feat = nullptr;       // network input, 2 coarse planes (+ e_prev2's, adaptive)
+  float *Feat = nullptr;       // network input, 2 coarse planes
   float *Out = nullptr;        // fine-grid scratch
   float *Outc = nullptr;       // network output (coarse)
   MgHierarchy mg;              // coarse-grid correction (coarse_cycles > 0)
@@ -1591,6 +1602,9 @@ struct Solver {
   bool beta_pending = false;
   bool adapt_now = false;      // this step's guess is the adaptive extrapolation
   bool mean_pending = false;   // the mean is formed, its subtraction not yet done
+  // The rhs loses its weighted mean before the solve (x-Dirichlet, or the
+  // mixed form on a grid with a mirrored axis unless rhs_as_given).
+  bool project = false;
   // x-Dirichlet inflow / outflow (bc[0] == DIRICHLET), set by o3d_solver_set_inflow.
   bool xdir = false, in_set = false;
   double *in_base = nullptr;   // 3 planes of ny nz values: the inflow profile
@@ -1658,13 +1672,15 @@ int solver_jet_edges(Solver &S, int itime) {
   return 0;
 }
 
-// The compatibility projection of the x-Dirichlet Poisson rhs: minus its
-// weighted mean (trapezoid weights on the mirrored axes), the part of the
-// rhs the mirrored operator cannot reach, so SOR converges to eps.
+// The compatibility projection of the Poisson rhs on grids with a mirrored
+// axis (x-Dirichlet; the mixed form with free-slip axes): minus its weighted
+// mean (trapezoid weights on the mirrored axes, whose weighted column sums
+// of the mirrored operator vanish), the part of the rhs the operator cannot
+// reach, so SOR converges to eps.
 template <typename T>
 int solver_rhs_project(Solver &S) {
   const o3d_params &p = S.p;
-  const int mx = 1, my = p.bc[1] != PERIODIC, mz = p.bc[2] != PERIODIC;
+  const int mx = p.bc[0] != PERIODIC, my = p.bc[1] != PERIODIC, mz = p.bc[2] != PERIODIC;
   wsum_kernel<T><<<kWsumBlocks, 256, 0, g_stream>>>((const T *)S.rhs, p.nx, p.ny, p.nz, mx, my,
                                                      mz, S.wpart);
   HIPCHK(hipGetLastError());
@@ -2100,19 +2116,13 @@ __global__ void lw_guess_coarse_kernel(const double *__restrict__ p, const float
 
 // u0 = s * (out - in1) on the coarse grid: the network's estimate of the
 // coarse correction of the extrapolation, the multigrid's starting iterate.
-// u0 = s (out - in1 - beta (in1 - in2)): the network's increment estimate
-// relative to the guess p_prev + e_prev + beta (e_prev - e_prev2), on the
-// block grid (in1, in2: block averages of e_prev, e_prev2 over s; in2 null
-// when beta = 0).
+// u0 = s (out - in1): the network's correction of the guess whose residual
+// it was given (out estimates e_prev + A^-1 r on the block grid, in1 is the
+// block average of e_prev over s).
 __global__ void lw_coarse_init_kernel(float *__restrict__ u0, const float *__restrict__ out,
-                                      const float *__restrict__ in1,
-                                      const float *__restrict__ in2, float beta, int nc,
-                                      float s) {
+                                      const float *__restrict__ in1, int nc, float s) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nc) return;
-  float v = out[I] - in1[I];
-  if (in2) v -= beta * (in1[I] - in2[I]);
-  u0[I] = s * v;
+  if (I < nc) u0[I] = s * (out[I] - in1[I]);
 }
 
 // q += trilinear interpolation of the coarse correction c (the multigrid
@@ -2265,7 +2275,7 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.Eprev2, n));
   CHK(S.alloc(&S.Rl, n));
   CHK(S.alloc(&S.Out, n));
-  CHK(S.alloc(&S.Feat, 3 * nc));
+  CHK(S.alloc(&S.Feat, 2 * nc));
   CHK(S.alloc(&S.Outc, nc));
   if (S.p.coarse_cycles > 0) {
     const int F = kMgCoarsen;
@@ -2424,14 +2434,8 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
       // The network's coarse correction (on the multigrid's level 1, the
       // grid of F^3 blocks) starts the multigrid solve of the coarse-grid
       // correction below; q keeps the extrapolation.
-      const float *in2 = nullptr;
-      if (S.adapt_now) {
-        in2 = S.Feat + 2 * nc;
-        HIPCHK(launch_lw_pool(S.Eprev2, S.Feat + 2 * nc, S.p.nx, S.p.ny, S.p.nz, F,
-                              (float)(1.0 / re)));
-      }
-      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(
-          S.mg.u[1], S.Outc, S.Feat + nc, in2, (float)S.beta, nc, (float)re);
+      lw_coarse_init_kernel<<<(nc + 255) / 256, 256, 0, g_stream>>>(S.mg.u[1], S.Outc,
+                                                                    S.Feat + nc, nc, (float)re);
       nn_coarse = true;
     } else if (S.p.coarse_cycles == 0) {
       dim3 grid((S.p.nx + 255) / 256, S.p.ny, S.p.nz);
@@ -2505,7 +2509,7 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   CHK(solver_predict<T>(S, itime));
   if (S.xdir) CHK(solver_jet_edges<T>(S, itime));
   CHK(solver_divergence<T>(S, S.up, (T *)S.rhs, p.dt, S.part_div));
-  if (S.xdir) CHK(solver_rhs_project<T>(S));
+  if (S.project) CHK(solver_rhs_project<T>(S));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
   const bool guess =
       S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE || p.warm_start == O3D_WARM_EXTRAPOLATE2;
@@ -2693,6 +2697,21 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     if (!rc && hipHostMalloc((void **)&S->hbsum, 2 * sizeof(double)) != hipSuccess)
       rc = fail("hipHostMalloc failed");
   }
+  const bool mirrored =
+      p->bc[0] != PERIODIC || p->bc[1] != PERIODIC || p->bc[2] != PERIODIC;
+  S->project = p->bc[0] == DIRICHLET || (S->mixed && mirrored && !p->rhs_as_given);
+  if (!rc && S->project) {
+    rc |= S->alloc(&S->wpart, kWsumBlocks + 2);
+    if (!rc) {
+      // The sum of the weights, once, by the kernels that form the weighted sums.
+      wsum_kernel<double><<<kWsumBlocks, 256, 0, g_stream>>>(
+          nullptr, p->nx, p->ny, p->nz, p->bc[0] != PERIODIC, p->bc[1] != PERIODIC,
+          p->bc[2] != PERIODIC, S->wpart);
+      wsum_final_kernel<<<1, 1024, 0, g_stream>>>(S->wpart, nullptr, S->wpart + kWsumBlocks);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_stream) != hipSuccess)
+        rc = fail("weight sum failed");
+    }
+  }
   if (!rc && p->bc[0] == DIRICHLET) {
     S->xdir = true;
     const size_t P = (size_t)p->ny * p->nz;
@@ -2700,17 +2719,8 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     if (!rc) rc |= S->alloc(&S->in_w, P);
     if (!rc) rc |= S->alloc(&S->in_pl, 2 * P + 2 * (size_t)p->nz);
     if (!rc) rc |= S->alloc(&S->in_flux, 3);
-    if (!rc) rc |= S->alloc(&S->wpart, kWsumBlocks + 2);
     if (!rc && hipHostMalloc((void **)&S->hflux, 3 * sizeof(double)) != hipSuccess)
       rc = fail("hipHostMalloc failed");
-    if (!rc) {
-      // The sum of the weights, once, by the kernels that form the weighted sums.
-      wsum_kernel<double><<<kWsumBlocks, 256, 0, g_stream>>>(
-          nullptr, p->nx, p->ny, p->nz, 1, p->bc[1] != PERIODIC, p->bc[2] != PERIODIC, S->wpart);
-      wsum_final_kernel<<<1, 1024, 0, g_stream>>>(S->wpart, nullptr, S->wpart + kWsumBlocks);
-      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g_stream) != hipSuccess)
-        rc = fail("weight sum failed");
-    }
   }
   const int nb = std::max(S->tl.blocks(), std::max(S->tl_div.blocks(), S->tl_cor.blocks()));
   if (!rc) rc |= S->alloc(&S->part_div, nb);

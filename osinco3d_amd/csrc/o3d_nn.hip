@@ -170,6 +170,116 @@ __global__ void __launch_bounds__(256) conv_reduce_kernel(const ConvArgs<TO> a, 
   a.y[e] = (TO)v;
 }
 
+// The same implicit GEMM with the input staged through LDS, for the fine
+// levels.  A block owns a brick of 16 (x) x TY (y) x 4 (z) voxels and all
+// 16*NT output channels of its blockIdx.y group; wave w computes the z plane
+// w of the brick, TY voxel tiles of 16 (one per x row).  The brick and its
+// one-voxel periodic halo (18 x (TY+2) x 6 voxels, every input channel) are
+// loaded once, coalesced, into LDS as [channel group of 8][voxel] 16-byte
+// entries, so the 16 lanes that supply one MFMA A fragment (16 consecutive
+// x voxels, 8 channels of one tap) read 256 contiguous bytes: one pass over
+// the 64 banks.  Each input value is then read once from L2 instead of once
+// per tap.  Needs nx % 16 == 0, ny % TY == 0, nz % 4 == 0 and ci == CI.
+template <typename TO, int NT, int TY, int CI>
+__global__ void __launch_bounds__(256) conv_lds_kernel(const ConvArgs<TO> a) {
+  constexpr int HX = 18, HY = TY + 2, HZ = 6, NV = HX * HY * HZ, NG = CI / 8;
+  __shared__ bf16x8 tile[NG * NV];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, l16 = lane & 15;
+  const int ntx = a.nx >> 4, nty = a.ny / TY;
+  const int bx = (int)blockIdx.x % ntx, byz = (int)blockIdx.x / ntx;
+  const int x0 = bx * 16, y0 = (byz % nty) * TY, z0 = (byz / nty) * 4;
+  for (int e = threadIdx.x; e < NG * NV; e += 256) {
+    const int cg = e / NV, v = e - cg * NV;
+    const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+    const int gx = wrapi(x0 + hx - 1, a.nx), gy = wrapi(y0 + hy - 1, a.ny),
+              gz = wrapi(z0 + hz - 1, a.nz);
+    tile[e] = *(const bf16x8 *)(a.x + (size_t)(gx + a.nx * (gy + (size_t)a.ny * gz)) * CI +
+                                cg * 8);
+  }
+  __syncthreads();
+  const int co0 = blockIdx.y * 16 * NT;
+  f32x4 acc[TY][NT];
+#pragma unroll
+  for (int m = 0; m < TY; ++m)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  constexpr int ktot = 27 * CI;
+  const int nchunk = a.kp >> 5;
+  const bf16 *wrow[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) wrow[n] = a.w + (size_t)(co0 + 16 * n + l16) * a.kp;
+  for (int kc = 0; kc < nchunk; ++kc) {
+    const int k0 = (kc << 5) + (g << 3);
+    bf16x8 bv[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bv[n] = *(const bf16x8 *)(wrow[n] + k0);
+    bf16x8 av[TY];
+    if (k0 < ktot) {
+      const int t = k0 / CI, cg = (k0 % CI) >> 3;
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+      const int base = cg * NV + (l16 + 1 + dx) + HX * ((1 + dy) + HY * (wv + 1 + dz));
+#pragma unroll
+      for (int m = 0; m < TY; ++m) av[m] = tile[base + HX * m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < TY; ++m)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) av[m][q] = (bf16)0.f;
+    }
+#pragma unroll
+    for (int m = 0; m < TY; ++m)
+#pragma unroll
+      for (int n = 0; n < NT; ++n)
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
+  }
+  const int z = z0 + wv;
+#pragma unroll
+  for (int m = 0; m < TY; ++m)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const int c = co0 + 16 * n + l16;
+      const float b = a.bias ? a.bias[c] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int vox = x0 + 4 * g + i + a.nx * (y0 + m + a.ny * z);
+        const size_t o = (size_t)vox * a.co + c;
+        float v = acc[m][n][i] + b;
+        if (a.skip) v += (float)a.skip[o];
+        if (a.act) v = act_f(v);
+        if (a.accumulate) v += (float)a.y[o];
+        a.y[o] = (TO)v;
+      }
+    }
+}
+
+// LDS bricks of a grid (TY rows per brick) and whether the grid tiles.
+inline long long lds_bricks(int nx, int ny, int nz, int ty) {
+  if (nx % 16 || ny % ty || nz % 4) return 0;
+  return (long long)(nx / 16) * (ny / ty) * (nz / 4);
+}
+
+template <typename TO, int NT, int CI>
+bool launch_conv_lds(const ConvArgs<TO> &a, hipStream_t s) {
+  const int gy = a.co / (16 * NT);
+  // Four rows per brick while that leaves >= 512 blocks; one row (four
+  // times the blocks) down to 256; coarser grids take the split-K path.
+  // (64 input channels: one-row bricks only, 41 KB of LDS.)
+  const long long b4 = lds_bricks(a.nx, a.ny, a.nz, 4) * gy;
+  const long long b1 = lds_bricks(a.nx, a.ny, a.nz, 1) * gy;
+  if constexpr (CI <= 32) {
+    if (b4 >= 512) {
+      conv_lds_kernel<TO, NT, 4, CI><<<dim3((unsigned)(b4 / gy), gy), 256, 0, s>>>(a);
+      return true;
+    }
+  }
+  if (b1 >= 256) {
+    conv_lds_kernel<TO, NT, 1, CI><<<dim3((unsigned)(b1 / gy), gy), 256, 0, s>>>(a);
+    return true;
+  }
+  return false;
+}
+
 // Blocks a convolution launches without voxel-tile blocking or split-K.
 inline int conv_blocks(int V, int co) {
   const int nt = co >= 64 ? 4 : (co >= 32 ? 2 : 1);
@@ -212,6 +322,12 @@ template <typename TO>
 hipError_t launch_conv(const ConvArgs<TO> &a, float *ws, size_t ws_n, hipStream_t s) {
   const int V = a.nx * a.ny * a.nz;
   const int nt = a.co >= 64 ? 4 : (a.co >= 32 ? 2 : 1);
+  // Fine levels (16, 32 or 64 input channels): the LDS-staged kernel.
+  if (a.ci == 16 && nt == 1 && launch_conv_lds<TO, 1, 16>(a, s)) return hipGetLastError();
+  if (a.ci == 16 && nt == 2 && launch_conv_lds<TO, 2, 16>(a, s)) return hipGetLastError();
+  if (a.ci == 32 && nt == 1 && launch_conv_lds<TO, 1, 32>(a, s)) return hipGetLastError();
+  if (a.ci == 32 && nt == 2 && launch_conv_lds<TO, 2, 32>(a, s)) return hipGetLastError();
+  if (a.ci == 64 && nt == 2 && launch_conv_lds<TO, 2, 64>(a, s)) return hipGetLastError();
   if (nt == 4) return launch_conv_nt<TO, 4>(a, V, ws, ws_n, s);
   if (nt == 2) return launch_conv_nt<TO, 2>(a, V, ws, ws_n, s);
   return launch_conv_nt<TO, 1>(a, V, ws, ws_n, s);

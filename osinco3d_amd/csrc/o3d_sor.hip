@@ -717,6 +717,25 @@ hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s) 
 // residual keeps the operation order of the red-black sweep's Laplacian.
 constexpr int RBX = 64, RBY = 4, RZC = 8, RNT = RBX * RBY;
 
+// The tile of this block: the linear block index remapped XCD-aware (tiles
+// x fastest, then y, then z; see xcd_tile), so that tiles sharing halo rows
+// run on one XCD and find them in its L2.  b is the tile's linear index, the
+// slot of its per-block partial sums.
+struct RTile {
+  int bx, by, bz, b;
+};
+__device__ __forceinline__ RTile residual_tile() {
+  const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+  const int lin = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
+  const int t = xcd_tile(lin, gx * gy * (int)gridDim.z);
+  RTile r;
+  r.bx = t % gx;
+  r.by = (t / gx) % gy;
+  r.bz = t / (gx * gy);
+  r.b = t;
+  return r;
+}
+
 dim3 residual_grid(int nx, int ny, int nz) {
   return dim3((nx + RBX - 1) / RBX, (ny + RBY - 1) / RBY, (nz + RZC - 1) / RZC);
 }
@@ -734,10 +753,11 @@ __device__ __forceinline__ void residual_column_ld(LD &&ld, const TR *__restrict
                                                    int ny, int nz, int px, int py, int pz,
                                                    double ox, double oy, double oz, double A,
                                                    F &&f) {
-  const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
-  const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
+  const RTile T = residual_tile();
+  const int i = T.bx * RBX + (threadIdx.x % RBX);
+  const int j = T.by * RBY + (threadIdx.x / RBX);
   if (i >= nx || j >= ny) return;
-  const int k0 = blockIdx.z * RZC;
+  const int k0 = T.bz * RZC;
   const size_t plane = (size_t)nx * ny;
   const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
   const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
@@ -796,10 +816,11 @@ __global__ void __launch_bounds__(RNT) guess_residual_kernel(const double *__res
                                                              double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0;
-  const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
-  const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
+  const RTile T = residual_tile();
+  const int i = T.bx * RBX + (threadIdx.x % RBX);
+  const int j = T.by * RBY + (threadIdx.x / RBX);
   if (i < nx && j < ny) {
-    const int k0 = blockIdx.z * RZC;
+    const int k0 = T.bz * RZC;
     const size_t plane = (size_t)nx * ny;
     const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
     const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
@@ -849,8 +870,7 @@ __global__ void __launch_bounds__(RNT) guess_residual_kernel(const double *__res
     m0 = block_max<RNT / O3D_WAVE>(m0, sc);
     s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
     if (threadIdx.x == 0) {
-      const size_t b =
-          blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+      const size_t b = (size_t)T.b;
       part0[2 * b] = m0;
       part0[2 * b + 1] = s0;
     }
@@ -882,10 +902,11 @@ __global__ void __launch_bounds__(RNT) guess_adapt_kernel(
     int py, int pz, double ox, double oy, double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0, sb = 0.0, sd = 0.0;
-  const int i = blockIdx.x * RBX + (threadIdx.x % RBX);
-  const int j = blockIdx.y * RBY + (threadIdx.x / RBX);
+  const RTile T = residual_tile();
+  const int i = T.bx * RBX + (threadIdx.x % RBX);
+  const int j = T.by * RBY + (threadIdx.x / RBX);
   if (i < nx && j < ny) {
-    const int k0 = blockIdx.z * RZC;
+    const int k0 = T.bz * RZC;
     const size_t plane = (size_t)nx * ny;
     const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
     const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
@@ -944,8 +965,7 @@ __global__ void __launch_bounds__(RNT) guess_adapt_kernel(
       }
     }
   }
-  const size_t b =
-      blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+  const size_t b = (size_t)T.b;
   if (part0) {
     m0 = block_max<RNT / O3D_WAVE>(m0, sc);
     s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
@@ -1108,8 +1128,7 @@ __global__ void __launch_bounds__(RNT) residual_norm_pair_kernel(
   m1 = block_max<RNT / O3D_WAVE>(m1, sc);
   s1 = block_sum<RNT / O3D_WAVE>(s1, sc);
   if (threadIdx.x == 0) {
-    const size_t b =
-        blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+    const size_t b = (size_t)residual_tile().b;
     part[4 * b] = m0;
     part[4 * b + 1] = s0;
     part[4 * b + 2] = m1;

@@ -92,9 +92,10 @@ def test_statistics_full_size(orc, case):
 def test_mixing_layer_steps_mixed_tracks_fp64():
     """Ten full steps of the mixing layer (LES, free-slip y, AB2) in the FP64
     and mixed forms.  The y-free-slip Poisson problem of this driver is not
-    compatible (the reference's rhs keeps its mean), so the solves end on the
-    reference's stagnation rule in both forms; the velocity fields and the
-    kinetic energy still agree closely."""
+    compatible (the reference's rhs keeps its mean), so the FP64 solves end
+    on the reference's stagnation rule; the mixed form removes the rhs's
+    weighted mean and ends on eps.  The velocity fields and the kinetic
+    energy agree closely."""
     from osinco3d_amd.solver import FP64, MIXED, Solver, mixing_layer
     cfg = mixing_layer_config(steps=10)
     init = mixing_layer(cfg)
@@ -123,8 +124,9 @@ def test_mixing_layer_full_size_fp64_and_coarse_grid_forms_50_steps():
     """Config 3 at full size (384x256x128, LES, free-slip y, AB2) for 50
     steps in the FP64 form and the mixed form with the extrapolated warm
     start and the coarse-grid correction: every solve ends on the
-    reference's rules (eps, or stagnation for this incompatible free-slip
-    rhs; never kmax), the coarse-grid form needs fewer sweeps than FP64 and
+    reference's rules, never kmax (FP64: the stagnation rule, the free-slip
+    rhs being incompatible; the mixed form removes the rhs's weighted mean
+    and ends on eps), the coarse-grid form needs fewer sweeps than FP64 and
     E_k agrees within 1e-8."""
     from osinco3d_amd.solver import FP64, MIXED, WARM_EXTRAPOLATE, Solver, mixing_layer
     cfg = mixing_layer_config(steps=50)
@@ -135,17 +137,19 @@ def test_mixing_layer_full_size_fp64_and_coarse_grid_forms_50_steps():
         print(f"[fullsize] mixing layer 50 steps, form {name}", flush=True)
         s = Solver(cfg, prec, warm, coarse_cycles=coarse[0], smooth_sweeps=coarse[1])
         s.upload(*init)
-        sweeps, ek = [], []
+        sweeps, ek, reasons = [], [], set()
         for it in range(1, 51):
             info = s.step(it)
             assert info["sor_reason"] in (0, 1) and not info["diverged"], (name, it, info)
             sweeps.append(info["sor_exit_iter"])
+            reasons.add(info["sor_reason"])
             if it % 10 == 0:
                 ek.append(s.statistics(it * s.dt)[1])
         s.close()
-        res[name] = (np.array(sweeps), np.array(ek))
+        res[name] = (np.array(sweeps), np.array(ek), reasons)
         print(f"[fullsize] {name}: sweeps/step {np.mean(sweeps):.1f} (max {max(sweeps)})",
               flush=True)
-    (s64, e64), (smg, emg) = res["fp64"], res["mg"]
+    (s64, e64, r64), (smg, emg, rmg) = res["fp64"], res["mg"]
+    assert rmg == {0}
     assert smg.mean() < s64.mean()
     assert np.abs(emg - e64).max() <= 1e-8 * np.abs(e64).max()

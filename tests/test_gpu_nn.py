@@ -39,7 +39,9 @@ def _net(torch, shape, P):
     return net
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32)])
+# (32, 64, 64): its 64x64x32 and 32x32x16 levels run the LDS-staged
+# convolution (four- and one-row bricks), the coarser ones the split-K path.
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32), (32, 64, 64)])
 def test_forward_matches_oracle(torch, shape):
     print(f"[gpu] nn forward {shape}", flush=True)
     P = _params()
@@ -57,8 +59,9 @@ def test_forward_matches_oracle(torch, shape):
     assert np.corrcoef(got.ravel(), exp.ravel())[0, 1] > 0.9995
 
 
-def test_backward_matches_oracle(torch):
-    shape = (16, 16, 16)
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 64, 64)])
+def test_backward_matches_oracle(torch, shape):
+    print(f"[gpu] nn backward {shape}", flush=True)
     P = _params(1)
     rng = np.random.default_rng(4)
     inp = rng.standard_normal((2,) + shape).astype(np.float32)
