@@ -11,7 +11,6 @@ Each requested form is timed over the same steps:
   fp64   plain FP64 (the drop-in; bit-identical operators to the reference)
   mixed  FP32 stencils / AB update / SOR sweeps, FP64-refined Poisson
   extrap mixed + warm start by linear extrapolation p_prev + (p_prev - p_prev2)
-  extrap2 mixed + quadratic extrapolation p_prev + 2 e_prev - e_prev2
   mg     the adaptive extrapolation p_prev + e_prev + beta (e_prev - e_prev2)
          (beta the least-squares weight of the previous step), then the
          coarse-grid correction of the guess (multigrid V-cycles on the 2^3
@@ -24,7 +23,6 @@ Each requested form is timed over the same steps:
          Gauss-Seidel opening (the headline)
   mgl    mg with the linear extrapolation (beta = 0)
   nnl    nn with the linear extrapolation
-  mg2    mg with the quadratic extrapolation (beta = 1)
   nn1    nn with one V-cycle (instead of coarse_cycles) in the first coarse-grid
          correction, which starts from the network's coarse estimate
 The JSON line's value is the steps/s of the headline form (the last of
@@ -34,6 +32,7 @@ late_window: the forms fp64, mg and nn run the first 975 steps of the
 1000-step run untimed by events (wall-clocked as a block), then steps
 976-1000 are timed: steps/s of that late window, the 1000-step mean steps/s,
 sweeps per step, and E_k / enstrophy at step 1000 against the FP64 form.
+The mixing workload runs the same series over 500 steps (--late-total).
 
 roofline: the dominant work of the headline form is the red-black SOR
 iteration (in the mixed forms the FP32 colour-split red and black launches,
@@ -101,8 +100,10 @@ def parse_args():
     ap.add_argument("--cpu-kmax", type=int, default=5000,
                     help="kmax of the reference's measured 256^3 step (5000: a full step)")
     ap.add_argument("--late-forms", default="fp64,mg,nn",
-                    help="forms run through the 1000-step late window ('' to skip)")
-    ap.add_argument("--late-total", type=int, default=1000)
+                    help="forms run through the late-window series ('' to skip)")
+    ap.add_argument("--late-total", type=int, default=None,
+                    help="steps of the late-window series (default: 1000 for tgv, 500 for "
+                         "mixing, none for jet)")
     ap.add_argument("--late-window", type=int, default=25)
     ap.add_argument("--model", default=DEFAULT_MODEL)
     return ap.parse_args()
@@ -136,8 +137,8 @@ def pct(v, q):
 
 
 LEARNED_FORMS = ("nn", "nn0", "nn1", "nnl")
-COARSE_FORMS = ("mg", "mgl", "mg2", "nn", "nn1", "nnl")
-WARM_FORMS = LEARNED_FORMS + ("extrap", "extrap2", "mg", "mgl", "mg2")
+COARSE_FORMS = ("mg", "mgl", "nn", "nn1", "nnl")
+WARM_FORMS = LEARNED_FORMS + ("extrap", "mg", "mgl")
 
 
 def make_solver(form, cfg, model_path, coarse):
@@ -151,8 +152,6 @@ def make_solver(form, cfg, model_path, coarse):
         warm = S.WARM_LEARNED
     elif form in ("extrap", "mg", "mgl"):
         warm = S.WARM_EXTRAPOLATE
-    elif form in ("extrap2", "mg2"):
-        warm = S.WARM_EXTRAPOLATE2
     cc, gs = coarse if form in COARSE_FORMS else (0, 0)
     net = None
     if form in LEARNED_FORMS:
@@ -559,7 +558,11 @@ def main():
                     "stream_probe_GBps": copy_gbs}
     late = None
     late_forms = [f.strip() for f in args.late_forms.split(",") if f.strip()]
-    if late_forms and args.workload == "tgv":
+    late_total = args.late_total
+    if late_total is None:
+        late_total = {"tgv": 1000, "mixing": 500}.get(args.workload, 0)
+    args.late_total = late_total
+    if late_forms and late_total > args.late_window:
         late = {}
         for form in late_forms:
             print(f"[bench] late window {form}: {args.late_total} steps, last "

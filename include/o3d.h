@@ -254,8 +254,9 @@ int o3d_q_criterion(double *q, const double *ux, const double *uy, const double 
 /* Warm start of the Poisson solve. */
 #define O3D_WARM_PREVIOUS 0 /* previous step's pressure (the reference's behaviour) */
 #define O3D_WARM_LEARNED 1  /* learned correction, previous pressure as fallback */
-#define O3D_WARM_EXTRAPOLATE 2 /* p_prev + (p_prev - p_prev2), previous pressure as fallback */
-#define O3D_WARM_EXTRAPOLATE2 3 /* quadratic: p_prev + 2 e_prev - e_prev2, same fallback */
+#define O3D_WARM_EXTRAPOLATE 2 /* p_prev + (p_prev - p_prev2), previous pressure as fallback
+                                  (see adaptive_extrapolation for the weighted second
+                                  difference) */
 
 typedef struct o3d_params {
   int nx, ny, nz;
@@ -269,7 +270,7 @@ typedef struct o3d_params {
   double omega, eps;
   int kmax, idyn;
   int precision;        /* O3D_FP64 or O3D_MIXED */
-  int warm_start;       /* O3D_WARM_PREVIOUS, _LEARNED, _EXTRAPOLATE or _EXTRAPOLATE2 */
+  int warm_start;       /* O3D_WARM_PREVIOUS, _LEARNED or _EXTRAPOLATE */
   int inner_max;        /* O3D_MIXED: FP32 sweeps between FP64 refinements (0 = default) */
   int coarse_cycles;    /* O3D_MIXED with a warm start: multigrid V-cycles of the coarse-grid
                            correction added to the guess, on 2^3 block averages (0 = none;

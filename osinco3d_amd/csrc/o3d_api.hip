@@ -2159,11 +2159,6 @@ __global__ void __launch_bounds__(256) mg_prolong_fine_kernel(double *__restrict
   *o = w;
 }
 
-__global__ void lw_guess2_kernel(const double *p, const float *e1, const float *e2, double *q,
-                                 int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) q[i] = p[i] + (2.0 * (double)e1[i] - (double)e2[i]);
-}
 // e = p - p0 (float32) and the per-block sum of squares of e; pairs of
 // cells per thread (16-byte loads), the odd last cell by thread 0.
 constexpr int kIncBlocks = 2048;
@@ -2348,7 +2343,7 @@ int solver_learning_prepare(Solver &S) {
   // With a network, the adaptive guess is used only under the coarse-grid
   // correction, whose start takes the network's estimate relative to it.
   S.adapt_now = S.hist >= 2 && S.p.adaptive_extrapolation &&
-                S.p.warm_start != O3D_WARM_EXTRAPOLATE2 && (!S.nn || S.p.coarse_cycles > 0);
+                (!S.nn || S.p.coarse_cycles > 0);
   if (S.adapt_now) {
     // p_prev + e_prev + beta (e_prev - e_prev2), beta the least-squares
     // weight of the previous step's rhs, and its residual, with the
@@ -2403,9 +2398,9 @@ int solver_learning_prepare(Solver &S) {
 //  * a network attached: p_prev + e_prev + RMS(e_prev) * trilinear(NN(features)
 //    - features[1]), the network's coarse-grid correction of the linear
 //    extrapolation, from the extrapolation's residual and e_prev;
-//  * O3D_WARM_EXTRAPOLATE: p_prev + e_prev;
-//  * O3D_WARM_EXTRAPOLATE2: p_prev + 2 e_prev - e_prev2 (linear until two
-//    increments exist).
+//  * O3D_WARM_EXTRAPOLATE: p_prev + e_prev, or with adaptive_extrapolation
+//    p_prev + e_prev + beta (e_prev - e_prev2) (formed before, with its
+//    residual, by solver_learning_prepare).
 int solver_warm_guess(Solver &S, int *used, double *warm) {
   *used = 0;
   warm[0] = warm[1] = 0.0;
@@ -2443,10 +2438,8 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
                                                          S.Feat + nc, q, S.p.nx, S.p.ny,
                                                          S.p.nz, F, per[0], per[1], per[2], re);
     }
-  } else if (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2) {
-    lw_guess2_kernel<<<(n + 255) / 256, 256, 0, g_stream>>>(S.P[S.cur], S.Eprev, S.Eprev2, q, n);
   }
-  // Otherwise q already holds p_prev + e_prev (solver_learning_prepare).
+  // Otherwise q already holds the extrapolation (solver_learning_prepare).
   HIPCHK(hipGetLastError());
   if (S.p.coarse_cycles > 0) {
     // Coarse-grid correction: the residual of q averaged over F^3 blocks,
@@ -2456,8 +2449,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     const int F = kMgCoarsen;
     int per[3];
     solver_periodic(S.kind, per);
-    const bool own = (S.nn && !nn_coarse) ||
-                     (S.p.warm_start == O3D_WARM_EXTRAPOLATE2 && S.hist >= 2);
+    const bool own = S.nn && !nn_coarse;
     if (own)
       HIPCHK(launch_residual_f32<float>(q, (const float *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
                                         S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
@@ -2512,7 +2504,7 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   if (S.project) CHK(solver_rhs_project<T>(S));
   HIPCHK(hipEventRecord(S.ev[4], g_stream));
   const bool guess =
-      S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE || p.warm_start == O3D_WARM_EXTRAPOLATE2;
+      S.nn || p.warm_start == O3D_WARM_EXTRAPOLATE;
   const bool learning = guess || S.cap_r || S.cap_eprev || S.cap_e;
   int warm = 0;
   if (learning) CHK(solver_learning_prepare(S));
@@ -2642,7 +2634,7 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
   }
   if (p->precision != O3D_FP64 && p->precision != O3D_MIXED)
     return fail("unknown precision form %d", p->precision);
-  if (p->warm_start < O3D_WARM_PREVIOUS || p->warm_start > O3D_WARM_EXTRAPOLATE2)
+  if (p->warm_start < O3D_WARM_PREVIOUS || p->warm_start > O3D_WARM_EXTRAPOLATE)
     return fail("unknown warm start %d", p->warm_start);
   const int kind = solver_from_bc(p->bc);
   if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");

@@ -22,7 +22,6 @@ MIXED = 1
 WARM_PREVIOUS = 0
 WARM_LEARNED = 1
 WARM_EXTRAPOLATE = 2
-WARM_EXTRAPOLATE2 = 3
 
 SOR_REASONS = {0: "eps", 1: "stagnation", 2: "kmax"}
 
