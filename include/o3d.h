@@ -348,6 +348,15 @@ void *o3d_solver_field(void *handle, int which);
  * colour-split storage the red and the black launch counted as one).
  * launches2 may be NULL. */
 int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, int *launches2);
+/* Mixed form with colour-split sweeps: `iterations` back-to-back FP32
+ * red-black iterations (two launches each) of the solver's correction
+ * problem, eps 0 so none stops, on its correction and residual buffers
+ * (scratch between solves), timed with HIP events on the launch stream from
+ * the end of the first iteration to the end of the last; us = the mean time
+ * per iteration.  For measurement: the event interval spans many iterations,
+ * so it holds the kernels and the gaps between back-to-back launches only.
+ * No counterpart in the reference. */
+int o3d_solver_sor_period(void *handle, int iterations, double *us);
 /* Adams-Bashforth history for checkpoint / restart (the reference has none:
  * it restarts with an Euler step).  hist: host array of 9 n elements, float64
  * in O3D_FP64 and float32 in O3D_MIXED, ordered by age (0 = the newest

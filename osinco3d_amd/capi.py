@@ -88,7 +88,8 @@ EXPORTS = [
     "o3d_statistics", "o3d_statistics_noalloc", "o3d_transeq", "o3d_rotational", "o3d_q_criterion", "o3d_solver_create",
     "o3d_solver_destroy", "o3d_solver_upload",
     "o3d_solver_download", "o3d_solver_step", "o3d_solver_statistics", "o3d_solver_field",
-    "o3d_solver_sor_timing", "o3d_solver_set_warm_model", "o3d_solver_capture",
+    "o3d_solver_sor_timing", "o3d_solver_sor_period", "o3d_solver_set_warm_model",
+    "o3d_solver_capture",
     "o3d_warm_features", "o3d_solver_history", "o3d_multigrid", "o3d_poisson_multigrid",
     "o3d_solver_set_inflow",
 ]

@@ -2888,6 +2888,40 @@ extern "C" int o3d_solver_sor_timing(void *handle, double *ms2, int *sweeps2, in
   return 0;
 }
 
+extern "C" int o3d_solver_sor_period(void *handle, int iterations, double *us) {
+  if (!handle || !us) return fail("NULL argument");
+  Solver &S = *(Solver *)handle;
+  if (!S.mixed || !S.dslot || !S.R || !S.E[0] || !S.E[1])
+    return fail("sor period: needs the mixed form with colour-split sweeps");
+  if (iterations < 2 || iterations > 100000)
+    return fail("sor period: iterations %d outside [2, 100000]", iterations);
+  SorRun<float, float> r;
+  r.st = S.st;
+  r.hst = S.hst;
+  r.events = &S.ev;
+  r.buf[0] = S.E[0];
+  r.buf[1] = S.E[1];
+  sor_args_common<float, float>(S, r.a, S.R, S.st, iterations);
+  r.a.eps = 0.0;  // never converged, never stagnant: every launch sweeps
+  const int h = S.n / 2;
+  r.split = true;
+  r.sa = split_args<float>(r.a, S.E[0], S.E[0] + h, S.R, S.R + h, S.dslot);
+  HIPCHK(launch_sor_init(S.st, S.omega, 0, g_stream));
+  HIPCHK(launch_sor_split_init(S.dslot, g_stream));
+  hipEvent_t e0 = S.ev[0], e1 = S.ev[1];
+  for (int it = 1; it <= iterations; ++it) {
+    r.a.it = it;
+    HIPCHK(r.launch());
+    if (it == 1) HIPCHK(hipEventRecord(e0, g_stream));
+  }
+  HIPCHK(hipEventRecord(e1, g_stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *us = 1e3 * (double)ms / (iterations - 1);
+  return 0;
+}
+
 extern "C" int o3d_solver_set_warm_model(void *handle, void *nn) {
   if (!handle) return fail("NULL handle");
   Solver &S = *(Solver *)handle;

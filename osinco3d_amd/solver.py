@@ -241,6 +241,13 @@ class Solver:
                                                out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def sor_period(self, iterations: int = 200) -> float:
+        """Mean time (us) of one FP32 red-black iteration over `iterations`
+        back-to-back iterations (o3d_solver_sor_period; mixed form)."""
+        us = C.c_double(0)
+        capi.check(self.lib.o3d_solver_sor_period(self.handle, int(iterations), C.byref(us)))
+        return us.value
+
     def sor_timing(self, launches: bool = False):
         """(ms, sweeps) of the last step's complete SOR batches per precision
         [FP64, FP32]; with launches=True also the kernel launch counts."""
