@@ -453,34 +453,43 @@ __global__ void __launch_bounds__(256) mg_pool_split_kernel(const float *__restr
   f0[idx] = 0.125f * t;
 }
 
+// The trilinear interpolation of the level-0 solution onto the fine grid,
+// stored colour-split (the first FP32 correction solve's starting iterate):
+// one thread per pair of cells (2c, 2c + 1) of a row, which are one red and
+// one black cell, so both halves of the split array are written coalesced;
+// 2D block (64 x 4) over (c, j), blockIdx.z = k: no 64-bit index division.
 __global__ void __launch_bounds__(256) mg_prolong_split_kernel(float *__restrict__ e,
                                                                const float *__restrict__ u0,
                                                                int nx, int ny, int nz, int px,
                                                                int py, int pz) {
-  const size_t half = (size_t)nx * ny * nz / 2;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 2 * half) return;
-  const int black = idx >= half ? 1 : 0;
-  const size_t o = idx - (black ? half : 0);
   const int hx = nx / 2;
-  const int c = (int)(o % hx);
-  const size_t row = o / hx;
-  const int j = (int)(row % ny), k = (int)(row / ny);
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (c >= hx || j >= ny) return;
+  const size_t half = (size_t)nx * ny * nz / 2;
+  const size_t o = (size_t)hx * (j + (size_t)ny * k) + c;
   const int q = (j + k) & 1;
-  const int i = 2 * c + (black ? 1 - q : q);
   const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
-  int x0, x1, y0, y1, z0, z1;
-  float wx, wy, wz;
-  mg_axis(i, cx, px, x0, x1, wx);
+  int y0, y1, z0, z1;
+  float wy, wz;
   mg_axis(j, cy, py, y0, y1, wy);
   mg_axis(k, cz, pz, z0, z1, wz);
   auto cv = [&](int a, int b, int d) {
     return u0[(size_t)a + (size_t)cx * (b + (size_t)cy * d)];
   };
-  e[idx] = (1.f - wz) * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z0) + wx * cv(x1, y0, z0)) +
+  auto at = [&](int i) {
+    int x0, x1;
+    float wx;
+    mg_axis(i, cx, px, x0, x1, wx);
+    return (1.f - wz) * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z0) + wx * cv(x1, y0, z0)) +
                          wy * ((1.f - wx) * cv(x0, y1, z0) + wx * cv(x1, y1, z0))) +
            wz * ((1.f - wy) * ((1.f - wx) * cv(x0, y0, z1) + wx * cv(x1, y0, z1)) +
                  wy * ((1.f - wx) * cv(x0, y1, z1) + wx * cv(x1, y1, z1)));
+  };
+  const float va = at(2 * c), vb = at(2 * c + 1);
+  // Red cell: i = 2c + q; black: i = 2c + 1 - q (entry c of its colour's row).
+  e[o] = q ? vb : va;
+  e[half + o] = q ? va : vb;
 }
 
 }  // namespace
@@ -495,9 +504,9 @@ hipError_t mg_pool_split(const float *r, float *f0, int nx, int ny, int nz, hipS
 hipError_t mg_prolong_split(float *e, const float *u0, int nx, int ny, int nz, const int per[3],
                             hipStream_t s) {
   if (nx % 2 || ny % 2 || nz % 2) return hipErrorInvalidValue;
-  const size_t n = (size_t)nx * ny * nz;
-  mg_prolong_split_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(e, u0, nx, ny, nz, per[0],
-                                                                       per[1], per[2]);
+  dim3 grid((nx / 2 + 63) / 64, (ny + 3) / 4, nz);
+  mg_prolong_split_kernel<<<grid, dim3(64, 4), 0, s>>>(e, u0, nx, ny, nz, per[0], per[1],
+                                                       per[2]);
   return hipGetLastError();
 }
 

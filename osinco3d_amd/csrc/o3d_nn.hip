@@ -170,34 +170,49 @@ __global__ void __launch_bounds__(256) conv_reduce_kernel(const ConvArgs<TO> a, 
   a.y[e] = (TO)v;
 }
 
-// The same implicit GEMM with the input staged through LDS, for the fine
+// The same implicit GEMM with its operands staged through LDS, for the fine
 // levels.  A block owns a brick of 16 (x) x TY (y) x 4 (z) voxels and all
 // 16*NT output channels of its blockIdx.y group; wave w computes the z plane
-// w of the brick, TY voxel tiles of 16 (one per x row).  The brick and its
-// one-voxel periodic halo (18 x (TY+2) x 6 voxels, every input channel) are
-// loaded once, coalesced, into LDS as [channel group of 8][voxel] 16-byte
-// entries, so the 16 lanes that supply one MFMA A fragment (16 consecutive
-// x voxels, 8 channels of one tap) read 256 contiguous bytes: one pass over
-// the 64 banks.  Each input value is then read once from L2 instead of once
-// per tap.  Needs nx % 16 == 0, ny % TY == 0, nz % 4 == 0 and ci == CI.
-template <typename TO, int NT, int TY, int CI>
+// w of the brick, TY voxel tiles of 16 (one per x row).
+//  * Input: the brick and its one-voxel periodic halo (18 x (TY+2) x 6
+//    voxels, every input channel) are loaded once, coalesced, as [channel
+//    group of 8][voxel] 16-byte entries, so the 16 lanes that supply one
+//    MFMA A fragment (16 consecutive x voxels, 8 channels of one tap) read
+//    256 contiguous bytes: one pass over the 64 banks.  Each input value is
+//    read once from L2 instead of once per tap.  With UP the input is the
+//    grid of half the size, upsampled 2x (nearest) as it is staged, which is
+//    the decoder's up-sampling fused into the convolution that consumes it.
+//  * Weights: slices of SK K-chunks of the block's 16*NT rows, loaded
+//    cooperatively between two barriers; rows are padded by 16 bytes so the
+//    16 lanes reading one B fragment (16 rows, same k) hit distinct banks.
+//    The K loop then waits on LDS only, not on a global load per chunk.
+// Needs nx % 16 == 0, ny % TY == 0, nz % 4 == 0 and ci == CI (UP: even
+// dimensions, x of (nx/2, ny/2, nz/2) voxels).
+template <typename TO, int NT, int TY, int CI, bool UP>
 __global__ void __launch_bounds__(256) conv_lds_kernel(const ConvArgs<TO> a) {
   constexpr int HX = 18, HY = TY + 2, HZ = 6, NV = HX * HY * HZ, NG = CI / 8;
+  constexpr int SK = 8;              // K chunks per weight slice
+  constexpr int WROW = SK * 4 + 1;   // 16-byte entries per padded weight row
   __shared__ bf16x8 tile[NG * NV];
+  __shared__ bf16x8 wts[NT * 16 * WROW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = lane >> 4, l16 = lane & 15;
   const int ntx = a.nx >> 4, nty = a.ny / TY;
   const int bx = (int)blockIdx.x % ntx, byz = (int)blockIdx.x / ntx;
   const int x0 = bx * 16, y0 = (byz % nty) * TY, z0 = (byz / nty) * 4;
+  const int sx = UP ? a.nx >> 1 : a.nx, sy = UP ? a.ny >> 1 : a.ny;
   for (int e = threadIdx.x; e < NG * NV; e += 256) {
     const int cg = e / NV, v = e - cg * NV;
     const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
-    const int gx = wrapi(x0 + hx - 1, a.nx), gy = wrapi(y0 + hy - 1, a.ny),
-              gz = wrapi(z0 + hz - 1, a.nz);
-    tile[e] = *(const bf16x8 *)(a.x + (size_t)(gx + a.nx * (gy + (size_t)a.ny * gz)) * CI +
-                                cg * 8);
+    int gx = wrapi(x0 + hx - 1, a.nx), gy = wrapi(y0 + hy - 1, a.ny),
+        gz = wrapi(z0 + hz - 1, a.nz);
+    if (UP) {
+      gx >>= 1;
+      gy >>= 1;
+      gz >>= 1;
+    }
+    tile[e] = *(const bf16x8 *)(a.x + (size_t)(gx + sx * (gy + (size_t)sy * gz)) * CI + cg * 8);
   }
-  __syncthreads();
   const int co0 = blockIdx.y * 16 * NT;
   f32x4 acc[TY][NT];
 #pragma unroll
@@ -206,32 +221,46 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(const ConvArgs<TO> a) {
     for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
   constexpr int ktot = 27 * CI;
   const int nchunk = a.kp >> 5;
-  const bf16 *wrow[NT];
+  for (int s0 = 0; s0 < nchunk; s0 += SK) {
+    __syncthreads();  // every wave is done with the previous slice
+    for (int v = threadIdx.x; v < NT * 16 * SK * 4; v += 256) {
+      const int row = v / (SK * 4), c8 = v - row * (SK * 4);
+      const int k = s0 * 32 + c8 * 8;
+      bf16x8 w8;
+      if (k < a.kp) {
+        w8 = *(const bf16x8 *)(a.w + (size_t)(co0 + row) * a.kp + k);
+      } else {
 #pragma unroll
-  for (int n = 0; n < NT; ++n) wrow[n] = a.w + (size_t)(co0 + 16 * n + l16) * a.kp;
-  for (int kc = 0; kc < nchunk; ++kc) {
-    const int k0 = (kc << 5) + (g << 3);
-    bf16x8 bv[NT];
+        for (int q = 0; q < 8; ++q) w8[q] = (bf16)0.f;
+      }
+      wts[row * WROW + c8] = w8;
+    }
+    __syncthreads();  // the slice (and, the first time, the input tile) is in LDS
+    const int ce = min(SK, nchunk - s0);
+    for (int c = 0; c < ce; ++c) {
+      const int k0 = ((s0 + c) << 5) + (g << 3);
+      bf16x8 bv[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) bv[n] = *(const bf16x8 *)(wrow[n] + k0);
-    bf16x8 av[TY];
-    if (k0 < ktot) {
-      const int t = k0 / CI, cg = (k0 % CI) >> 3;
-      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-      const int base = cg * NV + (l16 + 1 + dx) + HX * ((1 + dy) + HY * (wv + 1 + dz));
+      for (int n = 0; n < NT; ++n) bv[n] = wts[(16 * n + l16) * WROW + c * 4 + g];
+      bf16x8 av[TY];
+      if (k0 < ktot) {
+        const int t = k0 / CI, cg = (k0 % CI) >> 3;
+        const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+        const int base = cg * NV + (l16 + 1 + dx) + HX * ((1 + dy) + HY * (wv + 1 + dz));
 #pragma unroll
-      for (int m = 0; m < TY; ++m) av[m] = tile[base + HX * m];
-    } else {
+        for (int m = 0; m < TY; ++m) av[m] = tile[base + HX * m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < TY; ++m)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) av[m][q] = (bf16)0.f;
+      }
 #pragma unroll
       for (int m = 0; m < TY; ++m)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) av[m][q] = (bf16)0.f;
+        for (int n = 0; n < NT; ++n)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
     }
-#pragma unroll
-    for (int m = 0; m < TY; ++m)
-#pragma unroll
-      for (int n = 0; n < NT; ++n)
-        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
   }
   const int z = z0 + wv;
 #pragma unroll
@@ -259,25 +288,56 @@ inline long long lds_bricks(int nx, int ny, int nz, int ty) {
   return (long long)(nx / 16) * (ny / ty) * (nz / 4);
 }
 
-template <typename TO, int NT, int CI>
-bool launch_conv_lds(const ConvArgs<TO> &a, hipStream_t s) {
+// Shape of the LDS-staged kernel for a convolution: rows per brick (4 while
+// that leaves >= 512 blocks, else 1 while >= 256) and output tiles per block
+// (the natural 16*NT channels, at most 2 tiles for LDS, or 1 tile when that
+// gives the coarser grids enough blocks); ty = 0 when it does not apply
+// (64 input channels: one-row bricks only; 128: never; the coarsest grids
+// take the split-K path).
+struct LdsPlan {
+  int ty = 0, nt = 0;
+};
+inline LdsPlan lds_plan(int nx, int ny, int nz, int ci, int co) {
+  LdsPlan p;
+  if (ci != 16 && ci != 32 && ci != 64) return p;
+  const int nat = std::min(2, co >= 64 ? 4 : (co >= 32 ? 2 : 1));
+  const long long b4 = ci <= 32 ? lds_bricks(nx, ny, nz, 4) : 0;
+  const long long b1 = lds_bricks(nx, ny, nz, 1);
+  if (b4 * (co / (16 * nat)) >= 512) {
+    p.ty = 4;
+    p.nt = nat;
+  } else if (b1 * (co / (16 * nat)) >= 256) {
+    p.ty = 1;
+    p.nt = nat;
+  } else if (b1 * (co / 16) >= 256) {
+    p.ty = 1;
+    p.nt = 1;
+  }
+  return p;
+}
+
+template <typename TO, int NT, int CI, bool UP>
+void launch_conv_lds(const ConvArgs<TO> &a, int ty, hipStream_t s) {
   const int gy = a.co / (16 * NT);
-  // Four rows per brick while that leaves >= 512 blocks; one row (four
-  // times the blocks) down to 256; coarser grids take the split-K path.
-  // (64 input channels: one-row bricks only, 41 KB of LDS.)
-  const long long b4 = lds_bricks(a.nx, a.ny, a.nz, 4) * gy;
-  const long long b1 = lds_bricks(a.nx, a.ny, a.nz, 1) * gy;
   if constexpr (CI <= 32) {
-    if (b4 >= 512) {
-      conv_lds_kernel<TO, NT, 4, CI><<<dim3((unsigned)(b4 / gy), gy), 256, 0, s>>>(a);
-      return true;
+    if (ty == 4) {
+      conv_lds_kernel<TO, NT, 4, CI, UP>
+          <<<dim3((unsigned)lds_bricks(a.nx, a.ny, a.nz, 4), gy), 256, 0, s>>>(a);
+      return;
     }
   }
-  if (b1 >= 256) {
-    conv_lds_kernel<TO, NT, 1, CI><<<dim3((unsigned)(b1 / gy), gy), 256, 0, s>>>(a);
-    return true;
-  }
-  return false;
+  conv_lds_kernel<TO, NT, 1, CI, UP>
+      <<<dim3((unsigned)lds_bricks(a.nx, a.ny, a.nz, 1), gy), 256, 0, s>>>(a);
+}
+
+template <typename TO, bool UP>
+void launch_conv_lds_any(const ConvArgs<TO> &a, LdsPlan p, hipStream_t s) {
+  if (a.ci == 16 && p.nt == 1) launch_conv_lds<TO, 1, 16, UP>(a, p.ty, s);
+  else if (a.ci == 16) launch_conv_lds<TO, 2, 16, UP>(a, p.ty, s);
+  else if (a.ci == 32 && p.nt == 1) launch_conv_lds<TO, 1, 32, UP>(a, p.ty, s);
+  else if (a.ci == 32) launch_conv_lds<TO, 2, 32, UP>(a, p.ty, s);
+  else if (p.nt == 1) launch_conv_lds<TO, 1, 64, UP>(a, p.ty, s);
+  else launch_conv_lds<TO, 2, 64, UP>(a, p.ty, s);
 }
 
 // Blocks a convolution launches without voxel-tile blocking or split-K.
@@ -322,12 +382,13 @@ template <typename TO>
 hipError_t launch_conv(const ConvArgs<TO> &a, float *ws, size_t ws_n, hipStream_t s) {
   const int V = a.nx * a.ny * a.nz;
   const int nt = a.co >= 64 ? 4 : (a.co >= 32 ? 2 : 1);
-  // Fine levels (16, 32 or 64 input channels): the LDS-staged kernel.
-  if (a.ci == 16 && nt == 1 && launch_conv_lds<TO, 1, 16>(a, s)) return hipGetLastError();
-  if (a.ci == 16 && nt == 2 && launch_conv_lds<TO, 2, 16>(a, s)) return hipGetLastError();
-  if (a.ci == 32 && nt == 1 && launch_conv_lds<TO, 1, 32>(a, s)) return hipGetLastError();
-  if (a.ci == 32 && nt == 2 && launch_conv_lds<TO, 2, 32>(a, s)) return hipGetLastError();
-  if (a.ci == 64 && nt == 2 && launch_conv_lds<TO, 2, 64>(a, s)) return hipGetLastError();
+  // Fine levels (16, 32 or 64 input channels, enough blocks): the LDS-staged
+  // kernel.
+  const LdsPlan lp = lds_plan(a.nx, a.ny, a.nz, a.ci, a.co);
+  if (lp.ty) {
+    launch_conv_lds_any<TO, false>(a, lp, s);
+    return hipGetLastError();
+  }
   if (nt == 4) return launch_conv_nt<TO, 4>(a, V, ws, ws_n, s);
   if (nt == 2) return launch_conv_nt<TO, 2>(a, V, ws, ws_n, s);
   return launch_conv_nt<TO, 1>(a, V, ws, ws_n, s);
@@ -621,6 +682,9 @@ struct Net {
   void *act_arena = nullptr;
   hipEvent_t ev[2];
   double last_ms = 0.0, last_flops = 0.0;
+  // u0 / u1 (the up-sampled d1 / d2) not formed by the last forward pass:
+  // its convolutions 12 / 10 read d1 / d2 up-sampled as they staged them.
+  bool up_stale[2] = {false, false};
   std::vector<void *> allocs;
 
   ~Net() {
@@ -793,6 +857,27 @@ int conv_fwd(Net &n, int li, const bf16 *x, const bf16 *skip, bf16 *y) {
   return 0;
 }
 
+// Convolution li of the decoder on xc up-sampled 2x (nearest), fused into
+// the LDS-staged kernel's staging; false (nothing launched) when that kernel
+// does not apply to the layer's grid.
+bool conv_fwd_up(Net &n, int li, const bf16 *xc, const bf16 *skip, bf16 *y, int *rc) {
+  const Conv &c = n.conv[li];
+  const int L = c.level;
+  const LdsPlan lp = lds_plan(n.nx[L], n.ny[L], n.nz[L], c.ci, c.co);
+  *rc = 0;
+  if (!lp.ty) return false;
+  ConvArgs<bf16> a;
+  a.x = xc; a.w = c.wf; a.bias = n.W + c.boff; a.skip = skip; a.y = y;
+  a.nx = n.nx[L]; a.ny = n.ny[L]; a.nz = n.nz[L];
+  a.lci = ilog2(c.ci); a.ci = c.ci; a.co = c.co; a.kp = c.kpf;
+  a.act = 1; a.accumulate = 0;
+  launch_conv_lds_any<bf16, true>(a, lp, st());
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) *rc = o3d::set_error(hipGetErrorString(e));
+  n.last_flops += 54.0 * c.ci * c.co * (double)n.V[L];
+  return true;
+}
+
 // Backward of y = act(conv(x) + b + skip): gy (FP32) -> dz; gskip += dz;
 // db, dW += ...; gx += conv^T(dz) when gx is given.
 int conv_bwd(Net &n, int li, const float *gy, const bf16 *y, const bf16 *x, float *gskip,
@@ -851,13 +936,22 @@ int forward(Net &n, const float *in, float *out) {
                                                               n.nz[2], 128);
   NCHK(conv_fwd(n, 8, n.u2, n.e2, n.d2a));
   NCHK(conv_fwd(n, 9, n.d2a, nullptr, n.d2));
-  up_fwd_kernel<<<elem_blocks(n.V[1] * 64), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
-                                                             n.nz[1], 64);
-  NCHK(conv_fwd(n, 10, n.u1, n.e1, n.d1a));
+  int rc = 0;
+  n.up_stale[1] = conv_fwd_up(n, 10, n.d2, n.e1, n.d1a, &rc);
+  NCHK(rc);
+  if (!n.up_stale[1]) {
+    up_fwd_kernel<<<elem_blocks(n.V[1] * 64), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
+                                                               n.nz[1], 64);
+    NCHK(conv_fwd(n, 10, n.u1, n.e1, n.d1a));
+  }
   NCHK(conv_fwd(n, 11, n.d1a, nullptr, n.d1));
-  up_fwd_kernel<<<elem_blocks(n.V[0] * 32), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
-                                                             n.nz[0], 32);
-  NCHK(conv_fwd(n, 12, n.u0, n.e0, n.d0a));
+  n.up_stale[0] = conv_fwd_up(n, 12, n.d1, n.e0, n.d0a, &rc);
+  NCHK(rc);
+  if (!n.up_stale[0]) {
+    up_fwd_kernel<<<elem_blocks(n.V[0] * 32), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
+                                                               n.nz[0], 32);
+    NCHK(conv_fwd(n, 12, n.u0, n.e0, n.d0a));
+  }
   NCHK(conv_fwd(n, 13, n.d0a, nullptr, n.d0));
   head_fwd_kernel<<<elem_blocks(n.V[0]), 256, 0, st()>>>(n.d0, n.W + n.head_off, in + n.V[0],
                                                           out, (int)n.V[0], 16);
@@ -872,10 +966,22 @@ int backward(Net &n, const float *dout) {
       dout, n.d0, n.W + n.head_off, n.g_d0, n.G + n.head_off, (int)n.V[0], 16);
   NNCHK(hipGetLastError());
   NCHK(conv_bwd(n, 13, n.g_d0, n.d0, n.d0a, nullptr, n.g_d0a));
+  if (n.up_stale[0]) {  // the forward pass read d1 up-sampled without forming u0
+    up_fwd_kernel<<<elem_blocks(n.V[0] * 32), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
+                                                               n.nz[0], 32);
+    NNCHK(hipGetLastError());
+    n.up_stale[0] = false;
+  }
   NCHK(conv_bwd(n, 12, n.g_d0a, n.d0a, n.u0, n.g_e0, n.g_u0));
   up_bwd_kernel<<<elem_blocks(n.V[1] * 32), 256, 0, st()>>>(n.g_u0, n.g_d1, n.nx[0], n.ny[0],
                                                              n.nz[0], 32);
   NCHK(conv_bwd(n, 11, n.g_d1, n.d1, n.d1a, nullptr, n.g_d1a));
+  if (n.up_stale[1]) {
+    up_fwd_kernel<<<elem_blocks(n.V[1] * 64), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
+                                                               n.nz[1], 64);
+    NNCHK(hipGetLastError());
+    n.up_stale[1] = false;
+  }
   NCHK(conv_bwd(n, 10, n.g_d1a, n.d1a, n.u1, n.g_e1, n.g_u1));
   up_bwd_kernel<<<elem_blocks(n.V[2] * 64), 256, 0, st()>>>(n.g_u1, n.g_d2, n.nx[1], n.ny[1],
                                                              n.nz[1], 64);
