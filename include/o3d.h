@@ -296,6 +296,9 @@ typedef struct o3d_params {
                                  does (its solves then end on the stagnation rule).  The FP64
                                  form always keeps the reference's rhs; x-Dirichlet grids
                                  always project */
+  int refine_cycles;          /* with the coarse-grid correction: V-cycles of the correction
+                                 that opens each later refinement cycle of a step (0 =
+                                 coarse_cycles) */
 } o3d_params;
 
 typedef struct o3d_step_info {

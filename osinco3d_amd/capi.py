@@ -48,6 +48,7 @@ class O3DParams(C.Structure):
         ("coarse_cycles_learned", C.c_int),
         ("adaptive_extrapolation", C.c_int),
         ("rhs_as_given", C.c_int),
+        ("refine_cycles", C.c_int),
     ]
 
 

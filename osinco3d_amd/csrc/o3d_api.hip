@@ -1916,7 +1916,8 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
         // The start of this correction solve: the coarse-grid correction of
         // its residual (cycle 0 starts from a guess that already had one).
         HIPCHK(mg_pool_split(S.R, S.mg.f[0], p.nx, p.ny, p.nz, g_stream));
-        HIPCHK(mg_solve(S.mg, p.coarse_cycles, 1, 1, g_stream));
+        HIPCHK(mg_solve(S.mg, p.refine_cycles > 0 ? p.refine_cycles : p.coarse_cycles, 1, 1,
+                        g_stream));
         HIPCHK(mg_prolong_split(S.E[0], S.mg.u[0], p.nx, p.ny, p.nz, per, g_stream));
       }
     }
@@ -1958,13 +1959,23 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     sw32 += sweeps;
     if (cycles == 0) S.last_first = sweeps;
     ++cycles;
-    if (split) HIPCHK(launch_fold_split(S.P[S.cur], S.E[0], p.nx, p.ny, p.nz, g_stream));
-    else HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
     // A solve that ended on its sweep cap continues from a refreshed
     // residual; every fourth such solve in a row is followed by the judge
     // sweep anyway, so a problem that cannot converge (an rhs incompatible
     // with the free-slip mirror) still reaches the stagnation rule.
-    if (rin == 2 && total < p.kmax && ++capped < 4) continue;
+    const bool to_judge = !(rin == 2 && total < p.kmax) || capped + 1 >= 4;
+    // p += e: with colour-split sweeps and a judge sweep next, folded into
+    // the judge's loads (it reads p + e and writes the other buffer) instead
+    // of a pass of its own; the sums are the same, so are the iterates.
+    const bool fuse = split && to_judge;
+    if (!fuse) {
+      if (split) HIPCHK(launch_fold_split(S.P[S.cur], S.E[0], p.nx, p.ny, p.nz, g_stream));
+      else HIPCHK(launch_fold(S.P[S.cur], r.result(sweeps), n, g_stream));
+    }
+    if (!to_judge) {
+      ++capped;
+      continue;
+    }
     capped = 0;
     // FP64 judge sweep (applied, like the reference's last sweep).
     SorRun<double, float> j;
@@ -1974,6 +1985,7 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     j.buf[0] = S.P[S.cur];
     j.buf[1] = S.P[1 - S.cur];
     sor_args_common<double, float>(S, j.a, (const float *)S.rhs, S.st64, 1);
+    if (fuse) j.a.efold = S.E[0];
     int jsw = 0;
     CHK(j.run(S.omega, 2, &jsw));
     if (jsw & 1) S.cur = 1 - S.cur;
@@ -2638,8 +2650,10 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
     return fail("unknown warm start %d", p->warm_start);
   const int kind = solver_from_bc(p->bc);
   if (kind < 0) return fail("no Poisson solver is bound for x free-slip with y periodic");
-  if (p->coarse_cycles < 0 || p->smooth_sweeps < 0 || p->coarse_cycles_learned < 0)
-    return fail("coarse_cycles, coarse_cycles_learned and smooth_sweeps must be >= 0");
+  if (p->coarse_cycles < 0 || p->smooth_sweeps < 0 || p->coarse_cycles_learned < 0 ||
+      p->refine_cycles < 0)
+    return fail("coarse_cycles, coarse_cycles_learned, refine_cycles and smooth_sweeps must be "
+                ">= 0");
   if ((p->coarse_cycles > 0 || p->smooth_sweeps > 0) && p->precision != O3D_MIXED)
     return fail("coarse_cycles and smooth_sweeps apply to the mixed form only");
   if (p->coarse_cycles > 0 && (p->nx % 2 || p->ny % 2 || p->nz % 2))

@@ -70,6 +70,10 @@ struct SorArgs {
   int it;        // iteration number of this launch (added to st->base)
   int gs;        // iterations 1..gs use omega = 1 (Gauss-Seidel smoothing) and
                  // end the solve only on kmax
+  // Optional (fused kernel, T = double): a float32 correction in colour-split
+  // storage added to pin as it is loaded (pin + e is the iterate the sweep
+  // starts from; pin itself is not modified), nx even.
+  const float *efold = nullptr;
 };
 
 template <typename T, typename TR>

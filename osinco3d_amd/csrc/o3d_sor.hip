@@ -310,7 +310,29 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   const __amdgpu_buffer_rsrc_t rrh = buf_rsrc(a.rhs, plane * nz);
   const unsigned o2a = (unsigned)col2a * sizeof(T), o2b = (unsigned)col2b * sizeof(T);
   const unsigned o1a = (unsigned)col1a * sizeof(TR), o1b = (unsigned)col1b * sizeof(TR);
-  auto ld_old = [&](unsigned o, unsigned zoff) { return bload<T>(rin, o, zoff * sizeof(T)); };
+  // With a.efold, old values are pin + e, e in colour-split storage: node
+  // (x, y, z) at entry (nx/2)(y + ny z) + x/2 of the red (x + y + z even) or
+  // black half.  Per R2 column: its entry in a plane and its (x + y) parity;
+  // the plane's parity picks the half.
+  const bool fold = a.efold != nullptr;
+  const __amdgpu_buffer_rsrc_t rfe = buf_rsrc(fold ? a.efold : (const float *)a.pin, plane * nz);
+  const int hxn = nx >> 1;
+  auto ecol = [&](int c) {
+    const int gx = sor_ghost(x0 - 2 + c % R2W, nx, a.per[0]);
+    const int gy = sor_ghost(y0 - 2 + c / R2W, ny, a.per[1]);
+    return 2u * (unsigned)(hxn * gy + (gx >> 1)) + (unsigned)((gx + gy) & 1);
+  };
+  const unsigned eca = ecol(tid), ecb = v2b ? ecol(c2b) : eca;
+  const unsigned halfB = (unsigned)(plane * nz / 2) * 4u;
+  auto ld_old = [&](unsigned o, unsigned ec, unsigned zoff) {
+    T v = bload<T>(rin, o, zoff * sizeof(T));
+    if (fold) {
+      const unsigned gz = zoff / (unsigned)plane;
+      const unsigned eo = (ec >> 1) * 4u + (((ec ^ gz) & 1u) ? halfB : 0u);
+      v = v + (T)bload<float>(rfe, eo, (unsigned)hxn * (unsigned)ny * gz * 4u);
+    }
+    return v;
+  };
   auto ld_rhs = [&](unsigned o, unsigned zoff) { return bload<TR>(rrh, o, zoff * sizeof(TR)); };
   
   // Prefetch registers: step k of the march publishes old planes
@@ -319,11 +341,11 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   TR pr[PD][PPS][2];
   {
     const unsigned g2 = zq_lo(z0 - 2), g1 = zq_lo(z0 - 1);
-    oldr[slot<NO>(-2)][tid] = ld_old(o2a, g2);
-    oldr[slot<NO>(-1)][tid] = ld_old(o2a, g1);
+    oldr[slot<NO>(-2)][tid] = ld_old(o2a, eca, g2);
+    oldr[slot<NO>(-1)][tid] = ld_old(o2a, eca, g1);
     if (v2b) {
-      oldr[slot<NO>(-2)][c2b] = ld_old(o2b, g2);
-      oldr[slot<NO>(-1)][c2b] = ld_old(o2b, g1);
+      oldr[slot<NO>(-2)][c2b] = ld_old(o2b, ecb, g2);
+      oldr[slot<NO>(-1)][c2b] = ld_old(o2b, ecb, g1);
     }
 #pragma unroll
     for (int k = 0; k < PD; ++k)
@@ -331,8 +353,8 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
       for (int q = 0; q < PPS; ++q) {
         const int sk = z0 - 1 + k * PPS + q;
         const unsigned go = zq_lo(sk + 1), gr = zq_lo(sk);
-        po[k][q][0] = ld_old(o2a, go);
-        po[k][q][1] = ld_old(o2b, go);
+        po[k][q][0] = ld_old(o2a, eca, go);
+        po[k][q][1] = ld_old(o2b, ecb, go);
         pr[k][q][0] = ld_rhs(o1a, gr);
         pr[k][q][1] = ld_rhs(o1b, gr);
       }
@@ -361,8 +383,8 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
       if (PAD || v2b) od[c2b] = po[k][q][1];
       if (PAD || v1b) rd[c2b] = pr[k][q][1];
       const unsigned go = zq(s + 1 + q + PD * PPS), gr = zq(s + q + PD * PPS);
-      po[k][q][0] = ld_old(o2a, go);
-      po[k][q][1] = ld_old(o2b, go);
+      po[k][q][0] = ld_old(o2a, eca, go);
+      po[k][q][1] = ld_old(o2b, ecb, go);
       pr[k][q][0] = ld_rhs(o1a, gr);
       pr[k][q][1] = ld_rhs(o1b, gr);
     }

@@ -129,7 +129,8 @@ def jet_inflow(cfg: Config, amplitude: float = 0.05, seed: int = 1234):
 def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
                 inner_max: int = 0, coarse_cycles: int = 0,
                 smooth_sweeps: int = 0, coarse_cycles_learned: int = 0,
-                adaptive: int = 0, rhs_as_given: int = 0) -> capi.O3DParams:
+                adaptive: int = 0, rhs_as_given: int = 0,
+                refine_cycles: int = 0) -> capi.O3DParams:
     p = capi.O3DParams()
     p.nx, p.ny, p.nz = cfg.nx, cfg.ny, cfg.nz
     p.dx, p.dy, p.dz = cfg.spacing()
@@ -158,6 +159,7 @@ def make_params(cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
     p.coarse_cycles_learned = coarse_cycles_learned
     p.adaptive_extrapolation = adaptive
     p.rhs_as_given = rhs_as_given
+    p.refine_cycles = refine_cycles
     return p
 
 
@@ -166,7 +168,8 @@ class Solver:
 
     def __init__(self, cfg: Config, precision: int = FP64, warm: int = WARM_PREVIOUS,
                  inner_max: int = 0, coarse_cycles: int = 0, smooth_sweeps: int = 0,
-                 coarse_cycles_learned: int = 0, adaptive: int = 0, rhs_as_given: int = 0):
+                 coarse_cycles_learned: int = 0, adaptive: int = 0, rhs_as_given: int = 0,
+                 refine_cycles: int = 0):
         """rhs_as_given: in the mixed form on a grid with a free-slip axis,
         solve the divergence's rhs unprojected, as the FP64 form does (0:
         remove its weighted mean, so every solve can end on eps)."""
@@ -174,7 +177,8 @@ class Solver:
         self.precision = precision
         self.lib = capi.load()
         self.params = make_params(cfg, precision, warm, inner_max, coarse_cycles,
-                                  smooth_sweeps, coarse_cycles_learned, adaptive, rhs_as_given)
+                                  smooth_sweeps, coarse_cycles_learned, adaptive, rhs_as_given,
+                                  refine_cycles)
         self.handle = C.c_void_p(0)
         capi.check(self.lib.o3d_solver_create(C.byref(self.params), C.byref(self.handle)))
         self.dt = self.params.dt
