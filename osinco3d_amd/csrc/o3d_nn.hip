@@ -182,16 +182,24 @@ __global__ void __launch_bounds__(256) conv_reduce_kernel(const ConvArgs<TO> a, 
 //    read once from L2 instead of once per tap.  With UP the input is the
 //    grid of half the size, upsampled 2x (nearest) as it is staged, which is
 //    the decoder's up-sampling fused into the convolution that consumes it.
-//  * Weights: slices of SK K-chunks of the block's 16*NT rows, loaded
-//    cooperatively between two barriers; rows are padded by 16 bytes so the
-//    16 lanes reading one B fragment (16 rows, same k) hit distinct banks.
-//    The K loop then waits on LDS only, not on a global load per chunk.
+//  * Weights: slices of SK K-chunks of the block's 16*NT rows (as many as
+//    fit in 48 KB of LDS next to the input tile, at least 8: one slice for
+//    16 input channels), loaded cooperatively; the first slice's loads are
+//    issued with the input tile's, before one barrier.  Rows are padded by
+//    16 bytes so the 16 lanes reading one B fragment (16 rows, same k) hit
+//    distinct banks.  The K loop then waits on LDS only, not on a global
+//    load per chunk.
 // Needs nx % 16 == 0, ny % TY == 0, nz % 4 == 0 and ci == CI (UP: even
 // dimensions, x of (nx/2, ny/2, nz/2) voxels).
 template <typename TO, int NT, int TY, int CI, bool UP>
 __global__ void __launch_bounds__(256) conv_lds_kernel(const ConvArgs<TO> a) {
   constexpr int HX = 18, HY = TY + 2, HZ = 6, NV = HX * HY * HZ, NG = CI / 8;
-  constexpr int SK = 8;              // K chunks per weight slice
+  // K chunks per weight slice: as many as fit in 48 KB of LDS with the
+  // input tile (three blocks per CU), at least 8, at most all of them.
+  constexpr int KCH = (27 * CI + 31) / 32;
+  constexpr int SKF = ((48 * 1024 - NG * NV * 16) / (NT * 16 * 16) - 1) / 4;
+  constexpr int SK = (SKF > 8 ? SKF : 8) < KCH ? (SKF > 8 ? SKF : 8) : KCH;
+  static_assert(SK >= 1, "no room for a weight slice");
   constexpr int WROW = SK * 4 + 1;   // 16-byte entries per padded weight row
   __shared__ bf16x8 tile[NG * NV];
   __shared__ bf16x8 wts[NT * 16 * WROW];
@@ -222,7 +230,7 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(const ConvArgs<TO> a) {
   constexpr int ktot = 27 * CI;
   const int nchunk = a.kp >> 5;
   for (int s0 = 0; s0 < nchunk; s0 += SK) {
-    __syncthreads();  // every wave is done with the previous slice
+    if (s0 > 0) __syncthreads();  // every wave is done with the previous slice
     for (int v = threadIdx.x; v < NT * 16 * SK * 4; v += 256) {
       const int row = v / (SK * 4), c8 = v - row * (SK * 4);
       const int k = s0 * 32 + c8 * 8;
