@@ -344,6 +344,11 @@ int o3d_solver_statistics(void *handle, double t, double *out17);
  * 8 nu_t.  Element type is float64 for O3D_FP64 and float32 otherwise
  * (p is float64 in both forms). */
 void *o3d_solver_field(void *handle, int which);
+/* Copy `bytes` between host and/or device memory (either side may be a
+ * device pointer such as o3d_solver_field's, or host memory), on the launch
+ * stream, and wait for it.  No counterpart in the reference: how a C, Fortran
+ * or Python caller reads a solver field without a HIP binding of its own. */
+int o3d_copy(void *dst, const void *src, size_t bytes);
 /* Time of SOR sweeps measured with HIP events on the launch stream over the
  * last step: total ms of complete sweep batches, the number of sweeps
  * (iterations) and of kernel launches in them, per precision (index 0 FP64,

@@ -92,7 +92,7 @@ EXPORTS = [
     "o3d_solver_sor_timing", "o3d_solver_sor_period", "o3d_solver_set_warm_model",
     "o3d_solver_capture",
     "o3d_warm_features", "o3d_solver_history", "o3d_multigrid", "o3d_poisson_multigrid",
-    "o3d_solver_set_inflow",
+    "o3d_solver_set_inflow", "o3d_copy",
 ]
 NN_EXPORTS = ["o3d_nn_create", "o3d_nn_destroy", "o3d_nn_resize", "o3d_nn_num_params",
               "o3d_nn_get_params", "o3d_nn_set_params", "o3d_nn_get_grads", "o3d_nn_zero_grad",

@@ -2848,6 +2848,15 @@ extern "C" int o3d_solver_statistics(void *handle, double t, double *out17) {
                                 S.p.bc, out17, s);
 }
 
+extern "C" int o3d_copy(void *dst, const void *src, size_t bytes) {
+  CHK(ensure_init());
+  if (bytes == 0) return 0;
+  if (!dst || !src) return fail("o3d_copy: NULL pointer");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, g_stream));
+  HIPCHK(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
 extern "C" void *o3d_solver_field(void *handle, int which) {
   if (!handle) return nullptr;
   Solver &S = *(Solver *)handle;

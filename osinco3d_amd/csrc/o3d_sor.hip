@@ -298,14 +298,14 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   const int b2i = (ty + 2) * R2W + tx + 2;
   const int bpar = (bi + bj) & 1;
 
-  // Byte offset of plane zp >= 0 (the march never goes below z0 - 2; only
+  // Stored plane of plane zp >= 0 (the march never goes below z0 - 2; only
   // the upper ghost rule can apply inside the loop), for old and rhs.
   const int perz = a.per[2];
   auto zq = [&](int zp) {
     const int hi = zp > nz + 1 ? nz - 1 : (perz ? zp - nz : 2 * (nz - 1) - zp);
-    return (unsigned)(plane * (zp < nz ? zp : hi));
+    return (unsigned)(zp < nz ? zp : hi);
   };
-  auto zq_lo = [&](int zp) { return (unsigned)(plane * sor_ghost(zp, nz, perz)); };
+  auto zq_lo = [&](int zp) { return (unsigned)sor_ghost(zp, nz, perz); };
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(a.pin, plane * nz);
   const __amdgpu_buffer_rsrc_t rrh = buf_rsrc(a.rhs, plane * nz);
   const unsigned o2a = (unsigned)col2a * sizeof(T), o2b = (unsigned)col2b * sizeof(T);
@@ -324,16 +324,17 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   };
   const unsigned eca = ecol(tid), ecb = v2b ? ecol(c2b) : eca;
   const unsigned halfB = (unsigned)(plane * nz / 2) * 4u;
-  auto ld_old = [&](unsigned o, unsigned ec, unsigned zoff) {
-    T v = bload<T>(rin, o, zoff * sizeof(T));
+  auto ld_old = [&](unsigned o, unsigned ec, unsigned gz) {
+    T v = bload<T>(rin, o, (unsigned)plane * gz * sizeof(T));
     if (fold) {
-      const unsigned gz = zoff / (unsigned)plane;
       const unsigned eo = (ec >> 1) * 4u + (((ec ^ gz) & 1u) ? halfB : 0u);
       v = v + (T)bload<float>(rfe, eo, (unsigned)hxn * (unsigned)ny * gz * 4u);
     }
     return v;
   };
-  auto ld_rhs = [&](unsigned o, unsigned zoff) { return bload<TR>(rrh, o, zoff * sizeof(TR)); };
+  auto ld_rhs = [&](unsigned o, unsigned gz) {
+    return bload<TR>(rrh, o, (unsigned)plane * gz * sizeof(TR));
+  };
   
   // Prefetch registers: step k of the march publishes old planes
   // s_k+1 .. s_k+PPS and rhs planes s_k .. s_k+PPS-1, s_k = z0-1 + k*PPS.
@@ -733,10 +734,11 @@ hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s) 
 }
 
 // ------------------------------------------------------ mixed precision
-// The residual kernels march z along one (i, j) column per thread (RZC
-// planes; the p values of planes k-1, k, k+1 stay in registers) over 64 x 4
-// (i, j) blocks, so a launch has many blocks and no index divisions.  The
-// residual keeps the operation order of the red-black sweep's Laplacian.
+// The residual kernels march z over RZC planes of a 64 x 4 (i, j) tile per
+// block (tile_march: the plane's tile and halo in LDS, the column's planes
+// k-1, k, k+1 in registers), so a launch has many blocks and no index
+// divisions.  The residual keeps the operation order of the red-black
+// sweep's Laplacian.
 constexpr int RBX = 64, RBY = 4, RZC = 8, RNT = RBX * RBY;
 
 // The tile of this block: the linear block index remapped XCD-aware (tiles
@@ -766,48 +768,122 @@ int residual_blocks(int nx, int ny, int nz) {
   return (int)(g.x * g.y * g.z);
 }
 
-// f(offset of (i, j, k), i, j, k, p(i, j, k), residual) for the planes of
-// this thread's column; the iterate's value at offset o is ld(o).  Every
-// load of the column is issued before the first f (planes past nz are
-// loaded clamped and dropped), so no store of f sits between loads.
+// LDS-tiled z march of the residual kernels over the block's RBX x RBY tile
+// (XCD-aware order, RZC planes).  Per plane, the NF node values of the tile
+// and of its one-node x and y halo (no corners) are staged in LDS, double
+// buffered (one barrier per plane); each thread keeps its own column's
+// planes k-1, k, k+1 (and k+2 in flight) and the rhs in registers.  A node
+// value is then loaded once per plane, plus the halo, instead of once per
+// neighbour reading it.  ld(o, v) fills v[0..NF) for node offset o;
+// f(o, i, j, k, nb, rh) receives, per field, the neighbour values
+// nb[f][0..6] = (x-1, x+1, y-1, y+1, z-1, z+1, centre) and the rhs, in the
+// operation order every residual kernel uses.  Every thread of the block
+// runs the march (it holds barriers); threads outside the grid load nothing
+// and call f for nothing.
+template <int NF, typename TR, typename LD, typename F>
+__device__ __forceinline__ void tile_march(LD &&ld, const TR *__restrict__ rhs, int nx, int ny,
+                                           int nz, int px, int py, int pz, F &&f) {
+  __shared__ double tl[2][NF][RBY + 2][RBX + 2];
+  const RTile T = residual_tile();
+  const int tx = threadIdx.x % RBX, ty = threadIdx.x / RBX;
+  const int i = T.bx * RBX + tx, j = T.by * RBY + ty;
+  const bool act = i < nx && j < ny;
+  const int k0 = T.bz * RZC;
+  const int kend = min(k0 + RZC, nz);
+  const size_t plane = (size_t)nx * ny;
+  // Halo duties: the first and last valid column and row of the tile.
+  const int ie = min(T.bx * RBX + RBX, nx) - 1, je = min(T.by * RBY + RBY, ny) - 1;
+  const bool hxl = act && tx == 0, hxr = act && i == ie;
+  const bool hyl = act && ty == 0, hyr = act && j == je;
+  const int c = act ? i + nx * j : 0;
+  const int cxl = act ? sor_ghost(i - 1, nx, px) + nx * j : 0;
+  const int cxr = act ? sor_ghost(i + 1, nx, px) + nx * j : 0;
+  const int cyl = act ? i + nx * sor_ghost(j - 1, ny, py) : 0;
+  const int cyr = act ? i + nx * sor_ghost(j + 1, ny, py) : 0;
+  auto zo = [&](int k) { return plane * (size_t)sor_ghost(k, nz, pz); };
+  double cm[NF], c0[NF], cp[NF], cn[NF], h[4][NF];
+  double rh = 0.0, rn = 0.0;
+  __syncthreads();  // a previous march of this block may still read the tile
+  if (act) {
+    ld(c + zo(k0 - 1), cm);
+    ld(c + zo(k0), c0);
+    ld(c + zo(k0 + 1), cp);
+    rh = (double)rhs[c + plane * k0];
+    if (hxl) ld(cxl + zo(k0), h[0]);
+    if (hxr) ld(cxr + zo(k0), h[1]);
+    if (hyl) ld(cyl + zo(k0), h[2]);
+    if (hyr) ld(cyr + zo(k0), h[3]);
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      tl[0][q][ty + 1][tx + 1] = c0[q];
+      if (hxl) tl[0][q][ty + 1][tx] = h[0][q];
+      if (hxr) tl[0][q][ty + 1][tx + 2] = h[1][q];
+      if (hyl) tl[0][q][ty][tx + 1] = h[2][q];
+      if (hyr) tl[0][q][ty + 2][tx + 1] = h[3][q];
+    }
+  }
+  __syncthreads();
+  for (int k = k0; k < kend; ++k) {
+    const int b = (k - k0) & 1;
+    const bool more = k + 1 < kend;
+    if (act) {
+      // Loads for later planes first: the column's plane k+2, the rhs and
+      // the halo of plane k+1.
+      ld(c + zo(k + 2), cn);
+      if (more) {
+        rn = (double)rhs[c + plane * (k + 1)];
+        if (hxl) ld(cxl + zo(k + 1), h[0]);
+        if (hxr) ld(cxr + zo(k + 1), h[1]);
+        if (hyl) ld(cyl + zo(k + 1), h[2]);
+        if (hyr) ld(cyr + zo(k + 1), h[3]);
+      }
+      double nb[NF][7];
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        nb[q][0] = tl[b][q][ty + 1][tx];
+        nb[q][1] = tl[b][q][ty + 1][tx + 2];
+        nb[q][2] = tl[b][q][ty][tx + 1];
+        nb[q][3] = tl[b][q][ty + 2][tx + 1];
+        nb[q][4] = cm[q];
+        nb[q][5] = cp[q];
+        nb[q][6] = c0[q];
+      }
+      f(c + plane * k, i, j, k, nb, rh);
+      if (more) {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          tl[1 - b][q][ty + 1][tx + 1] = cp[q];
+          if (hxl) tl[1 - b][q][ty + 1][tx] = h[0][q];
+          if (hxr) tl[1 - b][q][ty + 1][tx + 2] = h[1][q];
+          if (hyl) tl[1 - b][q][ty][tx + 1] = h[2][q];
+          if (hyr) tl[1 - b][q][ty + 2][tx + 1] = h[3][q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        cm[q] = c0[q];
+        c0[q] = cp[q];
+        cp[q] = cn[q];
+      }
+      rh = rn;
+    }
+    if (more) __syncthreads();
+  }
+}
+
+// f(offset of (i, j, k), i, j, k, p(i, j, k), residual) for every node of
+// the block's tile, the iterate's value at offset o being ld(o).
 template <typename TR, typename LD, typename F>
 __device__ __forceinline__ void residual_column_ld(LD &&ld, const TR *__restrict__ rhs, int nx,
                                                    int ny, int nz, int px, int py, int pz,
                                                    double ox, double oy, double oz, double A,
                                                    F &&f) {
-  const RTile T = residual_tile();
-  const int i = T.bx * RBX + (threadIdx.x % RBX);
-  const int j = T.by * RBY + (threadIdx.x / RBX);
-  if (i >= nx || j >= ny) return;
-  const int k0 = T.bz * RZC;
-  const size_t plane = (size_t)nx * ny;
-  const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
-  const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
-  const int c = i + nx * j;
-  const int cim = im + nx * j, cip = ip + nx * j, cjm = i + nx * jm, cjp = i + nx * jp;
-  double zc[RZC + 2], xm[RZC], xp[RZC], ym[RZC], yp[RZC], rh[RZC];
-#pragma unroll
-  for (int kk = 0; kk < RZC + 2; ++kk)
-    zc[kk] = ld(c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz));
-#pragma unroll
-  for (int kk = 0; kk < RZC; ++kk) {
-    const size_t o = plane * min(k0 + kk, nz - 1);
-    xm[kk] = ld(cim + o);
-    xp[kk] = ld(cip + o);
-    ym[kk] = ld(cjm + o);
-    yp[kk] = ld(cjp + o);
-    rh[kk] = (double)rhs[c + o];
-  }
-#pragma unroll
-  for (int kk = 0; kk < RZC; ++kk) {
-    const int k = k0 + kk;
-    if (k < nz) {
-      const size_t o = plane * k;
-      const double lap = ox * (xm[kk] + xp[kk]) + oy * (ym[kk] + yp[kk]) +
-                         oz * (zc[kk] + zc[kk + 2]) + A * zc[kk + 1];
-      f(c + o, i, j, k, zc[kk + 1], rh[kk] - lap);
-    }
-  }
+  tile_march<1>([&](size_t o, double *v) { v[0] = ld(o); }, rhs, nx, ny, nz, px, py, pz,
+                [&](size_t o, int i, int j, int k, const double (*nb)[7], double rh) {
+                  const double lap = ox * (nb[0][0] + nb[0][1]) + oy * (nb[0][2] + nb[0][3]) +
+                                     oz * (nb[0][4] + nb[0][5]) + A * nb[0][6];
+                  f(o, i, j, k, nb[0][6], rh - lap);
+                });
 }
 
 template <typename TR, typename F>
@@ -838,61 +914,33 @@ __global__ void __launch_bounds__(RNT) guess_residual_kernel(const double *__res
                                                              double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0;
-  const RTile T = residual_tile();
-  const int i = T.bx * RBX + (threadIdx.x % RBX);
-  const int j = T.by * RBY + (threadIdx.x / RBX);
-  if (i < nx && j < ny) {
-    const int k0 = T.bz * RZC;
-    const size_t plane = (size_t)nx * ny;
-    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
-    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
-    const int c = i + nx * j;
-    const int cl[4] = {im + nx * j, ip + nx * j, i + nx * jm, i + nx * jp};
-    double zp[RZC + 2], lp[4][RZC], rh[RZC];
-    float ze[RZC + 2], le[4][RZC];
-#pragma unroll
-    for (int kk = 0; kk < RZC + 2; ++kk) {
-      const size_t o = c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz);
-      zp[kk] = p[o];
-      ze[kk] = e[o];
-    }
-#pragma unroll
-    for (int kk = 0; kk < RZC; ++kk) {
-      const size_t o = plane * min(k0 + kk, nz - 1);
-#pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        lp[d][kk] = p[cl[d] + o];
-        le[d][kk] = e[cl[d] + o];
-      }
-      rh[kk] = (double)rhs[c + o];
-    }
-    auto qv = [](double a, float b) { return a + 1.0 * (double)b; };
-#pragma unroll
-    for (int kk = 0; kk < RZC; ++kk) {
-      const int k = k0 + kk;
-      if (k < nz) {
-        const size_t o = c + plane * k;
-        const double lapq = ox * (qv(lp[0][kk], le[0][kk]) + qv(lp[1][kk], le[1][kk])) +
-                            oy * (qv(lp[2][kk], le[2][kk]) + qv(lp[3][kk], le[3][kk])) +
-                            oz * (qv(zp[kk], ze[kk]) + qv(zp[kk + 2], ze[kk + 2])) +
-                            A * qv(zp[kk + 1], ze[kk + 1]);
-        q[o] = qv(zp[kk + 1], ze[kk + 1]);
-        r[o] = (float)(rh[kk] - lapq);
+  auto qv = [](double a, double b) { return a + 1.0 * b; };
+  tile_march<2>(
+      [&](size_t o, double *v) {
+        v[0] = p[o];
+        v[1] = (double)e[o];
+      },
+      rhs, nx, ny, nz, px, py, pz,
+      [&](size_t o, int, int, int, const double (*nb)[7], double rh) {
+        const double lapq = ox * (qv(nb[0][0], nb[1][0]) + qv(nb[0][1], nb[1][1])) +
+                            oy * (qv(nb[0][2], nb[1][2]) + qv(nb[0][3], nb[1][3])) +
+                            oz * (qv(nb[0][4], nb[1][4]) + qv(nb[0][5], nb[1][5])) +
+                            A * qv(nb[0][6], nb[1][6]);
+        q[o] = qv(nb[0][6], nb[1][6]);
+        r[o] = (float)(rh - lapq);
         if (part0) {
-          const double lapp = ox * (lp[0][kk] + lp[1][kk]) + oy * (lp[2][kk] + lp[3][kk]) +
-                              oz * (zp[kk] + zp[kk + 2]) + A * zp[kk + 1];
-          const double r0 = rh[kk] - lapp;
+          const double lapp = ox * (nb[0][0] + nb[0][1]) + oy * (nb[0][2] + nb[0][3]) +
+                              oz * (nb[0][4] + nb[0][5]) + A * nb[0][6];
+          const double r0 = rh - lapp;
           m0 = fmax(m0, fabs(r0));
           s0 += r0 * r0;
         }
-      }
-    }
-  }
+      });
   if (part0) {
     m0 = block_max<RNT / O3D_WAVE>(m0, sc);
     s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
     if (threadIdx.x == 0) {
-      const size_t b = (size_t)T.b;
+      const size_t b = (size_t)residual_tile().b;
       part0[2 * b] = m0;
       part0[2 * b + 1] = s0;
     }
@@ -924,70 +972,36 @@ __global__ void __launch_bounds__(RNT) guess_adapt_kernel(
     int py, int pz, double ox, double oy, double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0, sb = 0.0, sd = 0.0;
-  const RTile T = residual_tile();
-  const int i = T.bx * RBX + (threadIdx.x % RBX);
-  const int j = T.by * RBY + (threadIdx.x / RBX);
-  if (i < nx && j < ny) {
-    const int k0 = T.bz * RZC;
-    const size_t plane = (size_t)nx * ny;
-    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
-    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
-    const int c = i + nx * j;
-    const int cl[4] = {im + nx * j, ip + nx * j, i + nx * jm, i + nx * jp};
-    // The column (planes k0-1 .. k0+RZC) of p, e1 and d = e1 - e2, then the
-    // lateral neighbours plane by plane.
-    double zp[RZC + 2];
-    float ze[RZC + 2], zd[RZC + 2];
-#pragma unroll
-    for (int kk = 0; kk < RZC + 2; ++kk) {
-      const size_t o = c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz);
-      zp[kk] = p[o];
-      const float a = e1[o], b = e2[o];
-      ze[kk] = a;
-      zd[kk] = a - b;
-    }
-#pragma unroll
-    for (int kk = 0; kk < RZC; ++kk) {
-      const int k = k0 + kk;
-      const size_t ok = plane * min(k, nz - 1);
-      double lp[4];
-      float le[4], ld[4];
-#pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        lp[d] = p[cl[d] + ok];
-        const float a = e1[cl[d] + ok], b = e2[cl[d] + ok];
-        le[d] = a;
-        ld[d] = a - b;
-      }
-      const double rh = (double)rhs[c + ok];
-      if (k < nz) {
-        auto qv = [&](double pv, float ev, float dv) {
-          return pv + (double)ev + beta * (double)dv;
-        };
-        const double lapq =
-            ox * (qv(lp[0], le[0], ld[0]) + qv(lp[1], le[1], ld[1])) +
-            oy * (qv(lp[2], le[2], ld[2]) + qv(lp[3], le[3], ld[3])) +
-            oz * (qv(zp[kk], ze[kk], zd[kk]) + qv(zp[kk + 2], ze[kk + 2], zd[kk + 2])) +
-            A * qv(zp[kk + 1], ze[kk + 1], zd[kk + 1]);
+  // Fields: p, e1 and d = e1 - e2 (the FP32 difference).
+  auto qv = [&](double pv, double ev, double dv) { return pv + ev + beta * dv; };
+  tile_march<3>(
+      [&](size_t o, double *v) {
+        const float a = e1[o], b = e2[o];
+        v[0] = p[o];
+        v[1] = (double)a;
+        v[2] = (double)(a - b);
+      },
+      rhs, nx, ny, nz, px, py, pz,
+      [&](size_t o, int, int, int, const double (*nb)[7], double rh) {
+        auto qn = [&](int t) { return qv(nb[0][t], nb[1][t], nb[2][t]); };
+        const double lapq = ox * (qn(0) + qn(1)) + oy * (qn(2) + qn(3)) +
+                            oz * (qn(4) + qn(5)) + A * qn(6);
         const double rq = rh - lapq;
-        q[c + plane * k] = qv(zp[kk + 1], ze[kk + 1], zd[kk + 1]);
-        r[c + plane * k] = (float)rq;
-        const double lapd = ox * ((double)ld[0] + (double)ld[1]) +
-                            oy * ((double)ld[2] + (double)ld[3]) +
-                            oz * ((double)zd[kk] + (double)zd[kk + 2]) + A * (double)zd[kk + 1];
+        q[o] = qn(6);
+        r[o] = (float)rq;
+        const double lapd = ox * (nb[2][0] + nb[2][1]) + oy * (nb[2][2] + nb[2][3]) +
+                            oz * (nb[2][4] + nb[2][5]) + A * nb[2][6];
         sb += (rq + beta * lapd) * lapd;
         sd += lapd * lapd;
         if (part0) {
-          const double lapp = ox * (lp[0] + lp[1]) + oy * (lp[2] + lp[3]) +
-                              oz * (zp[kk] + zp[kk + 2]) + A * zp[kk + 1];
+          const double lapp = ox * (nb[0][0] + nb[0][1]) + oy * (nb[0][2] + nb[0][3]) +
+                              oz * (nb[0][4] + nb[0][5]) + A * nb[0][6];
           const double r0 = rh - lapp;
           m0 = fmax(m0, fabs(r0));
           s0 += r0 * r0;
         }
-      }
-    }
-  }
-  const size_t b = (size_t)T.b;
+      });
+  const size_t b = (size_t)residual_tile().b;
   if (part0) {
     m0 = block_max<RNT / O3D_WAVE>(m0, sc);
     s0 = block_sum<RNT / O3D_WAVE>(s0, sc);

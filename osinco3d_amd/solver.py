@@ -236,6 +236,15 @@ class Solver:
     def field_ptr(self, which: int) -> int:
         return int(self.lib.o3d_solver_field(self.handle, int(which)) or 0)
 
+    def field(self, which: int) -> np.ndarray:
+        """Host copy of solver field `which` (o3d_solver_field numbering):
+        float64 for the pressure and the FP64 form, float32 otherwise."""
+        dt = np.float64 if which == 6 or self.precision == FP64 else np.float32
+        out = np.empty((self.cfg.nz, self.cfg.ny, self.cfg.nx), dt)
+        capi.check(self.lib.o3d_copy(out.ctypes.data_as(C.c_void_p),
+                                     C.c_void_p(self.field_ptr(which)), C.c_size_t(out.nbytes)))
+        return out
+
     def pressure_stats(self) -> np.ndarray:
         """function_stats of the solver's pressure on the device (min, max,
         mean, then the 1-based position of the first maximum)."""

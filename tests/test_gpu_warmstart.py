@@ -187,3 +187,54 @@ def test_adaptive_extrapolation_weight_is_least_squares():
         assert infos[k + 1]["extrap_beta"] == min(1.5, max(0.0, got))
         checked += 1
     assert checked >= 8
+
+
+def _residual_order(q, rhs32, d):
+    """rhs - L q in FP64 with the residual kernels' operation order
+    (ox (xm + xp) + oy (ym + yp) + oz (zm + zp) + A c), rounded to FP32."""
+    ox, oy, oz = (1.0 / h ** 2 for h in d)
+    A = -(2.0 * ox + 2.0 * oy + 2.0 * oz)
+    lap = (ox * (np.roll(q, 1, 2) + np.roll(q, -1, 2)) + oy * (np.roll(q, 1, 1) + np.roll(q, -1, 1))
+           + oz * (np.roll(q, 1, 0) + np.roll(q, -1, 0)) + A * q)
+    return (rhs32.astype(np.float64) - lap).astype(np.float32)
+
+
+@pytest.mark.parametrize("adaptive", [0, 1])
+def test_guess_residual_is_bit_exact(adaptive):
+    """The guess the extrapolated warm start forms and its FP32 residual,
+    which open every solve of the warm forms: q = p_prev + e_prev (+ beta
+    (e_prev - e_prev2), the FP32 difference) in FP64 and r = rhs - L q with
+    the kernels' operation order, recomputed in numpy from the solver's own
+    fields and compared bit for bit."""
+    import torch
+
+    from osinco3d_amd import capi
+    n, steps = 32, 8
+    cfg = tgv_config(n, steps, 3)
+    d = cfg.spacing()
+    s = Solver(cfg, MIXED, WARM_EXTRAPOLATE, adaptive=adaptive)
+    s.init_taylor_green()
+    r = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+    ep = torch.empty_like(r)
+    e = torch.empty_like(r)
+    capi.check(capi.load().o3d_solver_capture(s.handle, capi.ptr(r), capi.ptr(ep), capi.ptr(e)))
+    eps_prev = None
+    checked = 0
+    for it in range(1, steps + 1):
+        p_prev = s.field(6)
+        info = s.step(it)
+        rhs = s.field(7)
+        e1 = ep.cpu().numpy()
+        if it >= 2:
+            q = p_prev + e1.astype(np.float64)
+            if adaptive and it >= 3 and info["extrap_beta"] != 0.0:
+                dd = (e1 - eps_prev).astype(np.float64)
+                q = (p_prev + e1.astype(np.float64)) + info["extrap_beta"] * dd
+            want = _residual_order(q, rhs, d)
+            got = r.cpu().numpy()
+            assert np.array_equal(got, want), (it, np.abs(got - want).max())
+            checked += 1
+        eps_prev = e1
+    capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
+    s.close()
+    assert checked == steps - 1
