@@ -735,9 +735,13 @@ hipError_t launch_sor_init(SorState *st, double omega, int base, hipStream_t s) 
 
 // ------------------------------------------------------ mixed precision
 // The residual kernels march z over RZC planes of a 64 x 4 (i, j) tile per
-// block (tile_march: the plane's tile and halo in LDS, the column's planes
-// k-1, k, k+1 in registers), so a launch has many blocks and no index
-// divisions.  The residual keeps the operation order of the red-black
+// block, so a launch has many blocks and no index divisions: the plain
+// residuals with the plane's tile and halo in LDS (tile_march), the guess
+// kernels (two and three fields per node) with one column per thread and
+// every plane's loads issued before the first use, which keeps more loads in
+// flight than a plane-by-plane LDS march at their register count (measured:
+// the adaptive guess took 210 us per 256^3 launch as an LDS march, 163 us
+// as columns).  The residual keeps the operation order of the red-black
 // sweep's Laplacian.
 constexpr int RBX = 64, RBY = 4, RZC = 8, RNT = RBX * RBY;
 
@@ -914,33 +918,61 @@ __global__ void __launch_bounds__(RNT) guess_residual_kernel(const double *__res
                                                              double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0;
-  auto qv = [](double a, double b) { return a + 1.0 * b; };
-  tile_march<2>(
-      [&](size_t o, double *v) {
-        v[0] = p[o];
-        v[1] = (double)e[o];
-      },
-      rhs, nx, ny, nz, px, py, pz,
-      [&](size_t o, int, int, int, const double (*nb)[7], double rh) {
-        const double lapq = ox * (qv(nb[0][0], nb[1][0]) + qv(nb[0][1], nb[1][1])) +
-                            oy * (qv(nb[0][2], nb[1][2]) + qv(nb[0][3], nb[1][3])) +
-                            oz * (qv(nb[0][4], nb[1][4]) + qv(nb[0][5], nb[1][5])) +
-                            A * qv(nb[0][6], nb[1][6]);
-        q[o] = qv(nb[0][6], nb[1][6]);
-        r[o] = (float)(rh - lapq);
+  const RTile T = residual_tile();
+  const int i = T.bx * RBX + (threadIdx.x % RBX);
+  const int j = T.by * RBY + (threadIdx.x / RBX);
+  if (i < nx && j < ny) {
+    const int k0 = T.bz * RZC;
+    const size_t plane = (size_t)nx * ny;
+    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+    const int c = i + nx * j;
+    const int cl[4] = {im + nx * j, ip + nx * j, i + nx * jm, i + nx * jp};
+    double zp[RZC + 2], lp[4][RZC], rh[RZC];
+    float ze[RZC + 2], le[4][RZC];
+#pragma unroll
+    for (int kk = 0; kk < RZC + 2; ++kk) {
+      const size_t o = c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz);
+      zp[kk] = p[o];
+      ze[kk] = e[o];
+    }
+#pragma unroll
+    for (int kk = 0; kk < RZC; ++kk) {
+      const size_t o = plane * min(k0 + kk, nz - 1);
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        lp[d][kk] = p[cl[d] + o];
+        le[d][kk] = e[cl[d] + o];
+      }
+      rh[kk] = (double)rhs[c + o];
+    }
+    auto qv = [](double a, float b) { return a + 1.0 * (double)b; };
+#pragma unroll
+    for (int kk = 0; kk < RZC; ++kk) {
+      const int k = k0 + kk;
+      if (k < nz) {
+        const size_t o = c + plane * k;
+        const double lapq = ox * (qv(lp[0][kk], le[0][kk]) + qv(lp[1][kk], le[1][kk])) +
+                            oy * (qv(lp[2][kk], le[2][kk]) + qv(lp[3][kk], le[3][kk])) +
+                            oz * (qv(zp[kk], ze[kk]) + qv(zp[kk + 2], ze[kk + 2])) +
+                            A * qv(zp[kk + 1], ze[kk + 1]);
+        q[o] = qv(zp[kk + 1], ze[kk + 1]);
+        r[o] = (float)(rh[kk] - lapq);
         if (part0) {
-          const double lapp = ox * (nb[0][0] + nb[0][1]) + oy * (nb[0][2] + nb[0][3]) +
-                              oz * (nb[0][4] + nb[0][5]) + A * nb[0][6];
-          const double r0 = rh - lapp;
+          const double lapp = ox * (lp[0][kk] + lp[1][kk]) + oy * (lp[2][kk] + lp[3][kk]) +
+                              oz * (zp[kk] + zp[kk + 2]) + A * zp[kk + 1];
+          const double r0 = rh[kk] - lapp;
           m0 = fmax(m0, fabs(r0));
           s0 += r0 * r0;
         }
-      });
+      }
+    }
+  }
   if (part0) {
     m0 = block_max<RNT / O3D_WAVE>(m0, sc);
     s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
     if (threadIdx.x == 0) {
-      const size_t b = (size_t)residual_tile().b;
+      const size_t b = (size_t)T.b;
       part0[2 * b] = m0;
       part0[2 * b + 1] = s0;
     }
@@ -972,36 +1004,70 @@ __global__ void __launch_bounds__(RNT) guess_adapt_kernel(
     int py, int pz, double ox, double oy, double oz, double A) {
   __shared__ double sc[RNT / O3D_WAVE];
   double m0 = 0.0, s0 = 0.0, sb = 0.0, sd = 0.0;
-  // Fields: p, e1 and d = e1 - e2 (the FP32 difference).
-  auto qv = [&](double pv, double ev, double dv) { return pv + ev + beta * dv; };
-  tile_march<3>(
-      [&](size_t o, double *v) {
-        const float a = e1[o], b = e2[o];
-        v[0] = p[o];
-        v[1] = (double)a;
-        v[2] = (double)(a - b);
-      },
-      rhs, nx, ny, nz, px, py, pz,
-      [&](size_t o, int, int, int, const double (*nb)[7], double rh) {
-        auto qn = [&](int t) { return qv(nb[0][t], nb[1][t], nb[2][t]); };
-        const double lapq = ox * (qn(0) + qn(1)) + oy * (qn(2) + qn(3)) +
-                            oz * (qn(4) + qn(5)) + A * qn(6);
+  const RTile T = residual_tile();
+  const int i = T.bx * RBX + (threadIdx.x % RBX);
+  const int j = T.by * RBY + (threadIdx.x / RBX);
+  if (i < nx && j < ny) {
+    const int k0 = T.bz * RZC;
+    const size_t plane = (size_t)nx * ny;
+    const int im = sor_ghost(i - 1, nx, px), ip = sor_ghost(i + 1, nx, px);
+    const int jm = sor_ghost(j - 1, ny, py), jp = sor_ghost(j + 1, ny, py);
+    const int c = i + nx * j;
+    const int cl[4] = {im + nx * j, ip + nx * j, i + nx * jm, i + nx * jp};
+    // The column (planes k0-1 .. k0+RZC) of p, e1 and d = e1 - e2, then the
+    // lateral neighbours plane by plane.
+    double zp[RZC + 2];
+    float ze[RZC + 2], zd[RZC + 2];
+#pragma unroll
+    for (int kk = 0; kk < RZC + 2; ++kk) {
+      const size_t o = c + plane * sor_ghost(min(k0 + kk - 1, nz), nz, pz);
+      zp[kk] = p[o];
+      const float a = e1[o], b = e2[o];
+      ze[kk] = a;
+      zd[kk] = a - b;
+    }
+#pragma unroll
+    for (int kk = 0; kk < RZC; ++kk) {
+      const int k = k0 + kk;
+      const size_t ok = plane * min(k, nz - 1);
+      double lp[4];
+      float le[4], ld[4];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        lp[d] = p[cl[d] + ok];
+        const float a = e1[cl[d] + ok], b = e2[cl[d] + ok];
+        le[d] = a;
+        ld[d] = a - b;
+      }
+      const double rh = (double)rhs[c + ok];
+      if (k < nz) {
+        auto qv = [&](double pv, float ev, float dv) {
+          return pv + (double)ev + beta * (double)dv;
+        };
+        const double lapq =
+            ox * (qv(lp[0], le[0], ld[0]) + qv(lp[1], le[1], ld[1])) +
+            oy * (qv(lp[2], le[2], ld[2]) + qv(lp[3], le[3], ld[3])) +
+            oz * (qv(zp[kk], ze[kk], zd[kk]) + qv(zp[kk + 2], ze[kk + 2], zd[kk + 2])) +
+            A * qv(zp[kk + 1], ze[kk + 1], zd[kk + 1]);
         const double rq = rh - lapq;
-        q[o] = qn(6);
-        r[o] = (float)rq;
-        const double lapd = ox * (nb[2][0] + nb[2][1]) + oy * (nb[2][2] + nb[2][3]) +
-                            oz * (nb[2][4] + nb[2][5]) + A * nb[2][6];
+        q[c + plane * k] = qv(zp[kk + 1], ze[kk + 1], zd[kk + 1]);
+        r[c + plane * k] = (float)rq;
+        const double lapd = ox * ((double)ld[0] + (double)ld[1]) +
+                            oy * ((double)ld[2] + (double)ld[3]) +
+                            oz * ((double)zd[kk] + (double)zd[kk + 2]) + A * (double)zd[kk + 1];
         sb += (rq + beta * lapd) * lapd;
         sd += lapd * lapd;
         if (part0) {
-          const double lapp = ox * (nb[0][0] + nb[0][1]) + oy * (nb[0][2] + nb[0][3]) +
-                              oz * (nb[0][4] + nb[0][5]) + A * nb[0][6];
+          const double lapp = ox * (lp[0] + lp[1]) + oy * (lp[2] + lp[3]) +
+                              oz * (zp[kk] + zp[kk + 2]) + A * zp[kk + 1];
           const double r0 = rh - lapp;
           m0 = fmax(m0, fabs(r0));
           s0 += r0 * r0;
         }
-      });
-  const size_t b = (size_t)residual_tile().b;
+      }
+    }
+  }
+  const size_t b = (size_t)T.b;
   if (part0) {
     m0 = block_max<RNT / O3D_WAVE>(m0, sc);
     s0 = block_sum<RNT / O3D_WAVE>(s0, sc);
