@@ -314,7 +314,9 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   // (x, y, z) at entry (nx/2)(y + ny z) + x/2 of the red (x + y + z even) or
   // black half.  Per R2 column: its entry in a plane and its (x + y) parity;
   // the plane's parity picks the half.
-  const bool fold = a.efold != nullptr;
+  // (Folding applies to the FP64 judge sweep only; for T = float it is
+  // compiled out, keeping that kernel within its register budget.)
+  const bool fold = sizeof(T) == 8 && a.efold != nullptr;
   const __amdgpu_buffer_rsrc_t rfe = buf_rsrc(fold ? a.efold : (const float *)a.pin, plane * nz);
   const int hxn = nx >> 1;
   auto ecol = [&](int c) {
@@ -324,14 +326,18 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   };
   const unsigned eca = ecol(tid), ecb = v2b ? ecol(c2b) : eca;
   const unsigned halfB = (unsigned)(plane * nz / 2) * 4u;
-  auto ld_old = [&](unsigned o, unsigned ec, unsigned gz) {
-    T v = bload<T>(rin, o, (unsigned)plane * gz * sizeof(T));
-    if (fold) {
-      const unsigned eo = (ec >> 1) * 4u + (((ec ^ gz) & 1u) ? halfB : 0u);
-      v = v + (T)bload<float>(rfe, eo, (unsigned)hxn * (unsigned)ny * gz * 4u);
-    }
-    return v;
+  // The raw loads; the sum p + e is formed only when a plane is published
+  // to LDS, so prefetched loads stay in flight across the march steps.
+  auto ld_old = [&](unsigned o, unsigned gz) {
+    return bload<T>(rin, o, (unsigned)plane * gz * sizeof(T));
   };
+  auto ld_e = [&](unsigned ec, unsigned gz) {
+    if constexpr (sizeof(T) != 8) return 0.0f;
+    if (!fold) return 0.0f;
+    const unsigned eo = (ec >> 1) * 4u + (((ec ^ gz) & 1u) ? halfB : 0u);
+    return bload<float>(rfe, eo, (unsigned)hxn * (unsigned)ny * gz * 4u);
+  };
+  auto pe_add = [&](T v, float e) { return fold ? v + (T)e : v; };
   auto ld_rhs = [&](unsigned o, unsigned gz) {
     return bload<TR>(rrh, o, (unsigned)plane * gz * sizeof(TR));
   };
@@ -339,14 +345,15 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   // Prefetch registers: step k of the march publishes old planes
   // s_k+1 .. s_k+PPS and rhs planes s_k .. s_k+PPS-1, s_k = z0-1 + k*PPS.
   T po[PD][PPS][2];
+  float pe[PD][PPS][2];
   TR pr[PD][PPS][2];
   {
     const unsigned g2 = zq_lo(z0 - 2), g1 = zq_lo(z0 - 1);
-    oldr[slot<NO>(-2)][tid] = ld_old(o2a, eca, g2);
-    oldr[slot<NO>(-1)][tid] = ld_old(o2a, eca, g1);
+    oldr[slot<NO>(-2)][tid] = pe_add(ld_old(o2a, g2), ld_e(eca, g2));
+    oldr[slot<NO>(-1)][tid] = pe_add(ld_old(o2a, g1), ld_e(eca, g1));
     if (v2b) {
-      oldr[slot<NO>(-2)][c2b] = ld_old(o2b, ecb, g2);
-      oldr[slot<NO>(-1)][c2b] = ld_old(o2b, ecb, g1);
+      oldr[slot<NO>(-2)][c2b] = pe_add(ld_old(o2b, g2), ld_e(ecb, g2));
+      oldr[slot<NO>(-1)][c2b] = pe_add(ld_old(o2b, g1), ld_e(ecb, g1));
     }
 #pragma unroll
     for (int k = 0; k < PD; ++k)
@@ -354,8 +361,10 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
       for (int q = 0; q < PPS; ++q) {
         const int sk = z0 - 1 + k * PPS + q;
         const unsigned go = zq_lo(sk + 1), gr = zq_lo(sk);
-        po[k][q][0] = ld_old(o2a, eca, go);
-        po[k][q][1] = ld_old(o2b, ecb, go);
+        po[k][q][0] = ld_old(o2a, go);
+        po[k][q][1] = ld_old(o2b, go);
+        pe[k][q][0] = ld_e(eca, go);
+        pe[k][q][1] = ld_e(ecb, go);
         pr[k][q][0] = ld_rhs(o1a, gr);
         pr[k][q][1] = ld_rhs(o1b, gr);
       }
@@ -379,13 +388,15 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
     for (int q = 0; q < PPS; ++q) {
       T *od = oldr[so(1 + q)];
       TR *rd = rhsr[sh(q)];
-      od[tid] = po[k][q][0];
+      od[tid] = pe_add(po[k][q][0], pe[k][q][0]);
       rd[tid] = pr[k][q][0];
-      if (PAD || v2b) od[c2b] = po[k][q][1];
+      if (PAD || v2b) od[c2b] = pe_add(po[k][q][1], pe[k][q][1]);
       if (PAD || v1b) rd[c2b] = pr[k][q][1];
       const unsigned go = zq(s + 1 + q + PD * PPS), gr = zq(s + q + PD * PPS);
-      po[k][q][0] = ld_old(o2a, eca, go);
-      po[k][q][1] = ld_old(o2b, ecb, go);
+      po[k][q][0] = ld_old(o2a, go);
+      po[k][q][1] = ld_old(o2b, go);
+      pe[k][q][0] = ld_e(eca, go);
+      pe[k][q][1] = ld_e(ecb, go);
       pr[k][q][0] = ld_rhs(o1a, gr);
       pr[k][q][1] = ld_rhs(o1b, gr);
     }
