@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -270,24 +271,67 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(const ConvArgs<TO> a) {
           acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
     }
   }
+  // Epilogue through LDS, one 16-channel output tile at a time: each wave
+  // writes its accumulators as [row m][voxel][channel] floats over the input
+  // tile (no longer read), then every lane forms 8 consecutive channels of
+  // one voxel, so the skip is read and the result written as 16-byte
+  // vectors instead of one 2- or 4-byte access per value.  Same operations
+  // per value, in the same order, as a per-value epilogue.
   const int z = z0 + wv;
+  static_assert(TY * 16 * 16 * 4 * 4 <= NG * NV * 16, "epilogue does not fit the tile");
+  float *ob = reinterpret_cast<float *>(tile) + wv * (TY * 16 * 16);
 #pragma unroll
-  for (int m = 0; m < TY; ++m)
+  for (int n = 0; n < NT; ++n) {
+    __syncthreads();  // the tile (first pass) or the previous pass is done with
 #pragma unroll
-    for (int n = 0; n < NT; ++n) {
-      const int c = co0 + 16 * n + l16;
-      const float b = a.bias ? a.bias[c] : 0.f;
+    for (int m = 0; m < TY; ++m)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int vox = x0 + 4 * g + i + a.nx * (y0 + m + a.ny * z);
-        const size_t o = (size_t)vox * a.co + c;
-        float v = acc[m][n][i] + b;
-        if (a.skip) v += (float)a.skip[o];
-        if (a.act) v = act_f(v);
-        if (a.accumulate) v += (float)a.y[o];
-        a.y[o] = (TO)v;
+      for (int i = 0; i < 4; ++i) ob[(m * 16 + 4 * g + i) * 16 + l16] = acc[m][n][i];
+    __syncthreads();
+    for (int t = lane; t < TY * 16 * 2; t += 64) {
+      const int m = t >> 5, v16 = (t >> 1) & 15, c8 = t & 1;
+      const int c = co0 + 16 * n + 8 * c8;
+      const int vox = x0 + v16 + a.nx * (y0 + m + a.ny * z);
+      const size_t o = (size_t)vox * a.co + c;
+      const float *src = ob + (m * 16 + v16) * 16 + 8 * c8;
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = src[q] + (a.bias ? a.bias[c + q] : 0.f);
+      if (a.skip) {
+        const bf16x8 sk = *(const bf16x8 *)(a.skip + o);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] += (float)sk[q];
+      }
+      if (a.act) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = act_f(v[q]);
+      }
+      if constexpr (std::is_same<TO, float>::value) {
+        f32x4 *yo = (f32x4 *)(a.y + o);
+        if (a.accumulate) {
+          const f32x4 y0v = yo[0], y1v = yo[1];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            v[q] += y0v[q];
+            v[4 + q] += y1v[q];
+          }
+        }
+        yo[0] = f32x4{v[0], v[1], v[2], v[3]};
+        yo[1] = f32x4{v[4], v[5], v[6], v[7]};
+      } else {
+        bf16x8 *yo = (bf16x8 *)(a.y + o);
+        if (a.accumulate) {
+          const bf16x8 yv = *yo;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] += (float)yv[q];
+        }
+        bf16x8 w;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = (bf16)v[q];
+        *yo = w;
       }
     }
+  }
 }
 
 // LDS bricks of a grid (TY rows per brick) and whether the grid tiles.

@@ -459,3 +459,32 @@ def test_poisson_multigrid_limits():
     with pytest.raises(capi.O3DError, match="npre"):
         capi.poisson_multigrid(np.zeros(sh), np.zeros(sh), (0.1, 0.1, 0.1), "0000", npre=0,
                                npost=0)
+
+
+def test_measurement_entry_points():
+    """The bench's ceilings: the stream probe (a hand-written triad) reports
+    a bandwidth between a copy's plausible floor and the 8 TB/s peak; the
+    back-to-back SOR period of a mixed solver is positive, below the
+    in-step batches' figure plus slack, and leaves the solver able to step
+    on (its buffers are scratch between solves)."""
+    from osinco3d_amd.config import tgv_config
+    from osinco3d_amd.solver import MIXED, WARM_EXTRAPOLATE, Solver
+    gbs, ms = capi.stream_probe(1 << 28, 5)
+    assert 1000.0 < gbs < 8000.0 and ms > 0.0
+    cfg = tgv_config(64, 6, 3)
+    s = Solver(cfg, MIXED, WARM_EXTRAPOLATE)
+    s.init_taylor_green()
+    for it in range(1, 4):
+        s.step(it)
+    us = s.sor_period(50)
+    assert 0.0 < us < 1000.0
+    for it in range(4, 7):
+        info = s.step(it)
+        assert info["sor_reason"] in (0, 1) and not info["diverged"]
+    s.close()
+    with pytest.raises(capi.O3DError):
+        s2 = Solver(cfg, MIXED)
+        try:
+            s2.sor_period(1)
+        finally:
+            s2.close()
