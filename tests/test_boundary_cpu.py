@@ -54,13 +54,35 @@ def test_runtime_fails_loudly_without_gpu():
     assert rc < 0 and "device" in capi.last_error().lower()
 
 
-def test_struct_layouts_match_header():
-    # o3d_params: 3 ints, 3 doubles, int[3], int, 2 doubles, 3x double[3], int,
-    # 2 doubles, 2 doubles, 2 ints, 3 ints -> the ctypes mirror computes the
-    # same natural alignment as the C compiler.
-    assert ctypes.sizeof(capi.O3DParams) % 8 == 0
-    assert capi.O3DParams.adt.offset % 8 == 0
-    assert ctypes.sizeof(capi.O3DStepInfo) % 8 == 0
+def test_struct_layouts_match_header(tmp_path):
+    """Every field of o3d_params and o3d_step_info at the same offset and the
+    same total size in the ctypes mirror as in the C compiler's layout of
+    include/o3d.h (a small C program prints sizeof and offsetof)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {"o3d_params": capi.O3DParams, "o3d_step_info": capi.O3DStepInfo}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "o3d.h"', "int main(void) {"]
+    for sname, cls in structs.items():
+        lines.append(f'  printf("{sname} size %zu\\n", sizeof({sname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{sname} {fname} %zu\\n", offsetof({sname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for line in out.splitlines():
+        sname, key, val = line.split()
+        got[(sname, key)] = int(val)
+    for sname, cls in structs.items():
+        assert got[(sname, "size")] == ctypes.sizeof(cls), sname
+        for fname, _ in cls._fields_:
+            assert got[(sname, fname)] == getattr(cls, fname).offset, (sname, fname)
 
 
 def test_namelist_round_trip(tmp_path):
