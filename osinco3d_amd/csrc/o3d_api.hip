@@ -1594,6 +1594,8 @@ struct Solver {
   double *epart = nullptr;     // sum-of-squares partials of e_prev
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
   double *mpart = nullptr;     // mean-pressure partials + the mean (mixed, every axis periodic)
+  double *jpart = nullptr;     // per-tile sums of p from the FP64 judge sweep (with mpart)
+  int jblocks = 0;             // tiles of the judge sweep
   // Adaptive extrapolation (o3d_params.adaptive_extrapolation): the weight of
   // e_prev - e_prev2 and the device sums behind the next one.
   double beta = 0.0;
@@ -1884,6 +1886,12 @@ int pressure_mean(double *p, int n, double *part) {
   HIPCHK(hipGetLastError());
   return 0;
 }
+// The same mean from the judge sweep's per-tile sums of the p it wrote.
+int pressure_mean_from(const double *tiles, int ntiles, int n, double *mp) {
+  p_mean_kernel<<<1, 1024, 0, g_stream>>>(tiles, ntiles, n, mp);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 
 int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   const o3d_params &p = S.p;
@@ -1899,7 +1907,10 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   int reason = 2;
   double judge_prev = 1609.0, dlast = 0.0;
   int capped = 0;  // consecutive correction solves that ended on their sweep cap
+  // The judge sweep summed the p it wrote and nothing has changed p since.
+  bool sum_ok = false;
   while (total < p.kmax) {
+    sum_ok = false;
     int per[3];
     solver_periodic(S.kind, per);
     // Colour-split correction sweeps where the grid allows (nx a multiple
@@ -1986,9 +1997,11 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
     j.buf[1] = S.P[1 - S.cur];
     sor_args_common<double, float>(S, j.a, (const float *)S.rhs, S.st64, 1);
     if (fuse) j.a.efold = S.E[0];
+    j.a.psum = S.jpart;
     int jsw = 0;
     CHK(j.run(S.omega, 2, &jsw));
     if (jsw & 1) S.cur = 1 - S.cur;
+    sum_ok = S.jpart != nullptr && jsw == 1;
     sw64 += jsw;
     total += jsw;
     dlast = S.hst->dmax_final;
@@ -1999,7 +2012,8 @@ int solver_poisson_mixed(Solver &S, o3d_step_info *info) {
   S.last_sweeps[1] = sw32;
   S.r_ready = false;
   if (S.mpart) {
-    CHK(pressure_mean(S.P[S.cur], n, S.mpart));
+    if (sum_ok) CHK(pressure_mean_from(S.jpart, S.jblocks, n, S.mpart + kMeanBlocks));
+    else CHK(pressure_mean(S.P[S.cur], n, S.mpart));
     S.mean_pending = true;
   }
   info->sor_exit_iter = reason == 2 ? p.kmax + 1 : total;
@@ -2691,7 +2705,11 @@ extern "C" int o3d_solver_create(const o3d_params *p, void **handle) {
       rc |= S->alloc(&S->dslot, sor_split_slot_words());
     int per[3];
     solver_periodic(kind, per);
-    if (!rc && per[0] && per[1] && per[2]) rc |= S->alloc(&S->mpart, kMeanBlocks + 1);
+    if (!rc && per[0] && per[1] && per[2]) {
+      rc |= S->alloc(&S->mpart, kMeanBlocks + 1);
+      S->jblocks = (int)sor_tiling<double, float>(p->nx, p->ny, p->nz).blocks();
+      if (!rc) rc |= S->alloc(&S->jpart, (size_t)S->jblocks);
+    }
   }
   if (!rc && !S->mixed && sor_split_supported<double>(p->nx, p->ny, p->nz)) {
     rc |= S->alloc(&S->dslot, sor_split_slot_words());

@@ -86,6 +86,13 @@ __device__ __forceinline__ void mg_relax(float *u, const float *f, const MgGrid 
   u[q] = (f[q] - mg_lap_off(u, g, i, j, k)) * g.invd;
 }
 
+// The same update from the neighbour sums (the operation order of
+// mg_lap_off and mg_relax, so both give the same value).
+__device__ __forceinline__ float mg_relax_value(float fq, float sx, float sy, float sz,
+                                                const MgGrid &g) {
+  return (fq - (g.cx * sx + g.cy * sy + g.cz * sz)) * g.invd;
+}
+
 // Residual f - L u at cell (i, j, k).
 __device__ __forceinline__ float mg_res(const float *u, const float *f, const MgGrid &g, int i,
                                         int j, int k) {
@@ -281,6 +288,46 @@ __device__ __forceinline__ void lds_smooth(float *u, const float *f, const MgGri
     }
 }
 
+// `sweeps` red-black sweeps of a level of at most 64 cells by wave 0, one
+// cell per lane in registers, the neighbours fetched across lanes: no block
+// barrier per colour.  The same updates in the same order as lds_smooth.
+// Called by every thread of the block (it ends with the block barrier).
+__device__ __forceinline__ void wave_smooth(float *u, const float *f, const MgGrid &g,
+                                            int sweeps) {
+  if (threadIdx.x < O3D_WAVE) {
+    const int n = g.mx * g.my * g.mz;
+    const int q = threadIdx.x;
+    const bool act = q < n;
+    int i = 0, j = 0, k = 0;
+    if (act) {
+      const int row = q / g.mx;
+      i = q - row * g.mx;
+      k = row / g.my;
+      j = row - k * g.my;
+    }
+    // Neighbour lanes (inactive lanes read their own).
+    const int rj = g.mx * (j + g.my * k);
+    const int lxm = act ? rj + mg_nb(i - 1, g.mx, g.px) : q;
+    const int lxp = act ? rj + mg_nb(i + 1, g.mx, g.px) : q;
+    const int lym = act ? i + g.mx * (mg_nb(j - 1, g.my, g.py) + g.my * k) : q;
+    const int lyp = act ? i + g.mx * (mg_nb(j + 1, g.my, g.py) + g.my * k) : q;
+    const int lzm = act ? i + g.mx * (j + g.my * mg_nb(k - 1, g.mz, g.pz)) : q;
+    const int lzp = act ? i + g.mx * (j + g.my * mg_nb(k + 1, g.mz, g.pz)) : q;
+    const int col = (i + j + k) & 1;
+    float uq = act ? u[q] : 0.f;
+    const float fq = act ? f[q] : 0.f;
+    for (int sw = 0; sw < sweeps; ++sw)
+      for (int colour = 0; colour < 2; ++colour) {
+        const float sx = __shfl(uq, lxm) + __shfl(uq, lxp);
+        const float sy = __shfl(uq, lym) + __shfl(uq, lyp);
+        const float sz = __shfl(uq, lzm) + __shfl(uq, lzp);
+        if (act && col == colour) uq = mg_relax_value(fq, sx, sy, sz, g);
+      }
+    if (act) u[q] = uq;
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(kMgSmallThreads) mg_small_vcycle_kernel(
     float *__restrict__ u_out, const float *__restrict__ f_in, MgSmall S) {
   __shared__ float su[kMgSmallCells], sf[kMgSmallCells];
@@ -310,7 +357,13 @@ __global__ void __launch_bounds__(kMgSmallThreads) mg_small_vcycle_kernel(
   // Coarsest.
 #pragma unroll
   for (int l = 1; l < kMgSmallLevels; ++l)
-    if (l == S.levels - 1) lds_smooth(su + S.off[l], sf + S.off[l], S.g[l], S.coarse_sweeps);
+    if (l == S.levels - 1) {
+      const MgGrid &g = S.g[l];
+      if (g.mx * g.my * g.mz <= O3D_WAVE)
+        wave_smooth(su + S.off[l], sf + S.off[l], g, S.coarse_sweeps);
+      else
+        lds_smooth(su + S.off[l], sf + S.off[l], g, S.coarse_sweeps);
+    }
   // Up: prolong, smooth.
 #pragma unroll
   for (int l = kMgSmallLevels - 2; l >= 0; --l) {

@@ -35,6 +35,7 @@ namespace nn {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr float kSlope = 0.1f;  // leaky-ReLU slope
@@ -55,11 +56,6 @@ struct ConvArgs {
   int lci, ci, co, kp;
   int act;            // 1: leaky ReLU
   int accumulate;     // y += result (FP32 gradient buffers)
-  // Split-K (coarse grids): grid.z = nsplit slices of the K chunks; each
-  // writes raw FP32 sums to part[slice][V][co] and conv_reduce_kernel applies
-  // bias, skip, activation and accumulation.  part = null: one slice.
-  float *part = nullptr;
-  int nsplit = 1;
 };
 
 // One wave computes 16*MV consecutive voxels (linear order, so coarse levels
@@ -81,8 +77,6 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
     for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ktot = 27 * a.ci;
   const int nchunk = a.kp >> 5;
-  const int kc0 = (int)blockIdx.z * nchunk / a.nsplit;
-  const int kc1 = ((int)blockIdx.z + 1) * nchunk / a.nsplit;
   // A-operand voxels of this lane, one per voxel tile.
   int xm[MV], ym[MV], zm[MV];
   bool vin[MV];
@@ -97,7 +91,7 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
   const bf16 *wrow[NT];
 #pragma unroll
   for (int n = 0; n < NT; ++n) wrow[n] = a.w + (size_t)(co0 + 16 * n + l16) * a.kp;
-  for (int kc = kc0; kc < kc1; ++kc) {
+  for (int kc = 0; kc < nchunk; ++kc) {
     const int k0 = (kc << 5) + (g << 3);
     bf16x8 bv[NT];
 #pragma unroll
@@ -122,19 +116,6 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
         acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[m][n], 0, 0, 0);
     }
   }
-  if (a.part) {
-    float *pp = a.part + (size_t)blockIdx.z * V * a.co;
-#pragma unroll
-    for (int m = 0; m < MV; ++m)
-#pragma unroll
-      for (int n = 0; n < NT; ++n)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int vox = vbase + 16 * m + 4 * g + i;
-          if (vox < V) pp[(size_t)vox * a.co + co0 + 16 * n + l16] = acc[m][n][i];
-        }
-    return;
-  }
 #pragma unroll
   for (int m = 0; m < MV; ++m)
 #pragma unroll
@@ -155,20 +136,71 @@ __global__ void __launch_bounds__(256) conv_kernel(const ConvArgs<TO> a) {
     }
 }
 
-// y[v][c] (+)= act(sum_s part[s][v][c] + bias[c] + skip[v][c]).
-template <typename TO>
-__global__ void __launch_bounds__(256) conv_reduce_kernel(const ConvArgs<TO> a, int V) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t nel = (size_t)V * a.co;
-  if (e >= nel) return;
-  const int c = (int)(e % a.co);
-  float v = 0.f;
-  for (int q = 0; q < a.nsplit; ++q) v += a.part[(size_t)q * nel + e];
-  if (a.bias) v += a.bias[c];
-  if (a.skip) v += (float)a.skip[e];
-  if (a.act) v = act_f(v);
-  if (a.accumulate) v += (float)a.y[e];
-  a.y[e] = (TO)v;
+// The same implicit GEMM for the coarse grids, whose few voxel tiles would
+// leave most of the machine idle: a block of KW waves owns one tile of 16
+// consecutive voxels and 16*NT output channels and splits the K chunks among
+// its waves (wave w takes chunks w, w + KW, ...); the waves' FP32
+// accumulators are added in LDS, in wave order, and the block's threads
+// apply bias, skip, activation and accumulation to the sums.
+template <typename TO, int NT, int KW>
+__global__ void __launch_bounds__(64 * KW) conv_kw_kernel(const ConvArgs<TO> a) {
+  __shared__ float red[KW][NT][4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int V = a.nx * a.ny * a.nz;
+  const int vbase = blockIdx.x * 16;
+  const int co0 = blockIdx.y * 16 * NT;
+  const int g = lane >> 4, l16 = lane & 15;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ktot = 27 * a.ci;
+  const int nchunk = a.kp >> 5;
+  const int vm = vbase + l16;
+  const bool vin = vm < V;
+  const int xm = vm % a.nx, ym = (vm / a.nx) % a.ny, zm = vm / (a.nx * a.ny);
+  const bf16 *wrow[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) wrow[n] = a.w + (size_t)(co0 + 16 * n + l16) * a.kp;
+  for (int kc = wv; kc < nchunk; kc += KW) {
+    const int k0 = (kc << 5) + (g << 3);
+    bf16x8 bv[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bv[n] = *(const bf16x8 *)(wrow[n] + k0);
+    const int t = k0 >> a.lci;
+    const int c0 = k0 & (a.ci - 1);
+    bf16x8 av;
+    if (k0 < ktot && vin) {
+      const int vx = wrapi(xm + t % 3 - 1, a.nx), vy = wrapi(ym + (t / 3) % 3 - 1, a.ny),
+                vz = wrapi(zm + t / 9 - 1, a.nz);
+      av = *(const bf16x8 *)(a.x + (size_t)(vx + a.nx * (vy + a.ny * vz)) * a.ci + c0);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) av[q] = (bf16)0.f;
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+      acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[n], 0, 0, 0);
+  }
+#pragma unroll
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wv][n][i][lane] = acc[n][i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < NT * 4 * 64; e += 64 * KW) {
+    const int n = e >> 8, i = (e >> 6) & 3, ln = e & 63;
+    const int vox = vbase + 4 * (ln >> 4) + i;
+    if (vox >= V) continue;
+    float v = red[0][n][i][ln];
+#pragma unroll
+    for (int w = 1; w < KW; ++w) v += red[w][n][i][ln];
+    const int c = co0 + 16 * n + (ln & 15);
+    const size_t o = (size_t)vox * a.co + c;
+    if (a.bias) v += a.bias[c];
+    if (a.skip) v += (float)a.skip[o];
+    if (a.act) v = act_f(v);
+    if (a.accumulate) v += (float)a.y[o];
+    a.y[o] = (TO)v;
+  }
 }
 
 // The same implicit GEMM with its operands staged through LDS, for the fine
@@ -345,7 +377,7 @@ inline long long lds_bricks(int nx, int ny, int nz, int ty) {
 // (the natural 16*NT channels, at most 2 tiles for LDS, or 1 tile when that
 // gives the coarser grids enough blocks); ty = 0 when it does not apply
 // (64 input channels: one-row bricks only; 128: never; the coarsest grids
-// take the split-K path).
+// split the K chunks among the waves of a block, conv_kw_kernel).
 struct LdsPlan {
   int ty = 0, nt = 0;
 };
@@ -392,46 +424,38 @@ void launch_conv_lds_any(const ConvArgs<TO> &a, LdsPlan p, hipStream_t s) {
   else launch_conv_lds<TO, 2, 64, UP>(a, p.ty, s);
 }
 
-// Blocks a convolution launches without voxel-tile blocking or split-K.
+// Blocks a convolution launches with one 16-voxel tile per wave.
 inline int conv_blocks(int V, int co) {
   const int nt = co >= 64 ? 4 : (co >= 32 ? 2 : 1);
   return ((V + 63) / 64) * (co / (16 * nt));
 }
 
-// K slices for a coarse grid: enough blocks to fill the machine (>= 2048),
-// at most one slice per 32-wide K chunk, at most 27.
-inline int conv_splits(int V, int co, int kp) {
-  const int b = conv_blocks(V, co);
-  if (b >= 1024) return 1;
-  return std::max(1, std::min({(2048 + b - 1) / b, kp / 32, 27}));
-}
+// Waves per block of the coarse-grid kernel.
+constexpr int kConvKW = 8;
 
 template <typename TO, int NT>
-hipError_t launch_conv_nt(ConvArgs<TO> a, int V, float *ws, size_t ws_n, hipStream_t s) {
+hipError_t launch_conv_nt(const ConvArgs<TO> &a, int V, hipStream_t s) {
   // Four voxel tiles per wave (four independent gathers per K chunk) once
-  // the grid still has >= 1024 blocks (4096 waves); on coarse grids, split K
-  // instead.
+  // the grid still has >= 1024 blocks (4096 waves); one tile per wave while
+  // it has >= 1024 blocks that way; on coarser grids, the K chunks split
+  // among the waves of a block.
   constexpr int kMV = 4;
   const int gy = a.co / (16 * NT);
-  const int split = ws ? conv_splits(V, a.co, a.kp) : 1;
-  if (split > 1 && (size_t)split * V * a.co <= ws_n) {
-    a.part = ws;
-    a.nsplit = split;
-    dim3 grid((V + 63) / 64, gy, split);
-    conv_kernel<TO, NT, 1><<<grid, 256, 0, s>>>(a);
-    conv_reduce_kernel<TO><<<(int)(((size_t)V * a.co + 255) / 256), 256, 0, s>>>(a, V);
-  } else if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 1024) {
+  if ((long long)((V + 64 * kMV - 1) / (64 * kMV)) * gy >= 1024) {
     dim3 grid((V + 64 * kMV - 1) / (64 * kMV), gy);
     conv_kernel<TO, NT, kMV><<<grid, 256, 0, s>>>(a);
-  } else {
+  } else if (conv_blocks(V, a.co) >= 1024) {
     dim3 grid((V + 63) / 64, gy);
     conv_kernel<TO, NT, 1><<<grid, 256, 0, s>>>(a);
+  } else {
+    dim3 grid((V + 15) / 16, gy);
+    conv_kw_kernel<TO, NT, kConvKW><<<grid, 64 * kConvKW, 0, s>>>(a);
   }
   return hipGetLastError();
 }
 
 template <typename TO>
-hipError_t launch_conv(const ConvArgs<TO> &a, float *ws, size_t ws_n, hipStream_t s) {
+hipError_t launch_conv(const ConvArgs<TO> &a, hipStream_t s) {
   const int V = a.nx * a.ny * a.nz;
   const int nt = a.co >= 64 ? 4 : (a.co >= 32 ? 2 : 1);
   // Fine levels (16, 32 or 64 input channels, enough blocks): the LDS-staged
@@ -441,9 +465,9 @@ hipError_t launch_conv(const ConvArgs<TO> &a, float *ws, size_t ws_n, hipStream_
     launch_conv_lds_any<TO, false>(a, lp, s);
     return hipGetLastError();
   }
-  if (nt == 4) return launch_conv_nt<TO, 4>(a, V, ws, ws_n, s);
-  if (nt == 2) return launch_conv_nt<TO, 2>(a, V, ws, ws_n, s);
-  return launch_conv_nt<TO, 1>(a, V, ws, ws_n, s);
+  if (nt == 4) return launch_conv_nt<TO, 4>(a, V, s);
+  if (nt == 2) return launch_conv_nt<TO, 2>(a, V, s);
+  return launch_conv_nt<TO, 1>(a, V, s);
 }
 
 // Weight gradient: dW[co][tap*ci + c] = sum_v dz[v][co] * x[v + off(tap)][c],
@@ -481,14 +505,20 @@ __global__ void __launch_bounds__(256) reduce_parts_kernel(const float *part, in
 }
 
 // ------------------------------------------------------- elementwise pieces
+// x[v] = (in[0][v], in[1][v], 0, ..., 0), C a multiple of 8 (16-byte stores).
 __global__ void pack_input_kernel(const float *__restrict__ in, bf16 *__restrict__ x, int V,
                                   int C) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
-  bf16 *row = x + (size_t)v * C;
-  row[0] = (bf16)in[v];
-  row[1] = (bf16)in[V + v];
-  for (int c = 2; c < C; ++c) row[c] = (bf16)0.f;
+  bf16x8 *row = (bf16x8 *)(x + (size_t)v * C);
+  bf16x8 z;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) z[q] = (bf16)0.f;
+  bf16x8 h = z;
+  h[0] = (bf16)in[v];
+  h[1] = (bf16)in[V + v];
+  row[0] = h;
+  for (int g = 1; g < C / 8; ++g) row[g] = z;
 }
 
 // The first convolution in the forward pass: only 2 of its 16 (padded)
@@ -521,8 +551,8 @@ __global__ void __launch_bounds__(256) conv_first_kernel(const bf16 *__restrict_
   for (int t = 0; t < 27; ++t) {
     const int vx = wrapi(i + t % 3 - 1, nx), vy = wrapi(j + (t / 3) % 3 - 1, ny),
               vz = wrapi(k + t / 9 - 1, nz);
-    const bf16 *src = x + (size_t)(vx + nx * (vy + ny * vz)) * 16;
-    const float a0 = (float)src[0], a1 = (float)src[1];
+    const bf16x2 pr = *(const bf16x2 *)(x + (size_t)(vx + nx * (vy + ny * vz)) * 16);
+    const float a0 = (float)pr[0], a1 = (float)pr[1];
 #pragma unroll
     for (int co = 0; co < 16; ++co) acc[co] += a0 * ws[t][0][co] + a1 * ws[t][1][co];
   }
@@ -537,21 +567,33 @@ __global__ void __launch_bounds__(256) conv_first_kernel(const bf16 *__restrict_
   dst[1] = o1;
 }
 
+// 2x average pooling onto the grid (nx/2, ny/2, nz/2): one thread per
+// coarse voxel and group of 8 channels (16-byte loads and stores), each
+// channel's 8 children summed in the same order.
 __global__ void pool_fwd_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int nx,
                                 int ny, int nz, int C) {
-  // out grid (nx/2, ny/2, nz/2)
   const int mx = nx / 2, my = ny / 2, mz = nz / 2;
+  const int G = C / 8;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)mx * my * mz * C) return;
-  const int c = idx % C;
-  const int v = idx / C;
+  if (idx >= (size_t)mx * my * mz * G) return;
+  const int cg = idx % G;
+  const int v = idx / G;
   const int i = v % mx, j = (v / mx) % my, k = v / (mx * my);
-  float s = 0.f;
+  bf16x8 ch[8];
+#pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int u = (2 * i + (q & 1)) + nx * ((2 * j + ((q >> 1) & 1)) + ny * (2 * k + (q >> 2)));
-    s += (float)in[(size_t)u * C + c];
+    ch[q] = *(const bf16x8 *)(in + (size_t)u * C + 8 * cg);
   }
-  out[idx] = (bf16)(0.125f * s);
+  bf16x8 o;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += (float)ch[q][c];
+    o[c] = (bf16)(0.125f * s);
+  }
+  *(bf16x8 *)(out + (size_t)v * C + 8 * cg) = o;
 }
 
 // g_in (fine, FP32) += g_out(coarse)/8
@@ -566,16 +608,18 @@ __global__ void pool_bwd_kernel(const float *__restrict__ gout, float *__restric
   gin[idx] += 0.125f * gout[(size_t)p * C + c];
 }
 
-// nearest up-sampling: fine (nx, ny, nz) from coarse (nx/2, ...)
+// nearest up-sampling: fine (nx, ny, nz) from coarse (nx/2, ...), one thread
+// per fine voxel and group of 8 channels.
 __global__ void up_fwd_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int nx,
                               int ny, int nz, int C) {
+  const int G = C / 8;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)nx * ny * nz * C) return;
-  const int c = idx % C;
-  const int v = idx / C;
+  if (idx >= (size_t)nx * ny * nz * G) return;
+  const int cg = idx % G;
+  const int v = idx / G;
   const int i = v % nx, j = (v / nx) % ny, k = v / (nx * ny);
   const int p = i / 2 + (nx / 2) * (j / 2 + (ny / 2) * (k / 2));
-  out[idx] = in[(size_t)p * C + c];
+  *(bf16x8 *)(out + (size_t)v * C + 8 * cg) = *(const bf16x8 *)(in + (size_t)p * C + 8 * cg);
 }
 
 // g_in (coarse) += sum of the 8 fine children
@@ -727,8 +771,6 @@ struct Net {
   size_t gbytes = 0;
   float *part = nullptr;   // wgrad partial sums
   size_t part_n = 0;
-  float *ws = nullptr;     // split-K partial sums of coarse-grid convolutions
-  size_t ws_n = 0;
   double *mse_part = nullptr;
   double *hpart = nullptr;
   void *act_arena = nullptr;
@@ -744,7 +786,6 @@ struct Net {
     if (act_arena) (void)hipFree(act_arena);
     if (gbuf) (void)hipFree(gbuf);
     if (part) (void)hipFree(part);
-    if (ws) (void)hipFree(ws);
     if (hpart) (void)hipHostFree(hpart);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
@@ -837,16 +878,6 @@ int alloc_activations(Net &n, int nx, int ny, int nz) {
   const size_t slabs = std::max<size_t>(1, std::min<size_t>(512, n.V[0] / 4096));
   n.part_n = maxe * slabs;
   NNCHK(hipMalloc((void **)&n.part, n.part_n * 4));
-  // Split-K workspace: the largest slices * V * co over forward and
-  // data-gradient convolutions.
-  if (n.ws) { (void)hipFree(n.ws); n.ws = nullptr; }
-  n.ws_n = 0;
-  for (const Conv &c : n.conv) {
-    const int V = (int)n.V[c.level];
-    n.ws_n = std::max(n.ws_n, (size_t)conv_splits(V, c.co, c.kpf) * V * c.co);
-    n.ws_n = std::max(n.ws_n, (size_t)conv_splits(V, c.ci, c.kpb) * V * c.ci);
-  }
-  if (n.ws_n) NNCHK(hipMalloc((void **)&n.ws, n.ws_n * 4));
   return 0;
 }
 
@@ -904,7 +935,7 @@ int conv_fwd(Net &n, int li, const bf16 *x, const bf16 *skip, bf16 *y) {
   a.nx = n.nx[c.level]; a.ny = n.ny[c.level]; a.nz = n.nz[c.level];
   a.lci = ilog2(c.ci); a.ci = c.ci; a.co = c.co; a.kp = c.kpf;
   a.act = 1; a.accumulate = 0;
-  NNCHK(launch_conv<bf16>(a, n.ws, n.ws_n, st()));
+  NNCHK(launch_conv<bf16>(a, st()));
   n.last_flops += 54.0 * c.ci * c.co * (double)n.V[c.level];
   return 0;
 }
@@ -959,7 +990,7 @@ int conv_bwd(Net &n, int li, const float *gy, const bf16 *y, const bf16 *x, floa
     a.nx = n.nx[L]; a.ny = n.ny[L]; a.nz = n.nz[L];
     a.lci = ilog2(c.co); a.ci = c.co; a.co = c.ci; a.kp = c.kpb;
     a.act = 0; a.accumulate = 1;
-    NNCHK(launch_conv<float>(a, n.ws, n.ws_n, st()));
+    NNCHK(launch_conv<float>(a, st()));
   }
   return 0;
 }
@@ -972,19 +1003,19 @@ int forward(Net &n, const float *in, float *out) {
   pack_input_kernel<<<elem_blocks(n.V[0]), 256, 0, st()>>>(in, n.xin, (int)n.V[0], 16);
   NCHK(conv_fwd(n, 0, n.xin, nullptr, n.e0a));
   NCHK(conv_fwd(n, 1, n.e0a, nullptr, n.e0));
-  pool_fwd_kernel<<<elem_blocks(n.V[1] * 16), 256, 0, st()>>>(n.e0, n.p1, n.nx[0], n.ny[0],
+  pool_fwd_kernel<<<elem_blocks(n.V[1] * 2), 256, 0, st()>>>(n.e0, n.p1, n.nx[0], n.ny[0],
                                                                n.nz[0], 16);
   NCHK(conv_fwd(n, 2, n.p1, nullptr, n.e1a));
   NCHK(conv_fwd(n, 3, n.e1a, nullptr, n.e1));
-  pool_fwd_kernel<<<elem_blocks(n.V[2] * 32), 256, 0, st()>>>(n.e1, n.p2, n.nx[1], n.ny[1],
+  pool_fwd_kernel<<<elem_blocks(n.V[2] * 4), 256, 0, st()>>>(n.e1, n.p2, n.nx[1], n.ny[1],
                                                                n.nz[1], 32);
   NCHK(conv_fwd(n, 4, n.p2, nullptr, n.e2a));
   NCHK(conv_fwd(n, 5, n.e2a, nullptr, n.e2));
-  pool_fwd_kernel<<<elem_blocks(n.V[3] * 64), 256, 0, st()>>>(n.e2, n.p3, n.nx[2], n.ny[2],
+  pool_fwd_kernel<<<elem_blocks(n.V[3] * 8), 256, 0, st()>>>(n.e2, n.p3, n.nx[2], n.ny[2],
                                                                n.nz[2], 64);
   NCHK(conv_fwd(n, 6, n.p3, nullptr, n.e3a));
   NCHK(conv_fwd(n, 7, n.e3a, nullptr, n.e3));
-  up_fwd_kernel<<<elem_blocks(n.V[2] * 128), 256, 0, st()>>>(n.e3, n.u2, n.nx[2], n.ny[2],
+  up_fwd_kernel<<<elem_blocks(n.V[2] * 16), 256, 0, st()>>>(n.e3, n.u2, n.nx[2], n.ny[2],
                                                               n.nz[2], 128);
   NCHK(conv_fwd(n, 8, n.u2, n.e2, n.d2a));
   NCHK(conv_fwd(n, 9, n.d2a, nullptr, n.d2));
@@ -992,7 +1023,7 @@ int forward(Net &n, const float *in, float *out) {
   n.up_stale[1] = conv_fwd_up(n, 10, n.d2, n.e1, n.d1a, &rc);
   NCHK(rc);
   if (!n.up_stale[1]) {
-    up_fwd_kernel<<<elem_blocks(n.V[1] * 64), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
+    up_fwd_kernel<<<elem_blocks(n.V[1] * 8), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
                                                                n.nz[1], 64);
     NCHK(conv_fwd(n, 10, n.u1, n.e1, n.d1a));
   }
@@ -1000,7 +1031,7 @@ int forward(Net &n, const float *in, float *out) {
   n.up_stale[0] = conv_fwd_up(n, 12, n.d1, n.e0, n.d0a, &rc);
   NCHK(rc);
   if (!n.up_stale[0]) {
-    up_fwd_kernel<<<elem_blocks(n.V[0] * 32), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
+    up_fwd_kernel<<<elem_blocks(n.V[0] * 4), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
                                                                n.nz[0], 32);
     NCHK(conv_fwd(n, 12, n.u0, n.e0, n.d0a));
   }
@@ -1019,7 +1050,7 @@ int backward(Net &n, const float *dout) {
   NNCHK(hipGetLastError());
   NCHK(conv_bwd(n, 13, n.g_d0, n.d0, n.d0a, nullptr, n.g_d0a));
   if (n.up_stale[0]) {  // the forward pass read d1 up-sampled without forming u0
-    up_fwd_kernel<<<elem_blocks(n.V[0] * 32), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
+    up_fwd_kernel<<<elem_blocks(n.V[0] * 4), 256, 0, st()>>>(n.d1, n.u0, n.nx[0], n.ny[0],
                                                                n.nz[0], 32);
     NNCHK(hipGetLastError());
     n.up_stale[0] = false;
@@ -1029,7 +1060,7 @@ int backward(Net &n, const float *dout) {
                                                              n.nz[0], 32);
   NCHK(conv_bwd(n, 11, n.g_d1, n.d1, n.d1a, nullptr, n.g_d1a));
   if (n.up_stale[1]) {
-    up_fwd_kernel<<<elem_blocks(n.V[1] * 64), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
+    up_fwd_kernel<<<elem_blocks(n.V[1] * 8), 256, 0, st()>>>(n.d2, n.u1, n.nx[1], n.ny[1],
                                                                n.nz[1], 64);
     NNCHK(hipGetLastError());
     n.up_stale[1] = false;

@@ -74,6 +74,10 @@ struct SorArgs {
   // storage added to pin as it is loaded (pin + e is the iterate the sweep
   // starts from; pin itself is not modified), nx even.
   const float *efold = nullptr;
+  // Optional (T = double): psum[tile] = the FP64 sum of the new iterate over
+  // the nodes the tile owns, in a fixed order (tile = the XCD-remapped block
+  // index, tl.blocks() entries).
+  double *psum = nullptr;
 };
 
 template <typename T, typename TR>

@@ -374,6 +374,9 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   // maximum of |pn - pc| values that are themselves values of type T).
   T dmax = T(0);
   auto upd = [&](T d) { dmax = d > dmax ? d : dmax; };
+  // The owned nodes' new values, summed (FP64 judge sweep only).
+  const bool want_sum = sizeof(T) == 8 && a.psum != nullptr;
+  double vsum = 0.0;
   auto step = [&](int s, auto K) {
     constexpr int k = decltype(K)::value;
     // Slots of planes s + j: s = z0 - 1 + k*PPS + (a multiple of every
@@ -444,7 +447,10 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
         const bool blk = ((bpar + sb) & 1) != 0;
         const T v = blk ? w1 * bold[q] + w * pn : c;
         upd(blk && bown ? T(fabs(pn - bold[q])) : T(0));
-        if (bown) (a.pout + (size_t)plane * sb)[(unsigned)(bi + nx * bj)] = v;
+        if (bown) {
+          (a.pout + (size_t)plane * sb)[(unsigned)(bi + nx * bj)] = v;
+          if (want_sum) vsum += (double)v;
+        }
       }
     }
   };
@@ -463,6 +469,10 @@ __global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
   const double dm = block_max<SNW>((double)dmax, scr);
   if (threadIdx.x == 0 && dm != 0.0)
     atomicMax(&st->dmax_bits[it & 7], (unsigned long long)__double_as_longlong(dm));
+  if (want_sum) {
+    const double bs = block_sum<SNW>(vsum, scr);
+    if (threadIdx.x == 0) a.psum[t] = bs;
+  }
 }
 
 // ------------------------------------------------ colour-split half sweeps

@@ -199,6 +199,65 @@ def _residual_order(q, rhs32, d):
     return (rhs32.astype(np.float64) - lap).astype(np.float32)
 
 
+def _residual_bc(q, rhs32, d, per):
+    """rhs - L q with the SOR neighbour rules (periodic wrap, or the
+    free-slip mirror q(-1) = q(1), q(n) = q(n - 2)) and the residual
+    kernels' operation order, rounded to FP32; q is (nz, ny, nx) and per
+    holds the periodic flags of x, y, z."""
+    def nb(axis, s):
+        n = q.shape[axis]
+        idx = np.arange(n) + s
+        if per[2 - axis]:
+            idx = idx % n
+        else:
+            idx = np.where(idx < 0, -idx, np.where(idx > n - 1, 2 * (n - 1) - idx, idx))
+        return np.take(q, idx, axis=axis)
+    ox, oy, oz = (1.0 / h ** 2 for h in d)
+    A = -(2.0 * ox + 2.0 * oy + 2.0 * oz)
+    lap = (ox * (nb(2, -1) + nb(2, 1)) + oy * (nb(1, -1) + nb(1, 1))
+           + oz * (nb(0, -1) + nb(0, 1)) + A * q)
+    return (rhs32.astype(np.float64) - lap).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["periodic", "free-slip y"])
+def test_residual_kernel_is_bit_exact(kind):
+    """The LDS-tiled residual r = rhs - L p (the first step of a warm form,
+    before any increment exists) against numpy, bit for bit, from a random
+    pressure on a 72 x (38 or 26) x 22 grid: partial tiles in x (64 + 8) and
+    y (rows of 4) and a partial last z chunk (8 + 8 + 6 planes), so every
+    halo duty, the periodic wrap and the free-slip mirror are exercised."""
+    import torch
+
+    from osinco3d_amd import capi
+    from osinco3d_amd.config import mixing_layer_config
+    from osinco3d_amd.solver import taylor_green
+    if kind == "periodic":
+        cfg = tgv_config(32, 2, 3)
+        cfg.nx, cfg.ny, cfg.nz = 72, 38, 22
+        per = (1, 1, 1)
+    else:
+        cfg = mixing_layer_config(72, 26, 22, steps=2)
+        per = (1, 0, 1)
+    shape = (cfg.nz, cfg.ny, cfg.nx)
+    print(f"[gpu] residual kernel bit-exact {kind} {shape}", flush=True)
+    d = cfg.spacing()
+    u = taylor_green(cfg)[:3]
+    p0 = np.random.default_rng(17).standard_normal(shape)
+    s = Solver(cfg, MIXED, WARM_EXTRAPOLATE)
+    s.upload(*u, p0)
+    r = torch.empty(shape, dtype=torch.float32, device="cuda")
+    ep = torch.empty_like(r)
+    e = torch.empty_like(r)
+    capi.check(capi.load().o3d_solver_capture(s.handle, capi.ptr(r), capi.ptr(ep), capi.ptr(e)))
+    s.step(1)
+    rhs = s.field(7)
+    want = _residual_bc(p0, rhs, d, per)
+    got = r.cpu().numpy()
+    capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
+    s.close()
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
 @pytest.mark.parametrize("adaptive", [0, 1])
 def test_guess_residual_is_bit_exact(adaptive):
     """The guess the extrapolated warm start forms and its FP32 residual,
