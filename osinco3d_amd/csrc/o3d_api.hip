@@ -1591,8 +1591,9 @@ struct Solver {
   double *p0part = nullptr;    // per-block norms of the residual of p0_of (guess pass)
   const double *p0_of = nullptr;  // the pressure those norms belong to (null: none)
   double *hnpart = nullptr;    // pinned copy
-  double *epart = nullptr;     // sum-of-squares partials of e_prev
+  double *epart = nullptr;     // sum-of-squares partials of e_prev (one per correction block)
   double *hepart = nullptr;    // pinned copy, valid after the step's final sync
+  int ne = 0;                  // entries of epart / hepart
   double *mpart = nullptr;     // mean-pressure partials + the mean (mixed, every axis periodic)
   double *jpart = nullptr;     // per-tile sums of p from the FP64 judge sweep (with mpart)
   int jblocks = 0;             // tiles of the judge sweep
@@ -1751,8 +1752,15 @@ int solver_divergence(Solver &S, const void *const f[3], T *out, double divisor,
   return 0;
 }
 
+// Partials of the FP64 pressure sum (p_sum_kernel); the mean follows them.
+constexpr int kMeanBlocks = 1024;
+
+// The velocity correction; with a pending mean, the pressure's shift
+// (p - mean into the spare pressure buffer, which becomes the current one);
+// with incr, the increment e_prev = p - p_prev and its sum-of-squares
+// partials, all in the one pass over p.
 template <typename T>
-int solver_correct(Solver &S) {
+int solver_correct(Solver &S, bool incr = false) {
   CorrectArgs<T, double> a;
   for (int c = 0; c < 3; ++c) { a.u[c] = (T *)S.u[c]; a.up[c] = (const T *)S.up[c]; }
   a.p = S.P[S.cur];
@@ -1763,7 +1771,18 @@ int solver_correct(Solver &S) {
   const double h[3] = {S.p.dx, S.p.dy, S.p.dz};
   fill_coef<double>(a.d1, h, false);
   a.dt = S.p.dt;
+  if (S.mean_pending) {
+    a.mean = S.mpart + kMeanBlocks;
+    a.pout = S.P[1 - S.cur];
+  }
+  if (incr) {
+    a.p0 = S.Pstart;
+    a.e = S.Eprev;
+    a.epart = S.epart;
+  }
   HIPCHK(launch_correct<T, double>(a, g_stream));
+  if (a.pout) S.cur = 1 - S.cur;
+  S.mean_pending = false;
   return 0;
 }
 
@@ -1835,7 +1854,6 @@ int solver_poisson_f64(Solver &S, o3d_step_info *info) {
 // pressure is defined up to a constant, which the FP32 sweeps let drift).
 // The FP64 sum is formed in a fixed order: per block over a contiguous
 // chunk (2 doubles per thread and load), then the partials in index order.
-constexpr int kMeanBlocks = 1024;
 __global__ void __launch_bounds__(256) p_sum_kernel(const double *__restrict__ p, int n,
                                                     double *__restrict__ part) {
   __shared__ double sv[4];
@@ -1863,22 +1881,9 @@ __global__ void __launch_bounds__(1024) p_mean_kernel(const double *__restrict__
   s = block_sum<16>(s, sv);
   if (threadIdx.x == 0) mp[0] = s / (double)n;
 }
-__global__ void __launch_bounds__(256) p_sub_mean_kernel(double *__restrict__ p, int n,
-                                                         const double *__restrict__ mp) {
-  const double mean = mp[0];
-  double2 *p2 = (double2 *)p;
-  const int n2 = n / 2;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
-    double2 v = p2[i];
-    v.x -= mean;
-    v.y -= mean;
-    p2[i] = v;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) p[n - 1] -= mean;
-}
 
-// The mean of the pressure into part[kMeanBlocks] (the subtraction follows,
-// fused into the increment pass of the warm-start forms, else on its own).
+// The mean of the pressure into part[kMeanBlocks] (the velocity correction
+// pass applies the shift).
 int pressure_mean(double *p, int n, double *part) {
   p_sum_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(p, n, part);
   HIPCHK(hipGetLastError());
@@ -2185,45 +2190,6 @@ __global__ void __launch_bounds__(256) mg_prolong_fine_kernel(double *__restrict
   *o = w;
 }
 
-// e = p - p0 (float32) and the per-block sum of squares of e; pairs of
-// cells per thread (16-byte loads), the odd last cell by thread 0.
-constexpr int kIncBlocks = 2048;
-// With mp (the periodic mean-pressure removal pending), p -= mp[0] first,
-// written back, and e is formed from the shifted p.
-__global__ void __launch_bounds__(256) lw_increment_kernel(double *__restrict__ p,
-                                                           const double *__restrict__ p0,
-                                                           float *__restrict__ e, int n,
-                                                           double *__restrict__ part,
-                                                           const double *__restrict__ mp) {
-  __shared__ double sc[4];
-  double s = 0.0;
-  const int np = n / 2;
-  const double m = mp ? mp[0] : 0.0;
-  double2 *p2 = reinterpret_cast<double2 *>(p);
-  const double2 *q2 = reinterpret_cast<const double2 *>(p0);
-  float2 *e2 = reinterpret_cast<float2 *>(e);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
-    double2 a = p2[i];
-    const double2 b = q2[i];
-    if (mp) {
-      a.x -= m;
-      a.y -= m;
-      p2[i] = a;
-    }
-    const float2 v = make_float2((float)(a.x - b.x), (float)(a.y - b.y));
-    e2[i] = v;
-    s += (double)v.x * v.x;
-    s += (double)v.y * v.y;
-  }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-    if (mp) p[n - 1] -= m;
-    const float v = (float)(p[n - 1] - p0[n - 1]);
-    e[n - 1] = v;
-    s += (double)v * v;
-  }
-  s = block_sum<4>(s, sc);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
 __global__ void __launch_bounds__(256) lw_sumsq_kernel(const float *a, int n, double *part) {
   __shared__ double sc[4];
   double s = 0.0;
@@ -2314,8 +2280,9 @@ int lw_alloc(Solver &S) {
   CHK(S.alloc(&S.npart, np));
   CHK(S.alloc(&S.p0part, 2 * residual_blocks(S.p.nx, S.p.ny, S.p.nz)));
   HIPCHK(hipHostMalloc((void **)&S.hnpart, 2 * kLwBlocks * sizeof(double)));
-  CHK(S.alloc(&S.epart, kIncBlocks));
-  HIPCHK(hipHostMalloc((void **)&S.hepart, kIncBlocks * sizeof(double)));
+  S.ne = S.tl_cor.blocks();
+  CHK(S.alloc(&S.epart, (size_t)S.ne));
+  HIPCHK(hipHostMalloc((void **)&S.hepart, (size_t)S.ne * sizeof(double)));
   return 0;
 }
 
@@ -2441,7 +2408,7 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
     // Sum of squares of e_prev: reduced when e_prev was formed, read back by
     // the previous step's final synchronisation.
     double se2 = 0.0;
-    for (int b = 0; b < kIncBlocks; ++b) se2 += S.hepart[b];
+    for (int b = 0; b < S.ne; ++b) se2 += S.hepart[b];
     const int rc = lw_features(w, S.Rl, S.Eprev, nullptr, S.p.nx, S.p.ny, S.p.nz, h2, S.Feat,
                                nullptr, &re, se2);
     if (rc < 0) return rc;
@@ -2480,7 +2447,6 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
       HIPCHK(launch_residual_f32<float>(q, (const float *)S.rhs, S.Rl, S.Out, S.p.nx, S.p.ny,
                                         S.p.nz, per, S.sc.ox, S.sc.oy, S.sc.oz, S.sc.A,
                                         g_stream));
-    const int nc = n / (F * F * F);
     HIPCHK(launch_lw_pool(S.Rl, S.mg.f[0], S.p.nx, S.p.ny, S.p.nz, F, 1.0f));
     if (nn_coarse) HIPCHK(mg_seed_from_level1(S.mg, g_stream));
     // From the network's coarse estimate the first correction may run fewer
@@ -2503,19 +2469,14 @@ int solver_warm_guess(Solver &S, int *used, double *warm) {
   return 0;
 }
 
-// After the solve: e = p_new - p_prev becomes the next step's e_prev.
+// After the correction pass formed e_prev = p_new - p_prev: its sum-of-squares
+// partials to the host (read at the next step), the capture, the history.
 int solver_learning_finish(Solver &S) {
-  const int n = S.n;
-  std::swap(S.Eprev, S.Eprev2);
-  lw_increment_kernel<<<kIncBlocks, 256, 0, g_stream>>>(
-      S.P[S.cur], S.Pstart, S.Eprev, n, S.epart,
-      S.mean_pending ? S.mpart + kMeanBlocks : nullptr);
-  S.mean_pending = false;
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(S.hepart, S.epart, kIncBlocks * sizeof(double), hipMemcpyDeviceToHost,
-                        g_stream));
+  HIPCHK(hipMemcpyAsync(S.hepart, S.epart, (size_t)S.ne * sizeof(double),
+                        hipMemcpyDeviceToHost, g_stream));
   if (S.cap_e)
-    HIPCHK(hipMemcpyAsync(S.cap_e, S.Eprev, (size_t)n * 4, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(S.cap_e, S.Eprev, (size_t)S.n * 4, hipMemcpyDeviceToDevice,
+                          g_stream));
   S.hist++;
   return 0;
 }
@@ -2548,14 +2509,11 @@ int solver_step_t(Solver &S, int itime, o3d_step_info *info) {
   else CHK(solver_poisson_f64(S, info));
   info->warm_used = warm;
   HIPCHK(hipEventRecord(S.ev[5], g_stream));
+  // The correction pass also applies the pending mean shift and, in the
+  // warm forms, forms e_prev = p - p_prev (the next step's increment).
+  if (learning) std::swap(S.Eprev, S.Eprev2);
+  CHK(solver_correct<T>(S, learning));
   if (learning) CHK(solver_learning_finish(S));
-  if (S.mean_pending) {
-    p_sub_mean_kernel<<<kMeanBlocks, 256, 0, g_stream>>>(S.P[S.cur], S.n,
-                                                         S.mpart + kMeanBlocks);
-    HIPCHK(hipGetLastError());
-    S.mean_pending = false;
-  }
-  CHK(solver_correct<T>(S));
   CHK(solver_divergence<T>(S, S.u, nullptr, 1.0, S.part_div2));
   // The step's statistics (their partials stay untouched until here): the
   // rhs divergence, the diagnostic divergence, the three corrected

@@ -73,6 +73,17 @@ struct CorrectArgs {
   Tiling tl;
   TP d1[3][3];
   TP dt;
+  // Optional, after a solve of the mixed forms.  mean: p - mean[0] is the
+  // pressure (every loaded value shifted, so the gradient is the shifted
+  // pressure's) and pout receives it at the owned nodes (another buffer:
+  // neighbouring tiles still load p).  p0: the FP32 increment
+  // e = (p - mean) - p0 of the owned nodes into e, with the block's sum of
+  // e^2 in epart[block].
+  const double *mean = nullptr;
+  TP *pout = nullptr;
+  const TP *p0 = nullptr;
+  float *e = nullptr;
+  double *epart = nullptr;
 };
 
 template <typename T>

@@ -619,6 +619,9 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
 
   FieldPartial st[3];
   partial_init(st[0]); partial_init(st[1]); partial_init(st[2]);
+  // The shift of every loaded pressure value (p - 0 = p exactly without one).
+  const TP mv = A.mean ? (TP)A.mean[0] : TP(0);
+  double es = 0.0;  // sum of e^2 over the owned nodes
   // p column queue (planes k-3..k+3) and raw loads in flight in two-slot
   // rings (slots fixed at compile time by the unrolled pair of steps): the
   // load of plane k+5 is issued at step k and joins the queue at the end of
@@ -631,7 +634,7 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
   };
   TP q[7];
 #pragma unroll
-  for (int s = 0; s < 7; ++s) q[s] = A.p[pz(P.z0 - 3 + s)];
+  for (int s = 0; s < 7; ++s) q[s] = A.p[pz(P.z0 - 3 + s)] - mv;
   TP pr[2];
   pr[1] = A.p[pz(P.z0 + 4)];
   TP nhx[2] = {TP(0), TP(0)}, nhy[2] = {TP(0), TP(0)};
@@ -644,11 +647,13 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
     nhy[sl] = A.p[hyi + pk];
   };
   T upv[2][3];
+  TP p0v[2] = {TP(0), TP(0)};
   const int own = P.valid ? P.i + nx * P.j : 0;
   auto load_up = [&](int kp, int sl) {
     const int idx = own + plane * min(kp, nz - 1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) upv[sl][c] = A.up[c][idx];
+    if (A.p0) p0v[sl] = A.p0[idx];
   };
   load_halo(P.z0, 0);
   load_halo(P.z0 + 1, 1);
@@ -658,8 +663,8 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
     constexpr int sl = decltype(SL)::value;
     const int pk = plane * k;
     pl[ty + 3][tx + 3] = q[3];
-    if (hx_col >= 0) pl[ty + 3][hx_col] = nhx[sl];
-    if (hy_row >= 0) pl[hy_row][tx + 3] = nhy[sl];
+    if (hx_col >= 0) pl[ty + 3][hx_col] = nhx[sl] - mv;
+    if (hy_row >= 0) pl[hy_row][tx + 3] = nhy[sl] - mv;
     __syncthreads();
     if (P.valid) {
       const int idx = P.i + nx * P.j + pk;
@@ -679,6 +684,12 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
         A.u[c][idx] = un;
         partial_add(st[c], (double)un, idx);
       }
+      if (A.pout) A.pout[idx] = q[3];
+      if (A.p0) {
+        const float ev = (float)(q[3] - p0v[sl]);
+        A.e[idx] = ev;
+        es += (double)ev * ev;
+      }
     }
     // k + 5 <= nz + 4 stays a valid mirror / wrap index for nz >= 6.
     pr[sl] = A.p[pz(k + 5)];
@@ -687,7 +698,7 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 6; ++s) q[s] = q[s + 1];
-    q[6] = pr[1 - sl];
+    q[6] = pr[1 - sl] - mv;
   };
   // Whole pairs in the loop (every back edge leaves slot 1's loads last in
   // flight), the odd last plane after it.
@@ -699,6 +710,10 @@ __global__ void __launch_bounds__(kTX *kTY) correct_kernel(const CorrectArgs<T, 
   if (k < P.z1) step(k, std::integral_constant<int, 0>{});
 #pragma unroll
   for (int c = 0; c < 3; ++c) partial_store(st[c], A.part + 3 * blockIdx.x + c, sc);
+  if (A.epart) {
+    es = block_sum<kNW>(es, sc.v);
+    if (threadIdx.x == 0) A.epart[blockIdx.x] = es;
+  }
 }
 
 template <typename T, typename TP>

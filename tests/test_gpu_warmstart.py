@@ -264,7 +264,9 @@ def test_guess_residual_is_bit_exact(adaptive):
     which open every solve of the warm forms: q = p_prev + e_prev (+ beta
     (e_prev - e_prev2), the FP32 difference) in FP64 and r = rhs - L q with
     the kernels' operation order, recomputed in numpy from the solver's own
-    fields and compared bit for bit."""
+    fields and compared bit for bit; and each step's increment
+    e = p_new - p_prev, with p_new after the periodic mean shift (its mean
+    at round-off)."""
     import torch
 
     from osinco3d_amd import capi
@@ -284,6 +286,11 @@ def test_guess_residual_is_bit_exact(adaptive):
         info = s.step(it)
         rhs = s.field(7)
         e1 = ep.cpu().numpy()
+        # The step's increment, formed in the velocity correction pass from
+        # the pressure after its mean shift: e = p_new - p_prev in FP32.
+        p_new = s.field(6)
+        assert abs(p_new.mean()) <= 1e-12 * np.abs(p_new).max(), it
+        assert np.array_equal(e.cpu().numpy(), (p_new - p_prev).astype(np.float32)), it
         if it >= 2:
             q = p_prev + e1.astype(np.float64)
             if adaptive and it >= 3 and info["extrap_beta"] != 0.0:
