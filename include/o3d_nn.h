@@ -31,7 +31,7 @@ int o3d_nn_zero_grad(void *nn);
 /* in: 2 float32 planes (channel 0 then channel 1), out: 1 float32 plane. */
 int o3d_nn_forward(void *nn, const float *in, float *out);
 /* Back-propagates d(loss)/d(out) through the last forward; accumulates the
- * weight gradients. */
+ * weight gradients.  The input of that forward must still hold its values. */
 int o3d_nn_backward(void *nn, const float *dout);
 /* Mean squared error of out against target (FP64 sum), writing
  * dout = scale * 2 (out - target) / n.  loss (host) receives the mean. */

@@ -522,49 +522,68 @@ __global__ void pack_input_kernel(const float *__restrict__ in, bf16 *__restrict
 }
 
 // The first convolution in the forward pass: only 2 of its 16 (padded)
-// input channels are non-zero, so it runs as a direct 3x3x3 convolution on
-// the vector ALUs instead of an implicit GEMM whose K would be 8/9 zeros.
-// One thread per voxel computes all 16 output channels from the 27 taps x 2
-// channels (the same BF16 inputs and weights as the MFMA path, FP32 sums),
-// then bias and leaky ReLU; the weights (16 x 27 x 2) are staged in LDS.
-__global__ void __launch_bounds__(256) conv_first_kernel(const bf16 *__restrict__ x,
+// input channels are non-zero, so its K is made dense: k = 2*tap + channel
+// over the 54 real pairs, zero-padded to 64 (two MFMA K-chunks instead of
+// the padded layout's 14).  A wave computes kFirstTiles tiles of 16
+// consecutive voxels x 16 output channels (the block's weight staging
+// serves 64 * kFirstTiles voxels); lane (l16, g) gathers, per chunk, the
+// two channels of taps 16*chunk + 4g .. +3 of voxel l16 straight from the
+// network's FP32 input planes (the 16 lanes of a tap read 16 consecutive
+// floats), rounded to BF16 as the packed input holds them.  The dense
+// weights are staged in LDS from the padded layout (rows padded by 16
+// bytes: the 16 lanes of a B fragment hit distinct banks); bias and leaky
+// ReLU follow.
+constexpr int kFirstTiles = 4;
+__global__ void __launch_bounds__(256) conv_first_kernel(const float *__restrict__ in,
                                                          const bf16 *__restrict__ w,
                                                          const float *__restrict__ bias,
                                                          bf16 *__restrict__ y, int nx, int ny,
                                                          int nz, int kp) {
-  __shared__ float ws[27][2][16];
-  __shared__ float bs[16];
-  for (int e = threadIdx.x; e < 27 * 2 * 16; e += blockDim.x) {
-    const int co = e & 15, c = (e >> 4) & 1, t = e >> 5;
-    ws[t][c][co] = (float)w[(size_t)co * kp + t * 16 + c];
+  __shared__ bf16 wd[16][64 + 8];
+  for (int e = threadIdx.x; e < 16 * 64; e += blockDim.x) {
+    const int co = e >> 6, k = e & 63;
+    wd[co][k] = k < 54 ? w[(size_t)co * kp + (k >> 1) * 16 + (k & 1)] : (bf16)0.f;
   }
-  if (threadIdx.x < 16) bs[threadIdx.x] = bias[threadIdx.x];
   __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, l16 = lane & 15;
   const int V = nx * ny * nz;
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  const int i = v % nx, j = (v / nx) % ny, k = v / (nx * ny);
-  float acc[16];
+  const float b = bias[l16];
+  for (int tt = 0; tt < kFirstTiles; ++tt) {
+    const int vbase = ((blockIdx.x * 4 + wv) * kFirstTiles + tt) * 16;
+    if (vbase >= V) return;  // after the kernel's only barrier
+    const int vm = vbase + l16;
+    const bool vin = vm < V;
+    const int i = vm % nx, j = (vm / nx) % ny, k = vm / (nx * ny);
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int co = 0; co < 16; ++co) acc[co] = bs[co];
+    for (int kc = 0; kc < 2; ++kc) {
+      bf16x8 av;
 #pragma unroll
-  for (int t = 0; t < 27; ++t) {
-    const int vx = wrapi(i + t % 3 - 1, nx), vy = wrapi(j + (t / 3) % 3 - 1, ny),
-              vz = wrapi(k + t / 9 - 1, nz);
-    const bf16x2 pr = *(const bf16x2 *)(x + (size_t)(vx + nx * (vy + ny * vz)) * 16);
-    const float a0 = (float)pr[0], a1 = (float)pr[1];
+      for (int q = 0; q < 4; ++q) {
+        const int t = 16 * kc + 4 * g + q;
+        bf16x2 pr;
+        pr[0] = pr[1] = (bf16)0.f;
+        if (t < 27 && vin) {
+          const int vx = wrapi(i + t % 3 - 1, nx), vy = wrapi(j + (t / 3) % 3 - 1, ny),
+                    vz = wrapi(k + t / 9 - 1, nz);
+          const size_t u = (size_t)vx + (size_t)nx * (vy + (size_t)ny * vz);
+          pr[0] = (bf16)in[u];
+          pr[1] = (bf16)in[(size_t)V + u];
+        }
+        av[2 * q] = pr[0];
+        av[2 * q + 1] = pr[1];
+      }
+      const bf16x8 bv = *(const bf16x8 *)&wd[l16][32 * kc + 8 * g];
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    // acc[q] = C[voxel vbase + 4g + q][channel l16]
 #pragma unroll
-    for (int co = 0; co < 16; ++co) acc[co] += a0 * ws[t][0][co] + a1 * ws[t][1][co];
+    for (int q = 0; q < 4; ++q) {
+      const int vox = vbase + 4 * g + q;
+      if (vox < V) y[(size_t)vox * 16 + l16] = (bf16)act_f(acc[q] + b);
+    }
   }
-  bf16x8 o0, o1;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    o0[q] = (bf16)act_f(acc[q]);
-    o1[q] = (bf16)act_f(acc[8 + q]);
-  }
-  bf16x8 *dst = (bf16x8 *)(y + (size_t)v * 16);
-  dst[0] = o0;
-  dst[1] = o1;
 }
 
 // 2x average pooling onto the grid (nx/2, ny/2, nz/2): one thread per
@@ -662,13 +681,19 @@ __global__ void __launch_bounds__(256) act_bwd_kernel(const float *__restrict__ 
 
 // out[v] = sum_c wh[c] d0[v][c] + bh + id[v]: the network learns a
 // correction to its second input plane (the previous increment).
+// C a multiple of 8: 16-byte loads of d0, the channels summed in order.
 __global__ void head_fwd_kernel(const bf16 *__restrict__ d0, const float *__restrict__ wh,
                                 const float *__restrict__ id, float *__restrict__ out, int V,
                                 int C) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
   float s = wh[C];
-  for (int c = 0; c < C; ++c) s += wh[c] * (float)d0[(size_t)v * C + c];
+  const bf16x8 *row = (const bf16x8 *)(d0 + (size_t)v * C);
+  for (int gq = 0; gq < C / 8; ++gq) {
+    const bf16x8 h = row[gq];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += wh[8 * gq + q] * (float)h[q];
+  }
   out[v] = s + id[v];
 }
 
@@ -779,6 +804,10 @@ struct Net {
   // u0 / u1 (the up-sampled d1 / d2) not formed by the last forward pass:
   // its convolutions 12 / 10 read d1 / d2 up-sampled as they staged them.
   bool up_stale[2] = {false, false};
+  // The input of the last forward pass (its caller keeps it for the
+  // backward pass), not yet packed into xin.
+  const float *fwd_in = nullptr;
+  bool xin_stale = false;
   std::vector<void *> allocs;
 
   ~Net() {
@@ -839,6 +868,9 @@ int alloc_activations(Net &n, int nx, int ny, int nz) {
     n.V[l] = (size_t)n.nx[l] * n.ny[l] * n.nz[l];
   }
   if (n.act_arena) { (void)hipFree(n.act_arena); n.act_arena = nullptr; }
+  n.fwd_in = nullptr;  // a backward pass needs a forward pass on the new grid
+  n.xin_stale = false;
+  n.up_stale[0] = n.up_stale[1] = false;
   if (n.gbuf) { (void)hipFree(n.gbuf); n.gbuf = nullptr; }
   if (n.part) { (void)hipFree(n.part); n.part = nullptr; }
   // bf16 activations: channels per level.
@@ -919,17 +951,22 @@ void init_params(std::vector<float> &w, const Net &n, uint64_t seed) {
   for (int c = 0; c < 16; ++c) w[n.head_off + c] = (float)((2 * next() - 1) * 0.02);
 }
 
+// The first convolution from the network's FP32 input planes.
+int conv_first_fwd(Net &n, const float *in, bf16 *y) {
+  const Conv &c = n.conv[0];
+  if (c.ci != 16 || c.co != 16) return o3d::set_error("first convolution: 16 -> 16 expected");
+  const int V = (int)n.V[c.level];
+  conv_first_kernel<<<(V + 64 * kFirstTiles - 1) / (64 * kFirstTiles), 256, 0, st()>>>(
+      in, c.wf, n.W + c.boff, y,
+                                                       n.nx[c.level], n.ny[c.level],
+                                                       n.nz[c.level], c.kpf);
+  NNCHK(hipGetLastError());
+  n.last_flops += 54.0 * 2 * c.co * (double)V;  // the two real input channels
+  return 0;
+}
+
 int conv_fwd(Net &n, int li, const bf16 *x, const bf16 *skip, bf16 *y) {
   const Conv &c = n.conv[li];
-  if (li == 0 && c.ci == 16 && c.co == 16 && !skip) {
-    const int V = (int)n.V[c.level];
-    conv_first_kernel<<<(V + 255) / 256, 256, 0, st()>>>(x, c.wf, n.W + c.boff, y,
-                                                         n.nx[c.level], n.ny[c.level],
-                                                         n.nz[c.level], c.kpf);
-    NNCHK(hipGetLastError());
-    n.last_flops += 54.0 * 2 * c.co * (double)V;  // the two real input channels
-    return 0;
-  }
   ConvArgs<bf16> a;
   a.x = x; a.w = c.wf; a.bias = n.W + c.boff; a.skip = skip; a.y = y;
   a.nx = n.nx[c.level]; a.ny = n.ny[c.level]; a.nz = n.nz[c.level];
@@ -1000,8 +1037,11 @@ int elem_blocks(size_t n) { return (int)((n + 255) / 256); }
 int forward(Net &n, const float *in, float *out) {
   n.last_flops = 0.0;
   NNCHK(hipEventRecord(n.ev[0], st()));
-  pack_input_kernel<<<elem_blocks(n.V[0]), 256, 0, st()>>>(in, n.xin, (int)n.V[0], 16);
-  NCHK(conv_fwd(n, 0, n.xin, nullptr, n.e0a));
+  // The packed BF16 input (the first layer's weight gradient reads it) is
+  // formed only when a backward pass follows.
+  n.fwd_in = in;
+  n.xin_stale = true;
+  NCHK(conv_first_fwd(n, in, n.e0a));
   NCHK(conv_fwd(n, 1, n.e0a, nullptr, n.e0));
   pool_fwd_kernel<<<elem_blocks(n.V[1] * 2), 256, 0, st()>>>(n.e0, n.p1, n.nx[0], n.ny[0],
                                                                n.nz[0], 16);
@@ -1085,6 +1125,11 @@ int backward(Net &n, const float *dout) {
   pool_bwd_kernel<<<elem_blocks(n.V[0] * 16), 256, 0, st()>>>(n.g_p1, n.g_e0, n.nx[0], n.ny[0],
                                                                n.nz[0], 16);
   NCHK(conv_bwd(n, 1, n.g_e0, n.e0, n.e0a, nullptr, n.g_e0a));
+  if (n.xin_stale) {
+    pack_input_kernel<<<elem_blocks(n.V[0]), 256, 0, st()>>>(n.fwd_in, n.xin, (int)n.V[0], 16);
+    NNCHK(hipGetLastError());
+    n.xin_stale = false;
+  }
   NCHK(conv_bwd(n, 0, n.g_e0a, n.e0a, n.xin, nullptr, nullptr));
   NNCHK(hipGetLastError());
   return 0;

@@ -68,10 +68,13 @@ def test_forward_matches_oracle(torch, shape):
 # BF16 roundings and leaky-ReLU slopes it flips.  At 16^3 the GPU's FP32
 # sums round like the oracle's almost everywhere (forward 7e-4, gradients
 # within 8e-3); on the larger grid, whose fine levels run the LDS-staged
-# kernel forward and backward, more roundings differ (forward 2.4e-3), and
-# the check is the one a wrong tap, halo or channel would fail: every
-# layer's gradient within cos 0.995 and 15 % of the oracle's.
-BWD_TOL = {(16, 16, 16): (5e-2, 0.999), (32, 64, 64): (0.15, 0.995)}
+# kernel forward and backward, more roundings differ (forward 1.7e-3).
+# There the oracle's own gradients, under 3e-4 and 1e-3 relative input
+# noise, move by up to 16.5 % with cosines down to 0.9934 (layers 2, 4, 5),
+# so the check is set at that floor, the one a wrong tap, halo or channel
+# would fail by far: every layer's gradient within cos 0.99 and 20 % of
+# the oracle's.
+BWD_TOL = {(16, 16, 16): (5e-2, 0.999), (32, 64, 64): (0.2, 0.99)}
 
 
 @pytest.mark.parametrize("shape", list(BWD_TOL))
