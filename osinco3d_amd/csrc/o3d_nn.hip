@@ -374,10 +374,12 @@ inline long long lds_bricks(int nx, int ny, int nz, int ty) {
 
 // Shape of the LDS-staged kernel for a convolution: rows per brick (4 while
 // that leaves >= 512 blocks, else 1 while >= 256) and output tiles per block
-// (the natural 16*NT channels, at most 2 tiles for LDS, or 1 tile when that
-// gives the coarser grids enough blocks); ty = 0 when it does not apply
-// (64 input channels: one-row bricks only; 128: never; the coarsest grids
-// split the K chunks among the waves of a block, conv_kw_kernel).
+// (the natural 16*NT channels, at most 2 tiles for LDS); ty = 0 when it does
+// not apply (64 input channels: one-row bricks only; 128: never; coarser
+// grids split the K chunks among the waves of a block, conv_kw_kernel,
+// which measured faster there: the 64^3 network's forward pass took 0.246 ms
+// with its 16^3 layers staged through LDS one output tile per block, 0.228
+// ms with them on conv_kw_kernel).
 struct LdsPlan {
   int ty = 0, nt = 0;
 };
@@ -393,9 +395,6 @@ inline LdsPlan lds_plan(int nx, int ny, int nz, int ci, int co) {
   } else if (b1 * (co / (16 * nat)) >= 256) {
     p.ty = 1;
     p.nt = nat;
-  } else if (b1 * (co / 16) >= 256) {
-    p.ty = 1;
-    p.nt = 1;
   }
   return p;
 }

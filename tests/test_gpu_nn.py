@@ -41,9 +41,9 @@ def _net(torch, shape, P):
 
 # (32, 64, 64): its 64x64x32 and 32x32x16 levels run the LDS-staged
 # convolution (four- and one-row bricks, the decoder's up-sampling fused
-# into it), the coarser ones the split-K path; (64, 64, 64), the network of
-# the 256^3 solver: the LDS-staged kernel down to 16^3 (one output tile per
-# block there), split-K at 8^3.
+# into it), the coarser ones the K split among a block's waves; (64, 64,
+# 64), the network of the 256^3 solver: the LDS-staged kernel at 64^3 and
+# 32^3, the K split at 16^3 and 8^3.
 @pytest.mark.parametrize("shape", [(16, 16, 16), (8, 16, 32), (32, 64, 64), (64, 64, 64)])
 def test_forward_matches_oracle(torch, shape):
     print(f"[gpu] nn forward {shape}", flush=True)
