@@ -99,6 +99,8 @@ class Network:
         capi.check(self.lib.o3d_nn_forward(self.h, capi.ptr(inp), capi.ptr(out)))
 
     def backward(self, dout):
+        """Back-propagates dout through the last forward pass (whose input
+        tensor must still hold its values) and accumulates the gradients."""
         capi.check(self.lib.o3d_nn_backward(self.h, capi.ptr(dout)))
 
     def zero_grad(self):
