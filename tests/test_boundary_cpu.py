@@ -148,3 +148,25 @@ def test_fields_bin_round_trip(tmp_path):
     assert np.allclose(d["y"], -1.0 + np.arange(6) * 0.4)
     with pytest.raises(ValueError):
         read_fields(str(p), Config(nx=8, ny=6, nz=5))
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py's contract on the command line: it has no multi-GPU mode, so
+    --gpus above 1 (or a launcher's WORLD_SIZE above 1) ends with a message
+    and exit status 2 before touching a device; --help lists the flags the
+    harness passes."""
+    import subprocess
+    import sys
+    bench = os.path.join(ROOT, "bench.py")
+    r = subprocess.run([sys.executable, bench, "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr, (r.returncode, r.stderr)
+    env = dict(os.environ, WORLD_SIZE="4")
+    r = subprocess.run([sys.executable, bench], capture_output=True, text=True, timeout=120,
+                       env=env)
+    assert r.returncode == 2 and "WORLD_SIZE 4" in r.stderr, (r.returncode, r.stderr)
+    r = subprocess.run([sys.executable, bench, "--help"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0
+    for flag in ("--gpus", "--steps", "--warmup", "--workload", "--forms"):
+        assert flag in r.stdout, flag
