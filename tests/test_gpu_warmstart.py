@@ -304,3 +304,45 @@ def test_guess_residual_is_bit_exact(adaptive):
     capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
     s.close()
     assert checked == steps - 1
+
+
+def test_fp64_form_with_extrapolated_guess():
+    """The FP64 form accepts the extrapolated warm start too (the guess and
+    residual kernels in their FP64-rhs instances, the increment formed in
+    the FP64 correction pass): the solves end on the reference's rules, the
+    fields track the FP64 form started from the previous pressure to the
+    eps-tied tolerance with fewer sweeps, and each step's increment is
+    p_new - p_prev in FP32, bit for bit."""
+    import torch
+
+    from osinco3d_amd import capi
+    from osinco3d_amd.solver import FP64
+    n, steps = 32, 8
+    cfg = tgv_config(n, steps, 3)
+    runs = {}
+    for name, warm in (("previous", WARM_PREVIOUS), ("extrapolation", WARM_EXTRAPOLATE)):
+        s = Solver(cfg, FP64, warm)
+        s.init_taylor_green()
+        cap = None
+        if warm == WARM_EXTRAPOLATE:
+            cap = [torch.empty((n, n, n), dtype=torch.float32, device="cuda") for _ in range(3)]
+            capi.check(capi.load().o3d_solver_capture(s.handle, *[capi.ptr(t) for t in cap]))
+        sweeps = []
+        for it in range(1, steps + 1):
+            p_prev = s.field(6)
+            info = s.step(it)
+            assert info["sor_reason"] in (0, 1) and not info["diverged"], (name, it)
+            sweeps.append(info["sor_exit_iter"])
+            if cap is not None:
+                e = cap[2].cpu().numpy()
+                assert np.array_equal(e, (s.field(6) - p_prev).astype(np.float32)), it
+        if cap is not None:
+            capi.check(capi.load().o3d_solver_capture(s.handle, None, None, None))
+        runs[name] = (np.array(sweeps), s.download())
+        s.close()
+    (sp, fp), (se, fe) = runs["previous"], runs["extrapolation"]
+    print(f"[warm] FP64 form at {n}^3: sweeps/step previous {sp.mean():.1f}, extrapolation "
+          f"{se.mean():.1f}", flush=True)
+    assert se[2:].mean() < sp[2:].mean()
+    for c in range(3):
+        assert np.abs(fe[c] - fp[c]).max() <= 1e-6
