@@ -831,7 +831,6 @@ struct Net {
     if (r_ != 0) return r_;  \
   } while (0)
 
-const int kCh[4] = {16, 32, 64, 128};
 
 void build_layers(Net &n) {
   // (ci, co, level): encoder, bottleneck, decoder (14 convolutions).

@@ -210,10 +210,12 @@ __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsi
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
-// At most 64 VGPRs where the compiler can fit them (8 waves per SIMD: four
-// 512-thread blocks per CU).
+// FP32: at most 64 VGPRs (8 waves per SIMD: four 512-thread blocks per CU).
+// FP64 needs more registers and runs at 5 waves per SIMD.
+template <typename T>
+constexpr int sor_waves() { return sizeof(T) == 4 ? 8 : 5; }
 template <typename T, typename TR>
-__global__ void __launch_bounds__(SNT, 8) sor_kernel(const SorArgs<T, TR> a) {
+__global__ void __launch_bounds__(SNT, sor_waves<T>()) sor_kernel(const SorArgs<T, TR> a) {
   constexpr int PPS = SorShape<T>::PPS;
   constexpr int PD = SorShape<T>::PD;
   constexpr int NO = SorShape<T>::NO, NRD = SorShape<T>::NRD, NRH = SorShape<T>::NRH;
