@@ -153,12 +153,17 @@ __device__ __forceinline__ float2 mg_interp_pair(const float *uc, int I, int j, 
 }
 
 // One colour of a red-black Gauss-Seidel sweep, one thread per cell of that
-// colour (mx even: every row holds mx / 2 cells of each colour).
+// colour (mx even: every row holds mx / 2 cells of each colour).  The
+// smoothing, restriction and prolongation kernels take their blocks in
+// XCD-aware order (xcd_tile): each XCD works on a contiguous run of planes
+// and finds most neighbour planes in its own L2 (at 256^3: the 128^3
+// prolongation 10.2 -> 8.0 us, restriction 10.1 -> 9.7 us, smoothing 7.5 ->
+// 7.2 us).  The pooling kernels only stream and keep the plain order.
 __global__ void __launch_bounds__(256) mg_smooth_kernel(float *__restrict__ u,
                                                         const float *__restrict__ f, MgGrid g,
                                                         int colour) {
   const int hx = g.mx >> 1;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= hx * g.my * g.mz) return;
   const int row = p / hx, t = p - row * hx;
   const int k = row / g.my, j = row - k * g.my;
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(256) mg_smooth_kernel(float *__restrict__ u,
 __global__ void __launch_bounds__(256) mg_smooth_any_kernel(float *__restrict__ u,
                                                             const float *__restrict__ f,
                                                             MgGrid g, int colour) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (idx >= g.mx * g.my * g.mz) return;
   const int row = idx / g.mx, i = idx - row * g.mx;
   const int k = row / g.my, j = row - k * g.my;
@@ -185,7 +190,7 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(float *__restrict__ fc
                                                           const float *__restrict__ u,
                                                           const float *__restrict__ f, MgGrid g) {
   const int cx = g.mx >> 1, cy = g.my >> 1, cz = g.mz >> 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (idx >= cx * cy * cz) return;
   const int crow = idx / cx, I = idx - crow * cx;
   const int K = crow / cy, J = crow - K * cy;
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(256) mg_prolong_kernel(float *__restrict__ u,
                                                          const float *__restrict__ uc, int mx,
                                                          int my, int mz, int px, int py, int pz) {
   const int cx = mx >> 1;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= cx * my * mz) return;
   const int row = p / cx, I = p - row * cx;
   const int k = row / my, j = row - k * my;
